@@ -1,0 +1,109 @@
+"""ctypes binding of libcaffe_hip.so, the C ABI declared in include/caffe_hip.h.
+
+The library is built in-tree by ``caffe_old_amd.build`` (hipcc, gfx950).  Loading
+fails loudly when it is missing: there is no CPU fallback for the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcaffe_hip.so")
+
+CAFFE_OK = 0
+CAFFE_ERR_ARG = -1
+CAFFE_ERR_HIP = -2
+CAFFE_ERR_STATE = -3
+CAFFE_ERR_NOT_CONVERGED = -4
+
+SOLVE_PLAIN_FP64 = 0
+SOLVE_IR_FP32 = 1
+SOLVE_LEARNED_FCG = 2
+
+# every symbol include/caffe_hip.h declares
+EXPORTS = [
+    "caffe_ctx_create", "caffe_ctx_destroy", "caffe_last_error", "caffe_case_setup", "caffe_set_solve_opts",
+    "caffe_solve", "caffe_save_u0", "caffe_momentum_assemble", "caffe_momentum_solve", "caffe_rhiechow",
+    "caffe_pcorr_assemble", "caffe_pcorr_solve", "caffe_correct", "caffe_ib_field", "caffe_ib_coeffs",
+    "caffe_grad_u", "caffe_step_report", "caffe_last_reports", "caffe_field_download", "caffe_field_upload",
+    "caffe_metric_download", "caffe_matrix_download", "caffe_rhs_download", "caffe_ghost_rows", "caffe_sync",
+    "caffe_padded_size",
+]
+
+
+class CaffeCase(C.Structure):
+    _fields_ = [
+        ("nI", C.c_int), ("nJ", C.c_int), ("nK", C.c_int),
+        ("algo", C.c_int), ("steady", C.c_int),
+        ("rho", C.c_double), ("mu", C.c_double), ("alphaU", C.c_double), ("alphaP", C.c_double),
+        ("nInner", C.c_int), ("nPCorr", C.c_int),
+        ("bcType", C.c_int * 6), ("bcVec", C.c_double * 18), ("bcVal", C.c_double * 6),
+        ("uInit", C.c_double * 3), ("pInit", C.c_double),
+        ("sphereR", C.c_double), ("sphereC", C.c_double * 3),
+        ("uniform", C.c_int),
+    ]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("mode", C.c_int), ("rtol", C.c_double), ("maxit", C.c_int), ("inner_rtol", C.c_double)]
+
+
+class SolveReport(C.Structure):
+    _fields_ = [("iters", C.c_int), ("refinements", C.c_int), ("relres", C.c_double), ("converged", C.c_int),
+                ("fell_back", C.c_int), ("ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class CaffeError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load the HIP library.  Raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise CaffeError(f"{path} is not built: run __graft_entry__.build() (hipcc, gfx950)")
+    L = C.CDLL(path)
+    vp, dp, ip, lp = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_long)
+    sig = {
+        "caffe_ctx_create": ([C.c_int, C.POINTER(C.c_void_p)], C.c_int),
+        "caffe_ctx_destroy": ([vp], C.c_int),
+        "caffe_last_error": ([vp], C.c_char_p),
+        "caffe_case_setup": ([vp, C.POINTER(CaffeCase), dp], C.c_int),
+        "caffe_set_solve_opts": ([vp, C.c_int, C.POINTER(SolveOpts)], C.c_int),
+        "caffe_solve": ([vp, C.c_double, dp], C.c_int),
+        "caffe_save_u0": ([vp], C.c_int),
+        "caffe_momentum_assemble": ([vp, C.c_double], C.c_int),
+        "caffe_momentum_solve": ([vp, C.POINTER(SolveReport)], C.c_int),
+        "caffe_rhiechow": ([vp], C.c_int),
+        "caffe_pcorr_assemble": ([vp], C.c_int),
+        "caffe_pcorr_solve": ([vp, C.POINTER(SolveOpts), C.POINTER(SolveReport)], C.c_int),
+        "caffe_correct": ([vp], C.c_int),
+        "caffe_ib_field": ([vp], C.c_int),
+        "caffe_ib_coeffs": ([vp], C.c_int),
+        "caffe_grad_u": ([vp], C.c_int),
+        "caffe_step_report": ([vp, C.c_double, dp, dp, dp], C.c_int),
+        "caffe_last_reports": ([vp, C.POINTER(SolveReport), C.POINTER(SolveReport)], C.c_int),
+        "caffe_field_download": ([vp, C.c_char_p, dp, C.c_long], C.c_int),
+        "caffe_field_upload": ([vp, C.c_char_p, dp, C.c_long], C.c_int),
+        "caffe_metric_download": ([vp, C.c_char_p, dp, C.c_long], C.c_int),
+        "caffe_matrix_download": ([vp, C.c_int, dp, C.c_long], C.c_int),
+        "caffe_rhs_download": ([vp, C.c_int, dp, C.c_long], C.c_int),
+        "caffe_ghost_rows": ([vp, ip, lp, lp, dp, dp], C.c_int),
+        "caffe_sync": ([vp], C.c_int),
+        "caffe_padded_size": ([vp], C.c_long),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L

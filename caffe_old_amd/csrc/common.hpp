@@ -1,0 +1,172 @@
+// common.hpp -- device-side layout, geometry access and deterministic reductions
+// shared by every CAFFE hot-path kernel.
+//
+// Layout: every cell array is padded to (nI+2)*(nJ+2)*(nK+2), i fastest.  The
+// one-cell halo is the Field's six boundary patches (Field.tpp:89-128: (-1,j,k)
+// is the west patch), so stencils never branch on the domain edge.  Face arrays
+// use the same shape: entry p of an I-face array is the WEST face of padded cell
+// p, so the east face of cell p is entry p+1 (the last one lands in the halo
+// slot).  This keeps faceE(i) == faceW(i+1) a single storage location, as
+// Field::facesI_ does (Field.h:96-101).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace caffe {
+
+enum : int8_t { ST_ACTIVE = 0, ST_GHOST = 1, ST_INACTIVE = 2 };  // IndexMap.h:39
+enum : int { BC_FIXED = 0, BC_ZERO_GRADIENT = 1, BC_EMPTY = 2 };    // PrimitiveBoundaryCondition.h:16
+enum : int { F_INLET = 0, F_OUTLET = 1, F_WALL = 2, F_EMPTY = 3, F_ZEROGRAD = 4 };  // FlowBoundaryConditions.h:10
+
+struct V3 { double x, y, z; };
+__host__ __device__ inline V3 mk(double x, double y, double z) { V3 v; v.x = x; v.y = y; v.z = z; return v; }
+__host__ __device__ inline V3 operator+(V3 a, V3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__host__ __device__ inline V3 operator-(V3 a, V3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__host__ __device__ inline V3 operator-(V3 a) { return mk(-a.x, -a.y, -a.z); }
+__host__ __device__ inline V3 operator*(V3 a, double s) { return mk(a.x * s, a.y * s, a.z * s); }
+__host__ __device__ inline V3 operator*(double s, V3 a) { return mk(a.x * s, a.y * s, a.z * s); }
+// Vector3D::operator/= multiplies by the reciprocal (Vector3D.cpp:98-107)
+__host__ __device__ inline V3 operator/(V3 a, double s) { s = 1. / s; return mk(a.x * s, a.y * s, a.z * s); }
+__host__ __device__ inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__host__ __device__ inline V3 cross(V3 u, V3 v) {
+    return mk(u.y * v.z - u.z * v.y, u.z * v.x - u.x * v.z, u.x * v.y - u.y * v.x);
+}
+__host__ __device__ inline double mag(V3 u) { return sqrt(u.x * u.x + u.y * u.y + u.z * u.z); }
+__host__ __device__ inline V3 unitv(V3 u) { double inv = 1. / mag(u); return mk(inv * u.x, inv * u.y, inv * u.z); }
+
+// Padded grid dimensions.
+struct Dims {
+    int nI, nJ, nK;
+    int sJ;      // nI + 2
+    long sK;     // (nI + 2) * (nJ + 2)
+    long P;      // padded cell count
+    long N;      // interior cell count
+    __host__ __device__ long pidx(int i, int j, int k) const { return (long)(i + 1) + (long)(j + 1) * sJ + (long)(k + 1) * sK; }
+    // interior linear id (i fastest) -> padded index and (i,j,k)
+    __host__ __device__ long cell(long q, int& i, int& j, int& k) const {
+        i = (int)(q % nI); long r = q / nI; j = (int)(r % nJ); k = (int)(r / nJ);
+        return pidx(i, j, k);
+    }
+};
+
+// Mesh metrics.  UNIFORM meshes (a box of equal cells) evaluate every metric in
+// closed form from (hx, hy, hz); general meshes read per-face/per-cell arrays
+// computed on device from the node array with the reference's formulas
+// (HexaFvmMesh.cpp:626-918).
+struct Geo {
+    Dims D;
+    double hx, hy, hz;
+    const double* vol;
+    const double* S[3][3];   // S[dir][xyz]: area normal of the low (W/S/B) face of each cell
+    const double* dF[3];     // d = |S|^2 / (S . r) of that face
+    const double* gF[3];     // gE/gN/gT of the low-side cell (weight of the low cell's value)
+    const double* cF[3][3];  // c = S - r d of that face, seen from the low-side cell
+    const double* xc[3];     // cell centroid
+    const double* fc[3][3];  // face centroid of the low face
+};
+
+template <bool U> struct G;
+
+template <> struct G<true> {
+    __device__ static double vol(const Geo& g, long) { return g.hx * g.hy * g.hz; }
+    __device__ static V3 S(const Geo& g, int dir, long) {
+        return dir == 0 ? mk(g.hy * g.hz, 0., 0.) : (dir == 1 ? mk(0., g.hx * g.hz, 0.) : mk(0., 0., g.hx * g.hy));
+    }
+    // d at a face: |S| / |r|, r = cell-to-cell distance, or cell-to-face (h/2) on the domain boundary
+    __device__ static double d(const Geo& g, int dir, long, bool bnd) {
+        double v = dir == 0 ? g.hy * g.hz / g.hx : (dir == 1 ? g.hx * g.hz / g.hy : g.hx * g.hy / g.hz);
+        return bnd ? 2. * v : v;
+    }
+    __device__ static double gw(const Geo&, int, long) { return 0.5; }
+    __device__ static V3 xc(const Geo& g, int i, int j, int k, long) {
+        return mk((i + 0.5) * g.hx, (j + 0.5) * g.hy, (k + 0.5) * g.hz);
+    }
+    __device__ static V3 fc(const Geo& g, int dir, int i, int j, int k, long) {
+        return dir == 0 ? mk(i * g.hx, (j + 0.5) * g.hy, (k + 0.5) * g.hz)
+             : (dir == 1 ? mk((i + 0.5) * g.hx, j * g.hy, (k + 0.5) * g.hz) : mk((i + 0.5) * g.hx, (j + 0.5) * g.hy, k * g.hz));
+    }
+};
+
+template <> struct G<false> {
+    __device__ static double vol(const Geo& g, long p) { return g.vol[p]; }
+    __device__ static V3 S(const Geo& g, int dir, long f) { return mk(g.S[dir][0][f], g.S[dir][1][f], g.S[dir][2][f]); }
+    __device__ static double d(const Geo& g, int dir, long f, bool) { return g.dF[dir][f]; }
+    __device__ static double gw(const Geo& g, int dir, long f) { return g.gF[dir][f]; }
+    __device__ static V3 xc(const Geo& g, int, int, int, long p) { return mk(g.xc[0][p], g.xc[1][p], g.xc[2][p]); }
+    __device__ static V3 fc(const Geo& g, int dir, int, int, int, long f) { return mk(g.fc[dir][0][f], g.fc[dir][1][f], g.fc[dir][2][f]); }
+};
+
+// ------------------------------------------------------------------ reductions
+// Deterministic block reduction: wave64 butterfly then a fixed LDS tree.
+template <int BS>
+__device__ inline double blockSum(double v, double* lds) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    double r = 0.;
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < BS / 64; ++w) r += lds[w];
+    }
+    __syncthreads();
+    return r;  // valid in thread 0
+}
+template <int BS>
+__device__ inline double blockMax(double v, double* lds) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    double r = 0.;
+    if (threadIdx.x == 0) {
+        r = lds[0];
+        for (int w = 1; w < BS / 64; ++w) r = fmax(r, lds[w]);
+    }
+    __syncthreads();
+    return r;
+}
+
+// Grid-wide reduction with "last block finishes": every block writes its K partial
+// sums, the block that takes the last ticket adds the partials in block order (a
+// fixed order, so the result does not depend on scheduling) and runs Fin on them.
+constexpr int MAX_RED_BLOCKS = 4096;
+struct RedBuf {
+    double* part;          // MAX_RED_BLOCKS * 8
+    unsigned int* ticket;  // one counter, returns to 0 after each use
+};
+
+template <int BS, int K, class Fin>
+__device__ inline void gridReduce(const double (&v)[K], RedBuf rb, Fin fin) {
+    __shared__ double lds[BS / 64];
+    __shared__ bool last;
+    double s[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) s[q] = blockSum<BS>(v[q], lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) rb.part[(long)blockIdx.x * 8 + q] = s[q];
+        __threadfence();
+        unsigned int t = atomicAdd(rb.ticket, 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double acc[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) acc[q] = 0.;
+    // fixed split: thread t sums blocks t, t+BS, ... in order; then a fixed tree
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) acc[q] += __hip_atomic_load(&rb.part[(long)b * 8 + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    double tot[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) tot[q] = blockSum<BS>(acc[q], lds);
+    if (threadIdx.x == 0) {
+        fin(tot);
+        *rb.ticket = 0u;
+    }
+}
+
+}  // namespace caffe

@@ -1,0 +1,613 @@
+// fv_kernels.hip -- finite-volume assembly, Rhie-Chow, correction, gradients,
+// boundary patches, continuity and Courant kernels of the SIMPLE/PISO step.
+//
+// Each kernel restates one loop of the reference with the same IEEE operation
+// order (the library is built with -ffp-contract=off), so assembled
+// coefficients match the reference arithmetic.  Face values of d, h, p, p' and u
+// are not stored: an interior face value is recomputed from the two cell values
+// with the volume weight g (FvScheme.tpp:12-40) exactly when the reference would
+// read the stored one, and a boundary face value is the halo (patch) value,
+// which Field::set*FacesFromPatch keeps equal to the face (Field.tpp:280-404).
+#include "internal.hpp"
+
+namespace caffe {
+
+__device__ inline double dmin(double a, double b) { return (b < a) ? b : a; }  // std::min
+__device__ inline double dmax(double a, double b) { return (a < b) ? b : a; }  // std::max
+
+constexpr int BS = 256;
+
+// ------------------------------------------------------------------ boundary patches
+// PrimitiveBoundaryCondition::setBoundaries (PrimitiveBoundaryCondition.tpp:42-142):
+// ZERO_GRADIENT and EMPTY faces copy the boundary cell plane into the patch
+// (Field.tpp:364-404); FIXED faces keep their patch.
+struct HaloArgs { double* f[12]; int nf; int types[6]; };
+
+__global__ void k_halo(Dims D, HaloArgs a) {
+    const int face = blockIdx.y;
+    if (a.types[face] == BC_FIXED) return;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int n1, n2;
+    if (face < 2) { n1 = D.nJ; n2 = D.nK; } else if (face < 4) { n1 = D.nI; n2 = D.nK; } else { n1 = D.nI; n2 = D.nJ; }
+    if (t >= (long)n1 * n2) return;
+    int x = (int)(t % n1), y = (int)(t / n1);
+    long pin, pout;
+    switch (face) {
+    case 0: pin = D.pidx(D.nI - 1, x, y); pout = pin + 1; break;
+    case 1: pin = D.pidx(0, x, y); pout = pin - 1; break;
+    case 2: pin = D.pidx(x, D.nJ - 1, y); pout = pin + D.sJ; break;
+    case 3: pin = D.pidx(x, 0, y); pout = pin - D.sJ; break;
+    case 4: pin = D.pidx(x, y, D.nK - 1); pout = pin + D.sK; break;
+    default: pin = D.pidx(x, y, 0); pout = pin - D.sK; break;
+    }
+    for (int q = 0; q < a.nf; ++q) a.f[q][pout] = a.f[q][pin];
+}
+
+void launch_halo(Ctx& c, double* const* f, int nf, const int* types) {
+    const Dims& D = c.D;
+    long mx = (long)D.nJ * D.nK;
+    mx = std::max(mx, (long)D.nI * D.nK);
+    mx = std::max(mx, (long)D.nI * D.nJ);
+    for (int base = 0; base < nf; base += 12) {
+        HaloArgs a;
+        a.nf = std::min(12, nf - base);
+        for (int q = 0; q < a.nf; ++q) a.f[q] = f[base + q];
+        for (int q = 0; q < 6; ++q) a.types[q] = types[q];
+        dim3 grid(blocksFor(mx, BS), 6);
+        hipLaunchKernelGGL(k_halo, grid, dim3(BS), 0, c.stream, D, a);
+    }
+}
+
+// ------------------------------------------------------------------ momentum assembly
+// Simple::computeMomentum assembly loop (Simple.cpp:117-179) with the BC fold of
+// PrimitiveBoundaryCondition::setImplicitBoundaryCoefficients (PrimitiveBoundaryCondition.tpp:144-255).
+// The non-orthogonal term mu_f (grad u)_f . c_f is identically zero in the reference:
+// gradUField_ faces are never assigned (Simple.cpp:153-158, 355), so it is not evaluated.
+struct MomArgs {
+    const int8_t* st;
+    const double *mI, *mJ, *mK;
+    const double *u[3], *u0[3], *gp[3];
+    double* A; double* b; double* d;
+    double rho, mu, om, dt;
+    int steady;
+    int uT[6];
+};
+
+template <bool U>
+__global__ void k_mom_assemble(Geo g, MomArgs a) {
+    const Dims& D = g.D;
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= D.N) return;
+    int i, j, k;
+    const long p = D.cell(q, i, j, k);
+    const long P = D.P, sJ = D.sJ, sK = D.sK;
+    const int8_t s = a.st[p];
+    if (s != ST_ACTIVE) {
+        if (s == ST_INACTIVE) {  // no row in the reference; inert identity row here
+            a.A[p] = 1.;
+            for (int c = 1; c < 7; ++c) a.A[c * P + p] = 0.;
+            a.b[p] = 0.; a.b[P + p] = 0.; a.b[2 * P + p] = 0.;
+        }
+        return;
+    }
+    const double vol = G<U>::vol(g, p);
+    const double a0P = a.steady ? 0. : a.rho * vol / a.dt;
+    const double mE = a.mI[p + 1], mW = a.mI[p], mN = a.mJ[p + sJ], mS = a.mJ[p], mT = a.mK[p + sK], mB = a.mK[p];
+    const double dE = G<U>::d(g, 0, p + 1, i == D.nI - 1), dW = G<U>::d(g, 0, p, i == 0);
+    const double dN = G<U>::d(g, 1, p + sJ, j == D.nJ - 1), dS = G<U>::d(g, 1, p, j == 0);
+    const double dT = G<U>::d(g, 2, p + sK, k == D.nK - 1), dB = G<U>::d(g, 2, p, k == 0);
+    const double mu = a.mu;
+    double A[7];
+    A[1] = dmin(mE, 0.) - mu * dE;
+    A[2] = -dmax(mW, 0.) - mu * dW;
+    A[3] = dmin(mN, 0.) - mu * dN;
+    A[4] = -dmax(mS, 0.) - mu * dS;
+    A[5] = dmin(mT, 0.) - mu * dT;
+    A[6] = -dmax(mB, 0.) - mu * dB;
+    A[0] = (dmax(mE, 0.) + mu * dE - dmin(mW, 0.) + mu * dW + dmax(mN, 0.) + mu * dN - dmin(mS, 0.) + mu * dS
+            + dmax(mT, 0.) + mu * dT - dmin(mB, 0.) + mu * dB + a0P) / a.om;
+    double bx = a.u0[0][p] * a0P, by = a.u0[1][p] * a0P, bz = a.u0[2][p] * a0P;
+    const double mv = -vol;
+    bx += a.gp[0][p] * mv; by += a.gp[1][p] * mv; bz += a.gp[2][p] * mv;
+    const double r = (1. - a.om) * A[0];
+    bx += a.u[0][p] * r; by += a.u[1][p] * r; bz += a.u[2][p] * r;
+    // boundary fold, in the reference's face order E, W, N, S, T, B
+    const long nb[7] = {0, p + 1, p - 1, p + sJ, p - sJ, p + sK, p - sK};
+    const bool atB[7] = {false, i == D.nI - 1, i == 0, j == D.nJ - 1, j == 0, k == D.nK - 1, k == 0};
+#pragma unroll
+    for (int f = 1; f < 7; ++f) {
+        if (!atB[f]) continue;
+        if (a.uT[f - 1] == BC_FIXED) {
+            const long h = nb[f];
+            bx -= a.u[0][h] * A[f]; by -= a.u[1][h] * A[f]; bz -= a.u[2][h] * A[f];
+        } else {
+            A[0] += A[f]; A[f] = 0.;
+        }
+    }
+    a.A[p] = A[0];
+#pragma unroll
+    for (int f = 1; f < 7; ++f) {
+        // PETSc drops columns -1: off-domain and INACTIVE neighbours (IndexMap.cpp:47-78)
+        const bool keep = !atB[f] && a.st[nb[f]] != ST_INACTIVE;
+        a.A[f * P + p] = keep ? A[f] : 0.;
+    }
+    a.b[p] = bx; a.b[P + p] = by; a.b[2 * P + p] = bz;
+    a.d[p] = vol / A[0];
+}
+
+void launch_mom_assemble(Ctx& c, double dt) {
+    MomArgs a;
+    a.st = c.st; a.mI = c.mF[0]; a.mJ = c.mF[1]; a.mK = c.mF[2];
+    for (int q = 0; q < 3; ++q) { a.u[q] = c.u[q]; a.u0[q] = c.u0[q]; a.gp[q] = c.gp[q]; }
+    a.A = c.Am; a.b = c.bm; a.d = c.d;
+    a.rho = c.cs.rho; a.mu = c.cs.mu; a.om = c.cs.alphaU; a.dt = dt; a.steady = c.cs.steady;
+    for (int q = 0; q < 6; ++q) a.uT[q] = c.uT[q];
+    int nb = blocksFor(c.D.N, BS);
+    if (c.uniform) hipLaunchKernelGGL(k_mom_assemble<true>, dim3(nb), dim3(BS), 0, c.stream, c.geo, a);
+    else hipLaunchKernelGGL(k_mom_assemble<false>, dim3(nb), dim3(BS), 0, c.stream, c.geo, a);
+}
+
+// Simple.cpp:189-206: u = x for non-inactive cells; h = u + d grad p (ACTIVE) or u (GHOST)
+__global__ void k_mom_copyback(Dims D, const int8_t* st, const double* x, double* u0, double* u1, double* u2,
+                               double* h0, double* h1, double* h2, const double* d,
+                               const double* g0, const double* g1, const double* g2) {
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= D.N) return;
+    int i, j, k;
+    const long p = D.cell(q, i, j, k);
+    const int8_t s = st[p];
+    if (s == ST_INACTIVE) return;
+    const long P = D.P;
+    double ux = x[p], uy = x[P + p], uz = x[2 * P + p];
+    u0[p] = ux; u1[p] = uy; u2[p] = uz;
+    if (s == ST_ACTIVE) {
+        const double dd = d[p];
+        h0[p] = ux + dd * g0[p]; h1[p] = uy + dd * g1[p]; h2[p] = uz + dd * g2[p];
+    } else {
+        h0[p] = ux; h1[p] = uy; h2[p] = uz;
+    }
+}
+
+void launch_mom_copyback(Ctx& c) {
+    hipLaunchKernelGGL(k_mom_copyback, dim3(blocksFor(c.D.N, BS)), dim3(BS), 0, c.stream, c.D, c.st, c.xm,
+                       c.u[0], c.u[1], c.u[2], c.h[0], c.h[1], c.h[2], c.d, c.gp[0], c.gp[1], c.gp[2]);
+}
+
+// face value of a cell field on the high (E/N/T) face of cell p in direction dir
+template <bool U>
+__device__ inline double faceHi(const Geo& g, const double* phi, long p, long s, int dir, bool bnd) {
+    if (bnd) return phi[p + s];
+    const double w = G<U>::gw(g, dir, p + s);
+    return w * phi[p] + (1. - w) * phi[p + s];
+}
+// face value on the low (W/S/B) face of cell p: the high face of the low neighbour
+template <bool U>
+__device__ inline double faceLo(const Geo& g, const double* phi, long p, long s, int dir, bool bnd) {
+    if (bnd) return phi[p - s];
+    const double w = G<U>::gw(g, dir, p);
+    return w * phi[p - s] + (1. - w) * phi[p];
+}
+
+// ------------------------------------------------------------------ Rhie-Chow
+// Simple::rhieChowInterpolateFaces (Simple.cpp:410-449)
+struct RcArgs {
+    const int8_t* st;
+    const double *h[3], *d, *p;
+    double *mF[3];
+    double rho;
+};
+
+template <bool U>
+__global__ void k_rhie_chow(Geo g, RcArgs a) {
+    const Dims& D = g.D;
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= D.N) return;
+    int ijk[3];
+    const long p = D.cell(q, ijk[0], ijk[1], ijk[2]);
+    if (a.st[p] == ST_INACTIVE) return;
+    const long str[3] = {1, D.sJ, D.sK};
+    const int nn[3] = {D.nI, D.nJ, D.nK};
+    const double pP = a.p[p];
+#pragma unroll
+    for (int dir = 0; dir < 3; ++dir) {
+        const long s = str[dir];
+        if (ijk[dir] == 0) {  // W/S/B domain face, area normal -S
+            V3 hf = mk(a.h[0][p - s], a.h[1][p - s], a.h[2][p - s]);
+            V3 Sw = -G<U>::S(g, dir, p);
+            double m = -dot(hf, Sw) + a.d[p - s] * (a.p[p - s] - pP) * G<U>::d(g, dir, p, true);
+            m *= a.rho;
+            a.mF[dir][p] = m;
+        }
+        const bool bnd = ijk[dir] == nn[dir] - 1;
+        V3 hf = mk(faceHi<U>(g, a.h[0], p, s, dir, bnd), faceHi<U>(g, a.h[1], p, s, dir, bnd), faceHi<U>(g, a.h[2], p, s, dir, bnd));
+        const double df = faceHi<U>(g, a.d, p, s, dir, bnd);
+        double m = dot(hf, G<U>::S(g, dir, p + s)) - df * (a.p[p + s] - pP) * G<U>::d(g, dir, p + s, bnd);
+        m *= a.rho;
+        a.mF[dir][p + s] = m;
+    }
+}
+
+void launch_rhie_chow(Ctx& c) {
+    RcArgs a;
+    a.st = c.st;
+    for (int q = 0; q < 3; ++q) { a.h[q] = c.h[q]; a.mF[q] = c.mF[q]; }
+    a.d = c.d; a.p = c.p; a.rho = c.cs.rho;
+    int nb = blocksFor(c.D.N, BS);
+    if (c.uniform) hipLaunchKernelGGL(k_rhie_chow<true>, dim3(nb), dim3(BS), 0, c.stream, c.geo, a);
+    else hipLaunchKernelGGL(k_rhie_chow<false>, dim3(nb), dim3(BS), 0, c.stream, c.geo, a);
+}
+
+// ------------------------------------------------------------------ pressure-correction assembly
+// Simple::computePCorr assembly (Simple.cpp:231-269).  Stored symmetric: aP per cell
+// and one coefficient per face (aI at entry p couples cells p-1 and p).  An interior
+// face shared by two ACTIVE cells gets the same bits from both sides (d_f is one face
+// value, dE(i) == dW(i+1)), so it is written once, by the high-side cell; a face
+// toward a non-ACTIVE neighbour is written by the ACTIVE side.
+struct PcArgs {
+    const int8_t* st;
+    const double *d, *mI, *mJ, *mK, *pc;
+    double *A, *b;
+    double rho;
+    int pT[6];
+};
+
+template <bool U>
+__global__ void k_pc_assemble(Geo g, PcArgs a) {
+    const Dims& D = g.D;
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= D.N) return;
+    int i, j, k;
+    const long p = D.cell(q, i, j, k);
+    const long P = D.P, sJ = D.sJ, sK = D.sK;
+    const int8_t s = a.st[p];
+    if (s != ST_ACTIVE) {
+        if (s == ST_INACTIVE) { a.A[p] = 1.; a.b[p] = 0.; }
+        return;
+    }
+    const bool atB[7] = {false, i == D.nI - 1, i == 0, j == D.nJ - 1, j == 0, k == D.nK - 1, k == 0};
+    const double rho = a.rho;
+    double A[7];
+    A[1] = rho * faceHi<U>(g, a.d, p, 1, 0, atB[1]) * G<U>::d(g, 0, p + 1, atB[1]);
+    A[2] = rho * faceLo<U>(g, a.d, p, 1, 0, atB[2]) * G<U>::d(g, 0, p, atB[2]);
+    A[3] = rho * faceHi<U>(g, a.d, p, sJ, 1, atB[3]) * G<U>::d(g, 1, p + sJ, atB[3]);
+    A[4] = rho * faceLo<U>(g, a.d, p, sJ, 1, atB[4]) * G<U>::d(g, 1, p, atB[4]);
+    A[5] = rho * faceHi<U>(g, a.d, p, sK, 2, atB[5]) * G<U>::d(g, 2, p + sK, atB[5]);
+    A[6] = rho * faceLo<U>(g, a.d, p, sK, 2, atB[6]) * G<U>::d(g, 2, p, atB[6]);
+    A[0] = -(A[1] + A[2] + A[3] + A[4] + A[5] + A[6]);
+    double b = a.mI[p + 1] - a.mI[p] + a.mJ[p + sJ] - a.mJ[p] + a.mK[p + sK] - a.mK[p];
+    const long nb[7] = {0, p + 1, p - 1, p + sJ, p - sJ, p + sK, p - sK};
+#pragma unroll
+    for (int f = 1; f < 7; ++f) {
+        if (!atB[f]) continue;
+        if (a.pT[f - 1] == BC_FIXED) b -= A[f] * a.pc[nb[f]];
+        else { A[0] += A[f]; A[f] = 0.; }
+    }
+    a.A[p] = A[0];
+    a.b[p] = b;
+    // west/south/bottom faces: always written by this cell
+    a.A[P + p] = (!atB[2] && a.st[p - 1] != ST_INACTIVE) ? A[2] : 0.;
+    a.A[2 * P + p] = (!atB[4] && a.st[p - sJ] != ST_INACTIVE) ? A[4] : 0.;
+    a.A[3 * P + p] = (!atB[6] && a.st[p - sK] != ST_INACTIVE) ? A[6] : 0.;
+    // east/north/top faces: written here unless an ACTIVE neighbour writes them
+    if (atB[1] || a.st[p + 1] != ST_ACTIVE) a.A[P + p + 1] = (!atB[1] && a.st[p + 1] != ST_INACTIVE) ? A[1] : 0.;
+    if (atB[3] || a.st[p + sJ] != ST_ACTIVE) a.A[2 * P + p + sJ] = (!atB[3] && a.st[p + sJ] != ST_INACTIVE) ? A[3] : 0.;
+    if (atB[5] || a.st[p + sK] != ST_ACTIVE) a.A[3 * P + p + sK] = (!atB[5] && a.st[p + sK] != ST_INACTIVE) ? A[5] : 0.;
+}
+
+void launch_pc_assemble(Ctx& c) {
+    PcArgs a;
+    a.st = c.st; a.d = c.d; a.mI = c.mF[0]; a.mJ = c.mF[1]; a.mK = c.mF[2]; a.pc = c.pc;
+    a.A = c.Ap; a.b = c.bp; a.rho = c.cs.rho;
+    for (int q = 0; q < 6; ++q) a.pT[q] = c.pT[q];
+    int nb = blocksFor(c.D.N, BS);
+    if (c.uniform) hipLaunchKernelGGL(k_pc_assemble<true>, dim3(nb), dim3(BS), 0, c.stream, c.geo, a);
+    else hipLaunchKernelGGL(k_pc_assemble<false>, dim3(nb), dim3(BS), 0, c.stream, c.geo, a);
+}
+
+// Simple.cpp:277-289
+__global__ void k_copy_noninactive(Dims D, const int8_t* st, const double* src, double* dst) {
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= D.N) return;
+    int i, j, k;
+    const long p = D.cell(q, i, j, k);
+    if (st[p] != ST_INACTIVE) dst[p] = src[p];
+}
+void launch_pc_copyback(Ctx& c) {
+    hipLaunchKernelGGL(k_copy_noninactive, dim3(blocksFor(c.D.N, BS)), dim3(BS), 0, c.stream, c.D, c.st, c.xp, c.pc);
+}
+
+// ------------------------------------------------------------------ gradients
+// FvScalarScheme::computeCellCenteredGradients, DIVERGENCE_THEOREM (FvScalarScheme.cpp:48-66),
+// every cell: grad = (phi_E S_E + phi_W S_W + ... + phi_B S_B) / V
+template <bool U>
+__global__ void k_grad_scalar(Geo g, const double* phi, double* o0, double* o1, double* o2) {
+    const Dims& D = g.D;
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= D.N) return;
+    int ijk[3];
+    const long p = D.cell(q, ijk[0], ijk[1], ijk[2]);
+    const long str[3] = {1, D.sJ, D.sK};
+    const int nn[3] = {D.nI, D.nJ, D.nK};
+    V3 acc = mk(0., 0., 0.);
+    bool first = true;
+#pragma unroll
+    for (int dir = 0; dir < 3; ++dir) {
+        const long s = str[dir];
+        const double fe = faceHi<U>(g, phi, p, s, dir, ijk[dir] == nn[dir] - 1);
+        const double fw = faceLo<U>(g, phi, p, s, dir, ijk[dir] == 0);
+        V3 te = G<U>::S(g, dir, p + s) * fe;
+        V3 tw = (-G<U>::S(g, dir, p)) * fw;
+        if (first) { acc = te + tw; first = false; }
+        else { acc = acc + te; acc = acc + tw; }
+    }
+    acc = acc / G<U>::vol(g, p);
+    o0[p] = acc.x; o1[p] = acc.y; o2[p] = acc.z;
+}
+
+void launch_grad_scalar(Ctx& c, const double* phi, double* const* out) {
+    int nb = blocksFor(c.D.N, BS);
+    if (c.uniform) hipLaunchKernelGGL(k_grad_scalar<true>, dim3(nb), dim3(BS), 0, c.stream, c.geo, phi, out[0], out[1], out[2]);
+    else hipLaunchKernelGGL(k_grad_scalar<false>, dim3(nb), dim3(BS), 0, c.stream, c.geo, phi, out[0], out[1], out[2]);
+}
+
+// FvVectorScheme::computeCellCenteredGradients, DIVERGENCE_THEOREM (FvVectorScheme.cpp:56-74):
+// grad u = sum_f tensor(u_f, S_f) / V, row-major (tensor: Vector3D.cpp:176-181)
+struct GuArgs { const double* u[3]; double* o[9]; };
+template <bool U>
+__global__ void k_grad_vector(Geo g, GuArgs a) {
+    const Dims& D = g.D;
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= D.N) return;
+    int ijk[3];
+    const long p = D.cell(q, ijk[0], ijk[1], ijk[2]);
+    const long str[3] = {1, D.sJ, D.sK};
+    const int nn[3] = {D.nI, D.nJ, D.nK};
+    double t[9];
+    bool first = true;
+#pragma unroll
+    for (int dir = 0; dir < 3; ++dir) {
+        const long s = str[dir];
+        const bool bh = ijk[dir] == nn[dir] - 1, bl = ijk[dir] == 0;
+        double ue[3], uw[3];
+        for (int c = 0; c < 3; ++c) { ue[c] = faceHi<U>(g, a.u[c], p, s, dir, bh); uw[c] = faceLo<U>(g, a.u[c], p, s, dir, bl); }
+        V3 Se = G<U>::S(g, dir, p + s), Sw = -G<U>::S(g, dir, p);
+        double te[9] = {ue[0] * Se.x, ue[0] * Se.y, ue[0] * Se.z, ue[1] * Se.x, ue[1] * Se.y, ue[1] * Se.z, ue[2] * Se.x, ue[2] * Se.y, ue[2] * Se.z};
+        double tw[9] = {uw[0] * Sw.x, uw[0] * Sw.y, uw[0] * Sw.z, uw[1] * Sw.x, uw[1] * Sw.y, uw[1] * Sw.z, uw[2] * Sw.x, uw[2] * Sw.y, uw[2] * Sw.z};
+        for (int c = 0; c < 9; ++c) {
+            if (first) t[c] = te[c] + tw[c];
+            else { t[c] = t[c] + te[c]; t[c] = t[c] + tw[c]; }
+        }
+        first = false;
+    }
+    const double v = G<U>::vol(g, p);
+    for (int c = 0; c < 9; ++c) a.o[c][p] = t[c] / v;
+}
+
+void launch_grad_vector(Ctx& c) {
+    GuArgs a;
+    for (int q = 0; q < 3; ++q) a.u[q] = c.u[q];
+    for (int q = 0; q < 9; ++q) a.o[q] = c.gu[q];
+    int nb = blocksFor(c.D.N, BS);
+    if (c.uniform) hipLaunchKernelGGL(k_grad_vector<true>, dim3(nb), dim3(BS), 0, c.stream, c.geo, a);
+    else hipLaunchKernelGGL(k_grad_vector<false>, dim3(nb), dim3(BS), 0, c.stream, c.geo, a);
+}
+
+// ------------------------------------------------------------------ correct
+// Simple::correct (Simple.cpp:303-346).  Each face is corrected once, by its low-side
+// cell when that cell is not INACTIVE (W/S/B domain faces by the boundary cell, only
+// for OUTLET faces).  The new fluxes go to a second buffer so every cell can form its
+// own continuity sum from the corrected values of all six faces without a race.
+struct CorrArgs {
+    const int8_t* st;
+    const double *mO[3];
+    double *mN[3];
+    double *mflow, *p, *u[3];
+    const double *pc, *d, *gpc[3];
+    double rho, ap;
+    int mc[6];
+};
+
+template <bool U>
+__global__ void k_correct(Geo g, CorrArgs a) {
+    const Dims& D = g.D;
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= D.N) return;
+    int ijk[3];
+    const long p = D.cell(q, ijk[0], ijk[1], ijk[2]);
+    const long str[3] = {1, D.sJ, D.sK};
+    const int nn[3] = {D.nI, D.nJ, D.nK};
+    const int8_t s = a.st[p];
+    if (s == ST_INACTIVE) {
+#pragma unroll
+        for (int dir = 0; dir < 3; ++dir) {
+            a.mN[dir][p + str[dir]] = a.mO[dir][p + str[dir]];
+            if (ijk[dir] == 0) a.mN[dir][p] = a.mO[dir][p];
+        }
+        return;
+    }
+    const double rho = a.rho, pcP = a.pc[p];
+    double mHi[3], mLo[3];
+#pragma unroll
+    for (int dir = 0; dir < 3; ++dir) {
+        const long s2 = str[dir];
+        // high face (E/N/T), owned by this cell
+        const bool bh = ijk[dir] == nn[dir] - 1;
+        double mh = a.mO[dir][p + s2];
+        if (!bh || a.mc[2 * dir]) {
+            const double df = faceHi<U>(g, a.d, p, s2, dir, bh);
+            mh += -rho * df * (a.pc[p + s2] - pcP) * G<U>::d(g, dir, p + s2, bh);
+        }
+        a.mN[dir][p + s2] = mh;
+        mHi[dir] = mh;
+        // low face (W/S/B)
+        const bool bl = ijk[dir] == 0;
+        double ml = a.mO[dir][p];
+        if (bl) {
+            if (a.mc[2 * dir + 1]) {
+                const double df = a.d[p - s2];
+                ml += rho * df * (a.pc[p - s2] - pcP) * G<U>::d(g, dir, p, true);
+            }
+            a.mN[dir][p] = ml;
+        } else if (a.st[p - s2] != ST_INACTIVE) {
+            // the low neighbour's high-face correction, evaluated as it evaluates it
+            const double df = faceLo<U>(g, a.d, p, s2, dir, false);
+            ml += -rho * df * (pcP - a.pc[p - s2]) * G<U>::d(g, dir, p, false);
+        }
+        mLo[dir] = ml;
+    }
+    a.mflow[p] = mHi[0] - mLo[0] + mHi[1] - mLo[1] + mHi[2] - mLo[2];
+    a.p[p] += a.ap * pcP;
+    const double md = -a.d[p];
+    a.u[0][p] += a.gpc[0][p] * md;
+    a.u[1][p] += a.gpc[1][p] * md;
+    a.u[2][p] += a.gpc[2][p] * md;
+}
+
+void launch_correct(Ctx& c) {
+    CorrArgs a;
+    a.st = c.st;
+    for (int q = 0; q < 3; ++q) { a.mO[q] = c.mF[q]; a.mN[q] = c.mF2[q]; a.u[q] = c.u[q]; a.gpc[q] = c.gpc[q]; }
+    a.mflow = c.mflow; a.p = c.p; a.pc = c.pc; a.d = c.d; a.rho = c.cs.rho; a.ap = c.cs.alphaP;
+    for (int q = 0; q < 6; ++q) a.mc[q] = c.massCorr[q] ? 1 : 0;
+    int nb = blocksFor(c.D.N, BS);
+    if (c.uniform) hipLaunchKernelGGL(k_correct<true>, dim3(nb), dim3(BS), 0, c.stream, c.geo, a);
+    else hipLaunchKernelGGL(k_correct<false>, dim3(nb), dim3(BS), 0, c.stream, c.geo, a);
+    for (int q = 0; q < 3; ++q) std::swap(c.mF[q], c.mF2[q]);
+}
+
+// ------------------------------------------------------------------ continuity and Courant
+// Simple::computeContinuityError (Simple.cpp:358-380) scans ACTIVE cells in k,j,i order:
+// if |m| > e then e = |m|/V.  The scan is order dependent; it is evaluated exactly as a
+// sequence of "first later cell with |m| > e" searches, each a parallel min-index
+// reduction inside one workgroup, starting after the previous hit.
+template <bool U>
+__global__ void __launch_bounds__(1024) k_continuity_quirk(Geo g, const int8_t* st, const double* mflow, double* out) {
+    const Dims& D = g.D;
+    __shared__ long smin[16];
+    __shared__ double se;
+    __shared__ long sstart;
+    __shared__ int sdone;
+    if (threadIdx.x == 0) { se = 0.; sstart = 0; sdone = 0; }
+    __syncthreads();
+    const long CH = 1024L * 8;
+    while (true) {
+        const double e = se;
+        long found = D.N;
+        for (long base = sstart; base < D.N && found == D.N; base += CH) {
+            long best = D.N;
+            for (int r = 0; r < 8; ++r) {
+                long qq = base + (long)r * 1024 + threadIdx.x;
+                if (qq < D.N) {
+                    int i, j, k;
+                    long p = D.cell(qq, i, j, k);
+                    if (st[p] == ST_ACTIVE && fabs(mflow[p]) > e && qq < best) best = qq;
+                }
+            }
+            for (int o = 32; o > 0; o >>= 1) { long t = __shfl_xor(best, o, 64); best = t < best ? t : best; }
+            if ((threadIdx.x & 63) == 0) smin[threadIdx.x >> 6] = best;
+            __syncthreads();
+            long b2 = D.N;
+            for (int w = 0; w < 16; ++w) b2 = smin[w] < b2 ? smin[w] : b2;
+            found = b2;
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            if (found >= D.N) sdone = 1;
+            else {
+                int i, j, k;
+                long p = D.cell(found, i, j, k);
+                se = fabs(mflow[p]) / G<U>::vol(g, p);
+                sstart = found + 1;
+            }
+        }
+        __syncthreads();
+        if (sdone) break;
+    }
+    if (threadIdx.x == 0) *out = se;
+}
+
+template <bool U, int MODE>
+__global__ void k_maxreduce(Geo g, const int8_t* st, const double* mflow, const double* u0, const double* u1,
+                            const double* u2, double dt, double* part) {
+    const Dims& D = g.D;
+    __shared__ double lds[BS / 64];
+    double mx = 0.;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < D.N; q += (long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        const long p = D.cell(q, i, j, k);
+        double v;
+        if (MODE == 0) {  // continuity max over ACTIVE cells
+            if (st[p] != ST_ACTIVE) continue;
+            v = fabs(mflow[p]) / G<U>::vol(g, p);
+        } else {  // Simple::computeCourantNumber (Simple.cpp:382-408), every cell
+            V3 xc = G<U>::xc(g, i, j, k, p);
+            V3 rI = (G<U>::fc(g, 0, i + 1, j, k, p + 1) - xc) - (G<U>::fc(g, 0, i, j, k, p) - xc);
+            V3 rJ = (G<U>::fc(g, 1, i, j + 1, k, p + D.sJ) - xc) - (G<U>::fc(g, 1, i, j, k, p) - xc);
+            V3 rK = (G<U>::fc(g, 2, i, j, k + 1, p + D.sK) - xc) - (G<U>::fc(g, 2, i, j, k, p) - xc);
+            V3 uu = mk(u0[p], u1[p], u2[p]);
+            double uI = fabs(dot(uu, unitv(rI))), uJ = fabs(dot(uu, unitv(rJ))), uK = fabs(dot(uu, unitv(rK)));
+            v = dt * (uI / mag(rI) + uJ / mag(rJ) + uK / mag(rK));
+        }
+        mx = v > mx ? v : mx;
+    }
+    double r = blockMax<BS>(mx, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+__global__ void k_max_final(const double* part, int n, double* out) {
+    __shared__ double lds[BS / 64];
+    double mx = 0.;
+    for (int q = threadIdx.x; q < n; q += BS) mx = part[q] > mx ? part[q] : mx;
+    double r = blockMax<BS>(mx, lds);
+    if (threadIdx.x == 0) *out = r;
+}
+
+static double readScalar(Ctx& c, int slot) {
+    HIPCHECK(hipMemcpyAsync(c.hscal + slot, c.scal + slot, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    HIPCHECK(hipStreamSynchronize(c.stream));
+    return c.hscal[slot];
+}
+
+double continuity_quirk(Ctx& c) {
+    if (c.uniform) hipLaunchKernelGGL(k_continuity_quirk<true>, dim3(1), dim3(1024), 0, c.stream, c.geo, c.st, c.mflow, c.scal + SC_MISC);
+    else hipLaunchKernelGGL(k_continuity_quirk<false>, dim3(1), dim3(1024), 0, c.stream, c.geo, c.st, c.mflow, c.scal + SC_MISC);
+    return readScalar(c, SC_MISC);
+}
+
+static double maxReduce(Ctx& c, int mode, double dt) {
+    const int nb = blocksFor(c.D.N, BS, 2048);
+    double* part = c.red.part;
+    if (mode == 0) {
+        if (c.uniform) hipLaunchKernelGGL((k_maxreduce<true, 0>), dim3(nb), dim3(BS), 0, c.stream, c.geo, c.st, c.mflow, c.u[0], c.u[1], c.u[2], dt, part);
+        else hipLaunchKernelGGL((k_maxreduce<false, 0>), dim3(nb), dim3(BS), 0, c.stream, c.geo, c.st, c.mflow, c.u[0], c.u[1], c.u[2], dt, part);
+    } else {
+        if (c.uniform) hipLaunchKernelGGL((k_maxreduce<true, 1>), dim3(nb), dim3(BS), 0, c.stream, c.geo, c.st, c.mflow, c.u[0], c.u[1], c.u[2], dt, part);
+        else hipLaunchKernelGGL((k_maxreduce<false, 1>), dim3(nb), dim3(BS), 0, c.stream, c.geo, c.st, c.mflow, c.u[0], c.u[1], c.u[2], dt, part);
+    }
+    hipLaunchKernelGGL(k_max_final, dim3(1), dim3(BS), 0, c.stream, part, nb, c.scal + SC_MISC + 1);
+    return readScalar(c, SC_MISC + 1);
+}
+double continuity_max(Ctx& c) { return maxReduce(c, 0, 0.); }
+double courant(Ctx& c, double dt) { return maxReduce(c, 1, dt); }
+
+__global__ void k_fill(double* a, double v, long n) {
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) a[q] = v;
+}
+void launch_fill(Ctx& c, double* a, double v, long n) {
+    hipLaunchKernelGGL(k_fill, dim3(blocksFor(n, BS)), dim3(BS), 0, c.stream, a, v, n);
+}
+
+// IbPiso.cpp:74-80: SOLID cells get u = 0, p = 0
+__global__ void k_zero_inactive(Dims D, const int8_t* st, double* u0, double* u1, double* u2, double* p) {
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= D.N) return;
+    int i, j, k;
+    const long pp = D.cell(q, i, j, k);
+    if (st[pp] == ST_INACTIVE) { u0[pp] = 0.; u1[pp] = 0.; u2[pp] = 0.; p[pp] = 0.; }
+}
+void launch_set_inactive_fields(Ctx& c) {
+    hipLaunchKernelGGL(k_zero_inactive, dim3(blocksFor(c.D.N, BS)), dim3(BS), 0, c.stream, c.D, c.st, c.u[0], c.u[1], c.u[2], c.p);
+}
+
+}  // namespace caffe

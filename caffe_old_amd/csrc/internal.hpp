@@ -1,0 +1,135 @@
+// internal.hpp -- the device-resident solver context behind the C ABI, and the
+// host-side launchers each kernel translation unit exports.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/caffe_hip.h"
+#include "common.hpp"
+
+namespace caffe {
+
+// Device scalar slots (double) used by the Krylov kernels.
+enum {
+    SC_M = 0,    // momentum system v uses SC_M + 8*v: rho, rhoOld, alpha, omega, beta, bb, rr
+    SC_P = 24,   // pressure system (FP64 and FP32 inner): rz, alpha, beta, bb, rr, pq
+    SC_IR = 32,  // refinement: rr64, bb64, scale
+    SC_MISC = 40,
+    N_SCAL = 64
+};
+enum { FL_MDONE = 0, FL_PDONE = 1, FL_MITER = 2, FL_PITER = 3, N_FLAGS = 8 };
+
+struct Overlay {          // ghost-cell rows (IbPiso.cpp:148-185), 9 entries each
+    int n = 0;
+    long* cell = nullptr;     // padded index of the ghost cell
+    long* cols = nullptr;     // 9 * n padded column cells (-1 = dropped)
+    double* valM = nullptr;   // 9 * n momentum-row values
+    double* valP = nullptr;   // 9 * n pressure-row values
+    float* valP32 = nullptr;
+    int* rowOf = nullptr;     // P: overlay row of a padded cell or -1
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool ready = false;
+
+    caffe_case cs{};
+    Dims D{};
+    Geo geo{};
+    bool uniform = true;
+    bool hasIB = false;
+    bool singularP = false;   // pure Neumann p' system (null space = constants)
+    int uT[6], pT[6], flowT[6];
+    bool massCorr[6];
+
+    // fields, padded layout
+    double *u[3], *u0[3], *h[3], *p, *pc, *d, *gp[3], *gpc[3], *gu[9], *mflow;
+    double *mF[3], *mF2[3];   // mass flux face arrays (ping-pong for correct())
+    int8_t* st = nullptr;     // IndexMap statuses, halo = INACTIVE
+    int* rowId = nullptr;     // host-visible row numbering is rebuilt on demand
+
+    // assembled systems
+    double* Am = nullptr;     // 7 * P  momentum band: aP, aE, aW, aN, aS, aT, aB
+    double* bm = nullptr;     // 3 * P
+    double* xm = nullptr;     // 3 * P
+    double* Ap = nullptr;     // 4 * P  p' system: aP, aI (west face), aJ, aK
+    double* bp = nullptr;     // P
+    double* xp = nullptr;     // P
+    float* Ap32 = nullptr;    // 4 * P
+    Overlay ov;
+
+    // Krylov work
+    double* wm = nullptr;     // momentum BiCGStab: 8 vectors * 3 * P
+    double* wp = nullptr;     // pressure: 6 * P
+    float* wp32 = nullptr;    // FP32 inner: 6 * P
+    RedBuf red{};
+    double* scal = nullptr;   // N_SCAL device scalars
+    int* flags = nullptr;     // N_FLAGS device ints
+    double* hscal = nullptr;  // pinned host mirror
+    int* hflags = nullptr;
+
+    // metric arrays of general meshes
+    double* metric = nullptr;
+    double* dnodes = nullptr; // node array of general meshes (IB cell location)
+    std::vector<void*> allocs;
+
+    caffe_solve_opts momOpts{}, pcOpts{};
+    caffe_solve_report lastMom{}, lastPc{};
+    int pcSolvesThisStep = 0;
+    bool ibReady = false;     // ghost rows computed for the current case
+    double courant = 0.;      // last Courant number (Simple.cpp:88)
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    // learned preconditioner (optional)
+    void* net = nullptr;
+};
+
+// ---- launchers (fv_kernels.hip)
+void launch_halo(Ctx& c, double* const* f, int nf, const int* types);
+void launch_mom_assemble(Ctx& c, double dt);
+void launch_mom_copyback(Ctx& c);
+void launch_rhie_chow(Ctx& c);
+void launch_pc_assemble(Ctx& c);
+void launch_pc_copyback(Ctx& c);
+void launch_grad_scalar(Ctx& c, const double* phi, double* const* out);
+void launch_grad_vector(Ctx& c);
+void launch_correct(Ctx& c);
+double continuity_quirk(Ctx& c);
+double continuity_max(Ctx& c);
+double courant(Ctx& c, double dt);
+void launch_fill(Ctx& c, double* a, double v, long n);
+void launch_set_inactive_fields(Ctx& c);
+
+// ---- mesh (mesh.hip)
+void compute_metrics(Ctx& c, const double* dNodes);
+
+// ---- Krylov (krylov.hip)
+int momentum_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep);
+int pcorr_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep);
+void pcorr_make_fp32(Ctx& c);
+
+// ---- immersed boundary (ib.hip)
+int ib_field(Ctx& c);
+int ib_coeffs(Ctx& c);
+
+// ---- learned preconditioner (learned.hip)
+int learned_apply_guess(Ctx& c, const double* r, double* x, double scale);
+
+#define HIPCHECK(x)                                                        \
+    do {                                                                   \
+        hipError_t e_ = (x);                                               \
+        if (e_ != hipSuccess) throw std::string(#x) + ": " + hipGetErrorString(e_); \
+    } while (0)
+
+inline int blocksFor(long n, int bs, int cap = 1 << 30) {
+    long b = (n + bs - 1) / bs;
+    if (b > cap) b = cap;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+
+}  // namespace caffe

@@ -1,0 +1,148 @@
+/*
+ * caffe_hip.h -- C ABI of the MI355X (gfx950) implementation of CAFFE's
+ * SIMPLE / PISO / IbPiso per-iteration hot path.
+ *
+ * One caffe_ctx owns one GPU-resident solver instance: mesh metrics, fields
+ * with their boundary patches, the assembled momentum and pressure-correction
+ * systems and the Krylov work space.  All work is queued on the context's HIP
+ * stream; calls that return host data (downloads, reports, solves that check
+ * convergence) synchronise that stream.  One host thread per context.
+ *
+ * Every function returns CAFFE_OK (0) or a negative status; the message of the
+ * last failure is caffe_last_error(ctx).  No C++ exception crosses this ABI.
+ *
+ * Reference interfaces replaced (paths relative to the CAFFE tree):
+ *   caffe_case_setup          Simple::Simple / Piso::Piso / IbPiso::IbPiso
+ *                             (src/Solvers/Simple.cpp:34-74, Piso.cpp:5-10,
+ *                             IbPiso.cpp:32-39) with HexaFvmMesh::initialize
+ *                             (src/Domains/HexaFvmMesh/HexaFvmMesh.cpp:51-58, 615-624)
+ *   caffe_solve               virtual double Solver::solve(double timeStep)
+ *                             (src/Solvers/Solver.h:20; Simple.cpp:78-91,
+ *                             Piso.cpp:12-29, IbPiso.cpp:43-61)
+ *   caffe_momentum_assemble   Simple::computeMomentum assembly loop (Simple.cpp:117-179)
+ *   caffe_momentum_solve      the 3 x SparseMatrix::solve + copy-back + BCs + face
+ *                             interpolation of computeMomentum (Simple.cpp:184-218)
+ *   caffe_rhiechow            Simple::rhieChowInterpolateFaces (Simple.cpp:410-449)
+ *   caffe_pcorr_assemble      Simple::computePCorr assembly loop (Simple.cpp:231-269)
+ *   caffe_pcorr_solve         SparseMatrix::solve on A_[1] (Simple.cpp:275,
+ *                             src/Math/SparseMatrix.cpp:86-106) + copy-back,
+ *                             BCs, interpolation and grad p' (Simple.cpp:277-298)
+ *   caffe_correct             Simple::correct (Simple.cpp:301-356)
+ *   caffe_ib_field            IbPiso::computeIbField (IbPiso.cpp:65-117)
+ *   caffe_ib_coeffs           IbPiso::computeIbCoeffs (IbPiso.cpp:119-190)
+ *   caffe_step_report         Simple::computeContinuityError / computeCourantNumber
+ *                             (Simple.cpp:358-408)
+ *   caffe_field_download      Field<T>::cellData() / patches / faces
+ *                             (src/Domains/Field/Field.h:53-131)
+ */
+#ifndef CAFFE_HIP_H
+#define CAFFE_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CAFFE_OK 0
+#define CAFFE_ERR_ARG (-1)
+#define CAFFE_ERR_HIP (-2)
+#define CAFFE_ERR_STATE (-3)
+#define CAFFE_ERR_NOT_CONVERGED (-4)
+
+typedef struct caffe_ctx caffe_ctx;
+
+/* FlowBoundaryConditions::Type (src/Domains/Field/FlowBoundaryConditions.h:10) */
+enum { CAFFE_BC_INLET = 0, CAFFE_BC_OUTLET = 1, CAFFE_BC_WALL = 2, CAFFE_BC_EMPTY = 3, CAFFE_BC_ZERO_GRADIENT = 4 };
+/* Solver family */
+enum { CAFFE_SIMPLE = 0, CAFFE_PISO = 1, CAFFE_IBPISO = 2 };
+/* Pressure-correction / momentum solve modes */
+enum {
+    CAFFE_SOLVE_PLAIN_FP64 = 0,   /* preconditioned Krylov in FP64 (the judge and fallback) */
+    CAFFE_SOLVE_IR_FP32 = 1,      /* FP32 inner Krylov inside FP64 iterative refinement */
+    CAFFE_SOLVE_LEARNED_FCG = 2   /* learned initial guess + IR_FP32; accepted only at rtol */
+};
+
+/* The keys of case/case.info and case/initialConditions.info the hot path reads
+ * (Solver.cpp:8, Simple.cpp:61-67, 104-108, Piso.cpp:9, FlowBoundaryConditions.cpp:25-29,
+ * IbPiso.cpp:37-38, InitialConditions.cpp:12-74). */
+typedef struct {
+    int nI, nJ, nK;          /* cells */
+    int algo;                /* CAFFE_SIMPLE / CAFFE_PISO / CAFFE_IBPISO */
+    int steady;              /* Solver.timeAccurate OFF */
+    double rho, mu;
+    double alphaU, alphaP;   /* relaxationFactorMomentum / relaxationFactorPCorr */
+    int nInner, nPCorr;
+    int bcType[6];           /* east, west, north, south, top, bottom */
+    double bcVec[18];        /* refVector per face */
+    double bcVal[6];         /* refValue per face */
+    double uInit[3];
+    double pInit;
+    double sphereR, sphereC[3];
+    int uniform;             /* 1: cells form a uniform box; metrics in closed form */
+} caffe_case;
+
+typedef struct {
+    int mode;                /* CAFFE_SOLVE_* */
+    double rtol;             /* target true FP64 relative residual ||b - Ax||_2 / ||b||_2 */
+    int maxit;               /* Krylov iteration cap (all refinement levels together) */
+    double inner_rtol;       /* IR_FP32: relative tolerance of each FP32 inner solve */
+} caffe_solve_opts;
+
+typedef struct {
+    int iters;               /* Krylov iterations (sum over inner solves for IR) */
+    int refinements;         /* FP64 refinement steps (IR modes) */
+    double relres;           /* true FP64 relative residual at exit */
+    int converged;           /* relres <= rtol */
+    int fell_back;           /* learned step rejected, plain solve used */
+    double ms;               /* device time of the solve */
+} caffe_solve_report;
+
+int caffe_ctx_create(int device, caffe_ctx** out);
+int caffe_ctx_destroy(caffe_ctx* ctx);
+const char* caffe_last_error(caffe_ctx* ctx);
+
+/* Build the device mesh (metrics from nodes, or closed form when c->uniform),
+ * allocate fields and systems, apply boundary and initial conditions.
+ * nodes: (nI+1)(nJ+1)(nK+1) x 3 doubles, i fastest, xyz interleaved; borrowed. */
+int caffe_case_setup(caffe_ctx* ctx, const caffe_case* c, const double* nodes);
+
+/* which: 0 = momentum, 1 = pressure correction */
+int caffe_set_solve_opts(caffe_ctx* ctx, int which, const caffe_solve_opts* o);
+
+/* Solver::solve(dt): returns the reference's continuity value (Simple.cpp:358-380) */
+int caffe_solve(caffe_ctx* ctx, double dt, double* continuity);
+
+int caffe_save_u0(caffe_ctx* ctx);
+int caffe_momentum_assemble(caffe_ctx* ctx, double dt);
+int caffe_momentum_solve(caffe_ctx* ctx, caffe_solve_report* rep);
+int caffe_rhiechow(caffe_ctx* ctx);
+int caffe_pcorr_assemble(caffe_ctx* ctx);
+int caffe_pcorr_solve(caffe_ctx* ctx, const caffe_solve_opts* o, caffe_solve_report* rep);
+int caffe_correct(caffe_ctx* ctx);
+int caffe_ib_field(caffe_ctx* ctx);
+int caffe_ib_coeffs(caffe_ctx* ctx);
+int caffe_grad_u(caffe_ctx* ctx);
+int caffe_step_report(caffe_ctx* ctx, double dt, double* continuity_ref, double* continuity_max, double* courant);
+int caffe_last_reports(caffe_ctx* ctx, caffe_solve_report* momentum, caffe_solve_report* pcorr);
+
+/* Field exchange in the padded layout (nI+2)(nJ+2)(nK+2), i fastest, halo = boundary
+ * patches; vector fields are component-major.  Names: u u0 h p pCorr d gradP gradPCorr
+ * gradU massFlow mI mJ mK status.  n = number of doubles in host. */
+int caffe_field_download(caffe_ctx* ctx, const char* name, double* host, long n);
+int caffe_field_upload(caffe_ctx* ctx, const char* name, const double* host, long n);
+/* Metric arrays (general meshes): vol xc SI SJ SK fcI fcJ fcK dI dJ dK gI gJ gK */
+int caffe_metric_download(caffe_ctx* ctx, const char* name, double* host, long n);
+/* Assembled systems.  which 0: band 7*P (aP aE aW aN aS aT aB); which 1: 4*P
+ * (aP, aI, aJ, aK: coefficient of the west/south/bottom face of each cell).
+ * rhs: 3*P (momentum) or P (pressure correction). */
+int caffe_matrix_download(caffe_ctx* ctx, int which, double* host, long n);
+int caffe_rhs_download(caffe_ctx* ctx, int which, double* host, long n);
+/* Ghost rows: n = number of ghost cells; cells[n], cols[9n] (padded indices, -1 dropped),
+ * valsM[9n], valsP[9n]; any pointer may be NULL. */
+int caffe_ghost_rows(caffe_ctx* ctx, int* n, long* cells, long* cols, double* valsM, double* valsP);
+int caffe_sync(caffe_ctx* ctx);
+long caffe_padded_size(caffe_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

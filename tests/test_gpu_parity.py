@@ -1,0 +1,190 @@
+"""GPU parity tests: libcaffe_hip.so (through its C ABI) against the FP64 oracle.
+
+Bars (from the hot-path specification):
+  * assembled coefficients and right-hand sides: max-norm relative difference <= 1e-12
+    on the same input state (the state is copied from the oracle into the device);
+  * fields after full SIMPLE / PISO / IbPiso runs with tight linear solves on both
+    sides: u, v, w relative L2 <= 1e-6, p relative L2 <= 1e-6 after removing the mean
+    (the pure-Neumann pressure is defined up to a constant);
+  * metrics of general meshes: bit-identical.
+"""
+import numpy as np
+import pytest
+
+from caffe_old_amd import case as cs
+from tests.helpers import (copy_state, interior, oracle_momentum_band, oracle_pcorr_sym, oracle_rhs_padded, rel_inf,
+                           rel_l2)
+
+pytestmark = pytest.mark.gpu
+
+TIGHT = 1e-13
+
+
+def _pair(case, uniform=None, gpu_rtol=1e-12):
+    from caffe_old_amd.solver import make_solver
+    from oracle.pyoracle import Oracle
+    g = make_solver(case, uniform=uniform)
+    g.set_solve_opts("momentum", rtol=gpu_rtol, maxit=20000)
+    g.set_solve_opts("pcorr", rtol=gpu_rtol, maxit=200000)
+    o = Oracle(case, rtol=TIGHT, n_threads=4, n_blocks=4)
+    return g, o
+
+
+def _cases():
+    uni = cs.lid_cavity(10, 0.01, 1.0, "simple", alpha_u=0.7, alpha_p=0.3, max_iterations=6)
+    gen_nodes = cs.tecplot_roundtrip(cs.transfinite_nodes(cs.box_vertices(1, 1, 1), 9, 10, 11))
+    gen = cs.lid_cavity(9, 0.01, 1.0, "simple", alpha_u=0.7, alpha_p=0.3, max_iterations=6, nodes=gen_nodes)
+    gen.nodes = gen_nodes
+    skew_nodes = cs.perturb_interior(cs.cartesian_nodes(1, 1, 1, 8, 8, 8), 0.15, 1 / 8, seed=0)
+    skew = cs.lid_cavity(8, 1e-3, 1.0, "piso", alpha_u=1.0, alpha_p=1.0, time_accurate=True, dt=0.05, n_pcorr=2,
+                         max_iterations=4, nodes=skew_nodes)
+    return {"uniform": (uni, True), "transfinite": (gen, False), "skewed": (skew, False)}
+
+
+CASES = _cases()
+
+
+@pytest.mark.parametrize("name", ["transfinite", "skewed"])
+def test_metrics_bit_identical(name):
+    case, uni = CASES[name]
+    g, o = _pair(case, uniform=uni)
+    shape = g.shape
+    assert np.array_equal(interior(g.metric("vol"), shape), interior(o.metric("vol"), shape))
+    assert np.array_equal(interior(g.metric("xc"), shape), interior(o.metric("xc"), shape))
+    for a, nm in enumerate("IJK"):
+        gs, os_ = g.metric("S" + nm), o.metric("S" + nm)
+        assert np.array_equal(gs, os_), f"S{nm}"
+        assert np.array_equal(g.metric("fc" + nm), o.metric("fc" + nm)), f"fc{nm}"
+    # d and g per face: the oracle's per-cell dE/dW etc. map onto the face arrays
+    ni, nj, nk = shape
+    sJ, sK = ni + 2, (ni + 2) * (nj + 2)
+    for nm, lo, hi, gname, s in (("dI", "dW", "dE", "gE", 1), ("dJ", "dS", "dN", "gN", sJ), ("dK", "dB", "dT", "gT", sK)):
+        gd = g.metric(nm)
+        oW, oE = o.metric(lo), o.metric(hi)
+        cells = np.flatnonzero(interior(np.arange(g.P), shape).ravel() >= 0)
+        pad = interior(np.arange(g.P), shape).ravel()
+        assert np.array_equal(gd[pad], oW[pad]), nm + " low faces"
+        assert np.array_equal(gd[pad + s], oE[pad]), nm + " high faces"
+        gg = g.metric("g" + nm[1])
+        og = o.metric(gname)
+        # interior faces only (domain faces hold g = 0 on both sides)
+        inner = pad[og[pad] != 0]
+        assert np.array_equal(gg[inner + s], og[inner]), "g" + nm[1]
+        del cells
+
+
+@pytest.mark.parametrize("name", ["uniform", "transfinite", "skewed"])
+def test_assembly_parity(name):
+    case, uni = CASES[name]
+    g, o = _pair(case, uniform=uni)
+    dt = case.time_step
+    for _ in range(3):  # build a nontrivial state on the oracle
+        o.solve(dt)
+    o.save_u0()
+    # momentum assembly (Simple.cpp:117-179)
+    copy_state(o, g)
+    g.momentum_assemble(dt)
+    o.assemble_momentum(dt)
+    band_o, status = oracle_momentum_band(o)
+    band_g = g.matrix(0)
+    act = status == 0
+    assert rel_inf(band_g, band_o, act) <= 1e-12
+    b_g = g.rhs(0)
+    for c in range(3):
+        b_o = oracle_rhs_padded(o, c, g.P)
+        assert rel_inf(b_g[c], b_o, act) <= 1e-12, f"momentum rhs {c}"
+    assert rel_inf(g.get("d"), o.get("d"), act) <= 1e-12
+    # pressure-correction assembly (Simple.cpp:231-269)
+    copy_state(o, g)
+    g.pcorr_assemble()
+    o.assemble_pcorr()
+    sym_o, status = oracle_pcorr_sym(o)
+    sym_g = g.matrix(1)
+    assert rel_inf(sym_g[0], sym_o[0], act) <= 1e-12
+    for a in (1, 2, 3):
+        assert rel_inf(sym_g[a], sym_o[a]) <= 1e-12, f"face coefficients {a}"
+    assert rel_inf(g.rhs(1), oracle_rhs_padded(o, 0, g.P), act) <= 1e-12
+
+
+@pytest.mark.parametrize("name", ["uniform", "skewed"])
+def test_rhie_chow_and_correct_parity(name):
+    case, uni = CASES[name]
+    g, o = _pair(case, uniform=uni)
+    for _ in range(2):
+        o.solve(case.time_step)
+    copy_state(o, g)
+    g.rhie_chow()
+    o.rhie_chow()
+    for n in ("mI", "mJ", "mK"):
+        assert rel_inf(g.get(n), o.get(n)) <= 1e-12, n
+    copy_state(o, g)
+    g.correct()
+    o.correct()
+    for n in ("mI", "mJ", "mK", "massFlow", "p", "u", "gradP"):
+        assert rel_inf(g.get(n), o.get(n)) <= 1e-12, n
+
+
+def _field_errors(g, o):
+    sh = g.shape
+    ug, uo = interior(g.get("u"), sh), interior(o.get("u"), sh)
+    pg, po = interior(g.get("p"), sh), interior(o.get("p"), sh)
+    errs = {c: rel_l2(ug[i], uo[i]) for i, c in enumerate("uvw") if np.linalg.norm(uo[i]) > 0}
+    errs["p"] = rel_l2(pg - pg.mean(), po - po.mean())
+    return errs
+
+
+@pytest.mark.parametrize("name", ["uniform", "transfinite"])
+def test_simple_fields(name):
+    case, uni = CASES[name]
+    case = cs.Case(**{**case.__dict__, "max_iterations": 21})
+    g, o = _pair(case, uniform=uni)
+    hist_g, hist_o = [], []
+    for _ in range(case.n_solves):
+        hist_g.append(g.solve())
+        hist_o.append(o.solve())
+        g.pcorr_assemble()  # pre-correction imbalance of the next iteration is physics-determined
+    errs = _field_errors(g, o)
+    assert max(errs.values()) <= 1e-6, errs
+
+
+def test_piso_skewed_fields():
+    case, uni = CASES["skewed"]
+    g, o = _pair(case, uniform=uni)
+    for _ in range(case.n_solves):
+        g.solve()
+        o.solve()
+    errs = _field_errors(g, o)
+    assert max(errs.values()) <= 1e-6, errs
+
+
+def test_piso_courant_and_continuity():
+    case = cs.lid_cavity(12, 1e-3, 1.0, "piso", alpha_u=1.0, alpha_p=1.0, time_accurate=True, dt=0.5 / 12,
+                         n_pcorr=2, max_iterations=3)
+    g, o = _pair(case, uniform=True)
+    for _ in range(case.n_solves):
+        g.solve()
+        o.solve()
+    rep = g.step_report()
+    assert abs(rep["courant"] - o.courant(case.time_step)) <= 1e-12 * o.courant(case.time_step)
+    # continuity: same state -> same order-dependent scan
+    copy_state(o, g)
+    rep = g.step_report()
+    assert rep["continuity"] == pytest.approx(o.continuity(), rel=1e-12, abs=0)
+    assert rep["continuity_max"] == pytest.approx(o.continuity_max(), rel=1e-12, abs=0)
+
+
+def test_ib_channel_small():
+    # the channel with an immersed sphere, scaled down: 4 x 1 x 1 on 64 x 16 x 16 (h = 1/16),
+    # sphere r = 0.2 at (1, .5, .5) so that it spans several cells
+    case = cs.channel_ib_sphere(64, 16, 16, steps=2)
+    case.sphere = (0.2, (1.0, 0.5, 0.5))
+    g, o = _pair(case, uniform=True)
+    for _ in range(case.n_solves):
+        g.solve()
+        o.solve()
+    sh = g.shape
+    st_o = o.imap()[1].reshape(sh[2], sh[1], sh[0])
+    st_g = interior(g.get("status"), sh)
+    assert np.array_equal(st_g.astype(int), st_o)
+    errs = _field_errors(g, o)
+    assert max(errs.values()) <= 1e-6, errs
