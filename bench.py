@@ -1,0 +1,144 @@
+#!/usr/bin/env python3
+"""Benchmark: PISO time steps per second on the 128^3 lid-driven cavity (Re 1000, two
+pressure correctors, dt = 0.5 h / U), plus the pressure-correction solve time to a true
+FP64 relative residual of 1e-8: FP32-inner / FP64-refinement (and learned) vs plain FP64.
+
+One "step" = Piso::solve(dt) (Piso.cpp:12-29): momentum assembly + 3 BiCGStab solves,
+Rhie-Chow, then 2 x (p' assembly + p' Krylov solve + correct), continuity and Courant.
+Inputs are resident in HBM before the timed region; the timed region contains K full
+steps bracketed by stream synchronisation.
+
+Prints one JSON line.  --gpus N > 1 is not supported: the whole domain lives on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _cpu_threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except Exception:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_baseline(n, steps=1):
+    """The oracle (C++ FP64 restatement, BiCGStab + block SSOR at rtol 1e-5 like the
+    reference's PETSc settings) on the same 128^3 case: `steps` PISO steps from rest."""
+    from caffe_old_amd import case as cs
+    from oracle.pyoracle import KSP_PETSC_LIKE, Oracle
+    th = _cpu_threads()
+    c = cs.cavity_piso_128(n, steps=steps)
+    o = Oracle(c, ksp=KSP_PETSC_LIKE, rtol=1e-5, maxit=10000, n_blocks=th, n_threads=th)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.solve()
+    dt = time.perf_counter() - t0
+    return {"value": steps / dt, "unit": "steps/s", "cores": th, "kind": "port",
+            "sample": f"{steps} PISO step(s) of the {n}^3 cavity from rest, oracle BiCGStab+SSOR rtol 1e-5, "
+                      f"{th} OpenMP threads, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--mode", choices=["ir", "plain", "learned"], default="ir")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quiet", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: --gpus {a.gpus} is not supported; this build runs the whole domain on one GPU "
+              f"(multi-GPU block decomposition is design-only).", file=sys.stderr)
+        sys.exit(2)
+
+    from caffe_old_amd import _lib as L
+    from caffe_old_amd import case as cs
+    from caffe_old_amd.solver import make_solver
+
+    mode = {"ir": L.SOLVE_IR_FP32, "plain": L.SOLVE_PLAIN_FP64, "learned": L.SOLVE_LEARNED_FCG}[a.mode]
+    case = cs.cavity_piso_128(a.n, steps=a.steps + a.warmup)
+    s = make_solver(case)
+    s.set_solve_opts("momentum", rtol=1e-8, maxit=5000)
+    s.set_solve_opts("pcorr", mode=mode, rtol=1e-8, maxit=100000, inner_rtol=1e-2)
+
+    def log(*x):
+        if not a.quiet:
+            print(*x, file=sys.stderr, flush=True)
+
+    for w in range(a.warmup):
+        s.solve()
+        log(f"warmup step {w}: continuity {s.continuityError:.3e}")
+    s.sync()
+    t0 = time.perf_counter()
+    iters_p = 0
+    for k in range(a.steps):
+        s.solve()
+        m, p = s.reports()
+        iters_p += p["iters"]
+    s.sync()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1000.0 * elapsed / a.steps
+    rep = s.step_report()
+    log(f"timed {a.steps} steps: {ms_per_step:.2f} ms/step, last p' iters {p['iters']}, courant {rep['courant']:.3f}")
+
+    # pressure-correction solve: plain FP64 vs FP32-inner / FP64-refinement, same system
+    psolve = {}
+    for name, md in (("plain_fp64", L.SOLVE_PLAIN_FP64), ("ir_fp32", L.SOLVE_IR_FP32), ("learned_fcg", L.SOLVE_LEARNED_FCG)):
+        s.pcorr_assemble()
+        r = s.pcorr_solve(mode=md, rtol=1e-8, maxit=100000, inner_rtol=1e-2)
+        psolve[name] = {"ms": round(r["ms"], 3), "iters": r["iters"], "refinements": r["refinements"],
+                        "relres": r["relres"], "fell_back": r["fell_back"]}
+    log("p' solves:", psolve)
+
+    # roofline of the dominant kernel (pressure-correction CG stencil kernel)
+    kname = "pcorr_cg_k1_fp64" if a.mode == "plain" else "pcorr_cg_k1_fp32"
+    s.pcorr_assemble()
+    kms, kbytes = s.time_kernel(kname, reps=200)
+    achieved = kbytes / (kms * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        try:
+            traffic = json.load(open(tfile)).get(f"{kname}_{a.n}")
+        except Exception:
+            traffic = None
+
+    out = {
+        "metric": "PISO steps/s at 128^3; p-solve ms to 1e-8 resid, learned+FP32 vs plain FP64",
+        "value": round(1000.0 / ms_per_step, 4),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if a.mode == "plain" else "f64+f32",
+        "data": "synthetic: lid-driven cavity from rest (no external data)",
+        "config": {"workload": f"3D lid-driven cavity {a.n}^3 uniform hex, Re 1000 (mu 1e-3, lid 1), PISO 2 correctors, "
+                               f"dt {case.time_step:g} (CFL 0.5), p' solve to true relres 1e-8 ({a.mode})",
+                   "cells": case.n_cells, "p_solve_ms": psolve},
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "ms_per_launch": round(kms, 5), "bytes_per_launch": kbytes},
+    }
+    if not a.no_cpu:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(a.n)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -27,7 +27,8 @@ EXPORTS = [
     "caffe_solve", "caffe_save_u0", "caffe_momentum_assemble", "caffe_momentum_solve", "caffe_rhiechow",
     "caffe_pcorr_assemble", "caffe_pcorr_solve", "caffe_correct", "caffe_ib_field", "caffe_ib_coeffs",
     "caffe_grad_u", "caffe_step_report", "caffe_last_reports", "caffe_field_download", "caffe_field_upload",
-    "caffe_metric_download", "caffe_matrix_download", "caffe_rhs_download", "caffe_ghost_rows", "caffe_sync",
+    "caffe_metric_download", "caffe_matrix_download", "caffe_rhs_download", "caffe_ghost_rows", "caffe_time_kernel",
+    "caffe_sync",
     "caffe_padded_size",
 ]
 
@@ -98,6 +99,7 @@ def load(path: str = LIB_PATH):
         "caffe_matrix_download": ([vp, C.c_int, dp, C.c_long], C.c_int),
         "caffe_rhs_download": ([vp, C.c_int, dp, C.c_long], C.c_int),
         "caffe_ghost_rows": ([vp, ip, lp, lp, dp, dp], C.c_int),
+        "caffe_time_kernel": ([vp, C.c_char_p, C.c_int, dp, dp], C.c_int),
         "caffe_sync": ([vp], C.c_int),
         "caffe_padded_size": ([vp], C.c_long),
     }

@@ -207,8 +207,7 @@ int setup(Ctx& c, const caffe_case* cs, const double* nodes) {
     (void)zero6;
 
     // ---- reductions, scalars, events
-    c.red.part = dalloc(c, (size_t)MAX_RED_BLOCKS * 8);
-    c.red.ticket = talloc<unsigned int>(c, 4);
+    c.red.part = dalloc(c, (size_t)3 * MAX_RED_BLOCKS * RED_STRIDE + 4096);
     c.scal = dalloc(c, N_SCAL);
     c.flags = talloc<int>(c, N_FLAGS);
     if (!c.hscal) {
@@ -520,12 +519,30 @@ int caffe_ghost_rows(caffe_ctx* x, int* n, long* cells, long* cols, double* vm, 
         const int m = c.ov.n;
         *n = m;
         if (m > 0) {
-            if (cells) HIPCHECK(hipMemcpyAsync(cells, c.ov.cell, sizeof(long) * m, hipMemcpyDeviceToHost, c.stream));
-            if (cols) HIPCHECK(hipMemcpyAsync(cols, c.ov.cols, sizeof(long) * 9 * m, hipMemcpyDeviceToHost, c.stream));
+            std::vector<int> ce(m), co(9 * (size_t)m);
+            HIPCHECK(hipMemcpyAsync(ce.data(), c.ov.cell, sizeof(int) * m, hipMemcpyDeviceToHost, c.stream));
+            HIPCHECK(hipMemcpyAsync(co.data(), c.ov.cols, sizeof(int) * 9 * m, hipMemcpyDeviceToHost, c.stream));
+            HIPCHECK(hipStreamSynchronize(c.stream));
+            if (cells) for (int q = 0; q < m; ++q) cells[q] = ce[q];
+            if (cols) for (int q = 0; q < 9 * m; ++q) cols[q] = co[q];
             if (vm) HIPCHECK(hipMemcpyAsync(vm, c.ov.valM, sizeof(double) * 9 * m, hipMemcpyDeviceToHost, c.stream));
             if (vp) HIPCHECK(hipMemcpyAsync(vp, c.ov.valP, sizeof(double) * 9 * m, hipMemcpyDeviceToHost, c.stream));
         }
         HIPCHECK(hipStreamSynchronize(c.stream));
+        return CAFFE_OK;
+    });
+}
+
+int caffe_time_kernel(caffe_ctx* x, const char* which, int reps, double* ms, double* bytes) {
+    return guard(x, [&](Ctx& c) {
+        READY(c);
+        std::string w(which ? which : "");
+        if (reps < 1 || !ms) { c.err = "bad timing arguments"; return CAFFE_ERR_ARG; }
+        const bool f32 = (w == "pcorr_cg_k1_fp32");
+        if (!f32 && w != "pcorr_cg_k1_fp64") { c.err = "unknown kernel " + w; return CAFFE_ERR_ARG; }
+        *ms = time_cg_k1(c, f32, reps);
+        // algorithmic bytes per interior cell: aP, aI, aJ, aK + z + p_old read, p_new + Ap written
+        if (bytes) *bytes = (double)c.D.N * (f32 ? 4.0 : 8.0) * 8.0;
         return CAFFE_OK;
     });
 }

@@ -34,17 +34,22 @@ __host__ __device__ inline V3 cross(V3 u, V3 v) {
 __host__ __device__ inline double mag(V3 u) { return sqrt(u.x * u.x + u.y * u.y + u.z * u.z); }
 __host__ __device__ inline V3 unitv(V3 u) { double inv = 1. / mag(u); return mk(inv * u.x, inv * u.y, inv * u.z); }
 
-// Padded grid dimensions.
+// Padded grid dimensions (32-bit indices: the padded grid holds < 2^31 cells).
 struct Dims {
     int nI, nJ, nK;
     int sJ;      // nI + 2
-    long sK;     // (nI + 2) * (nJ + 2)
-    long P;      // padded cell count
-    long N;      // interior cell count
-    __host__ __device__ long pidx(int i, int j, int k) const { return (long)(i + 1) + (long)(j + 1) * sJ + (long)(k + 1) * sK; }
+    int sK;      // (nI + 2) * (nJ + 2)
+    int P;       // padded cell count
+    int N;       // interior cell count
+    __host__ __device__ int pidx(int i, int j, int k) const { return (i + 1) + (j + 1) * sJ + (k + 1) * sK; }
     // interior linear id (i fastest) -> padded index and (i,j,k)
-    __host__ __device__ long cell(long q, int& i, int& j, int& k) const {
-        i = (int)(q % nI); long r = q / nI; j = (int)(r % nJ); k = (int)(r / nJ);
+    __host__ __device__ int cell(int q, int& i, int& j, int& k) const {
+        const unsigned uq = (unsigned)q, ni = (unsigned)nI, nj = (unsigned)nJ;
+        const unsigned r = uq / ni;
+        i = (int)(uq - r * ni);
+        const unsigned kk = r / nj;
+        j = (int)(r - kk * nj);
+        k = (int)kk;
         return pidx(i, j, k);
     }
 };
@@ -126,47 +131,39 @@ __device__ inline double blockMax(double v, double* lds) {
     return r;
 }
 
-// Grid-wide reduction with "last block finishes": every block writes its K partial
-// sums, the block that takes the last ticket adds the partials in block order (a
-// fixed order, so the result does not depend on scheduling) and runs Fin on them.
-constexpr int MAX_RED_BLOCKS = 4096;
-struct RedBuf {
-    double* part;          // MAX_RED_BLOCKS * 8
-    unsigned int* ticket;  // one counter, returns to 0 after each use
-};
+// Deterministic grid reductions without device-scope fences.  A producing kernel
+// writes one row of up to 8 partial sums per block; a consuming kernel (or a
+// one-block finalize kernel) adds the rows in block order.  Kernel boundaries give
+// the visibility, so no block ever waits for, or fences against, another.
+constexpr int MAX_RED_BLOCKS = 1024;
+constexpr int RED_STRIDE = 8;
 
-template <int BS, int K, class Fin>
-__device__ inline void gridReduce(const double (&v)[K], RedBuf rb, Fin fin) {
+template <int BS, int K>
+__device__ inline void writePartials(const double (&v)[K], double* buf) {
     __shared__ double lds[BS / 64];
-    __shared__ bool last;
-    double s[K];
 #pragma unroll
-    for (int q = 0; q < K; ++q) s[q] = blockSum<BS>(v[q], lds);
-    if (threadIdx.x == 0) {
+    for (int q = 0; q < K; ++q) {
+        double s = blockSum<BS>(v[q], lds);
+        if (threadIdx.x == 0) buf[blockIdx.x * RED_STRIDE + q] = s;
+    }
+}
+
+// every thread of the block receives the K totals of nb rows
+template <int BS, int K>
+__device__ inline void readPartials(const double* buf, int nb, double (&out)[K]) {
+    __shared__ double lds[BS / 64];
+    __shared__ double res[K];
 #pragma unroll
-        for (int q = 0; q < K; ++q) rb.part[(long)blockIdx.x * 8 + q] = s[q];
-        __threadfence();
-        unsigned int t = atomicAdd(rb.ticket, 1u);
-        last = (t == gridDim.x - 1);
+    for (int q = 0; q < K; ++q) {
+        double a = 0.;
+        for (int b = threadIdx.x; b < nb; b += BS) a += buf[b * RED_STRIDE + q];
+        double s = blockSum<BS>(a, lds);
+        if (threadIdx.x == 0) res[q] = s;
     }
     __syncthreads();
-    if (!last) return;
-    __threadfence();
-    double acc[K];
 #pragma unroll
-    for (int q = 0; q < K; ++q) acc[q] = 0.;
-    // fixed split: thread t sums blocks t, t+BS, ... in order; then a fixed tree
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) {
-#pragma unroll
-        for (int q = 0; q < K; ++q) acc[q] += __hip_atomic_load(&rb.part[(long)b * 8 + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    double tot[K];
-#pragma unroll
-    for (int q = 0; q < K; ++q) tot[q] = blockSum<BS>(acc[q], lds);
-    if (threadIdx.x == 0) {
-        fin(tot);
-        *rb.ticket = 0u;
-    }
+    for (int q = 0; q < K; ++q) out[q] = res[q];
+    __syncthreads();
 }
 
 }  // namespace caffe

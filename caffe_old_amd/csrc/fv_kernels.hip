@@ -598,13 +598,14 @@ void launch_fill(Ctx& c, double* a, double v, long n) {
     hipLaunchKernelGGL(k_fill, dim3(blocksFor(n, BS)), dim3(BS), 0, c.stream, a, v, n);
 }
 
-// IbPiso.cpp:74-80: SOLID cells get u = 0, p = 0
+// IbPiso.cpp:74-80: every SOLID cell (INACTIVE, and the IB/GHOST cells picked from
+// them afterwards) starts the time step with u = 0, p = 0
 __global__ void k_zero_inactive(Dims D, const int8_t* st, double* u0, double* u1, double* u2, double* p) {
     long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= D.N) return;
     int i, j, k;
     const long pp = D.cell(q, i, j, k);
-    if (st[pp] == ST_INACTIVE) { u0[pp] = 0.; u1[pp] = 0.; u2[pp] = 0.; p[pp] = 0.; }
+    if (st[pp] != ST_ACTIVE) { u0[pp] = 0.; u1[pp] = 0.; u2[pp] = 0.; p[pp] = 0.; }
 }
 void launch_set_inactive_fields(Ctx& c) {
     hipLaunchKernelGGL(k_zero_inactive, dim3(blocksFor(c.D.N, BS)), dim3(BS), 0, c.stream, c.D, c.st, c.u[0], c.u[1], c.u[2], c.p);

@@ -119,8 +119,8 @@ __device__ inline void trilinear(const V3* pts, const V3& ip, double* beta) {
 
 struct CoefArgs {
     int n;
-    const long* cell;
-    long* cols;
+    const int* cell;
+    int* cols;
     double *valM, *valP;
     double *Am, *Ap;     // diagonal of ghost rows for the Jacobi preconditioner
     const int8_t* st;
@@ -134,8 +134,8 @@ __global__ void k_ib_coeffs(Geo g, CoefArgs a) {
     const Dims& D = g.D;
     int gi = blockIdx.x * blockDim.x + threadIdx.x;
     if (gi >= a.n) return;
-    const long p = a.cell[gi];
-    long rem = p;
+    const int p = a.cell[gi];
+    int rem = p;
     const int k = (int)(rem / D.sK) - 1; rem %= D.sK;
     const int j = (int)(rem / D.sJ) - 1;
     const int i = (int)(rem % D.sJ) - 1;
@@ -188,12 +188,12 @@ __global__ void k_ib_coeffs(Geo g, CoefArgs a) {
     trilinear(pts, ip, beta);
     // IbPiso.cpp:148-185
     double vm[9], vp[9];
-    long cl[9];
+    int cl[9];
     for (int m = 0; m < 9; ++m) { vm[m] = 0.; vp[m] = 0.; cl[m] = -1; }
     vm[0] = 1.;
     cl[0] = p;
     for (int m = 0; m < 8; ++m) {
-        const long pm = D.pidx(ii[m], jj[m], kk[m]);
+        const int pm = D.pidx(ii[m], jj[m], kk[m]);
         if (ii[m] == i && jj[m] == j && kk[m] == k) { vm[0] += beta[m]; cl[m + 1] = -1; }
         else { vm[m + 1] = beta[m]; cl[m + 1] = a.st[pm] == ST_INACTIVE ? -1 : pm; }
     }
@@ -215,8 +215,8 @@ static void ensureOverlay(Ctx& c, int n) {
     auto al = [&](size_t bytes) { void* ptr = nullptr; HIPCHECK(hipMalloc(&ptr, bytes)); HIPCHECK(hipMemset(ptr, 0, bytes)); c.allocs.push_back(ptr); return ptr; };
     if (!c.ov.rowOf) c.ov.rowOf = (int*)al(sizeof(int) * c.D.P);
     if (n > 0) {
-        c.ov.cell = (long*)al(sizeof(long) * n);
-        c.ov.cols = (long*)al(sizeof(long) * 9 * n);
+        c.ov.cell = (int*)al(sizeof(int) * n);
+        c.ov.cols = (int*)al(sizeof(int) * 9 * n);
         c.ov.valM = (double*)al(sizeof(double) * 9 * n);
         c.ov.valP = (double*)al(sizeof(double) * 9 * n);
         c.ov.valP32 = (float*)al(sizeof(float) * 9 * n);
@@ -241,17 +241,17 @@ int ib_field(Ctx& c) {
     std::vector<int8_t> st(D.P);
     HIPCHECK(hipMemcpyAsync(st.data(), c.st, D.P, hipMemcpyDeviceToHost, c.stream));
     HIPCHECK(hipStreamSynchronize(c.stream));
-    std::vector<long> cells;
+    std::vector<int> cells;
     std::vector<int> rowOf(D.P, -1);
     for (int k = 0; k < D.nK; ++k)
         for (int j = 0; j < D.nJ; ++j)
             for (int i = 0; i < D.nI; ++i) {
-                long p = D.pidx(i, j, k);
+                int p = D.pidx(i, j, k);
                 if (st[p] == ST_GHOST) { rowOf[p] = (int)cells.size(); cells.push_back(p); }
             }
     ensureOverlay(c, (int)cells.size());
     HIPCHECK(hipMemcpyAsync(c.ov.rowOf, rowOf.data(), sizeof(int) * D.P, hipMemcpyHostToDevice, c.stream));
-    if (!cells.empty()) HIPCHECK(hipMemcpyAsync(c.ov.cell, cells.data(), sizeof(long) * cells.size(), hipMemcpyHostToDevice, c.stream));
+    if (!cells.empty()) HIPCHECK(hipMemcpyAsync(c.ov.cell, cells.data(), sizeof(int) * cells.size(), hipMemcpyHostToDevice, c.stream));
     // ghost and inactive rows carry a zero right-hand side (never assembled)
     HIPCHECK(hipStreamSynchronize(c.stream));
     return CAFFE_OK;

@@ -13,18 +13,17 @@ namespace caffe {
 
 // Device scalar slots (double) used by the Krylov kernels.
 enum {
-    SC_M = 0,    // momentum system v uses SC_M + 8*v: rho, rhoOld, alpha, omega, beta, bb, rr
-    SC_P = 24,   // pressure system (FP64 and FP32 inner): rz, alpha, beta, bb, rr, pq
-    SC_IR = 32,  // refinement: rr64, bb64, scale
-    SC_MISC = 40,
-    N_SCAL = 64
+    SC_BI = 0,     // BiCGStab state, 16 slots per system (up to 3 systems)
+    SC_CG = 48,    // CG state
+    SC_MISC = 64,  // reductions read back by the host
+    N_SCAL = 128
 };
 enum { FL_MDONE = 0, FL_PDONE = 1, FL_MITER = 2, FL_PITER = 3, N_FLAGS = 8 };
 
 struct Overlay {          // ghost-cell rows (IbPiso.cpp:148-185), 9 entries each
     int n = 0;
-    long* cell = nullptr;     // padded index of the ghost cell
-    long* cols = nullptr;     // 9 * n padded column cells (-1 = dropped)
+    int* cell = nullptr;      // padded index of the ghost cell
+    int* cols = nullptr;      // 9 * n padded column cells (-1 = dropped)
     double* valM = nullptr;   // 9 * n momentum-row values
     double* valP = nullptr;   // 9 * n pressure-row values
     float* valP32 = nullptr;
@@ -63,10 +62,10 @@ struct Ctx {
     Overlay ov;
 
     // Krylov work
-    double* wm = nullptr;     // momentum BiCGStab: 8 vectors * 3 * P
-    double* wp = nullptr;     // pressure: 6 * P
-    float* wp32 = nullptr;    // FP32 inner: 6 * P
-    RedBuf red{};
+    double* wm = nullptr;     // momentum BiCGStab: 12 vectors * 3 * P
+    double* wp = nullptr;     // pressure: 8 * P
+    float* wp32 = nullptr;    // FP32 inner: 8 * P
+    struct { double* part = nullptr; } red;  // 3 partial-sum buffers of MAX_RED_BLOCKS rows
     double* scal = nullptr;   // N_SCAL device scalars
     int* flags = nullptr;     // N_FLAGS device ints
     double* hscal = nullptr;  // pinned host mirror
@@ -111,6 +110,8 @@ void compute_metrics(Ctx& c, const double* dNodes);
 int momentum_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep);
 int pcorr_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep);
 void pcorr_make_fp32(Ctx& c);
+void add_vec(Ctx& c, double* x, const double* d, int n);
+double time_cg_k1(Ctx& c, bool fp32, int reps);
 
 // ---- immersed boundary (ib.hip)
 int ib_field(Ctx& c);

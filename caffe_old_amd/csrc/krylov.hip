@@ -2,71 +2,75 @@
 // correction systems (the replacement of SparseMatrix::solve, SparseMatrix.cpp:86-106).
 //
 // Every solve starts from x = 0, as KSPSolve does with PETSc's default initial guess.
-// Convergence is on the true residual norm ||b - A x||_2 <= rtol ||b||_2; the
-// recurrence residual is checked every iteration and the true residual at exit
-// (a restart from it if they disagree).
+// Convergence is judged on the true FP64 residual ||b - A x||_2 <= rtol ||b||_2.
 //
-// Design: all Krylov scalars live in device memory.  Each kernel both applies the
-// operator (7-point stencil, operands formed on the fly at the 7 stencil points from
-// the previous iterates, so no separate vector-update pass) and produces the partial
-// dot products; the last block to finish combines the partials in block order (a
-// fixed order) and computes the next scalar (alpha, omega, beta) on the device.  The
-// host queues a chunk of iterations and reads one flag per chunk.
+// Kernel design.  All Krylov scalars stay in device memory.  Each kernel
+//   1. adds up the per-block partial sums the previous kernel wrote (every block does
+//      this redundantly, in block order, so all blocks hold identical scalars and the
+//      result does not depend on scheduling),
+//   2. derives its step length (alpha, beta, omega) and the convergence flag,
+//   3. applies the 7-point operator to an operand formed on the fly at the stencil
+//      points from the previous iterates (so there is no separate vector-update pass),
+//   4. writes its own partial sums.
+// No kernel uses a device-scope fence or atomic: on CDNA the L2 of each XCD is not
+// coherent with the others, so such fences write back L2; kernel boundaries give the
+// ordering instead.  Block 0 records the scalars the next kernel or the host needs.
 //
 // Systems:
-//   momentum     7-point band (aP aE aW aN aS aT aB), 3 right-hand sides sharing the
-//                coefficients, plus 9-entry ghost rows; BiCGStab, Jacobi.
-//   pressure     symmetric storage (aP + one coefficient per face), pure Neumann or
-//                Dirichlet outlet; PCG with Jacobi in FP64, or FP32 PCG inside FP64
-//                iterative refinement.  With ghost rows the matrix is nonsymmetric and
-//                BiCGStab is used instead.
+//   momentum  7-point band (aP aE aW aN aS aT aB), three right-hand sides sharing the
+//             coefficients, 9-entry ghost rows; BiCGStab with Jacobi, FP64.
+//   pressure  symmetric storage (aP + one coefficient per face); CG with Jacobi.  FP64,
+//             or FP32 iterations with FP64 reliable updates (the true residual is
+//             recomputed in FP64 and the FP32 recurrence restarted from it, keeping the
+//             search direction).  With ghost rows (nonsymmetric) BiCGStab is used.
 #include "internal.hpp"
 
 #include <cmath>
+#include <type_traits>
 
 namespace caffe {
 
-constexpr int KBS = 256;
-constexpr int KGRID_CAP = 2048;
+constexpr int KBS = 1024;
+constexpr int KGRID = 512;   // at most 2 blocks of 16 waves per CU: the partial rows stay few
 
 // ------------------------------------------------------------------ operators
 struct OvView {
     const int* rowOf;      // nullptr when there are no ghost rows
-    const long* cols;
+    const int* cols;
     const void* vals;      // double or float, 9 per row
 };
 
-template <class T>
-struct Band7 {             // momentum matrix
-    const double* A; long P; long sJ, sK; OvView ov; const int8_t* st;
+struct Band7 {             // momentum matrix (FP64)
+    const double* A; int P, sJ, sK; OvView ov; const int8_t* st;
     template <class F>
-    __device__ double apply(F x, long p) const {
+    __device__ double apply(F x, int p) const {
         if (ov.rowOf) {
             int r = ov.rowOf[p];
             if (r >= 0) {
-                const double* v = (const double*)ov.vals + 9L * r;
+                const double* v = (const double*)ov.vals + 9 * r;
                 double s = v[0] * x(p);
-                for (int m = 1; m < 9; ++m) { long c = ov.cols[9L * r + m]; if (c >= 0) s += v[m] * x(c); }
+                for (int m = 1; m < 9; ++m) { int c = ov.cols[9 * r + m]; if (c >= 0) s += v[m] * x(c); }
                 return s;
             }
         }
         return A[p] * x(p) + A[P + p] * x(p + 1) + A[2 * P + p] * x(p - 1) + A[3 * P + p] * x(p + sJ)
              + A[4 * P + p] * x(p - sJ) + A[5 * P + p] * x(p + sK) + A[6 * P + p] * x(p - sK);
     }
+    __device__ double diag(int p) const { return A[p]; }
 };
 
 template <class T>
 struct Sym4 {              // pressure-correction matrix
-    const T* A; long P; long sJ, sK; OvView ov; const int8_t* st;
+    const T* A; int P, sJ, sK; OvView ov; const int8_t* st;
     template <class F>
-    __device__ double apply(F x, long p) const {
+    __device__ double apply(F x, int p) const {
         if (ov.rowOf) {
             if (st[p] == ST_INACTIVE) return x(p);
             int r = ov.rowOf[p];
             if (r >= 0) {
-                const T* v = (const T*)ov.vals + 9L * r;
+                const T* v = (const T*)ov.vals + 9 * r;
                 double s = (double)v[0] * x(p);
-                for (int m = 1; m < 9; ++m) { long c = ov.cols[9L * r + m]; if (c >= 0) s += (double)v[m] * x(c); }
+                for (int m = 1; m < 9; ++m) { int c = ov.cols[9 * r + m]; if (c >= 0) s += (double)v[m] * x(c); }
                 return s;
             }
         }
@@ -74,389 +78,448 @@ struct Sym4 {              // pressure-correction matrix
              + (double)A[2 * P + p] * x(p - sJ) + (double)A[2 * P + p + sJ] * x(p + sJ)
              + (double)A[3 * P + p] * x(p - sK) + (double)A[3 * P + p + sK] * x(p + sK);
     }
+    __device__ double diag(int p) const { return (double)A[p]; }
 };
 
-// ------------------------------------------------------------------ slot layouts
-enum { B_RHO = 0, B_ALPHA, B_OMEGA, B_BETA, B_BB, B_RR, B_DONE, B_IT };  // BiCGStab
-enum { C_RZ = 0, C_ALPHA, C_BETA, C_BB, C_RR, C_DONE, C_IT, C_PQ };     // CG
-
-struct LoopCtl { double tol2; int maxit; };
-
 // grid-stride over interior cells: q = interior id, p = padded index
-__device__ inline long cellIdx(const Dims& D, long q) { int i, j, k; return D.cell(q, i, j, k); }
-#define FOR_CELLS(D, q, p)                                                                    \
-    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x, p = 0;                        \
-         q < (D).N && ((p = cellIdx((D), q)), true); q += (long)gridDim.x * blockDim.x)
+__device__ inline int cellIdx(const Dims& D, int q) { int i, j, k; return D.cell(q, i, j, k); }
+#define FOR_CELLS(D, q, p)                                                                 \
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x, p = 0;                            \
+         q < (D).N && ((p = cellIdx((D), q)), true); q += gridDim.x * blockDim.x)
 
-// ================================================================== PCG (pressure)
-// init: r = b - mean (if singular), z = r / aP, p = 0, bb, rz
-template <class T>
-__global__ void k_cg_init(Dims D, Sym4<T> A, const T* b, double mean, T* r, T* z, T* p0, T* p1, T* x,
-                          double* sc, RedBuf rb) {
-    double v[2] = {0., 0.};
+#define BLOCK0 (blockIdx.x == 0 && threadIdx.x == 0)
+
+struct LoopCtl { double tol2; int maxit; double nLive; };
+
+// ================================================================== CG
+// state slots (doubles)
+enum { C_RZCUR = 0, C_RZPREV, C_BB, C_RR, C_IT, C_DONE, C_RATIO, C_PQ, C_PBUF };
+
+// r = s*(b - shift) on live cells, z = r / aP, p = 0 (zeroP), x = 0 (zeroX);
+// partials (r,z), (r,r)
+template <class T, class TB>
+__global__ void k_cg_init(Dims D, Sym4<T> A, const TB* b, double shift, double s, T* r, T* z, T* p0, T* p1, T* x,
+                          int zeroP, int zeroX, double* part, double* st, double ratio) {
+    double v[4] = {0., 0., 0., 0.};
     FOR_CELLS(D, q, p) {
         const bool live = !A.st || A.st[p] != ST_INACTIVE;
-        double bv = live ? (double)b[p] - mean : 0.;
-        T rr = (T)bv;
-        double zz = (double)rr / (double)A.A[p];
-        r[p] = rr; z[p] = (T)zz; p0[p] = 0; p1[p] = 0; x[p] = 0;
-        v[0] += (double)rr * (double)rr;
-        v[1] += (double)rr * zz;
+        const T rr = (T)(live ? ((double)b[p] - shift) * s : 0.);
+        const double zz = (double)rr / A.diag(p);
+        r[p] = rr; z[p] = (T)zz;
+        if (zeroP) { p0[p] = 0; p1[p] = 0; }
+        if (zeroX) x[p] = 0;
+        v[0] += (double)rr * zz;
+        v[1] += (double)rr * (double)rr;
+        v[2] += (double)(T)zz;
+        v[3] += (double)rr;
     }
-    gridReduce<KBS, 2>(v, rb, [&](const double* t) {
-        sc[C_BB] = t[0]; sc[C_RR] = t[0]; sc[C_RZ] = t[1]; sc[C_BETA] = 0.; sc[C_IT] = 0.;
-        sc[C_DONE] = (t[0] == 0.) ? 1. : 0.;
-    });
+    writePartials<KBS, 4>(v, part);
+    if (BLOCK0) {
+        if (zeroP) st[C_IT] = 0.;
+        st[C_DONE] = 0.; st[C_BB] = -1.; st[C_RATIO] = ratio;
+    }
 }
 
-// K1: p_new = z + beta p_old (on the fly at the stencil), q = A p_new, partial (p_new, q)
+// K1: [rz, rr from the previous partials] convergence; beta; p_new = z + beta p_old formed
+// at the stencil points; q = A p_new; partial (p_new, q)
+// For a singular (pure Neumann) system lc.nLive > 0: the preconditioned residual is
+// projected onto zero mean, z - mean(z), so that neither the search directions nor the
+// iterate drift along the constant null vector (rounding puts a small component there).
+// The two direction buffers alternate with the parity of the completed-iteration count
+// kept on the device, so a chunk of queued iterations that stops early (convergence)
+// leaves the latest direction where the next K1 looks for it.
 template <class T>
-__global__ void k_cg_k1(Dims D, Sym4<T> A, const T* z, const T* pold, T* pnew, T* qv, double* sc, RedBuf rb) {
-    if (sc[C_DONE] != 0.) return;
-    const double beta = sc[C_BETA];
+__global__ void k_cg_k1(Dims D, Sym4<T> A, const T* z, T* pb0, T* pb1, T* qv, const double* partIn, int nb,
+                        double* partOut, double* st, LoopCtl lc) {
+    if (st[C_DONE] != 0.) return;
+    const int par = ((int)st[C_IT]) & 1;
+    const T* pold = par ? pb1 : pb0;
+    T* pnew = par ? pb0 : pb1;
+    double t[4];
+    readPartials<KBS, 4>(partIn, nb, t);
+    const double zbar = lc.nLive > 0. ? t[2] / lc.nLive : 0.;
+    const double rz = t[0] - zbar * t[3];
+    // residual norm without its (unreachable) mean component
+    const double rr = lc.nLive > 0. ? fmax(t[1] - t[3] * t[3] / lc.nLive, 0.) : t[1];
+    const double bb = st[C_BB] < 0. ? rr : st[C_BB];
+    const bool first = st[C_IT] == 0.;
+    const bool done = rr <= lc.tol2 * bb || st[C_IT] >= lc.maxit || rr == 0.;
+    if (BLOCK0) {
+        st[C_BB] = bb; st[C_RR] = rr; st[C_RZCUR] = rz; st[C_PBUF] = par ? 0. : 1.;
+        if (done) st[C_DONE] = 1.;
+    }
+    if (done) return;
+    const double beta = first ? 0. : (rz / st[C_RZPREV]) * st[C_RATIO];
     double v[1] = {0.};
-    auto pf = [&](long c) { return (double)z[c] + beta * (double)pold[c]; };
+    // (singular systems have no ghost rows: halo cells meet zero coefficients, so the
+    // shifted value there is never used)
+    auto pf = [&](int c) { return ((double)z[c] - zbar) + beta * (double)pold[c]; };
     FOR_CELLS(D, q, p) {
-        double pn = pf(p);
-        double y = A.apply(pf, p);
+        const double pn = pf(p);
+        const double y = A.apply(pf, p);
         pnew[p] = (T)pn; qv[p] = (T)y;
         v[0] += pn * y;
     }
-    gridReduce<KBS, 1>(v, rb, [&](const double* t) {
-        sc[C_PQ] = t[0];
-        sc[C_ALPHA] = t[0] != 0. ? sc[C_RZ] / t[0] : 0.;
-    });
+    writePartials<KBS, 1>(v, partOut);
 }
 
-// K2: x += alpha p; r -= alpha q; z = r / aP; partials (r,z), (r,r); beta, convergence
+// K2: [alpha = rz / (p, q)] x += alpha p; r -= alpha q; z = r / aP; partials (r,z), (r,r)
 template <class T>
-__global__ void k_cg_k2(Dims D, Sym4<T> A, T* x, T* r, T* z, const T* pv, const T* qv, double* sc, RedBuf rb, LoopCtl lc) {
-    if (sc[C_DONE] != 0.) return;
-    const double alpha = sc[C_ALPHA];
-    double v[2] = {0., 0.};
-    FOR_CELLS(D, q, p) {
-        double xn = (double)x[p] + alpha * (double)pv[p];
-        double rn = (double)r[p] - alpha * (double)qv[p];
-        T rt = (T)rn;
-        double zn = (double)rt / (double)A.A[p];
-        x[p] = (T)xn; r[p] = rt; z[p] = (T)zn;
-        v[0] += (double)rt * zn;
-        v[1] += (double)rt * (double)rt;
+__global__ void k_cg_k2(Dims D, Sym4<T> A, T* x, T* r, T* z, const T* pb0, const T* pb1, const T* qv, const double* partIn,
+                        int nb, double* partOut, double* st) {
+    if (st[C_DONE] != 0.) return;
+    const T* pv = st[C_PBUF] != 0. ? pb1 : pb0;  // the direction K1 of this iteration wrote
+    double t[1];
+    readPartials<KBS, 1>(partIn, nb, t);
+    const double alpha = t[0] != 0. ? st[C_RZCUR] / t[0] : 0.;
+    if (BLOCK0) {
+        st[C_RZPREV] = st[C_RZCUR]; st[C_IT] += 1.; st[C_RATIO] = 1.; st[C_PQ] = t[0];
     }
-    gridReduce<KBS, 2>(v, rb, [&](const double* t) {
-        double rzOld = sc[C_RZ];
-        sc[C_RZ] = t[0];
-        sc[C_BETA] = rzOld != 0. ? t[0] / rzOld : 0.;
-        sc[C_RR] = t[1];
-        sc[C_IT] += 1.;
-        if (t[1] <= lc.tol2 * sc[C_BB] || sc[C_IT] >= lc.maxit || sc[C_PQ] == 0.) sc[C_DONE] = 1.;
-    });
+    double v[4] = {0., 0., 0., 0.};
+    FOR_CELLS(D, q, p) {
+        const double xn = (double)x[p] + alpha * (double)pv[p];
+        const T rt = (T)((double)r[p] - alpha * (double)qv[p]);
+        const T zn = (T)((double)rt / A.diag(p));
+        x[p] = (T)xn; r[p] = rt; z[p] = zn;
+        v[0] += (double)rt * (double)zn;
+        v[1] += (double)rt * (double)rt;
+        v[2] += (double)zn;
+        v[3] += (double)rt;
+    }
+    writePartials<KBS, 4>(v, partOut);
 }
 
 // ================================================================== BiCGStab, NV systems
-// Vectors of system v are at offset v*P.  Slots of system v at sc + 8v.
+enum { B_RHOCUR = 0, B_RHOPREV, B_ALPHA, B_OMEGA, B_BB, B_RR, B_IT, B_DONE, B_BRK, B_STRIDE = 16 };
+
 template <int NV, class T, class Op>
-__global__ void k_bi_init(Dims D, Op A, const T* b, double mean, T* r, T* rh, T* p0, T* p1, T* v0, T* v1, T* x,
-                          T* dinv, double* sc, RedBuf rb) {
-    double acc[NV];
-    for (int s = 0; s < NV; ++s) acc[s] = 0.;
-    const long P = D.P;
+__global__ void k_bi_init(Dims D, Op A, const double* b, double shift, T* r, T* rh, T* p0, T* v0, T* x, T* dinv,
+                          double* part, double* st) {
+    double acc[2 * NV];
+    for (int s = 0; s < 2 * NV; ++s) acc[s] = 0.;
+    const int P = D.P;
     FOR_CELLS(D, q, p) {
         const bool live = !A.st || A.st[p] != ST_INACTIVE;
-        dinv[p] = (T)(1. / (double)A.A[p]);
+        dinv[p] = (T)(1. / A.diag(p));
         for (int s = 0; s < NV; ++s) {
-            const long o = s * P + p;
-            double bv = live ? (double)b[o] - mean : 0.;
-            r[o] = (T)bv; rh[o] = (T)bv; p0[o] = 0; p1[o] = 0; v0[o] = 0; v1[o] = 0; x[o] = 0;
-            acc[s] += bv * bv;
+            const int o = s * P + p;
+            const T bv = (T)(live ? b[o] - shift : 0.);
+            r[o] = bv; rh[o] = bv; p0[o] = 0; v0[o] = 0; x[o] = 0;
+            acc[2 * s] += (double)bv * (double)bv;
+            acc[2 * s + 1] += (double)bv * (double)bv;
         }
     }
-    gridReduce<KBS, NV>(acc, rb, [&](const double* t) {
+    writePartials<KBS, 2 * NV>(acc, part);
+    if (BLOCK0) {
         for (int s = 0; s < NV; ++s) {
-            double* c = sc + 8 * s;
-            c[B_BB] = t[s]; c[B_RR] = t[s]; c[B_RHO] = t[s];  // rho = (rh, r) = (r, r)
-            c[B_ALPHA] = 1.; c[B_OMEGA] = 1.; c[B_BETA] = 0.; c[B_IT] = 0.;
-            c[B_DONE] = (t[s] == 0.) ? 1. : 0.;
+            double* c = st + B_STRIDE * s;
+            c[B_IT] = 0.; c[B_DONE] = 0.; c[B_BB] = -1.; c[B_RHOPREV] = 1.; c[B_ALPHA] = 1.; c[B_OMEGA] = 1.; c[B_BRK] = 0.;
         }
-    });
+    }
 }
 
-// KA: p_new = r + beta (p_old - omega v_old); v_new = A D^-1 p_new; partial (rh, v_new)
+// KA: [rho, rr] convergence; beta; p_new = r + beta (p - omega v) at the stencil points;
+// v_new = A D^-1 p_new; partial (rh, v_new)
 template <int NV, class T, class Op>
 __global__ void k_bi_a(Dims D, Op A, const T* r, const T* rh, const T* pold, T* pnew, const T* vold, T* vnew,
-                       const T* dinv, double* sc, RedBuf rb) {
+                       const T* dinv, const double* partIn, int nb, double* partOut, double* st, LoopCtl lc) {
     bool any = false;
-    for (int s = 0; s < NV; ++s) any |= sc[8 * s + B_DONE] == 0.;
+    for (int s = 0; s < NV; ++s) any |= st[B_STRIDE * s + B_DONE] == 0.;
     if (!any) return;
-    double acc[NV];
+    double t[2 * NV];
+    readPartials<KBS, 2 * NV>(partIn, nb, t);
     double beta[NV], om[NV];
     bool live[NV];
     for (int s = 0; s < NV; ++s) {
-        acc[s] = 0.; beta[s] = sc[8 * s + B_BETA]; om[s] = sc[8 * s + B_OMEGA]; live[s] = sc[8 * s + B_DONE] == 0.;
+        const double* c = st + B_STRIDE * s;
+        const double rho = t[2 * s], rr = t[2 * s + 1];
+        const double bb = c[B_BB] < 0. ? rr : c[B_BB];
+        double done = c[B_DONE];
+        if (done == 0.) {
+            if (rr <= lc.tol2 * bb || c[B_IT] >= lc.maxit || rr == 0.) done = 1.;
+            else if (c[B_BRK] != 0. || rho == 0.) done = 2.;
+        }
+        live[s] = done == 0.;
+        om[s] = c[B_OMEGA];
+        beta[s] = c[B_IT] == 0. ? 0. : (rho / c[B_RHOPREV]) * (c[B_ALPHA] / c[B_OMEGA]);
+        if (BLOCK0) {
+            double* w = st + B_STRIDE * s;
+            w[B_BB] = bb; w[B_RR] = rr; w[B_RHOCUR] = rho; w[B_DONE] = done;
+        }
     }
-    const long P = D.P;
+    double acc[NV];
+    for (int s = 0; s < NV; ++s) acc[s] = 0.;
+    const int P = D.P;
     FOR_CELLS(D, q, p) {
         for (int s = 0; s < NV; ++s) {
             if (!live[s]) continue;
-            const long o = s * P;
-            auto pf = [&](long c) { return (double)r[o + c] + beta[s] * ((double)pold[o + c] - om[s] * (double)vold[o + c]); };
-            auto ph = [&](long c) { return pf(c) * (double)dinv[c]; };
-            double pn = pf(p);
-            double vn = A.apply(ph, p);
+            const int o = s * P;
+            auto pf = [&](int c) { return (double)r[o + c] + beta[s] * ((double)pold[o + c] - om[s] * (double)vold[o + c]); };
+            auto ph = [&](int c) { return pf(c) * (double)dinv[c]; };
+            const double pn = pf(p);
+            const double vn = A.apply(ph, p);
             pnew[o + p] = (T)pn; vnew[o + p] = (T)vn;
             acc[s] += (double)rh[o + p] * vn;
         }
     }
-    gridReduce<KBS, NV>(acc, rb, [&](const double* t) {
-        for (int s = 0; s < NV; ++s) {
-            double* c = sc + 8 * s;
-            if (c[B_DONE] != 0.) continue;
-            if (t[s] == 0.) { c[B_DONE] = 2.; c[B_ALPHA] = 0.; }  // breakdown: host restarts
-            else c[B_ALPHA] = c[B_RHO] / t[s];
-        }
-    });
+    writePartials<KBS, NV>(acc, partOut);
 }
 
-// KB: s = r - alpha v (on the fly); t = A D^-1 s; partials (t,s), (t,t)
+// KB: [alpha = rho / (rh, v)] s = r - alpha v at the stencil points; t = A D^-1 s;
+// partials (t,s), (t,t)
 template <int NV, class T, class Op>
-__global__ void k_bi_b(Dims D, Op A, const T* r, const T* v, T* sv, T* tv, const T* dinv, double* sc, RedBuf rb) {
+__global__ void k_bi_b(Dims D, Op A, const T* r, const T* v, T* sv, T* tv, const T* dinv, const double* partIn, int nb,
+                       double* partOut, double* st) {
     bool any = false;
-    for (int s = 0; s < NV; ++s) any |= sc[8 * s + B_DONE] == 0.;
+    for (int s = 0; s < NV; ++s) any |= st[B_STRIDE * s + B_DONE] == 0.;
     if (!any) return;
-    double acc[2 * NV];
+    double t[NV];
+    readPartials<KBS, NV>(partIn, nb, t);
     double al[NV];
     bool live[NV];
-    for (int s = 0; s < NV; ++s) { acc[2 * s] = acc[2 * s + 1] = 0.; al[s] = sc[8 * s + B_ALPHA]; live[s] = sc[8 * s + B_DONE] == 0.; }
-    const long P = D.P;
+    for (int s = 0; s < NV; ++s) {
+        const double* c = st + B_STRIDE * s;
+        live[s] = c[B_DONE] == 0. && t[s] != 0.;
+        al[s] = live[s] ? c[B_RHOCUR] / t[s] : 0.;
+        if (BLOCK0) {
+            double* w = st + B_STRIDE * s;
+            w[B_ALPHA] = al[s];
+            if (c[B_DONE] == 0. && t[s] == 0.) w[B_BRK] = 1.;
+        }
+    }
+    double acc[2 * NV];
+    for (int s = 0; s < 2 * NV; ++s) acc[s] = 0.;
+    const int P = D.P;
     FOR_CELLS(D, q, p) {
         for (int s = 0; s < NV; ++s) {
             if (!live[s]) continue;
-            const long o = s * P;
-            auto sf = [&](long c) { return (double)r[o + c] - al[s] * (double)v[o + c]; };
-            auto sh = [&](long c) { return sf(c) * (double)dinv[c]; };
-            double ss = sf(p);
-            double tt = A.apply(sh, p);
+            const int o = s * P;
+            auto sf = [&](int c) { return (double)r[o + c] - al[s] * (double)v[o + c]; };
+            auto sh = [&](int c) { return sf(c) * (double)dinv[c]; };
+            const double ss = sf(p);
+            const double tt = A.apply(sh, p);
             sv[o + p] = (T)ss; tv[o + p] = (T)tt;
             acc[2 * s] += tt * ss;
             acc[2 * s + 1] += tt * tt;
         }
     }
-    gridReduce<KBS, 2 * NV>(acc, rb, [&](const double* t) {
-        for (int s = 0; s < NV; ++s) {
-            double* c = sc + 8 * s;
-            if (c[B_DONE] != 0.) continue;
-            c[B_OMEGA] = t[2 * s + 1] > 0. ? t[2 * s] / t[2 * s + 1] : 0.;
-        }
-    });
+    writePartials<KBS, 2 * NV>(acc, partOut);
 }
 
-// KC: x += alpha D^-1 p + omega D^-1 s; r = s - omega t; partials (rh, r), (r, r)
+// KC: [omega = (t,s)/(t,t)] x += alpha D^-1 p + omega D^-1 s; r = s - omega t;
+// partials (rh, r), (r, r)
 template <int NV, class T, class Op>
 __global__ void k_bi_c(Dims D, Op A, T* x, T* r, const T* rh, const T* pv, const T* sv, const T* tv, const T* dinv,
-                       double* sc, RedBuf rb, LoopCtl lc) {
+                       const double* partIn, int nb, double* partOut, double* st) {
     bool any = false;
-    for (int s = 0; s < NV; ++s) any |= sc[8 * s + B_DONE] == 0.;
+    for (int s = 0; s < NV; ++s) any |= st[B_STRIDE * s + B_DONE] == 0.;
     if (!any) return;
-    double acc[2 * NV];
+    double t[2 * NV];
+    readPartials<KBS, 2 * NV>(partIn, nb, t);
     double al[NV], om[NV];
     bool live[NV];
     for (int s = 0; s < NV; ++s) {
-        acc[2 * s] = acc[2 * s + 1] = 0.; al[s] = sc[8 * s + B_ALPHA]; om[s] = sc[8 * s + B_OMEGA]; live[s] = sc[8 * s + B_DONE] == 0.;
+        const double* c = st + B_STRIDE * s;
+        live[s] = c[B_DONE] == 0. && c[B_BRK] == 0.;
+        al[s] = c[B_ALPHA];
+        om[s] = t[2 * s + 1] > 0. ? t[2 * s] / t[2 * s + 1] : 0.;
+        if (BLOCK0 && live[s]) {
+            double* w = st + B_STRIDE * s;
+            w[B_OMEGA] = om[s]; w[B_RHOPREV] = c[B_RHOCUR]; w[B_IT] = c[B_IT] + 1.;
+            if (om[s] == 0.) w[B_BRK] = 1.;
+        }
     }
-    const long P = D.P;
+    double acc[2 * NV];
+    for (int s = 0; s < 2 * NV; ++s) acc[s] = 0.;
+    const int P = D.P;
     FOR_CELLS(D, q, p) {
         const double di = (double)dinv[p];
         for (int s = 0; s < NV; ++s) {
-            if (!live[s]) continue;
-            const long o = s * P + p;
-            double xn = (double)x[o] + al[s] * ((double)pv[o] * di) + om[s] * ((double)sv[o] * di);
-            double rn = (double)sv[o] - om[s] * (double)tv[o];
-            x[o] = (T)xn; r[o] = (T)rn;
+            const int o = s * P + p;
+            double rn;
+            if (live[s]) {
+                const double xn = (double)x[o] + al[s] * ((double)pv[o] * di) + om[s] * ((double)sv[o] * di);
+                rn = (double)sv[o] - om[s] * (double)tv[o];
+                x[o] = (T)xn; r[o] = (T)rn;
+            } else {
+                rn = (double)r[o];
+            }
             acc[2 * s] += (double)rh[o] * rn;
             acc[2 * s + 1] += rn * rn;
         }
     }
-    gridReduce<KBS, 2 * NV>(acc, rb, [&](const double* t) {
-        for (int s = 0; s < NV; ++s) {
-            double* c = sc + 8 * s;
-            if (c[B_DONE] != 0.) continue;
-            double rhoN = t[2 * s];
-            c[B_RR] = t[2 * s + 1];
-            c[B_BETA] = (c[B_RHO] != 0. && c[B_OMEGA] != 0.) ? (rhoN / c[B_RHO]) * (c[B_ALPHA] / c[B_OMEGA]) : 0.;
-            c[B_RHO] = rhoN;
-            c[B_IT] += 1.;
-            if (t[2 * s + 1] <= lc.tol2 * c[B_BB] || c[B_IT] >= lc.maxit) c[B_DONE] = 1.;
-            else if (c[B_OMEGA] == 0. || rhoN == 0.) c[B_DONE] = 2.;
-        }
-    });
+    writePartials<KBS, 2 * NV>(acc, partOut);
 }
 
 // ================================================================== residual and utilities
-// r = b - mean - A x in FP64 over NV systems; partials (r,r) per system, (b,b) per system
+// r = (b - shift) - A x in FP64 over NV systems; partials (r,r), (b,b) per system
 template <int NV, class Op>
-__global__ void k_resid(Dims D, Op A, const double* b, double mean, const double* x, double* r, double* sc, RedBuf rb) {
-    double acc[2 * NV];
-    for (int s = 0; s < 2 * NV; ++s) acc[s] = 0.;
-    const long P = D.P;
+__global__ void k_resid(Dims D, Op A, const double* b, double shift, const double* x, double* r, double* part) {
+    double acc[3 * NV];
+    for (int s = 0; s < 3 * NV; ++s) acc[s] = 0.;
+    const int P = D.P;
     FOR_CELLS(D, q, p) {
         const bool live = !A.st || A.st[p] != ST_INACTIVE;
         for (int s = 0; s < NV; ++s) {
-            const long o = s * P;
-            auto xf = [&](long c) { return x[o + c]; };
-            double bv = live ? b[o + p] - mean : 0.;
-            double rv = live ? bv - A.apply(xf, p) : 0.;
+            const int o = s * P;
+            auto xf = [&](int c) { return x[o + c]; };
+            const double bv = live ? b[o + p] - shift : 0.;
+            const double rv = live ? bv - A.apply(xf, p) : 0.;
             if (r) r[o + p] = rv;
-            acc[2 * s] += rv * rv;
-            acc[2 * s + 1] += bv * bv;
+            acc[3 * s] += rv * rv;
+            acc[3 * s + 1] += bv * bv;
+            acc[3 * s + 2] += rv;
         }
     }
-    gridReduce<KBS, 2 * NV>(acc, rb, [&](const double* t) {
-        for (int s = 0; s < 2 * NV; ++s) sc[s] = t[s];
-    });
+    writePartials<KBS, 3 * NV>(acc, part);
 }
 
-// mean of b over non-inactive cells (Neumann null-space projection)
-__global__ void k_mean(Dims D, const int8_t* st, const double* b, double* sc, RedBuf rb) {
+// sum and count of a field over non-inactive cells
+__global__ void k_sum(Dims D, const int8_t* st, const double* b, double* part) {
     double acc[2] = {0., 0.};
     FOR_CELLS(D, q, p) {
         if (st[p] == ST_INACTIVE) continue;
         acc[0] += b[p];
         acc[1] += 1.;
     }
-    gridReduce<KBS, 2>(acc, rb, [&](const double* t) { sc[0] = t[1] > 0. ? t[0] / t[1] : 0.; });
+    writePartials<KBS, 2>(acc, part);
 }
 
-// x += s * e (FP64 accumulate of the FP32 correction)
+template <int K>
+__global__ void k_finalize(const double* part, int nb, double* out) {
+    double t[K];
+    readPartials<KBS, K>(part, nb, t);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < K; ++q) out[q] = t[q];
+}
+
+__global__ void k_add_const(Dims D, const int8_t* st, double* x, double c) {
+    FOR_CELLS(D, q, p) { if (st[p] != ST_INACTIVE) x[p] += c; }
+}
 __global__ void k_axpy_mixed(Dims D, double* x, const float* e, double s) {
     FOR_CELLS(D, q, p) { x[p] += s * (double)e[p]; }
 }
-// r32 = r64 * s
-__global__ void k_to_f32(Dims D, const double* r, float* o, double s) {
-    FOR_CELLS(D, q, p) { o[p] = (float)(r[p] * s); }
-}
-__global__ void k_cast_f32(const double* a, float* o, long n) {
-    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_cast_f32(const double* a, float* o, int n) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q < n) o[q] = (float)a[q];
 }
-__global__ void k_zero_d(double* a, long n) {
-    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_zero_d(double* a, int n) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q < n) a[q] = 0.;
+}
+__global__ void k_add(double* x, const double* d, int n) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) x[q] += d[q];
 }
 
 // ================================================================== host drivers
-static int gridFor(const Dims& D) { return blocksFor(D.N, KBS, KGRID_CAP); }
+static int gridFor(const Dims& D) { return blocksFor(D.N, KBS, KGRID); }
 
-static void readBack(Ctx& c, int slot, int n) {
-    HIPCHECK(hipMemcpyAsync(c.hscal + slot, c.scal + slot, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+static void readBack(Ctx& c, const double* dev, int slot, int n) {
+    HIPCHECK(hipMemcpyAsync(c.hscal + slot, dev, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     HIPCHECK(hipStreamSynchronize(c.stream));
 }
 
-static double computeMean(Ctx& c, const double* b) {
-    hipLaunchKernelGGL(k_mean, dim3(gridFor(c.D)), dim3(KBS), 0, c.stream, c.D, c.st, b, c.scal + SC_MISC + 4, c.red);
-    readBack(c, SC_MISC + 4, 1);
-    return c.hscal[SC_MISC + 4];
+void add_vec(Ctx& c, double* x, const double* d, int n) {
+    hipLaunchKernelGGL(k_add, dim3(blocksFor(n, KBS)), dim3(KBS), 0, c.stream, x, d, n);
+}
+
+// mean over non-inactive cells
+static double meanOf(Ctx& c, const double* v) {
+    const int g = gridFor(c.D);
+    hipLaunchKernelGGL(k_sum, dim3(g), dim3(KBS), 0, c.stream, c.D, c.st, v, c.red.part);
+    hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(KBS), 0, c.stream, c.red.part, g, c.scal + SC_MISC + 4);
+    readBack(c, c.scal + SC_MISC + 4, SC_MISC + 4, 2);
+    return c.hscal[SC_MISC + 5] > 0. ? c.hscal[SC_MISC + 4] / c.hscal[SC_MISC + 5] : 0.;
 }
 
 static OvView ovM(Ctx& c) { return OvView{c.hasIB ? c.ov.rowOf : nullptr, c.ov.cols, c.ov.valM}; }
 static OvView ovP(Ctx& c) { return OvView{c.hasIB ? c.ov.rowOf : nullptr, c.ov.cols, c.ov.valP}; }
 static OvView ovP32(Ctx& c) { return OvView{c.hasIB ? c.ov.rowOf : nullptr, c.ov.cols, c.ov.valP32}; }
+static Band7 bandM(Ctx& c) { return Band7{c.Am, c.D.P, c.D.sJ, c.D.sK, ovM(c), c.st}; }
+template <class T> static Sym4<T> symP(Ctx& c);
+template <> Sym4<double> symP<double>(Ctx& c) { return Sym4<double>{c.Ap, c.D.P, c.D.sJ, c.D.sK, ovP(c), c.st}; }
+template <> Sym4<float> symP<float>(Ctx& c) { return Sym4<float>{c.Ap32, c.D.P, c.D.sJ, c.D.sK, ovP32(c), c.st}; }
 
-// Generic BiCGStab over NV systems.  Returns iterations (max over systems).
+// true FP64 residual: r = (b - shift) - A x; returns ||r||, ||b - shift|| per system.
+// For the singular (pure Neumann) pressure system the mean of r (rounding of the zero
+// row sums) is not part of the residual of the problem and is left out of ||r||.
+template <int NV, class Op>
+static void trueResid(Ctx& c, Op A, const double* b, double shift, const double* x, double* r, double* rr, double* bb,
+                      bool project = false) {
+    const int g = gridFor(c.D);
+    hipLaunchKernelGGL((k_resid<NV, Op>), dim3(g), dim3(KBS), 0, c.stream, c.D, A, b, shift, x, r, c.red.part);
+    hipLaunchKernelGGL(k_finalize<3 * NV>, dim3(1), dim3(KBS), 0, c.stream, c.red.part, g, c.scal + SC_MISC + 8);
+    readBack(c, c.scal + SC_MISC + 8, SC_MISC + 8, 3 * NV);
+    for (int s = 0; s < NV; ++s) {
+        const double r2 = c.hscal[SC_MISC + 8 + 3 * s], b2 = c.hscal[SC_MISC + 9 + 3 * s], rs = c.hscal[SC_MISC + 10 + 3 * s];
+        rr[s] = std::sqrt(project ? std::max(r2 - rs * rs / (double)c.D.N, 0.) : r2);
+        bb[s] = std::sqrt(b2);
+    }
+}
+
+// BiCGStab over NV systems on the work block w (10 * NV * P entries of T).
+// Returns the iteration count (max over systems); on exit done[] holds 1 (converged
+// on the recurrence), 2 (breakdown) or 0 (iteration cap).
 template <int NV, class T, class Op>
-static int runBicgstab(Ctx& c, Op A, const T* b, double mean, T* x, T* work, int slot, double rtol, int maxit,
-                       int chunk) {
+static int runBicgstab(Ctx& c, Op A, const double* b, double shift, T* x, T* w, double rtol, int maxit, int chunk,
+                       int* doneOut) {
     const Dims& D = c.D;
-    const long P = D.P, NVP = (long)NV * P;
-    T* r = work; T* rh = r + NVP; T* p0 = rh + NVP; T* p1 = p0 + NVP; T* v0 = p1 + NVP; T* v1 = v0 + NVP;
+    const int P = D.P, NVP = NV * P;
+    T* r = w; T* rh = r + NVP; T* p0 = rh + NVP; T* p1 = p0 + NVP; T* v0 = p1 + NVP; T* v1 = v0 + NVP;
     T* s = v1 + NVP; T* t = s + NVP; T* dinv = t + NVP;
-    double* sc = c.scal + slot;
+    double* st = c.scal + SC_BI;
+    double* partC = c.red.part;
+    double* partA = c.red.part + MAX_RED_BLOCKS * RED_STRIDE;
+    double* partB = c.red.part + 2 * MAX_RED_BLOCKS * RED_STRIDE;
     const int g = gridFor(D);
-    LoopCtl lc{rtol * rtol, maxit};
-    hipLaunchKernelGGL((k_bi_init<NV, T, Op>), dim3(g), dim3(KBS), 0, c.stream, D, A, b, mean, r, rh, p0, p1, v0, v1, x, dinv, sc, c.red);
-    int it = 0;
+    LoopCtl lc{rtol * rtol, maxit, 0.};
+    hipLaunchKernelGGL((k_bi_init<NV, T, Op>), dim3(g), dim3(KBS), 0, c.stream, D, A, b, shift, r, rh, p0, v0, x, dinv, partC, st);
     T *pc = p0, *pn = p1, *vc = v0, *vn = v1;
+    int it = 0;
     while (true) {
         for (int q = 0; q < chunk; ++q) {
-            hipLaunchKernelGGL((k_bi_a<NV, T, Op>), dim3(g), dim3(KBS), 0, c.stream, D, A, r, rh, pc, pn, vc, vn, dinv, sc, c.red);
-            hipLaunchKernelGGL((k_bi_b<NV, T, Op>), dim3(g), dim3(KBS), 0, c.stream, D, A, r, vn, s, t, dinv, sc, c.red);
-            hipLaunchKernelGGL((k_bi_c<NV, T, Op>), dim3(g), dim3(KBS), 0, c.stream, D, A, x, r, rh, pn, s, t, dinv, sc, c.red, lc);
+            hipLaunchKernelGGL((k_bi_a<NV, T, Op>), dim3(g), dim3(KBS), 0, c.stream, D, A, r, rh, pc, pn, vc, vn, dinv, partC, g, partA, st, lc);
+            hipLaunchKernelGGL((k_bi_b<NV, T, Op>), dim3(g), dim3(KBS), 0, c.stream, D, A, r, vn, s, t, dinv, partA, g, partB, st);
+            hipLaunchKernelGGL((k_bi_c<NV, T, Op>), dim3(g), dim3(KBS), 0, c.stream, D, A, x, r, rh, pn, s, t, dinv, partB, g, partC, st);
             std::swap(pc, pn); std::swap(vc, vn);
         }
-        readBack(c, slot, 8 * NV);
+        readBack(c, st, SC_BI, B_STRIDE * NV);
         bool all = true;
         it = 0;
         for (int q = 0; q < NV; ++q) {
-            all &= c.hscal[slot + 8 * q + B_DONE] != 0.;
-            it = std::max(it, (int)c.hscal[slot + 8 * q + B_IT]);
+            all &= c.hscal[SC_BI + B_STRIDE * q + B_DONE] != 0.;
+            it = std::max(it, (int)c.hscal[SC_BI + B_STRIDE * q + B_IT]);
         }
         if (all || it >= maxit) break;
+        if (chunk < 16) chunk *= 2;
     }
+    for (int q = 0; q < NV; ++q) doneOut[q] = (int)c.hscal[SC_BI + B_STRIDE * q + B_DONE];
     return it;
 }
 
-template <class T>
-static int runCg(Ctx& c, Sym4<T> A, const T* b, double mean, T* x, T* work, int slot, double rtol, int maxit, int chunk) {
-    const Dims& D = c.D;
-    const long P = D.P;
-    T* r = work; T* z = r + P; T* p0 = z + P; T* p1 = p0 + P; T* qv = p1 + P;
-    double* sc = c.scal + slot;
-    const int g = gridFor(D);
-    LoopCtl lc{rtol * rtol, maxit};
-    hipLaunchKernelGGL((k_cg_init<T>), dim3(g), dim3(KBS), 0, c.stream, D, A, b, mean, r, z, p0, p1, x, sc, c.red);
-    T *pc = p0, *pn = p1;
-    int it = 0;
-    while (true) {
-        for (int q = 0; q < chunk; ++q) {
-            hipLaunchKernelGGL((k_cg_k1<T>), dim3(g), dim3(KBS), 0, c.stream, D, A, z, pc, pn, qv, sc, c.red);
-            hipLaunchKernelGGL((k_cg_k2<T>), dim3(g), dim3(KBS), 0, c.stream, D, A, x, r, z, pn, qv, sc, c.red, lc);
-            std::swap(pc, pn);
-        }
-        readBack(c, slot, 8);
-        it = (int)c.hscal[slot + C_IT];
-        if (c.hscal[slot + C_DONE] != 0. || it >= maxit) break;
-    }
-    return it;
-}
-
-// true FP64 residual of the momentum systems; returns max relres over components
-static double momResid(Ctx& c, double* rbuf) {
-    Band7<double> A{c.Am, c.D.P, c.D.sJ, c.D.sK, ovM(c), c.st};
-    hipLaunchKernelGGL((k_resid<3, Band7<double>>), dim3(gridFor(c.D)), dim3(KBS), 0, c.stream, c.D, A, c.bm, 0., c.xm, rbuf,
-                       c.scal + SC_MISC + 8, c.red);
-    readBack(c, SC_MISC + 8, 6);
-    double mx = 0.;
-    for (int s = 0; s < 3; ++s) {
-        double rr = c.hscal[SC_MISC + 8 + 2 * s], bb = c.hscal[SC_MISC + 8 + 2 * s + 1];
-        double rel = bb > 0. ? std::sqrt(rr / bb) : 0.;
-        mx = std::max(mx, rel);
-    }
-    return mx;
-}
-
-__global__ void k_add(double* x, const double* d, long n) {
-    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < n) x[q] += d[q];
-}
-void add_vec(Ctx& c, double* x, const double* d, long n) {
-    hipLaunchKernelGGL(k_add, dim3(blocksFor(n, KBS)), dim3(KBS), 0, c.stream, x, d, n);
-}
-
+// ---------------------------------------------------------------- momentum
 int momentum_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep) {
     HIPCHECK(hipEventRecord(c.ev0, c.stream));
-    Band7<double> A{c.Am, c.D.P, c.D.sJ, c.D.sK, ovM(c), c.st};
-    const long P = c.D.P;
-    int iters = runBicgstab<3, double>(c, A, c.bm, 0., c.xm, c.wm, SC_M, o.rtol, o.maxit, 4);
-    double rel = momResid(c, c.wm);
-    // restart from the true residual if the recurrence drifted (or broke down)
+    const int P = c.D.P;
+    Band7 A = bandM(c);
+    int done[3];
+    double* w = c.wm;              // 10 * 3 * P
+    double* dx = c.wm + 30 * P;    // 3 * P
+    double* rt = c.wm + 33 * P;    // 3 * P
+    int iters = runBicgstab<3, double>(c, A, c.bm, 0., c.xm, w, o.rtol, o.maxit, 4, done);
+    double rr[3], bb[3];
+    trueResid<3>(c, A, c.bm, 0., c.xm, rt, rr, bb);
+    auto worst = [&]() { double m = 0.; for (int s = 0; s < 3; ++s) m = std::max(m, bb[s] > 0. ? rr[s] / bb[s] : 0.); return m; };
+    double rel = worst();
     int restarts = 0;
-    while (rel > o.rtol && restarts < 3 && iters < o.maxit) {
-        double* dx = c.wm + 10 * 3 * P;
-        double* rr = c.wm + 11 * 3 * P;
-        HIPCHECK(hipMemcpyAsync(rr, c.wm, 3 * P * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    while (rel > o.rtol && restarts < 10 && iters < o.maxit) {
+        // continue from the true residual: solve A dx = r_true, x += dx
         const double sub = std::min(0.5, 0.5 * o.rtol / rel);
-        iters += runBicgstab<3, double>(c, A, rr, 0., dx, c.wm, SC_M, sub, o.maxit - iters, 4);
+        iters += runBicgstab<3, double>(c, A, rt, 0., dx, w, sub, o.maxit - iters, 4, done);
         add_vec(c, c.xm, dx, 3 * P);
-        rel = momResid(c, c.wm);
+        trueResid<3>(c, A, c.bm, 0., c.xm, rt, rr, bb);
+        rel = worst();
         ++restarts;
     }
     HIPCHECK(hipEventRecord(c.ev1, c.stream));
@@ -470,92 +533,147 @@ int momentum_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep) {
     return rel <= o.rtol ? CAFFE_OK : CAFFE_ERR_NOT_CONVERGED;
 }
 
-
 void pcorr_make_fp32(Ctx& c) {
-    long n = 4 * c.D.P;
+    const int n = 4 * c.D.P;
     hipLaunchKernelGGL(k_cast_f32, dim3(blocksFor(n, KBS)), dim3(KBS), 0, c.stream, c.Ap, c.Ap32, n);
     if (c.hasIB && c.ov.n > 0) {
-        long m = 9L * c.ov.n;
+        const int m = 9 * c.ov.n;
         hipLaunchKernelGGL(k_cast_f32, dim3(blocksFor(m, KBS)), dim3(KBS), 0, c.stream, c.ov.valP, c.ov.valP32, m);
     }
 }
 
-// true FP64 residual of the pressure system: r = (b - mean) - A x; returns relres, stores rr, bb
-static double pcResid(Ctx& c, double mean, double* r, double* rrOut, double* bbOut) {
-    Sym4<double> A{c.Ap, c.D.P, c.D.sJ, c.D.sK, ovP(c), c.st};
-    hipLaunchKernelGGL((k_resid<1, Sym4<double>>), dim3(gridFor(c.D)), dim3(KBS), 0, c.stream, c.D, A, c.bp, mean, c.xp, r,
-                       c.scal + SC_IR, c.red);
-    readBack(c, SC_IR, 2);
-    double rr = c.hscal[SC_IR], bb = c.hscal[SC_IR + 1];
-    if (rrOut) *rrOut = rr;
-    if (bbOut) *bbOut = bb;
-    return bb > 0. ? std::sqrt(rr / bb) : 0.;
+// ---------------------------------------------------------------- pressure correction
+// CG on the symmetric p' system, continuing from the current x in c.xp.  T = double:
+// the iterate is c.xp itself.  T = float: the FP32 iterate e holds a correction in
+// units of the scale s (r32 = s r64); at every reliable update x64 += e / s, the true
+// residual r64 = b - A x64 is recomputed in FP64, r32 restarts from it with a new
+// scale and the search direction is rescaled.  Updates happen when the recurrence
+// residual has dropped by `delta` since the last one.
+template <class T>
+static int cgSolve(Ctx& c, double shift, double rtol, int maxit, double delta, int* updates, double* relOut) {
+    const Dims& D = c.D;
+    const int P = D.P;
+    const int g = gridFor(D);
+    constexpr bool mixed = std::is_same<T, float>::value;
+    Sym4<T> A = symP<T>(c);
+    Sym4<double> A64 = symP<double>(c);
+    T* w = mixed ? (T*)c.wp32 : (T*)c.wp;
+    T* r = w; T* z = r + P; T* p0 = z + P; T* p1 = p0 + P; T* qv = p1 + P; T* e = qv + P;
+    double* r64 = c.wp + 6 * P;
+    double* st = c.scal + SC_CG;
+    double* partB = c.red.part;                                   // (r,z), (r,r)
+    double* partA = c.red.part + MAX_RED_BLOCKS * RED_STRIDE;     // (p,q)
+    double rr, bb;
+    trueResid<1>(c, A64, c.bp, shift, c.xp, r64, &rr, &bb, c.singularP);
+    double rel = bb > 0. ? rr / bb : 0.;
+    *updates = 0;
+    if (rel <= rtol || bb == 0.) { *relOut = rel; return 0; }
+    double scale = mixed ? 1.0 / rr : 1.0;
+    T* x = mixed ? e : (T*)c.xp;
+    // the double iterate continues from xp (x is not zeroed); the FP32 correction starts at 0
+    hipLaunchKernelGGL((k_cg_init<T, double>), dim3(g), dim3(KBS), 0, c.stream, D, A, r64, 0., scale, r, z, p0, p1, x, 1,
+                       mixed ? 1 : 0, partB, st, 1.0);
+    const double tolIn = mixed ? delta : std::min(0.5, rtol / rel);
+    LoopCtl lc{tolIn * tolIn, maxit, c.singularP ? (double)D.N : 0.};
+    int it = 0;
+    int chunk = 8;
+    while (true) {
+        for (int q = 0; q < chunk; ++q) {
+            hipLaunchKernelGGL((k_cg_k1<T>), dim3(g), dim3(KBS), 0, c.stream, D, A, z, p0, p1, qv, partB, g, partA, st, lc);
+            hipLaunchKernelGGL((k_cg_k2<T>), dim3(g), dim3(KBS), 0, c.stream, D, A, x, r, z, p0, p1, qv, partA, g, partB, st);
+        }
+        readBack(c, st, SC_CG, 8);
+        it = (int)c.hscal[SC_CG + C_IT];
+        const bool recDone = c.hscal[SC_CG + C_DONE] != 0.;
+        if (chunk < 32) chunk *= 2;
+        if (!recDone && it < maxit) continue;
+        // reliable update / convergence check on the true FP64 residual
+        if (mixed) hipLaunchKernelGGL(k_axpy_mixed, dim3(g), dim3(KBS), 0, c.stream, D, c.xp, (const float*)e, 1.0 / scale);
+        trueResid<1>(c, A64, c.bp, shift, c.xp, r64, &rr, &bb, c.singularP);
+        rel = rr / bb;
+        ++*updates;
+        if (rel <= rtol || it >= maxit || *updates > 200) break;
+        // restart the recurrence from the true residual, keeping the search direction
+        const double ns = mixed ? 1.0 / rr : 1.0;
+        const double ratio = mixed ? scale / ns : 1.0;
+        hipLaunchKernelGGL((k_cg_init<T, double>), dim3(g), dim3(KBS), 0, c.stream, D, A, r64, 0., ns, r, z, p0, p1, x, 0,
+                           mixed ? 1 : 0, partB, st, ratio);
+        scale = ns;
+        const double tolNext = mixed ? delta : std::min(0.5, rtol / rel);
+        lc.tol2 = tolNext * tolNext;
+    }
+    *relOut = rel;
+    return it;
+}
+
+// BiCGStab on the nonsymmetric (ghost-row) p' system, restarted from the true FP64
+// residual; T = float runs each restart in FP32 to relative tolerance `inner`.
+template <class T>
+static int bicgPressure(Ctx& c, double shift, double rtol, int maxit, double inner, int* refs, double* relOut) {
+    const Dims& D = c.D;
+    const int P = D.P;
+    Sym4<T> A = symP<T>(c);
+    Sym4<double> A64 = symP<double>(c);
+    T* w = (T*)c.wm;
+    T* e = (T*)(c.wm + 12 * P);
+    double* r64 = c.wp + 6 * P;
+    double rr, bb;
+    trueResid<1>(c, A64, c.bp, shift, c.xp, r64, &rr, &bb);
+    double rel = bb > 0. ? rr / bb : 0.;
+    int it = 0, done[1];
+    *refs = 0;
+    while (rel > rtol && it < maxit && *refs < 60) {
+        const double tol = std::is_same<T, float>::value ? inner : std::min(0.5, 0.5 * rtol / rel);
+        it += runBicgstab<1, T>(c, A, r64, 0., e, w, tol, maxit - it, 8, done);
+        if (std::is_same<T, float>::value) hipLaunchKernelGGL(k_axpy_mixed, dim3(gridFor(D)), dim3(KBS), 0, c.stream, D, c.xp, (const float*)e, 1.0);
+        else add_vec(c, c.xp, (const double*)e, P);
+        const double prev = rel;
+        trueResid<1>(c, A64, c.bp, shift, c.xp, r64, &rr, &bb);
+        rel = rr / bb;
+        ++*refs;
+        if (!(rel < prev)) break;
+    }
+    *relOut = rel;
+    return it;
 }
 
 int pcorr_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep) {
     HIPCHECK(hipEventRecord(c.ev0, c.stream));
     const Dims& D = c.D;
-    const long P = D.P;
-    const double mean = c.singularP ? computeMean(c, c.bp) : 0.;
+    const int P = D.P;
+    // Neumann null space: the right-hand side loses its (rounding) mean, the solution is
+    // returned with zero mean over the cells that carry a row
+    const double shift = c.singularP ? meanOf(c, c.bp) : 0.;
     int iters = 0, refs = 0, fell = 0;
     double rel = 1.;
     const bool sym = !c.hasIB;
-    double* r64 = c.wp + 5 * P;
-    auto plain = [&](double tol, int cap) {
-        Sym4<double> A{c.Ap, P, D.sJ, D.sK, ovP(c), c.st};
-        if (sym) iters += runCg<double>(c, A, c.bp, mean, c.xp, c.wp, SC_P, tol, cap, 16);
-        else iters += runBicgstab<1, double>(c, A, c.bp, mean, c.xp, c.wm, SC_P, tol, cap, 8);
-        rel = pcResid(c, mean, r64, nullptr, nullptr);
-    };
+    hipLaunchKernelGGL(k_zero_d, dim3(blocksFor(P, KBS)), dim3(KBS), 0, c.stream, c.xp, P);
     if (o.mode == CAFFE_SOLVE_PLAIN_FP64) {
-        plain(o.rtol, o.maxit);
-        // restart from the true residual when the recurrence drifted
-        int restarts = 0;
-        while (rel > o.rtol && restarts < 3 && iters < o.maxit) {
-            double* dx = c.wp + 6 * P;
-            Sym4<double> A{c.Ap, P, D.sJ, D.sK, ovP(c), c.st};
-            double sub = std::min(0.5, 0.5 * o.rtol / rel);
-            if (sym) iters += runCg<double>(c, A, r64, 0., dx, c.wp, SC_P, sub, o.maxit - iters, 16);
-            else iters += runBicgstab<1, double>(c, A, r64, 0., dx, c.wm, SC_P, sub, o.maxit - iters, 8);
-            add_vec(c, c.xp, dx, P);
-            rel = pcResid(c, mean, r64, nullptr, nullptr);
-            ++restarts;
-        }
-        refs = restarts;
+        if (sym) iters += cgSolve<double>(c, shift, o.rtol, o.maxit, 0., &refs, &rel);
+        else iters += bicgPressure<double>(c, shift, o.rtol, o.maxit, 0., &refs, &rel);
     } else {
-        // FP32 inner solves inside FP64 iterative refinement.  x starts at 0 (or at the
-        // learned guess); each step solves A e = r / ||r|| in FP32 and adds ||r|| e.
-        hipLaunchKernelGGL(k_zero_d, dim3(blocksFor(P, KBS)), dim3(KBS), 0, c.stream, c.xp, P);
-        if (o.mode == CAFFE_SOLVE_LEARNED_FCG && c.net) {
-            double rr0, bb0;
-            pcResid(c, mean, r64, &rr0, &bb0);
-            if (learned_apply_guess(c, r64, c.xp, 1.0) != CAFFE_OK) fell = 1;
+        if (o.mode == CAFFE_SOLVE_LEARNED_FCG) {
+            if (!c.net || learned_apply_guess(c, c.bp, c.xp, shift) != CAFFE_OK) {
+                fell = 1;
+                hipLaunchKernelGGL(k_zero_d, dim3(blocksFor(P, KBS)), dim3(KBS), 0, c.stream, c.xp, P);
+            }
         }
         pcorr_make_fp32(c);
-        Sym4<float> A32{c.Ap32, P, D.sJ, D.sK, ovP32(c), c.st};
-        float* e32 = c.wp32 + 5 * P;
-        const double inner = o.inner_rtol > 0. ? o.inner_rtol : 1e-4;
-        double rr, bb;
-        rel = pcResid(c, mean, r64, &rr, &bb);
-        while (rel > o.rtol && iters < o.maxit && refs < 50) {
-            const double nr = std::sqrt(rr);
-            hipLaunchKernelGGL(k_to_f32, dim3(gridFor(D)), dim3(KBS), 0, c.stream, D, r64, c.wp32 + 6 * P, 1.0 / nr);
-            const float* rhs = c.wp32 + 6 * P;
-            if (sym) iters += runCg<float>(c, A32, rhs, 0., e32, c.wp32, SC_P, inner, o.maxit - iters, 16);
-            else iters += runBicgstab<1, float>(c, A32, rhs, 0., e32, (float*)c.wm, SC_P, inner, o.maxit - iters, 8);
-            hipLaunchKernelGGL(k_axpy_mixed, dim3(gridFor(D)), dim3(KBS), 0, c.stream, D, c.xp, e32, nr);
-            ++refs;
-            double relNew = pcResid(c, mean, r64, &rr, &bb);
-            if (!(relNew < rel)) {  // stagnation: fall back to the plain FP64 solve
-                rel = relNew;
-                break;
-            }
-            rel = relNew;
-        }
-        if (rel > o.rtol) {
+        const double delta = o.inner_rtol > 0. ? o.inner_rtol : 1e-2;
+        if (sym) iters += cgSolve<float>(c, shift, o.rtol, o.maxit, delta, &refs, &rel);
+        else iters += bicgPressure<float>(c, shift, o.rtol, o.maxit, delta, &refs, &rel);
+        if (rel > o.rtol) {  // the plain FP64 solve stays as fallback and judge, from x = 0
             fell = 1;
-            plain(o.rtol, o.maxit);
+            int r2 = 0;
+            hipLaunchKernelGGL(k_zero_d, dim3(blocksFor(P, KBS)), dim3(KBS), 0, c.stream, c.xp, P);
+            if (sym) iters += cgSolve<double>(c, shift, o.rtol, o.maxit, 0., &r2, &rel);
+            else iters += bicgPressure<double>(c, shift, o.rtol, o.maxit, 0., &r2, &rel);
+            refs += r2;
         }
+    }
+    if (c.singularP) {
+        const double m = meanOf(c, c.xp);
+        hipLaunchKernelGGL(k_add_const, dim3(gridFor(D)), dim3(KBS), 0, c.stream, D, c.st, c.xp, -m);
     }
     HIPCHECK(hipEventRecord(c.ev1, c.stream));
     HIPCHECK(hipEventSynchronize(c.ev1));
@@ -566,6 +684,36 @@ int pcorr_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep) {
         rep->fell_back = fell; rep->ms = ms;
     }
     return rel <= o.rtol ? CAFFE_OK : CAFFE_ERR_NOT_CONVERGED;
+}
+
+// Timing of the dominant kernel: K1 of the pressure-correction CG (stencil SpMV fused
+// with the search-direction update and the (p, Ap) partial sum) on the current p'
+// system, launched `reps` times between two events on the context's stream.
+template <class T>
+static void timeK1(Ctx& c, Sym4<T> A, T* w, int reps) {
+    const Dims& D = c.D;
+    const int P = D.P;
+    const int g = gridFor(D);
+    double* st = c.scal + SC_CG;
+    double* partB = c.red.part;
+    double* partA = c.red.part + MAX_RED_BLOCKS * RED_STRIDE;
+    LoopCtl lc{0., 1 << 30, 0.};  // never converges: every launch runs
+    T* r = w; T* z = r + P; T* p0 = z + P; T* p1 = p0 + P; T* q = p1 + P; T* x = q + P;
+    hipLaunchKernelGGL((k_cg_init<T, double>), dim3(g), dim3(KBS), 0, c.stream, D, A, c.bp, 0., 1.0, r, z, p0, p1, x, 1, 1, partB, st, 1.0);
+    hipLaunchKernelGGL((k_cg_k1<T>), dim3(g), dim3(KBS), 0, c.stream, D, A, z, p0, p1, q, partB, g, partA, st, lc);
+    HIPCHECK(hipEventRecord(c.ev0, c.stream));
+    for (int i = 0; i < reps; ++i)
+        hipLaunchKernelGGL((k_cg_k1<T>), dim3(g), dim3(KBS), 0, c.stream, D, A, z, p0, p1, q, partB, g, partA, st, lc);
+    HIPCHECK(hipEventRecord(c.ev1, c.stream));
+}
+
+double time_cg_k1(Ctx& c, bool fp32, int reps) {
+    if (fp32) { pcorr_make_fp32(c); timeK1<float>(c, symP<float>(c), c.wp32, reps); }
+    else timeK1<double>(c, symP<double>(c), c.wp, reps);
+    float ms = 0.f;
+    HIPCHECK(hipEventSynchronize(c.ev1));
+    HIPCHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    return (double)ms / reps;
 }
 
 }  // namespace caffe

@@ -151,6 +151,11 @@ class Solver:
         self._check(self.lib.caffe_last_reports(self._h, C.byref(m), C.byref(p)), "caffe_last_reports")
         return m.as_dict(), p.as_dict()
 
+    def time_kernel(self, which: str, reps: int = 50):
+        ms, by = C.c_double(), C.c_double()
+        self._check(self.lib.caffe_time_kernel(self._h, which.encode(), reps, C.byref(ms), C.byref(by)), "caffe_time_kernel")
+        return ms.value, by.value
+
     def sync(self):
         self._check(self.lib.caffe_sync(self._h), "caffe_sync")
 

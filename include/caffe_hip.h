@@ -139,6 +139,11 @@ int caffe_rhs_download(caffe_ctx* ctx, int which, double* host, long n);
 /* Ghost rows: n = number of ghost cells; cells[n], cols[9n] (padded indices, -1 dropped),
  * valsM[9n], valsP[9n]; any pointer may be NULL. */
 int caffe_ghost_rows(caffe_ctx* ctx, int* n, long* cells, long* cols, double* valsM, double* valsP);
+/* Diagnostics: average device time (ms) of `reps` back-to-back launches of one kernel on
+ * the context's stream, and its algorithmic HBM bytes per launch.  which:
+ * "pcorr_cg_k1_fp64" / "pcorr_cg_k1_fp32" (pressure-correction CG stencil kernel on the
+ * current p' system).  No counterpart in the reference (its timing is Time::tic/toc). */
+int caffe_time_kernel(caffe_ctx* ctx, const char* which, int reps, double* ms, double* bytes);
 int caffe_sync(caffe_ctx* ctx);
 long caffe_padded_size(caffe_ctx* ctx);
 

@@ -617,7 +617,7 @@ int bicgstabTight(const SpMat& A, const std::vector<double>& bIn, std::vector<do
     std::vector<double> r(n), rh(n), p(n), v(n), s(n), t(n), ph(n), sh(n);
     int total = 0;
     double rr = 1.;
-    for (int restart = 0; restart < 20 && total < o.maxit; ++restart) {
+    for (int restart = 0; restart < 50 && total < o.maxit; ++restart) {
         spmv(A, x, r, nt);
         for (int q = 0; q < n; ++q) r[q] = b[q] - r[q];
         rr = std::sqrt(dotv(r, r, nt)) / bn;
@@ -643,6 +643,12 @@ int bicgstabTight(const SpMat& A, const std::vector<double>& bIn, std::vector<do
             rho = rhoN;
             for (int q = 0; q < n; ++q) p[q] = r[q] + beta * (p[q] - omega * v[q]);
         }
+    }
+    if (project && n > 0) {  // gauge of the Neumann problem: zero-mean solution
+        double m = 0.;
+        for (int q = 0; q < n; ++q) m += x[q];
+        m /= n;
+        for (int q = 0; q < n; ++q) x[q] -= m;
     }
     spmv(A, x, r, nt);
     for (int q = 0; q < n; ++q) r[q] = b[q] - r[q];
@@ -1126,14 +1132,19 @@ long oc_set(void* h, const char* name, const double* in) {
     Oracle& o = *O(h);
     std::string n(name);
     size_t P = o.P();
+    // Fields whose face values the hot path reads (d, h, p, p', u) get faces consistent
+    // with the new cells: interior faces interpolated (FvScheme.tpp:12-40), domain faces
+    // equal to the patch (Field.tpp:280-362), as they are at every point of use.
     if (n == "u" || n == "u0" || n == "h" || n == "gradP" || n == "gradPCorr") {
         Field<V3>& F = n == "u" ? o.u : n == "u0" ? o.u0 : n == "h" ? o.h : n == "gradP" ? o.gradP : o.gradPC;
         for (int c = 0; c < 3; ++c) o.importCells(F, in + c * P, [c](V3& v, double x) { v(c) = x; });
+        if (n == "u" || n == "h") { o.interpolate(F); for (int f = 0; f < 6; ++f) F.facesFromPatch(f); }
         return (long)(3 * P);
     }
     if (n == "p" || n == "pCorr" || n == "d" || n == "massFlow") {
         Field<double>& F = n == "p" ? o.p : n == "pCorr" ? o.pC : n == "d" ? o.d : o.mflow;
         o.importCells(F, in, [](double& v, double x) { v = x; });
+        if (n != "massFlow") { o.interpolate(F); for (int f = 0; f < 6; ++f) F.facesFromPatch(f); }
         return (long)P;
     }
     if (n == "mI" || n == "mJ" || n == "mK") {

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from caffe_old_amd import case as cs
-from tests.helpers import (copy_state, interior, oracle_momentum_band, oracle_pcorr_sym, oracle_rhs_padded, rel_inf,
+from tests.helpers import (STATE, copy_state, interior, oracle_momentum_band, oracle_pcorr_sym, oracle_rhs_padded, rel_inf,
                            rel_l2)
 
 pytestmark = pytest.mark.gpu
@@ -81,6 +81,7 @@ def test_assembly_parity(name):
     for _ in range(3):  # build a nontrivial state on the oracle
         o.solve(dt)
     o.save_u0()
+    snap = {n: o.get(n) for n in STATE}
     # momentum assembly (Simple.cpp:117-179)
     copy_state(o, g)
     g.momentum_assemble(dt)
@@ -94,7 +95,10 @@ def test_assembly_parity(name):
         b_o = oracle_rhs_padded(o, c, g.P)
         assert rel_inf(b_g[c], b_o, act) <= 1e-12, f"momentum rhs {c}"
     assert rel_inf(g.get("d"), o.get("d"), act) <= 1e-12
-    # pressure-correction assembly (Simple.cpp:231-269)
+    # pressure-correction assembly (Simple.cpp:231-269), from the same state (the oracle's
+    # stored d faces belong to the state before the momentum assembly rewrote d)
+    for n, v in snap.items():
+        o.set(n, v)
     copy_state(o, g)
     g.pcorr_assemble()
     o.assemble_pcorr()
