@@ -30,21 +30,24 @@ def _cpu_threads():
     return max(1, min(n, 16))
 
 
-def cpu_baseline(n, steps=1):
-    """The oracle (C++ FP64 restatement, BiCGStab + block SSOR at rtol 1e-5 like the
-    reference's PETSc settings) on the same 128^3 case: `steps` PISO steps from rest."""
+def cpu_baseline(n, steps=2, warmup=1):
+    """The oracle (C++ FP64 restatement of the reference path, BiCGStab + block-local SSOR
+    at rtol 1e-5 like the reference's PETSc settings, OpenMP threads standing in for MPI
+    ranks) on the same 128^3 case: `warmup` untimed PISO steps, then `steps` timed ones."""
     from caffe_old_amd import case as cs
     from oracle.pyoracle import KSP_PETSC_LIKE, Oracle
     th = _cpu_threads()
-    c = cs.cavity_piso_128(n, steps=steps)
+    c = cs.cavity_piso_128(n, steps=steps + warmup)
     o = Oracle(c, ksp=KSP_PETSC_LIKE, rtol=1e-5, maxit=10000, n_blocks=th, n_threads=th)
+    for _ in range(warmup):
+        o.solve()
     t0 = time.perf_counter()
     for _ in range(steps):
         o.solve()
     dt = time.perf_counter() - t0
     return {"value": steps / dt, "unit": "steps/s", "cores": th, "kind": "port",
-            "sample": f"{steps} PISO step(s) of the {n}^3 cavity from rest, oracle BiCGStab+SSOR rtol 1e-5, "
-                      f"{th} OpenMP threads, {dt:.1f} s"}
+            "sample": f"steps {warmup + 1}-{warmup + steps} of the {n}^3 cavity PISO run from rest ({warmup} untimed), "
+                      f"oracle BiCGStab+SSOR rtol 1e-5, {th} OpenMP threads, {dt:.1f} s timed"}
 
 
 def main():
@@ -56,6 +59,8 @@ def main():
     ap.add_argument("--mode", choices=["ir", "plain", "learned"], default="ir")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="one warm-up step, then only the dominant kernel and the copy kernel (for counter runs)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: --gpus {a.gpus} is not supported; this build runs the whole domain on one GPU "
@@ -75,6 +80,14 @@ def main():
     def log(*x):
         if not a.quiet:
             print(*x, file=sys.stderr, flush=True)
+
+    if a.kernel_only:
+        s.solve()
+        s.pcorr_assemble()
+        for kn in ("pcorr_cg_k1_fp64", "pcorr_cg_k1_fp32", "copy_fp64"):
+            ms, by = s.time_kernel(kn, reps=20)
+            print(json.dumps({"kernel": kn, "ms": ms, "bytes": by, "GBs": by / ms / 1e6}), flush=True)
+        return
 
     for w in range(a.warmup):
         s.solve()

@@ -538,6 +538,12 @@ int caffe_time_kernel(caffe_ctx* x, const char* which, int reps, double* ms, dou
         READY(c);
         std::string w(which ? which : "");
         if (reps < 1 || !ms) { c.err = "bad timing arguments"; return CAFFE_ERR_ARG; }
+        if (w == "copy_fp64") {  // calibration: copy 8 * P doubles (read + write)
+            const int n = 8 * c.D.P;
+            *ms = time_copy(c, c.wm, c.wm + 12 * c.D.P, n, reps);
+            if (bytes) *bytes = 2.0 * 8.0 * n;
+            return CAFFE_OK;
+        }
         const bool f32 = (w == "pcorr_cg_k1_fp32");
         if (!f32 && w != "pcorr_cg_k1_fp64") { c.err = "unknown kernel " + w; return CAFFE_ERR_ARG; }
         *ms = time_cg_k1(c, f32, reps);

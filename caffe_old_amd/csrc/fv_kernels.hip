@@ -590,6 +590,22 @@ static double maxReduce(Ctx& c, int mode, double dt) {
 double continuity_max(Ctx& c) { return maxReduce(c, 0, 0.); }
 double courant(Ctx& c, double dt) { return maxReduce(c, 1, dt); }
 
+// plain copy of n doubles: the reference kernel that calibrates HBM byte counters
+__global__ void k_copy(const double* __restrict__ a, double* __restrict__ b, int n) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) b[q] = a[q];
+}
+double time_copy(Ctx& c, double* a, double* b, int n, int reps) {
+    hipLaunchKernelGGL(k_copy, dim3(blocksFor(n, BS)), dim3(BS), 0, c.stream, a, b, n);
+    HIPCHECK(hipEventRecord(c.ev0, c.stream));
+    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy, dim3(blocksFor(n, BS)), dim3(BS), 0, c.stream, a, b, n);
+    HIPCHECK(hipEventRecord(c.ev1, c.stream));
+    HIPCHECK(hipEventSynchronize(c.ev1));
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    return (double)ms / reps;
+}
+
 __global__ void k_fill(double* a, double v, long n) {
     long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q < n) a[q] = v;

@@ -101,6 +101,7 @@ double continuity_quirk(Ctx& c);
 double continuity_max(Ctx& c);
 double courant(Ctx& c, double dt);
 void launch_fill(Ctx& c, double* a, double v, long n);
+double time_copy(Ctx& c, double* a, double* b, int n, int reps);
 void launch_set_inactive_fields(Ctx& c);
 
 // ---- mesh (mesh.hip)

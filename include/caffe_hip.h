@@ -142,7 +142,7 @@ int caffe_ghost_rows(caffe_ctx* ctx, int* n, long* cells, long* cols, double* va
 /* Diagnostics: average device time (ms) of `reps` back-to-back launches of one kernel on
  * the context's stream, and its algorithmic HBM bytes per launch.  which:
  * "pcorr_cg_k1_fp64" / "pcorr_cg_k1_fp32" (pressure-correction CG stencil kernel on the
- * current p' system).  No counterpart in the reference (its timing is Time::tic/toc). */
+ * current p' system), "copy_fp64" (a plain copy of 8*P doubles, for counter calibration).  No counterpart in the reference (its timing is Time::tic/toc). */
 int caffe_time_kernel(caffe_ctx* ctx, const char* which, int reps, double* ms, double* bytes);
 int caffe_sync(caffe_ctx* ctx);
 long caffe_padded_size(caffe_ctx* ctx);
