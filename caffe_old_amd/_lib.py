@@ -30,6 +30,11 @@ EXPORTS = [
     "caffe_metric_download", "caffe_matrix_download", "caffe_rhs_download", "caffe_ghost_rows", "caffe_time_kernel",
     "caffe_sync",
     "caffe_padded_size",
+    # learned pressure-correction preconditioner
+    "caffe_net_create", "caffe_net_destroy", "caffe_net_last_error", "caffe_net_param_count", "caffe_net_get_params",
+    "caffe_net_set_params", "caffe_net_forward", "caffe_net_backward", "caffe_net_precond", "caffe_net_loss_grad",
+    "caffe_net_collect", "caffe_net_clear", "caffe_net_dataset_size", "caffe_net_train", "caffe_net_eval",
+    "caffe_ctx_attach_net",
 ]
 
 
@@ -74,6 +79,7 @@ def load(path: str = LIB_PATH):
         raise CaffeError(f"{path} is not built: run __graft_entry__.build() (hipcc, gfx950)")
     L = C.CDLL(path)
     vp, dp, ip, lp = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_long)
+    fp = C.POINTER(C.c_float)
     sig = {
         "caffe_ctx_create": ([C.c_int, C.POINTER(C.c_void_p)], C.c_int),
         "caffe_ctx_destroy": ([vp], C.c_int),
@@ -102,6 +108,22 @@ def load(path: str = LIB_PATH):
         "caffe_time_kernel": ([vp, C.c_char_p, C.c_int, dp, dp], C.c_int),
         "caffe_sync": ([vp], C.c_int),
         "caffe_padded_size": ([vp], C.c_long),
+        "caffe_net_create": ([C.c_int, C.c_ulonglong, C.POINTER(C.c_void_p)], C.c_int),
+        "caffe_net_destroy": ([vp], C.c_int),
+        "caffe_net_last_error": ([vp], C.c_char_p),
+        "caffe_net_param_count": ([], C.c_long),
+        "caffe_net_get_params": ([vp, fp, C.c_long], C.c_int),
+        "caffe_net_set_params": ([vp, fp, C.c_long], C.c_int),
+        "caffe_net_forward": ([vp, C.c_int, C.c_int, C.c_int, fp, fp], C.c_int),
+        "caffe_net_backward": ([vp, C.c_int, C.c_int, C.c_int, fp, fp, fp], C.c_int),
+        "caffe_net_precond": ([vp, C.c_int, C.c_int, C.c_int, fp, fp, fp], C.c_int),
+        "caffe_net_loss_grad": ([vp, C.c_int, C.c_int, C.c_int, fp, fp, fp, dp, fp], C.c_int),
+        "caffe_net_collect": ([vp, vp, lp], C.c_int),
+        "caffe_net_clear": ([vp], C.c_int),
+        "caffe_net_dataset_size": ([vp], C.c_long),
+        "caffe_net_train": ([vp, C.c_int, C.c_int, C.c_float, C.c_ulonglong, fp], C.c_int),
+        "caffe_net_eval": ([vp, fp, C.c_long], C.c_int),
+        "caffe_ctx_attach_net": ([vp, vp], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

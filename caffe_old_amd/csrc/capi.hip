@@ -9,10 +9,6 @@
 
 using namespace caffe;
 
-struct caffe_ctx {
-    Ctx c;
-};
-
 namespace {
 
 template <class F>

@@ -118,8 +118,13 @@ double time_cg_k1(Ctx& c, bool fp32, int reps);
 int ib_field(Ctx& c);
 int ib_coeffs(Ctx& c);
 
-// ---- learned preconditioner (learned.hip)
-int learned_apply_guess(Ctx& c, const double* r, double* x, double scale);
+// FP64 true residual of the p' system (r = b - shift - A x into wp + 6P): ||r||, ||b - shift||
+void pcorr_true_resid(Ctx& c, double shift, double* rr, double* bb);
+
+// ---- learned preconditioner (learned.hip): flexible CG with the learned two-grid
+// cycle, FP32 iterations inside FP64 refinement; returns the iteration count, or -1
+// when the system does not qualify (ghost rows, dimensions not divisible by 16)
+int learned_fcg(Ctx& c, double shift, double rtol, int maxit, double delta, int* refs, double* relOut);
 
 #define HIPCHECK(x)                                                        \
     do {                                                                   \
@@ -135,3 +140,7 @@ inline int blocksFor(long n, int bs, int cap = 1 << 30) {
 }
 
 }  // namespace caffe
+
+struct caffe_ctx {
+    caffe::Ctx c;
+};

@@ -637,6 +637,10 @@ static int bicgPressure(Ctx& c, double shift, double rtol, int maxit, double inn
     return it;
 }
 
+void pcorr_true_resid(Ctx& c, double shift, double* rr, double* bb) {
+    trueResid<1>(c, symP<double>(c), c.bp, shift, c.xp, c.wp + 6 * c.D.P, rr, bb, c.singularP);
+}
+
 int pcorr_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep) {
     HIPCHECK(hipEventRecord(c.ev0, c.stream));
     const Dims& D = c.D;
@@ -652,16 +656,20 @@ int pcorr_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep) {
         if (sym) iters += cgSolve<double>(c, shift, o.rtol, o.maxit, 0., &refs, &rel);
         else iters += bicgPressure<double>(c, shift, o.rtol, o.maxit, 0., &refs, &rel);
     } else {
-        if (o.mode == CAFFE_SOLVE_LEARNED_FCG) {
-            if (!c.net || learned_apply_guess(c, c.bp, c.xp, shift) != CAFFE_OK) {
-                fell = 1;
-                hipLaunchKernelGGL(k_zero_d, dim3(blocksFor(P, KBS)), dim3(KBS), 0, c.stream, c.xp, P);
-            }
-        }
         pcorr_make_fp32(c);
-        const double delta = o.inner_rtol > 0. ? o.inner_rtol : 1e-2;
-        if (sym) iters += cgSolve<float>(c, shift, o.rtol, o.maxit, delta, &refs, &rel);
-        else iters += bicgPressure<float>(c, shift, o.rtol, o.maxit, delta, &refs, &rel);
+        bool learnedDone = false;
+        if (o.mode == CAFFE_SOLVE_LEARNED_FCG) {
+            // learned two-grid cycle inside flexible CG; the inner FP32 solve restarts from
+            // the true FP64 residual each time it has gained `inner_rtol` (default 1e-4)
+            const int it = c.net ? learned_fcg(c, shift, o.rtol, o.maxit, o.inner_rtol > 0. ? o.inner_rtol : 1e-4, &refs, &rel) : -1;
+            if (it >= 0) { iters += it; learnedDone = true; }
+            else fell = 1;   // no network, or a system it does not apply to: IR below
+        }
+        if (!learnedDone) {
+            const double delta = o.inner_rtol > 0. ? o.inner_rtol : 1e-2;
+            if (sym) iters += cgSolve<float>(c, shift, o.rtol, o.maxit, delta, &refs, &rel);
+            else iters += bicgPressure<float>(c, shift, o.rtol, o.maxit, delta, &refs, &rel);
+        }
         if (rel > o.rtol) {  // the plain FP64 solve stays as fallback and judge, from x = 0
             fell = 1;
             int r2 = 0;

@@ -1,9 +1,821 @@
-// learned.hip -- placeholder for the learned pressure-correction initial guess;
-// replaced by the multilevel conv net kernels.
-#include "internal.hpp"
+// learned.hip -- the learned two-grid preconditioner of the pressure-correction solve,
+// its training loss, the device-resident training set, and flexible CG (FCG(1),
+// FP32 iterations inside FP64 iterative refinement) that uses it.  No counterpart in the
+// reference (SparseMatrix::solve, SparseMatrix.cpp:86-106, is PETSc BiCGStab + SOR);
+// the plain FP64 solve of krylov.hip stays the fallback and the judge.
+//
+// The cycle for a residual r on the fine grid (w = damped-Jacobi weight):
+//   z1 = w D^-1 r
+//   v  = P^T D^-1 (r - A z1)                 restriction (adjoint of P) to the coarse grid
+//   c  = net(v)                              the multilevel convolution network
+//   y  = z1 + P c                            P = cell-centred trilinear prolongation
+//   z  = y + w D^-1 (r - A y)
+// The network is linear, so z = M r with a fixed linear M.  Training minimises the
+// relative energy error of one cycle applied to a right-hand side b whose converged
+// solution x is known:  L = (x - M b)^T A (x - M b) / x^T A x.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "net.hpp"
+
 namespace caffe {
-int learned_apply_guess(Ctx& c, const double*, double*, double) {
-    c.err = "learned preconditioner not loaded";
-    return CAFFE_ERR_STATE;
+
+constexpr int FB = 512;     // threads per block of the fine-grid kernels
+constexpr int FGRID = 1024; // at most MAX_RED_BLOCKS blocks (partial-sum rows)
+
+struct SymF {  // symmetric p' matrix: aP, and the coefficient of each cell's low face
+    const float* A;
+    int P, sJ, sK;
+    __device__ float aP(int p) const { return A[p]; }
+    __device__ float inv(int p) const { const float a = A[p]; return a != 0.f ? 1.f / a : 0.f; }
+    template <class F>
+    __device__ float offdiag(F x, int p) const {
+        return A[P + p] * x(p - 1) + A[P + p + 1] * x(p + 1) + A[2 * P + p] * x(p - sJ) + A[2 * P + p + sJ] * x(p + sJ)
+             + A[3 * P + p] * x(p - sK) + A[3 * P + p + sK] * x(p + sK);
+    }
+    template <class F>
+    __device__ float apply(F x, int p) const { return A[p] * x(p) + offdiag(x, p); }
+};
+static SymF symF(const float* A, const Dims& D) { return SymF{A, D.P, D.sJ, D.sK}; }
+
+#define FOR_FINE(D, q, p)                                                                  \
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x, p = 0;                            \
+         q < (D).N && ((p = fineIdx((D), q)), true); q += gridDim.x * blockDim.x)
+__device__ inline int fineIdx(const Dims& D, int q) { int i, j, k; return D.cell(q, i, j, k); }
+__device__ __forceinline__ bool fdone(const double* done) { return done && *done != 0.; }
+static int fgrid(const Dims& D) { return blocksFor(D.N, FB, FGRID); }
+
+// 1-D cell-centred linear prolongation: fine f takes 3/4 of coarse f>>1 and 1/4 of its
+// other neighbour, clamped to [0, nc)
+__device__ __forceinline__ void pw(int f, int nc, int& c0, int& c1) {
+    c0 = f >> 1;
+    c1 = (f & 1) ? c0 + 1 : c0 - 1;
+    c1 = c1 < 0 ? 0 : (c1 >= nc ? nc - 1 : c1);
 }
+
+// ---------------------------------------------------------------- cycle kernels
+// w = D^-1 (r - A (w D^-1 r))
+__global__ void k_tg_pre(Dims D, SymF S, const float* r, float* w, const double* done) {
+    if (fdone(done)) return;
+    auto z1 = [&](int c) { return NET_OMEGA * S.inv(c) * r[c]; };
+    FOR_FINE(D, q, p) { w[p] = S.inv(p) * (r[p] - S.apply(z1, p)); }
+}
+
+// v = P^T w on the coarse grid (nI/2 x nJ/2 x nK/2), fine w padded
+__global__ void k_tg_restrict(Dims D, const float* w, float* v, const double* done) {
+    if (fdone(done)) return;
+    const int cx = D.nI >> 1, cy = D.nJ >> 1, cz = D.nK >> 1;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= cx * cy * cz) return;
+    const int I = e % cx, J = (e / cx) % cy, K = e / (cx * cy);
+    float wx[4], wy[4], wz[4];
+    int fx[4], fy[4], fz[4];
+    auto weights = [](int C, int nc, int nf, int* f, float* wt) {
+        for (int m = 0; m < 4; ++m) {
+            const int ff = 2 * C - 1 + m;
+            f[m] = ff;
+            float s = 0.f;
+            if (ff >= 0 && ff < nf) {
+                int c0, c1;
+                pw(ff, nc, c0, c1);
+                if (c0 == C) s += 0.75f;
+                if (c1 == C) s += 0.25f;
+            }
+            wt[m] = s;
+        }
+    };
+    weights(I, cx, D.nI, fx, wx);
+    weights(J, cy, D.nJ, fy, wy);
+    weights(K, cz, D.nK, fz, wz);
+    float s = 0.f;
+    for (int c = 0; c < 4; ++c) {
+        if (wz[c] == 0.f) continue;
+        for (int b = 0; b < 4; ++b) {
+            if (wy[b] == 0.f) continue;
+            const float wzy = wz[c] * wy[b];
+            for (int a = 0; a < 4; ++a) {
+                if (wx[a] == 0.f) continue;
+                s += wzy * wx[a] * w[D.pidx(fx[a], fy[b], fz[c])];
+            }
+        }
+    }
+    v[e] = s;
+}
+
+// (P c) at fine cell (i,j,k)
+__device__ __forceinline__ float prolongAt(const float* c, int cx, int cy, int cz, int i, int j, int k) {
+    int x0, x1, y0, y1, z0, z1;
+    pw(i, cx, x0, x1);
+    pw(j, cy, y0, y1);
+    pw(k, cz, z0, z1);
+    auto C = [&](int a, int b, int d) { return c[(d * cy + b) * cx + a]; };
+    const float ax = 0.75f, bx = 0.25f;
+    const float lo = ax * (ax * C(x0, y0, z0) + bx * C(x1, y0, z0)) + bx * (ax * C(x0, y1, z0) + bx * C(x1, y1, z0));
+    const float hi = ax * (ax * C(x0, y0, z1) + bx * C(x1, y0, z1)) + bx * (ax * C(x0, y1, z1) + bx * C(x1, y1, z1));
+    return ax * lo + bx * hi;
+}
+
+// y = w D^-1 r + P c
+__global__ void k_tg_post1(Dims D, SymF S, const float* r, const float* c, float* y, const double* done) {
+    if (fdone(done)) return;
+    const int cx = D.nI >> 1, cy = D.nJ >> 1, cz = D.nK >> 1;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < D.N; q += gridDim.x * blockDim.x) {
+        int i, j, k;
+        const int p = D.cell(q, i, j, k);
+        y[p] = NET_OMEGA * S.inv(p) * r[p] + prolongAt(c, cx, cy, cz, i, j, k);
+    }
+}
+
+// z = y + w D^-1 (r - A y); with xs: write e = xs - z instead
+__global__ void k_tg_post2(Dims D, SymF S, const float* r, const float* y, float* z, const float* xs) {
+    auto yf = [&](int c) { return y[c]; };
+    FOR_FINE(D, q, p) {
+        const float zz = y[p] + NET_OMEGA * S.inv(p) * (r[p] - S.apply(yf, p));
+        z[p] = xs ? xs[p] - zz : zz;
+    }
+}
+
+// ---------------------------------------------------------------- loss and its gradient
+// Ae = A e; partial (e, Ae)
+__global__ void k_l_spmv(Dims D, SymF S, const float* e, float* Ae, double* part) {
+    double v[1] = {0.};
+    auto ef = [&](int c) { return e[c]; };
+    FOR_FINE(D, q, p) {
+        const float a = S.apply(ef, p);
+        Ae[p] = a;
+        v[0] += (double)e[p] * (double)a;
+    }
+    writePartials<FB, 1>(v, part);
+}
+__global__ void k_l_loss(const double* part, int nb, double nrmInv, float* loss) {
+    double t[1];
+    readPartials<FB, 1>(part, nb, t);
+    if (threadIdx.x == 0) *loss += (float)(t[0] * nrmInv);
+}
+__global__ void k_l_final(const double* part, int nb, double* out) {
+    double t[1];
+    readPartials<FB, 1>(part, nb, t);
+    if (threadIdx.x == 0) *out = t[0];
+}
+// g = s Ae (s = -2 / x^T A x);  h = g - w A (D^-1 g)   (dL/dy = (I - w A D^-1) dL/dz)
+__global__ void k_l_h(Dims D, SymF S, const float* Ae, float s, float* h) {
+    auto gi = [&](int c) { return S.inv(c) * s * Ae[c]; };
+    FOR_FINE(D, q, p) { h[p] = s * Ae[p] - NET_OMEGA * S.apply(gi, p); }
+}
+
+// ---------------------------------------------------------------- host: cycle and loss
+enum { FV_R = 0, FV_E, FV_W, FV_Y, FV_Z, FV_P0, FV_P1, FV_Q0, FV_Q1, FV_H, FV_AE, N_FV = 12 };
+
+void tg_ensure_fine(Net& n, const Dims& D) {
+    if (n.fineP == D.P && n.fw) return;
+    HIPCHECK(hipDeviceSynchronize());
+    for (void* p : n.fineAllocs) (void)hipFree(p);
+    n.fineAllocs.clear();
+    n.fw = (float*)net_alloc(n, (size_t)N_FV * D.P * sizeof(float), n.fineAllocs);
+    n.fineP = D.P;
+}
+
+static bool netDimsOk(const Dims& D) { return D.nI % 16 == 0 && D.nJ % 16 == 0 && D.nK % 16 == 0 && D.nI >= 16; }
+
+static void restrictLaunch(hipStream_t s, const Dims& D, const float* w, float* v, const double* done) {
+    const int nc = (D.nI / 2) * (D.nJ / 2) * (D.nK / 2);
+    hipLaunchKernelGGL(k_tg_restrict, dim3((nc + 255) / 256), dim3(256), 0, s, D, w, v, done);
+}
+
+void tg_apply(Net& n, hipStream_t s, const Dims& D, const float* A, const float* r, float* z, const double* done) {
+    float* w = n.fw + (size_t)FV_W * D.P;
+    float* y = n.fw + (size_t)FV_Y * D.P;
+    const SymF S = symF(A, D);
+    const int g = fgrid(D);
+    hipLaunchKernelGGL(k_tg_pre, dim3(g), dim3(FB), 0, s, D, S, r, w, done);
+    restrictLaunch(s, D, w, n.vin, done);
+    net_forward(n, s, n.vin, n.vout, done);
+    hipLaunchKernelGGL(k_tg_post1, dim3(g), dim3(FB), 0, s, D, S, r, n.vout, y, done);
+    hipLaunchKernelGGL(k_tg_post2, dim3(g), dim3(FB), 0, s, D, S, r, y, z, (const float*)nullptr);
+}
+
+void tg_loss(Net& n, hipStream_t s, const Sample& smp, bool withGrad, float* lossDev) {
+    const Dims& D = smp.D;
+    float* w = n.fw + (size_t)FV_W * D.P;
+    float* y = n.fw + (size_t)FV_Y * D.P;
+    float* e = n.fw + (size_t)FV_E * D.P;
+    float* Ae = n.fw + (size_t)FV_AE * D.P;
+    float* h = n.fw + (size_t)FV_H * D.P;
+    const SymF S = symF(smp.A, D);
+    const int g = fgrid(D);
+    hipLaunchKernelGGL(k_tg_pre, dim3(g), dim3(FB), 0, s, D, S, smp.b, w, (const double*)nullptr);
+    restrictLaunch(s, D, w, n.vin, nullptr);
+    net_forward(n, s, n.vin, n.vout, nullptr);
+    hipLaunchKernelGGL(k_tg_post1, dim3(g), dim3(FB), 0, s, D, S, smp.b, n.vout, y, (const double*)nullptr);
+    hipLaunchKernelGGL(k_tg_post2, dim3(g), dim3(FB), 0, s, D, S, smp.b, y, e, (const float*)smp.x);
+    hipLaunchKernelGGL(k_l_spmv, dim3(g), dim3(FB), 0, s, D, S, e, Ae, n.part);
+    hipLaunchKernelGGL(k_l_loss, dim3(1), dim3(FB), 0, s, n.part, g, 1.0 / smp.nrm, lossDev);
+    if (!withGrad) return;
+    hipLaunchKernelGGL(k_l_h, dim3(g), dim3(FB), 0, s, D, S, Ae, (float)(-2.0 / smp.nrm), h);
+    restrictLaunch(s, D, h, n.gout, nullptr);
+    net_backward(n, s, n.vin, n.gout);
+}
+
+// x^T A x of a sample
+static double sampleNorm(Net& n, hipStream_t s, const Sample& smp) {
+    float* Ae = n.fw + (size_t)FV_AE * smp.D.P;
+    const int g = fgrid(smp.D);
+    hipLaunchKernelGGL(k_l_spmv, dim3(g), dim3(FB), 0, s, smp.D, symF(smp.A, smp.D), smp.x, Ae, n.part);
+    hipLaunchKernelGGL(k_l_final, dim3(1), dim3(FB), 0, s, n.part, g, n.scal);
+    HIPCHECK(hipMemcpyAsync(n.hscal, n.scal, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    return n.hscal[0];
+}
+
+// ---------------------------------------------------------------- flexible CG
+enum { F_RR = 0, F_BB, F_IT, F_DONE, F_PQ, F_SQ, F_PAR, F_FIRSTRR };
+struct FcgCtl { double tol2; int maxit; double nLive; };
+
+// r = s r64 (live cells), e = 0; partials (r,r), sum r
+__global__ void k_f_init(Dims D, const double* r64, double s, float* r, float* e, double* part, double* st) {
+    double v[2] = {0., 0.};
+    FOR_FINE(D, q, p) {
+        const float rr = (float)(s * r64[p]);
+        r[p] = rr;
+        e[p] = 0.f;
+        v[0] += (double)rr * (double)rr;
+        v[1] += (double)rr;
+    }
+    writePartials<FB, 2>(v, part);
+    if (blockIdx.x == 0 && threadIdx.x == 0) { st[F_IT] = 0.; st[F_DONE] = 0.; st[F_BB] = -1.; }
+}
+
+// convergence check on the partials of (r,r), sum r; then w = D^-1 (r - A w D^-1 r).
+// Block 0 raises F_DONE in this kernel, so no block returns on it before the block-wide
+// reduction: a block whose waves read the flag on either side of that write would
+// otherwise split at the barrier.  When the flag is read as set, the partials already
+// meet the criterion, so every wave reaches the same decision.
+__global__ void k_f_pre(Dims D, SymF S, const float* r, float* w, const double* partIn, int nb, double* st, FcgCtl lc) {
+    double t[2];
+    readPartials<FB, 2>(partIn, nb, t);
+    const double rr = lc.nLive > 0. ? fmax(t[0] - t[1] * t[1] / lc.nLive, 0.) : t[0];
+    const double bb = st[F_BB] < 0. ? rr : st[F_BB];
+    const bool done = st[F_DONE] != 0. || rr <= lc.tol2 * bb || st[F_IT] >= lc.maxit || rr == 0.;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st[F_BB] = bb; st[F_RR] = rr;
+        if (st[F_IT] == 1.) st[F_FIRSTRR] = rr;
+        if (done) st[F_DONE] = 1.;
+    }
+    if (done) return;
+    auto z1 = [&](int c) { return NET_OMEGA * S.inv(c) * r[c]; };
+    FOR_FINE(D, q, p) { w[p] = S.inv(p) * (r[p] - S.apply(z1, p)); }
+}
+
+// z = y + w D^-1 (r - A y); partials (z, q_old), sum z
+__global__ void k_f_post2(Dims D, SymF S, const float* r, const float* y, float* z, const float* q0, const float* q1,
+                          double* part, const double* st) {
+    if (st[F_DONE] != 0.) return;
+    const int it = (int)st[F_IT];
+    const float* qold = (it & 1) ? q0 : q1;
+    double v[2] = {0., 0.};
+    auto yf = [&](int c) { return y[c]; };
+    FOR_FINE(D, q, p) {
+        const float zz = y[p] + NET_OMEGA * S.inv(p) * (r[p] - S.apply(yf, p));
+        z[p] = zz;
+        v[0] += (double)zz * (double)qold[p];
+        v[1] += (double)zz;
+    }
+    writePartials<FB, 2>(v, part);
+}
+
+// beta = -(z - zbar, q_old) / (p_old, q_old); p = (z - zbar) + beta p_old formed at the
+// stencil points; q = A p; partials (p,q), (p,r), sum q
+__global__ void k_f_k1(Dims D, SymF S, const float* z, const float* r, float* p0, float* p1, float* q0, float* q1,
+                       const double* partIn, int nb, double* partOut, double* st, FcgCtl lc) {
+    if (st[F_DONE] != 0.) return;
+    double t[2];
+    readPartials<FB, 2>(partIn, nb, t);
+    const int it = (int)st[F_IT];
+    const double zbar = lc.nLive > 0. ? t[1] / lc.nLive : 0.;
+    const double zq = t[0] - zbar * st[F_SQ];
+    const double beta = it == 0 || st[F_PQ] == 0. ? 0. : -zq / st[F_PQ];
+    const float* pold = (it & 1) ? p0 : p1;
+    float* pnew = (it & 1) ? p1 : p0;
+    float* qnew = (it & 1) ? q1 : q0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) st[F_PAR] = (double)(it & 1);
+    const float zb = (float)zbar, bt = (float)beta;
+    auto pf = [&](int c) { return (z[c] - zb) + bt * pold[c]; };
+    double v[3] = {0., 0., 0.};
+    FOR_FINE(D, q, p) {
+        const float pn = pf(p);
+        const float qq = S.apply(pf, p);
+        pnew[p] = pn;
+        qnew[p] = qq;
+        v[0] += (double)pn * (double)qq;
+        v[1] += (double)pn * (double)r[p];
+        v[2] += (double)qq;
+    }
+    writePartials<FB, 3>(v, partOut);
+}
+
+// alpha = (p,r)/(p,q); e += alpha p; r -= alpha q; partials (r,r), sum r
+__global__ void k_f_k2(Dims D, float* e, float* r, const float* p0, const float* p1, const float* q0, const float* q1,
+                       const double* partIn, int nb, double* partOut, double* st) {
+    if (st[F_DONE] != 0.) return;
+    double t[3];
+    readPartials<FB, 3>(partIn, nb, t);
+    const int par = (int)st[F_PAR];
+    const float* pv = par ? p1 : p0;
+    const float* qv = par ? q1 : q0;
+    const double alpha = t[0] != 0. ? t[1] / t[0] : 0.;
+    if (blockIdx.x == 0 && threadIdx.x == 0) { st[F_PQ] = t[0]; st[F_SQ] = t[2]; st[F_IT] += 1.; }
+    const float al = (float)alpha;
+    double v[2] = {0., 0.};
+    FOR_FINE(D, q, p) {
+        e[p] += al * pv[p];
+        const float rn = r[p] - al * qv[p];
+        r[p] = rn;
+        v[0] += (double)rn * (double)rn;
+        v[1] += (double)rn;
+    }
+    writePartials<FB, 2>(v, partOut);
+}
+
+__global__ void k_f_axpy(Dims D, double* x, const float* e, double s) {
+    FOR_FINE(D, q, p) { x[p] += s * (double)e[p]; }
+}
+
+int learned_fcg(Ctx& c, double shift, double rtol, int maxit, double delta, int* refs, double* relOut) {
+    Net& n = *(Net*)c.net;
+    const Dims& D = c.D;
+    *refs = 0;
+    if (!netDimsOk(D) || c.hasIB) return -1;
+    HIPCHECK(hipSetDevice(n.device));
+    tg_ensure_fine(n, D);
+    if (!(n.nx == D.nI / 2 && n.ny == D.nJ / 2 && n.nz == D.nK / 2)) net_set_dims(n, D.nI / 2, D.nJ / 2, D.nK / 2, false);
+    hipStream_t s = c.stream;
+    if (!n.packed) net_pack(n, s);
+    const SymF S = symF(c.Ap32, D);
+    const int P = D.P;
+    float* r = n.fw + (size_t)FV_R * P;
+    float* e = n.fw + (size_t)FV_E * P;
+    float* w = n.fw + (size_t)FV_W * P;
+    float* y = n.fw + (size_t)FV_Y * P;
+    float* z = n.fw + (size_t)FV_Z * P;
+    float* p0 = n.fw + (size_t)FV_P0 * P;
+    float* p1 = n.fw + (size_t)FV_P1 * P;
+    float* q0 = n.fw + (size_t)FV_Q0 * P;
+    float* q1 = n.fw + (size_t)FV_Q1 * P;
+    double* st = c.scal + SC_CG;
+    const double* done = st + F_DONE;
+    double* partA = c.red.part;
+    double* partB = c.red.part + MAX_RED_BLOCKS * RED_STRIDE;
+    double* partC = c.red.part + 2 * MAX_RED_BLOCKS * RED_STRIDE;
+    const int g = fgrid(D);
+    double rr, bb;
+    pcorr_true_resid(c, shift, &rr, &bb);
+    double rel = bb > 0. ? rr / bb : 0.;
+    int its = 0;
+    FcgCtl lc{delta * delta, maxit, c.singularP ? (double)D.N : 0.};
+    while (rel > rtol && its < maxit && *refs < 40) {
+        const double scale = 1.0 / rr;
+        hipLaunchKernelGGL(k_f_init, dim3(g), dim3(FB), 0, s, D, (const double*)(c.wp + 6 * P), scale, r, e, partC, st);
+        lc.maxit = maxit - its;
+        int chunk = 4, it = 0;
+        while (true) {
+            for (int k = 0; k < chunk; ++k) {
+                hipLaunchKernelGGL(k_f_pre, dim3(g), dim3(FB), 0, s, D, S, r, w, partC, g, st, lc);
+                restrictLaunch(s, D, w, n.vin, done);
+                net_forward(n, s, n.vin, n.vout, done);
+                hipLaunchKernelGGL(k_tg_post1, dim3(g), dim3(FB), 0, s, D, S, r, n.vout, y, done);
+                hipLaunchKernelGGL(k_f_post2, dim3(g), dim3(FB), 0, s, D, S, r, y, z, q0, q1, partA, st);
+                hipLaunchKernelGGL(k_f_k1, dim3(g), dim3(FB), 0, s, D, S, z, r, p0, p1, q0, q1, partA, g, partB, st, lc);
+                hipLaunchKernelGGL(k_f_k2, dim3(g), dim3(FB), 0, s, D, e, r, p0, p1, q0, q1, partB, g, partC, st);
+            }
+            HIPCHECK(hipMemcpyAsync(c.hscal + SC_CG, st, 8 * sizeof(double), hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+            it = (int)c.hscal[SC_CG + F_IT];
+            if (c.hscal[SC_CG + F_DONE] != 0. || it >= lc.maxit) break;
+            if (chunk < 16) chunk *= 2;
+        }
+        its += it;
+        hipLaunchKernelGGL(k_f_axpy, dim3(g), dim3(FB), 0, s, D, c.xp, (const float*)e, 1.0 / scale);
+        const double prev = rel;
+        pcorr_true_resid(c, shift, &rr, &bb);
+        rel = rr / bb;
+        ++*refs;
+        if (!(rel < prev) || it == 0) break;
+    }
+    *relOut = rel;
+    return its;
+}
+
 }  // namespace caffe
+
+// ================================================================== C ABI
+using namespace caffe;
+
+namespace {
+
+template <class F>
+int nguard(caffe_net* x, F f) {
+    if (!x) return CAFFE_ERR_ARG;
+    try {
+        HIPCHECK(hipSetDevice(x->n.device));
+        return f(x->n);
+    } catch (const std::string& e) {
+        x->n.err = e;
+        return CAFFE_ERR_HIP;
+    } catch (const std::exception& e) {
+        x->n.err = e.what();
+        return CAFFE_ERR_HIP;
+    }
+}
+
+// splitmix64 + Box-Muller normal deviates (deterministic initialisation)
+struct Rng {
+    uint64_t s;
+    uint64_t next() {
+        uint64_t z = (s += 0x9e3779b97f4a7c15ULL);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+        return z ^ (z >> 31);
+    }
+    double uni() { return ((next() >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
+    double normal() { return std::sqrt(-2.0 * std::log(uni())) * std::cos(6.283185307179586 * uni()); }
+};
+
+void freeNet(Net& n) {
+    (void)hipDeviceSynchronize();
+    for (void* p : n.dimAllocs) (void)hipFree(p);
+    for (void* p : n.fineAllocs) (void)hipFree(p);
+    for (auto& smp : n.data) { (void)hipFree(smp.A); (void)hipFree(smp.b); (void)hipFree(smp.x); }
+    for (void* p : n.allocs) (void)hipFree(p);
+    n.dimAllocs.clear(); n.fineAllocs.clear(); n.data.clear(); n.allocs.clear();
+}
+
+Dims fineDims(int nI, int nJ, int nK) {
+    Dims D;
+    D.nI = nI; D.nJ = nJ; D.nK = nK;
+    D.sJ = nI + 2; D.sK = (nI + 2) * (nJ + 2);
+    D.P = D.sK * (nK + 2);
+    D.N = nI * nJ * nK;
+    return D;
+}
+
+__global__ void k_collect(Dims D, const double* Ap, const double* bp, const double* xp, const int8_t* st, double shift,
+                          double s, float* A, float* b, float* x) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= D.P) return;
+    for (int m = 0; m < 4; ++m) A[m * D.P + p] = (float)Ap[m * D.P + p];
+    int i = p % D.sJ, j = (p / D.sJ) % (D.nJ + 2), k = p / D.sK;
+    const bool in = i >= 1 && i <= D.nI && j >= 1 && j <= D.nJ && k >= 1 && k <= D.nK;
+    const bool live = in && st[p] != ST_INACTIVE;
+    b[p] = live ? (float)((bp[p] - shift) * s) : 0.f;
+    x[p] = live ? (float)(xp[p] * s) : 0.f;
+}
+// sum and count of b over the cells that carry a row
+__global__ void k_sum_live(Dims D, const double* b, const int8_t* st, double* part) {
+    double v[2] = {0., 0.};
+    FOR_FINE(D, q, p) {
+        if (st[p] == ST_INACTIVE) continue;
+        v[0] += b[p];
+        v[1] += 1.;
+    }
+    writePartials<FB, 2>(v, part);
+}
+__global__ void k_sumsq(Dims D, const double* b, const int8_t* st, double shift, double* part) {
+    double v[1] = {0.};
+    FOR_FINE(D, q, p) {
+        if (st[p] == ST_INACTIVE) continue;
+        const double d = b[p] - shift;
+        v[0] += d * d;
+    }
+    writePartials<FB, 1>(v, part);
+}
+__global__ void k_final2(const double* part, int nb, double* out) {
+    double t[2];
+    readPartials<FB, 2>(part, nb, t);
+    if (threadIdx.x == 0) { out[0] = t[0]; out[1] = t[1]; }
+}
+
+Sample uploadSample(Net& n, const Dims& D, const float* A, const float* b, const float* x) {
+    Sample smp;
+    smp.D = D;
+    HIPCHECK(hipMalloc(&smp.A, 4 * (size_t)D.P * sizeof(float)));
+    HIPCHECK(hipMalloc(&smp.b, (size_t)D.P * sizeof(float)));
+    HIPCHECK(hipMalloc(&smp.x, (size_t)D.P * sizeof(float)));
+    HIPCHECK(hipMemcpy(smp.A, A, 4 * (size_t)D.P * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(smp.b, b, (size_t)D.P * sizeof(float), hipMemcpyHostToDevice));
+    if (x) HIPCHECK(hipMemcpy(smp.x, x, (size_t)D.P * sizeof(float), hipMemcpyHostToDevice));
+    else HIPCHECK(hipMemset(smp.x, 0, (size_t)D.P * sizeof(float)));
+    (void)n;
+    return smp;
+}
+void freeSample(Sample& s) {
+    (void)hipFree(s.A); (void)hipFree(s.b); (void)hipFree(s.x);
+    s.A = s.b = s.x = nullptr;
+}
+
+int checkFine(Net& n, int nI, int nJ, int nK) {
+    if (nI < 16 || nJ < 16 || nK < 16 || nI % 16 || nJ % 16 || nK % 16) {
+        n.err = "the learned preconditioner needs fine-grid dimensions divisible by 16";
+        return CAFFE_ERR_ARG;
+    }
+    return CAFFE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int caffe_net_create(int device, unsigned long long seed, caffe_net** out) {
+    if (!out) return CAFFE_ERR_ARG;
+    *out = nullptr;
+    caffe_net* x = new caffe_net();
+    x->n.device = device;
+    int rc = nguard(x, [&](Net& n) {
+        HIPCHECK(hipStreamCreateWithFlags(&n.stream, hipStreamNonBlocking));
+        HIPCHECK(hipEventCreate(&n.ev0));
+        HIPCHECK(hipEventCreate(&n.ev1));
+        long o = 0, po = 0;
+        for (int l = 0; l < NET_LAYERS; ++l) {
+            n.off[l] = o;
+            n.poff[l] = po;
+            const long L = 27L * kNet[l].cin * kNet[l].cout;
+            o += L;
+            if (l != NET_IN && l != NET_OUT) po += L;
+        }
+        n.nParams = o;
+        n.theta = (float*)net_alloc(n, o * 4, n.allocs);
+        n.grad = (float*)net_alloc(n, o * 4, n.allocs);
+        n.m1 = (float*)net_alloc(n, o * 4, n.allocs);
+        n.m2 = (float*)net_alloc(n, o * 4, n.allocs);
+        n.wf = (uint16_t*)net_alloc(n, po * 2, n.allocs);
+        n.wb = (uint16_t*)net_alloc(n, po * 2, n.allocs);
+        n.part = (double*)net_alloc(n, 2 * MAX_RED_BLOCKS * RED_STRIDE * sizeof(double), n.allocs);
+        n.scal = (double*)net_alloc(n, 64 * sizeof(double), n.allocs);
+        HIPCHECK(hipHostMalloc((void**)&n.hscal, 64 * sizeof(double), hipHostMallocDefault));
+        // variance-preserving initialisation of a linear network; the output layer small
+        std::vector<float> h(o);
+        Rng rng{seed * 0x2545F4914F6CDD1DULL + 1};
+        for (int l = 0; l < NET_LAYERS; ++l) {
+            const long L = 27L * kNet[l].cin * kNet[l].cout;
+            const double sd = (l == NET_OUT ? 0.1 : 1.0) / std::sqrt(27.0 * kNet[l].cin);
+            for (long i = 0; i < L; ++i) h[n.off[l] + i] = (float)(sd * rng.normal());
+        }
+        HIPCHECK(hipMemcpy(n.theta, h.data(), o * 4, hipMemcpyHostToDevice));
+        net_pack(n, n.stream);
+        HIPCHECK(hipStreamSynchronize(n.stream));
+        return CAFFE_OK;
+    });
+    if (rc != CAFFE_OK) {
+        freeNet(x->n);
+        delete x;
+        return rc;
+    }
+    *out = x;
+    return CAFFE_OK;
+}
+
+int caffe_net_destroy(caffe_net* x) {
+    if (!x) return CAFFE_ERR_ARG;
+    Net& n = x->n;
+    (void)hipSetDevice(n.device);
+    freeNet(n);
+    if (n.hscal) (void)hipHostFree(n.hscal);
+    if (n.ev0) (void)hipEventDestroy(n.ev0);
+    if (n.ev1) (void)hipEventDestroy(n.ev1);
+    if (n.stream) (void)hipStreamDestroy(n.stream);
+    delete x;
+    return CAFFE_OK;
+}
+
+const char* caffe_net_last_error(caffe_net* x) { return x ? x->n.err.c_str() : "null network"; }
+
+long caffe_net_param_count(void) {
+    long o = 0;
+    for (int l = 0; l < NET_LAYERS; ++l) o += 27L * kNet[l].cin * kNet[l].cout;
+    return o;
+}
+
+int caffe_net_get_params(caffe_net* x, float* host, long n_) {
+    return nguard(x, [&](Net& n) {
+        if (!host || n_ != n.nParams) { n.err = "parameter count mismatch"; return CAFFE_ERR_ARG; }
+        HIPCHECK(hipStreamSynchronize(n.stream));
+        HIPCHECK(hipMemcpy(host, n.theta, n.nParams * 4, hipMemcpyDeviceToHost));
+        return CAFFE_OK;
+    });
+}
+
+int caffe_net_set_params(caffe_net* x, const float* host, long n_) {
+    return nguard(x, [&](Net& n) {
+        if (!host || n_ != n.nParams) { n.err = "parameter count mismatch"; return CAFFE_ERR_ARG; }
+        for (long i = 0; i < n_; ++i)
+            if (!std::isfinite(host[i])) { n.err = "non-finite parameter"; return CAFFE_ERR_ARG; }
+        HIPCHECK(hipStreamSynchronize(n.stream));
+        HIPCHECK(hipMemcpy(n.theta, host, n.nParams * 4, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemset(n.m1, 0, n.nParams * 4));
+        HIPCHECK(hipMemset(n.m2, 0, n.nParams * 4));
+        n.adamT = 0;
+        net_pack(n, n.stream);
+        HIPCHECK(hipStreamSynchronize(n.stream));
+        return CAFFE_OK;
+    });
+}
+
+int caffe_net_forward(caffe_net* x, int nx, int ny, int nz, const float* in, float* out) {
+    return nguard(x, [&](Net& n) {
+        if (!in || !out || nx < 8 || ny < 8 || nz < 8 || nx % 8 || ny % 8 || nz % 8) {
+            n.err = "coarse dimensions must be multiples of 8";
+            return CAFFE_ERR_ARG;
+        }
+        net_set_dims(n, nx, ny, nz, n.trainBufs);
+        const size_t nv = (size_t)nx * ny * nz;
+        HIPCHECK(hipMemcpyAsync(n.vin, in, nv * 4, hipMemcpyHostToDevice, n.stream));
+        net_forward(n, n.stream, n.vin, n.vout, nullptr);
+        HIPCHECK(hipMemcpyAsync(out, n.vout, nv * 4, hipMemcpyDeviceToHost, n.stream));
+        HIPCHECK(hipStreamSynchronize(n.stream));
+        return CAFFE_OK;
+    });
+}
+
+int caffe_net_backward(caffe_net* x, int nx, int ny, int nz, const float* in, const float* gout, float* gparams) {
+    return nguard(x, [&](Net& n) {
+        if (!in || !gout || !gparams || nx < 8 || ny < 8 || nz < 8 || nx % 8 || ny % 8 || nz % 8) {
+            n.err = "coarse dimensions must be multiples of 8";
+            return CAFFE_ERR_ARG;
+        }
+        net_set_dims(n, nx, ny, nz, true);
+        const size_t nv = (size_t)nx * ny * nz;
+        HIPCHECK(hipMemcpyAsync(n.vin, in, nv * 4, hipMemcpyHostToDevice, n.stream));
+        HIPCHECK(hipMemcpyAsync(n.gout, gout, nv * 4, hipMemcpyHostToDevice, n.stream));
+        HIPCHECK(hipMemsetAsync(n.grad, 0, n.nParams * 4, n.stream));
+        net_forward(n, n.stream, n.vin, n.vout, nullptr);
+        net_backward(n, n.stream, n.vin, n.gout);
+        HIPCHECK(hipMemcpyAsync(gparams, n.grad, n.nParams * 4, hipMemcpyDeviceToHost, n.stream));
+        HIPCHECK(hipStreamSynchronize(n.stream));
+        return CAFFE_OK;
+    });
+}
+
+int caffe_net_precond(caffe_net* x, int nI, int nJ, int nK, const float* A, const float* r, float* z) {
+    return nguard(x, [&](Net& n) {
+        if (!A || !r || !z) { n.err = "null argument"; return CAFFE_ERR_ARG; }
+        if (int rc = checkFine(n, nI, nJ, nK)) return rc;
+        const Dims D = fineDims(nI, nJ, nK);
+        HIPCHECK(hipStreamSynchronize(n.stream));
+        Sample smp = uploadSample(n, D, A, r, nullptr);
+        tg_ensure_fine(n, D);
+        net_set_dims(n, nI / 2, nJ / 2, nK / 2, n.trainBufs);
+        float* zd = n.fw + (size_t)FV_Z * D.P;
+        HIPCHECK(hipMemsetAsync(zd, 0, (size_t)D.P * 4, n.stream));
+        tg_apply(n, n.stream, D, smp.A, smp.b, zd, nullptr);
+        HIPCHECK(hipMemcpyAsync(z, zd, (size_t)D.P * 4, hipMemcpyDeviceToHost, n.stream));
+        HIPCHECK(hipStreamSynchronize(n.stream));
+        freeSample(smp);
+        return CAFFE_OK;
+    });
+}
+
+int caffe_net_loss_grad(caffe_net* x, int nI, int nJ, int nK, const float* A, const float* b, const float* xs,
+                        double* loss, float* gparams) {
+    return nguard(x, [&](Net& n) {
+        if (!A || !b || !xs || !loss) { n.err = "null argument"; return CAFFE_ERR_ARG; }
+        if (int rc = checkFine(n, nI, nJ, nK)) return rc;
+        const Dims D = fineDims(nI, nJ, nK);
+        HIPCHECK(hipStreamSynchronize(n.stream));
+        Sample smp = uploadSample(n, D, A, b, xs);
+        tg_ensure_fine(n, D);
+        net_set_dims(n, nI / 2, nJ / 2, nK / 2, gparams != nullptr || n.trainBufs);
+        smp.nrm = sampleNorm(n, n.stream, smp);
+        if (!(smp.nrm > 0.)) { freeSample(smp); n.err = "x^T A x is not positive"; return CAFFE_ERR_ARG; }
+        float* ld = (float*)(n.scal + 8);
+        HIPCHECK(hipMemsetAsync(ld, 0, sizeof(float), n.stream));
+        if (gparams) HIPCHECK(hipMemsetAsync(n.grad, 0, n.nParams * 4, n.stream));
+        tg_loss(n, n.stream, smp, gparams != nullptr, ld);
+        float lh = 0.f;
+        HIPCHECK(hipMemcpyAsync(&lh, ld, sizeof(float), hipMemcpyDeviceToHost, n.stream));
+        if (gparams) HIPCHECK(hipMemcpyAsync(gparams, n.grad, n.nParams * 4, hipMemcpyDeviceToHost, n.stream));
+        HIPCHECK(hipStreamSynchronize(n.stream));
+        freeSample(smp);
+        *loss = lh;
+        return CAFFE_OK;
+    });
+}
+
+int caffe_net_collect(caffe_net* x, caffe_ctx* cx, long* count) {
+    return nguard(x, [&](Net& n) {
+        if (!cx) { n.err = "null context"; return CAFFE_ERR_ARG; }
+        Ctx& c = cx->c;
+        if (!c.ready) { n.err = "context has no case"; return CAFFE_ERR_STATE; }
+        if (c.hasIB) { n.err = "ghost-cell (nonsymmetric) systems are not collected"; return CAFFE_ERR_STATE; }
+        if (c.device != n.device) { n.err = "context and network are on different devices"; return CAFFE_ERR_ARG; }
+        if (int rc = checkFine(n, c.D.nI, c.D.nJ, c.D.nK)) return rc;
+        const Dims& D = c.D;
+        hipStream_t s = c.stream;
+        const int g = fgrid(D);
+        double shift = 0.;
+        if (c.singularP) {
+            hipLaunchKernelGGL(k_sum_live, dim3(g), dim3(FB), 0, s, D, (const double*)c.bp, (const int8_t*)c.st, n.part);
+            hipLaunchKernelGGL(k_final2, dim3(1), dim3(FB), 0, s, n.part, g, n.scal);
+            HIPCHECK(hipMemcpyAsync(n.hscal, n.scal, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+            shift = n.hscal[1] > 0. ? n.hscal[0] / n.hscal[1] : 0.;
+        }
+        hipLaunchKernelGGL(k_sumsq, dim3(g), dim3(FB), 0, s, D, (const double*)c.bp, (const int8_t*)c.st, shift, n.part);
+        hipLaunchKernelGGL(k_l_final, dim3(1), dim3(FB), 0, s, n.part, g, n.scal);
+        HIPCHECK(hipMemcpyAsync(n.hscal, n.scal, sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        const double bn = std::sqrt(n.hscal[0]);
+        if (!(bn > 0.) || !std::isfinite(bn)) { n.err = "zero or non-finite right-hand side"; return CAFFE_ERR_STATE; }
+        Sample smp;
+        smp.D = D;
+        HIPCHECK(hipMalloc(&smp.A, 4 * (size_t)D.P * sizeof(float)));
+        HIPCHECK(hipMalloc(&smp.b, (size_t)D.P * sizeof(float)));
+        HIPCHECK(hipMalloc(&smp.x, (size_t)D.P * sizeof(float)));
+        hipLaunchKernelGGL(k_collect, dim3((D.P + 255) / 256), dim3(256), 0, s, D, (const double*)c.Ap, (const double*)c.bp,
+                           (const double*)c.xp, (const int8_t*)c.st, shift, 1.0 / bn, smp.A, smp.b, smp.x);
+        HIPCHECK(hipStreamSynchronize(s));
+        tg_ensure_fine(n, D);
+        smp.nrm = sampleNorm(n, n.stream, smp);
+        if (!(smp.nrm > 0.) || !std::isfinite(smp.nrm)) { freeSample(smp); n.err = "x^T A x is not positive"; return CAFFE_ERR_STATE; }
+        n.data.push_back(smp);
+        if (count) *count = (long)n.data.size();
+        return CAFFE_OK;
+    });
+}
+
+int caffe_net_clear(caffe_net* x) {
+    return nguard(x, [&](Net& n) {
+        HIPCHECK(hipStreamSynchronize(n.stream));
+        for (auto& smp : n.data) freeSample(smp);
+        n.data.clear();
+        return CAFFE_OK;
+    });
+}
+
+long caffe_net_dataset_size(caffe_net* x) { return x ? (long)x->n.data.size() : -1; }
+
+int caffe_net_train(caffe_net* x, int steps, int batch, float lr, unsigned long long seed, float* losses) {
+    return nguard(x, [&](Net& n) {
+        if (steps < 1 || batch < 1 || !(lr > 0.f)) { n.err = "bad training arguments"; return CAFFE_ERR_ARG; }
+        if (n.data.empty()) { n.err = "empty training set"; return CAFFE_ERR_STATE; }
+        const Dims D = n.data[0].D;
+        for (auto& smp : n.data)
+            if (smp.D.nI != D.nI || smp.D.nJ != D.nJ || smp.D.nK != D.nK) { n.err = "training samples differ in size"; return CAFFE_ERR_STATE; }
+        hipStream_t s = n.stream;
+        tg_ensure_fine(n, D);
+        net_set_dims(n, D.nI / 2, D.nJ / 2, D.nK / 2, true);
+        if ((long)steps > n.lossCap) {
+            n.losses = (float*)net_alloc(n, (size_t)steps * sizeof(float), n.allocs);
+            n.lossCap = steps;
+        }
+        HIPCHECK(hipMemsetAsync(n.losses, 0, (size_t)steps * sizeof(float), s));
+        Rng rng{seed + 0x51ed2701ULL};
+        const size_t ns = n.data.size();
+        for (int t = 0; t < steps; ++t) {
+            const float lrt = (float)(lr * 0.5 * (1.0 + std::cos(3.141592653589793 * t / steps)));
+            HIPCHECK(hipMemsetAsync(n.grad, 0, n.nParams * 4, s));
+            for (int b = 0; b < batch; ++b) {
+                const Sample& smp = n.data[rng.next() % ns];
+                tg_loss(n, s, smp, true, n.losses + t);
+            }
+            net_adam(n, s, lrt, 1.0f / batch);
+        }
+        std::vector<float> h(steps);
+        HIPCHECK(hipMemcpyAsync(h.data(), n.losses, (size_t)steps * 4, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        for (int t = 0; t < steps; ++t) {
+            h[t] /= (float)batch;
+            if (losses) losses[t] = h[t];
+        }
+        if (!std::isfinite(h[steps - 1])) { n.err = "training diverged (non-finite loss)"; return CAFFE_ERR_NOT_CONVERGED; }
+        return CAFFE_OK;
+    });
+}
+
+int caffe_net_eval(caffe_net* x, float* losses, long count) {
+    return nguard(x, [&](Net& n) {
+        if (!losses || count != (long)n.data.size()) { n.err = "loss buffer size mismatch"; return CAFFE_ERR_ARG; }
+        if (n.data.empty()) return CAFFE_OK;
+        hipStream_t s = n.stream;
+        const Dims D = n.data[0].D;
+        tg_ensure_fine(n, D);
+        net_set_dims(n, D.nI / 2, D.nJ / 2, D.nK / 2, n.trainBufs);
+        float* ld = (float*)(n.scal + 8);
+        for (long i = 0; i < count; ++i) {
+            HIPCHECK(hipMemsetAsync(ld, 0, sizeof(float), s));
+            tg_loss(n, s, n.data[i], false, ld);
+            HIPCHECK(hipMemcpyAsync(losses + i, ld, sizeof(float), hipMemcpyDeviceToHost, s));
+        }
+        HIPCHECK(hipStreamSynchronize(s));
+        return CAFFE_OK;
+    });
+}
+
+int caffe_ctx_attach_net(caffe_ctx* cx, caffe_net* x) {
+    if (!cx) return CAFFE_ERR_ARG;
+    Ctx& c = cx->c;
+    if (x && x->n.device != c.device) { c.err = "network and context are on different devices"; return CAFFE_ERR_ARG; }
+    c.net = x ? (void*)&x->n : nullptr;
+    return CAFFE_OK;
+}
+
+}  // extern "C"

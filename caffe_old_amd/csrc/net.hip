@@ -1,0 +1,645 @@
+// net.hip -- the multilevel convolution network of the learned pressure-correction
+// preconditioner (no counterpart in the reference, whose p' solve is PETSc BiCGStab +
+// SOR, SparseMatrix.cpp:86-106).
+//
+// Kernels
+//   k_conv         3x3x3 convolution on MFMA (v_mfma_f32_16x16x16_bf16): BF16 operands,
+//                  FP32 accumulation.  One workgroup computes a TXxTYxTZ output tile for
+//                  16 output channels; the K loop runs over 16-channel input blocks, each
+//                  staged with its one-voxel halo in LDS together with that block's
+//                  27x16x16 weights.  A wave owns TX*TY*TZ/64 groups of 16 voxels; per tap
+//                  it reads one B fragment (weights) and one A fragment per group.
+//                  The staging applies the layer's input operator on the fly: plain,
+//                  2x2x2 average pooling of the finer level, or nearest up-sampling of the
+//                  coarser level plus the skip features.  The same kernel computes the
+//                  input gradient with flipped taps and swapped channels (FP32 in/out).
+//   k_conv_wgrad   weight gradient on MFMA: D[co][ci] += G^T X over the voxels of a
+//                  chunk of tiles for all 27 taps, reduced across waves in LDS in a fixed
+//                  order, one partial row per workgroup, then k_reduce_parts.
+//   k_in_*/k_out_* the 1->16 input and 16->1 output layers in FP32 scalar code.
+// Layouts: activations [z][y][x][C] channels last (BF16), gradients FP32 in the same
+// layout, FP32 parameters [27][cout][cin] with tap t = (dz+1)*9 + (dy+1)*3 + (dx+1).
+#include <cmath>
+#include <cstring>
+
+#include "net.hpp"
+
+namespace caffe {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(uint32_t h) { return __uint_as_float(h << 16); }
+// round to nearest even (NaN stays NaN)
+__device__ __forceinline__ uint32_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (u >> 16) | 0x40u;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+__device__ __forceinline__ uint2 pack4(const float (&v)[4]) {
+    uint2 r;
+    r.x = f2bf(v[0]) | (f2bf(v[1]) << 16);
+    r.y = f2bf(v[2]) | (f2bf(v[3]) << 16);
+    return r;
+}
+__device__ __forceinline__ void ld4(const uint16_t* p, float (&v)[4]) {
+    const uint2 u = *(const uint2*)p;
+    v[0] = bf2f(u.x & 0xffffu); v[1] = bf2f(u.x >> 16); v[2] = bf2f(u.y & 0xffffu); v[3] = bf2f(u.y >> 16);
+}
+__device__ __forceinline__ void ld4(const float* p, float (&v)[4]) {
+    const float4 f = *(const float4*)p;
+    v[0] = f.x; v[1] = f.y; v[2] = f.z; v[3] = f.w;
+}
+__device__ __forceinline__ bool isDone(const double* done) { return done && *done != 0.; }
+
+constexpr int NB = 256;   // threads per block of the network kernels
+constexpr int LS = 20;    // LDS row: 16 BF16 channels + 4 padding (conflict-free 8-byte reads)
+
+// Channels [c, c+4) of the staged input at voxel (gx,gy,gz) of a level with dims
+// (nx,ny,nz), C channels, as 4 packed BF16.
+template <int MODE, class TIN>
+__device__ __forceinline__ uint2 stage4(const TIN* in, const TIN* skip, int C, int nx, int ny, int nz, int gx, int gy,
+                                        int gz, int c) {
+    if (MODE == MODE_PLAIN) {
+        const size_t o = ((size_t)(gz * ny + gy) * nx + gx) * C + c;
+        if constexpr (sizeof(TIN) == 2) {
+            return *(const uint2*)(in + o);
+        } else {
+            float v[4];
+            ld4(in + o, v);
+            return pack4(v);
+        }
+    } else if (MODE == MODE_POOL) {  // the finer level has dims (2nx, 2ny, 2nz)
+        float s[4] = {0.f, 0.f, 0.f, 0.f};
+        const int fx = 2 * nx, fy = 2 * ny;
+        for (int dz = 0; dz < 2; ++dz)
+            for (int dy = 0; dy < 2; ++dy)
+                for (int dx = 0; dx < 2; ++dx) {
+                    float v[4];
+                    ld4(in + ((size_t)((2 * gz + dz) * fy + 2 * gy + dy) * fx + 2 * gx + dx) * C + c, v);
+                    for (int q = 0; q < 4; ++q) s[q] += v[q];
+                }
+        for (int q = 0; q < 4; ++q) s[q] *= 0.125f;
+        return pack4(s);
+    } else {  // UPADD: coarser level has dims (nx/2, ny/2, nz/2)
+        float a[4], b[4];
+        ld4(in + ((size_t)((gz >> 1) * (ny >> 1) + (gy >> 1)) * (nx >> 1) + (gx >> 1)) * C + c, a);
+        ld4(skip + ((size_t)(gz * ny + gy) * nx + gx) * C + c, b);
+        for (int q = 0; q < 4; ++q) a[q] += b[q];
+        return pack4(a);
+    }
+}
+
+struct ConvArgs {
+    const void* in;
+    const void* skip;
+    const uint16_t* w;     // packed [cout/16][cin/16][27][16 co][16 ci]
+    void* out;
+    int nx, ny, nz, cin, cout;
+    int accum;             // FP32 output: add to it
+    const double* done;
+};
+
+template <int TX, int TY, int TZ, int MODE, class TIN, class TOUT>
+__global__ __launch_bounds__(NB) void k_conv(ConvArgs a) {
+    if (isDone(a.done)) return;
+    constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2, HV = HX * HY * HZ;
+    constexpr int TV = TX * TY * TZ, G = TV / 64;
+    static_assert(G >= 1 && TV % 64 == 0, "tile must hold a multiple of 64 voxels");
+    __shared__ __align__(16) uint16_t sIn[HV * LS];
+    __shared__ __align__(16) uint16_t sW[27 * 16 * LS];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int n = lane & 15, kq = lane >> 4;
+    const int ntx = (a.nx + TX - 1) / TX, nty = (a.ny + TY - 1) / TY;
+    int b = blockIdx.x;
+    const int bx = b % ntx;
+    b /= ntx;
+    const int by = b % nty, bz = b / nty;
+    const int ox = bx * TX, oy = by * TY, oz = bz * TZ;
+    const int cob = blockIdx.y, ncb = a.cin >> 4;
+    const TIN* in = (const TIN*)a.in;
+    const TIN* sk = (const TIN*)a.skip;
+
+    int base[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int vi = (wave * G + g) * 16 + n;
+        const int tx = vi % TX, ty = (vi / TX) % TY, tz = vi / (TX * TY);
+        base[g] = ((tz + 1) * HY + (ty + 1)) * HX + (tx + 1);
+    }
+    f32x4 acc[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) acc[g] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int cb = 0; cb < ncb; ++cb) {
+        __syncthreads();
+        for (int it = tid; it < HV * 4; it += NB) {
+            const int hv = it >> 2, c4 = it & 3;
+            const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+            const int gx = ox + hx - 1, gy = oy + hy - 1, gz = oz + hz - 1;
+            uint2 v = make_uint2(0u, 0u);
+            if (gx >= 0 && gx < a.nx && gy >= 0 && gy < a.ny && gz >= 0 && gz < a.nz)
+                v = stage4<MODE, TIN>(in, sk, a.cin, a.nx, a.ny, a.nz, gx, gy, gz, cb * 16 + c4 * 4);
+            *(uint2*)&sIn[hv * LS + c4 * 4] = v;
+        }
+        const uint16_t* ws = a.w + (size_t)(cob * ncb + cb) * (27 * 256);
+        for (int it = tid; it < 27 * 16 * 4; it += NB) {
+            const int row = it >> 2, c4 = it & 3;
+            *(uint2*)&sW[row * LS + c4 * 4] = *(const uint2*)(ws + row * 16 + c4 * 4);
+        }
+        __syncthreads();
+#pragma unroll 3
+        for (int t = 0; t < 27; ++t) {
+            const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+            const int toff = (dz * HY + dy) * HX + dx;
+            const bf16x4 bw = *(const bf16x4*)&sW[(t * 16 + n) * LS + kq * 4];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const bf16x4 av = *(const bf16x4*)&sIn[(base[g] + toff) * LS + kq * 4];
+                acc[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bw, acc[g], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int vi = (wave * G + g) * 16 + kq * 4 + q;
+            const int tx = vi % TX, ty = (vi / TX) % TY, tz = vi / (TX * TY);
+            const int gx = ox + tx, gy = oy + ty, gz = oz + tz;
+            if (gx < a.nx && gy < a.ny && gz < a.nz) {
+                const size_t o = ((size_t)(gz * a.ny + gy) * a.nx + gx) * a.cout + cob * 16 + n;
+                if constexpr (sizeof(TOUT) == 2) {
+                    ((uint16_t*)a.out)[o] = (uint16_t)f2bf(acc[g][q]);
+                } else {
+                    float* o32 = (float*)a.out;
+                    o32[o] = a.accum ? o32[o] + acc[g][q] : acc[g][q];
+                }
+            }
+        }
+    }
+}
+
+struct WgArgs {
+    const uint16_t* x;     // source activation (BF16) of the staged input
+    const uint16_t* skip;
+    const float* g;        // gradient of the layer output, FP32 [v][cout]
+    float* part;           // [chunk][27][cout][cin]
+    int nx, ny, nz, cin, cout;
+    int tilesPerChunk, ntiles;
+};
+
+template <int TX, int TY, int TZ, int MODE>
+__global__ __launch_bounds__(NB) void k_conv_wgrad(WgArgs a) {
+    constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2, HV = HX * HY * HZ;
+    constexpr int TV = TX * TY * TZ, G = TV / 64, GS = TV + 4;
+    __shared__ __align__(16) uint16_t sX[HV * LS];
+    __shared__ __align__(16) uint16_t sG[16 * GS];
+    __shared__ __align__(16) float sR[27 * 256];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int n = lane & 15, kq = lane >> 4;
+    const int ntx = (a.nx + TX - 1) / TX, nty = (a.ny + TY - 1) / TY;
+    const int ncib = a.cin >> 4;
+    const int cob = blockIdx.y / ncib, cib = blockIdx.y % ncib;
+
+    int hb[G][4];
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int vi = (wave * G + g) * 16 + kq * 4 + q;
+            const int tx = vi % TX, ty = (vi / TX) % TY, tz = vi / (TX * TY);
+            hb[g][q] = ((tz + 1) * HY + (ty + 1)) * HX + (tx + 1);
+        }
+    f32x4 acc[27];
+#pragma unroll
+    for (int t = 0; t < 27; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const int t0 = blockIdx.x * a.tilesPerChunk;
+    const int t1 = min(a.ntiles, t0 + a.tilesPerChunk);
+    for (int tile = t0; tile < t1; ++tile) {
+        int b = tile;
+        const int bx = b % ntx;
+        b /= ntx;
+        const int by = b % nty, bz = b / nty;
+        const int ox = bx * TX, oy = by * TY, oz = bz * TZ;
+        __syncthreads();
+        for (int it = tid; it < HV * 4; it += NB) {
+            const int hv = it >> 2, c4 = it & 3;
+            const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+            const int gx = ox + hx - 1, gy = oy + hy - 1, gz = oz + hz - 1;
+            uint2 v = make_uint2(0u, 0u);
+            if (gx >= 0 && gx < a.nx && gy >= 0 && gy < a.ny && gz >= 0 && gz < a.nz)
+                v = stage4<MODE, uint16_t>(a.x, a.skip, a.cin, a.nx, a.ny, a.nz, gx, gy, gz, cib * 16 + c4 * 4);
+            *(uint2*)&sX[hv * LS + c4 * 4] = v;
+        }
+        for (int it = tid; it < TV * 4; it += NB) {
+            const int vi = it >> 2, c4 = it & 3;
+            const int tx = vi % TX, ty = (vi / TX) % TY, tz = vi / (TX * TY);
+            const int gx = ox + tx, gy = oy + ty, gz = oz + tz;
+            float v[4] = {0.f, 0.f, 0.f, 0.f};
+            if (gx < a.nx && gy < a.ny && gz < a.nz)
+                ld4(a.g + ((size_t)(gz * a.ny + gy) * a.nx + gx) * a.cout + cob * 16 + c4 * 4, v);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) sG[(c4 * 4 + q) * GS + vi] = (uint16_t)f2bf(v[q]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const bf16x4 av = *(const bf16x4*)&sG[n * GS + (wave * G + g) * 16 + kq * 4];
+#pragma unroll
+            for (int t = 0; t < 27; ++t) {
+                const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+                const int toff = (dz * HY + dy) * HX + dx;
+                bf16x4 bx4;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) bx4[q] = (short)sX[(hb[g][q] + toff) * LS + n];
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bx4, acc[t], 0, 0, 0);
+            }
+        }
+    }
+    // fixed-order reduction over the four waves
+    for (int w = 1; w < NB / 64; ++w) {
+        __syncthreads();
+        if (wave == w)
+            for (int t = 0; t < 27; ++t)
+                for (int q = 0; q < 4; ++q) sR[(t * 4 + q) * 64 + lane] = acc[t][q];
+        __syncthreads();
+        if (wave == 0)
+            for (int t = 0; t < 27; ++t)
+                for (int q = 0; q < 4; ++q) acc[t][q] += sR[(t * 4 + q) * 64 + lane];
+    }
+    if (wave == 0) {
+        float* dst = a.part + (size_t)blockIdx.x * 27 * a.cout * a.cin;
+        for (int t = 0; t < 27; ++t)
+            for (int q = 0; q < 4; ++q) {
+                const int co = cob * 16 + kq * 4 + q, ci = cib * 16 + n;
+                dst[((size_t)t * a.cout + co) * a.cin + ci] = acc[t][q];
+            }
+    }
+}
+
+// dst[i] (+)= sum over chunks of part[chunk][i], chunks in order
+__global__ void k_reduce_parts(const float* part, int nChunk, int L, float* dst, int accum) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L) return;
+    float s = 0.f;
+    for (int c = 0; c < nChunk; ++c) s += part[(size_t)c * L + i];
+    dst[i] = accum ? dst[i] + s : s;
+}
+
+// ---------------------------------------------------------------- edge layers (FP32)
+__device__ __forceinline__ void vox(int v, int nx, int ny, int& x, int& y, int& z) {
+    x = v % nx;
+    const int r = v / nx;
+    y = r % ny;
+    z = r / ny;
+}
+
+// in: 1 -> 16 channels; W [27][16][1]
+__global__ void k_in_fwd(const float* x, const float* W, uint16_t* out, int nx, int ny, int nz, const double* done) {
+    if (isDone(done)) return;
+    __shared__ float sW[27 * 16];
+    for (int i = threadIdx.x; i < 27 * 16; i += blockDim.x) sW[i] = W[i];
+    __syncthreads();
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nx * ny * nz) return;
+    int i, j, k;
+    vox(v, nx, ny, i, j, k);
+    float acc[16];
+    for (int c = 0; c < 16; ++c) acc[c] = 0.f;
+    for (int t = 0; t < 27; ++t) {
+        const int a = i + t % 3 - 1, b = j + (t / 3) % 3 - 1, cz = k + t / 9 - 1;
+        if (a < 0 || a >= nx || b < 0 || b >= ny || cz < 0 || cz >= nz) continue;
+        const float xv = x[(cz * ny + b) * nx + a];
+        for (int c = 0; c < 16; ++c) acc[c] += sW[t * 16 + c] * xv;
+    }
+    uint32_t w[8];
+    for (int c = 0; c < 8; ++c) w[c] = f2bf(acc[2 * c]) | (f2bf(acc[2 * c + 1]) << 16);
+    uint4* o = (uint4*)(out + (size_t)v * 16);
+    o[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    o[1] = make_uint4(w[4], w[5], w[6], w[7]);
+}
+
+// out: 16 -> 1 channel; W [27][1][16]
+__global__ void k_out_fwd(const uint16_t* a, const float* W, float* y, int nx, int ny, int nz, const double* done) {
+    if (isDone(done)) return;
+    __shared__ float sW[27 * 16];
+    for (int i = threadIdx.x; i < 27 * 16; i += blockDim.x) sW[i] = W[i];
+    __syncthreads();
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nx * ny * nz) return;
+    int i, j, k;
+    vox(v, nx, ny, i, j, k);
+    float s = 0.f;
+    for (int t = 0; t < 27; ++t) {
+        const int ai = i + t % 3 - 1, bj = j + (t / 3) % 3 - 1, ck = k + t / 9 - 1;
+        if (ai < 0 || ai >= nx || bj < 0 || bj >= ny || ck < 0 || ck >= nz) continue;
+        const uint16_t* p = a + ((size_t)(ck * ny + bj) * nx + ai) * 16;
+        for (int c4 = 0; c4 < 4; ++c4) {
+            float v4[4];
+            ld4(p + 4 * c4, v4);
+            for (int q = 0; q < 4; ++q) s += sW[t * 16 + 4 * c4 + q] * v4[q];
+        }
+    }
+    y[v] = s;
+}
+
+// gradient of the out layer's input: ga[v][ci] += sum_t W[t][0][ci] g[v - off(t)]
+__global__ void k_out_ginput(const float* g, const float* W, float* ga, int nx, int ny, int nz) {
+    __shared__ float sW[27 * 16];
+    for (int i = threadIdx.x; i < 27 * 16; i += blockDim.x) sW[i] = W[i];
+    __syncthreads();
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nx * ny * nz) return;
+    int i, j, k;
+    vox(v, nx, ny, i, j, k);
+    float acc[16];
+    for (int c = 0; c < 16; ++c) acc[c] = 0.f;
+    for (int t = 0; t < 27; ++t) {
+        const int ai = i - (t % 3 - 1), bj = j - ((t / 3) % 3 - 1), ck = k - (t / 9 - 1);
+        if (ai < 0 || ai >= nx || bj < 0 || bj >= ny || ck < 0 || ck >= nz) continue;
+        const float gv = g[(ck * ny + bj) * nx + ai];
+        for (int c = 0; c < 16; ++c) acc[c] += sW[t * 16 + c] * gv;
+    }
+    float* o = ga + (size_t)v * 16;
+    for (int c = 0; c < 16; ++c) o[c] += acc[c];
+}
+
+// Weight gradients of the edge layers, one (tap, channel) pair per thread, a chunk of
+// voxels per block:
+//   IN  (x: [v] FP32 input, g: [v][16] FP32):  dW[t][c] = sum_v g[v][c] x[v+off(t)]
+//   OUT (x: [v][16] BF16 input, g: [v] FP32):  dW[t][c] = sum_v g[v] x[v+off(t)][c]
+template <bool IN>
+__global__ void k_edge_wgrad(const void* xv, const float* g, float* part, int nx, int ny, int nz, int vpc) {
+    const int pr = threadIdx.x;
+    if (pr >= 27 * 16) return;
+    const int t = pr / 16, c = pr % 16;
+    const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
+    const int nv = nx * ny * nz;
+    const int v0 = blockIdx.x * vpc, v1 = min(nv, v0 + vpc);
+    float s = 0.f;
+    for (int v = v0; v < v1; ++v) {
+        int i, j, k;
+        vox(v, nx, ny, i, j, k);
+        const int a = i + dx, b = j + dy, cz = k + dz;
+        if (a < 0 || a >= nx || b < 0 || b >= ny || cz < 0 || cz >= nz) continue;
+        const int u = (cz * ny + b) * nx + a;
+        if (IN) s += g[(size_t)v * 16 + c] * ((const float*)xv)[u];
+        else s += g[v] * bf2f(((const uint16_t*)xv)[(size_t)u * 16 + c]);
+    }
+    part[(size_t)blockIdx.x * 432 + pr] = s;
+}
+
+// ---------------------------------------------------------------- adjoints of staging (FP32)
+// g_fine[v][c] += g[v/2][c] / 8   (fine dims 2nx x 2ny x 2nz)
+__global__ void k_pool_adj(const float* g, float* gf, int nx, int ny, int nz, int C) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nf = 8L * nx * ny * nz * C;
+    if (e >= nf) return;
+    const int c = (int)(e % C);
+    const long v = e / C;
+    const int fx = 2 * nx, fy = 2 * ny;
+    const int i = (int)(v % fx), j = (int)((v / fx) % fy), k = (int)(v / ((long)fx * fy));
+    gf[e] += 0.125f * g[((size_t)((k >> 1) * ny + (j >> 1)) * nx + (i >> 1)) * C + c];
+}
+// g_coarse[V][c] += sum of the 8 children of V   (coarse dims nx x ny x nz)
+__global__ void k_up_adj(const float* g, float* gc, int nx, int ny, int nz, int C) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)nx * ny * nz * C) return;
+    const int c = (int)(e % C);
+    const long v = e / C;
+    const int i = (int)(v % nx), j = (int)((v / nx) % ny), k = (int)(v / ((long)nx * ny));
+    const int fx = 2 * nx, fy = 2 * ny;
+    float s = 0.f;
+    for (int dz = 0; dz < 2; ++dz)
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx)
+                s += g[((size_t)((2 * k + dz) * fy + 2 * j + dy) * fx + 2 * i + dx) * C + c];
+    gc[e] += s;
+}
+__global__ void k_addf(const float* a, float* b, long n) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) b[e] += a[e];
+}
+__global__ void k_zerof(float* a, long n) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) a[e] = 0.f;
+}
+
+// ---------------------------------------------------------------- parameters
+// forward packing: dst[cob][cib][t][co16][ci16] = W[t][co][ci]
+// input-gradient packing (flip): layer with cin' = cout, cout' = cin and
+//   W'[t][co' = ci][ci' = co] = W[26 - t][co][ci]
+__global__ void k_pack(const float* W, uint16_t* dst, int cin, int cout, int flip) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int L = 27 * cin * cout;
+    if (e >= L) return;
+    const int ci_ = cin, co_ = cout;
+    // e enumerates the packed layout of the (possibly flipped) layer
+    const int pin = flip ? co_ : ci_, pout = flip ? ci_ : co_;
+    int r = e;
+    const int cil = r % 16; r /= 16;
+    const int col = r % 16; r /= 16;
+    const int t = r % 27; r /= 27;
+    const int ncb = pin / 16;
+    const int cib = r % ncb, cob = r / ncb;
+    const int pco = cob * 16 + col, pci = cib * 16 + cil;
+    float v;
+    if (!flip) v = W[((size_t)t * co_ + pco) * ci_ + pci];
+    else v = W[((size_t)(26 - t) * co_ + pci) * ci_ + pco];
+    (void)pout;
+    dst[e] = (uint16_t)f2bf(v);
+}
+
+__global__ void k_adam(float* th, const float* g, float* m1, float* m2, long n, float lr, float gscale, float b1,
+                       float b2, float c1, float c2, float eps) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i] * gscale;
+    const float a = b1 * m1[i] + (1.f - b1) * gi;
+    const float b = b2 * m2[i] + (1.f - b2) * gi * gi;
+    m1[i] = a;
+    m2[i] = b;
+    th[i] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+}
+
+// ---------------------------------------------------------------- host side
+void* net_alloc(Net& n, size_t bytes, std::vector<void*>& list) {
+    void* p = nullptr;
+    HIPCHECK(hipMalloc(&p, bytes));
+    HIPCHECK(hipMemset(p, 0, bytes));
+    list.push_back(p);
+    return p;
+}
+
+static inline int lvlDim(int n0, int L) { return n0 >> L; }
+
+void net_set_dims(Net& n, int nx, int ny, int nz, bool train) {
+    if (n.nx == nx && n.ny == ny && n.nz == nz && (n.trainBufs || !train)) return;
+    HIPCHECK(hipDeviceSynchronize());
+    for (void* p : n.dimAllocs) (void)hipFree(p);
+    n.dimAllocs.clear();
+    for (int l = 0; l < NET_LAYERS; ++l) { n.act[l] = nullptr; n.gact[l] = nullptr; }
+    n.nx = nx; n.ny = ny; n.nz = nz;
+    n.trainBufs = train;
+    const size_t v0 = (size_t)nx * ny * nz;
+    size_t gst = 0;
+    for (int l = 0; l < NET_LAYERS; ++l) {
+        const LayerDef& d = kNet[l];
+        const size_t vl = (size_t)lvlDim(nx, d.level) * lvlDim(ny, d.level) * lvlDim(nz, d.level);
+        if (l != NET_OUT) {
+            n.act[l] = (uint16_t*)net_alloc(n, vl * d.cout * 2, n.dimAllocs);
+            if (train) n.gact[l] = (float*)net_alloc(n, vl * d.cout * 4, n.dimAllocs);
+        }
+        gst = std::max(gst, vl * d.cin);
+    }
+    n.vin = (float*)net_alloc(n, v0 * 4, n.dimAllocs);
+    n.vout = (float*)net_alloc(n, v0 * 4, n.dimAllocs);
+    n.gout = (float*)net_alloc(n, v0 * 4, n.dimAllocs);
+    if (train) n.gstage = (float*)net_alloc(n, gst * 4, n.dimAllocs);
+}
+
+void net_pack(Net& n, hipStream_t s) {
+    for (int l = 1; l < NET_OUT; ++l) {
+        const LayerDef& d = kNet[l];
+        const int L = 27 * d.cin * d.cout;
+        hipLaunchKernelGGL(k_pack, dim3((L + 255) / 256), dim3(256), 0, s, n.theta + n.off[l], n.wf + n.poff[l], d.cin, d.cout, 0);
+        hipLaunchKernelGGL(k_pack, dim3((L + 255) / 256), dim3(256), 0, s, n.theta + n.off[l], n.wb + n.poff[l], d.cin, d.cout, 1);
+    }
+    n.packed = true;
+}
+
+template <int MODE, class TIN, class TOUT>
+static void convLaunch(hipStream_t s, const ConvArgs& a) {
+    if (a.nx >= 16) {
+        const int nt = ((a.nx + 15) / 16) * ((a.ny + 3) / 4) * ((a.nz + 3) / 4);
+        hipLaunchKernelGGL((k_conv<16, 4, 4, MODE, TIN, TOUT>), dim3(nt, a.cout / 16), dim3(NB), 0, s, a);
+    } else if (a.nx >= 8) {
+        const int nt = ((a.nx + 7) / 8) * ((a.ny + 3) / 4) * ((a.nz + 3) / 4);
+        hipLaunchKernelGGL((k_conv<8, 4, 4, MODE, TIN, TOUT>), dim3(nt, a.cout / 16), dim3(NB), 0, s, a);
+    } else {
+        const int nt = ((a.nx + 3) / 4) * ((a.ny + 3) / 4) * ((a.nz + 3) / 4);
+        hipLaunchKernelGGL((k_conv<4, 4, 4, MODE, TIN, TOUT>), dim3(nt, a.cout / 16), dim3(NB), 0, s, a);
+    }
+}
+
+void net_forward(Net& n, hipStream_t s, const float* vin, float* vout, const double* done) {
+    const int nv0 = n.nx * n.ny * n.nz;
+    hipLaunchKernelGGL(k_in_fwd, dim3((nv0 + 255) / 256), dim3(256), 0, s, vin, n.theta + n.off[0], n.act[0], n.nx, n.ny, n.nz, done);
+    for (int l = 1; l < NET_OUT; ++l) {
+        const LayerDef& d = kNet[l];
+        ConvArgs a;
+        a.in = n.act[d.src];
+        a.skip = d.skip >= 0 ? n.act[d.skip] : nullptr;
+        a.w = n.wf + n.poff[l];
+        a.out = n.act[l];
+        a.nx = lvlDim(n.nx, d.level); a.ny = lvlDim(n.ny, d.level); a.nz = lvlDim(n.nz, d.level);
+        a.cin = d.cin; a.cout = d.cout; a.accum = 0; a.done = done;
+        if (d.mode == MODE_PLAIN) convLaunch<MODE_PLAIN, uint16_t, uint16_t>(s, a);
+        else if (d.mode == MODE_POOL) convLaunch<MODE_POOL, uint16_t, uint16_t>(s, a);
+        else convLaunch<MODE_UPADD, uint16_t, uint16_t>(s, a);
+    }
+    hipLaunchKernelGGL(k_out_fwd, dim3((nv0 + 255) / 256), dim3(256), 0, s, n.act[NET_OUT - 1], n.theta + n.off[NET_OUT], vout, n.nx, n.ny, n.nz, done);
+}
+
+static void ensurePart(Net& n, long need) {
+    if (need <= n.wpartCap) return;
+    HIPCHECK(hipDeviceSynchronize());
+    if (n.wpart) {
+        for (auto it = n.allocs.begin(); it != n.allocs.end(); ++it)
+            if (*it == n.wpart) { n.allocs.erase(it); break; }
+        (void)hipFree(n.wpart);
+    }
+    n.wpart = (float*)net_alloc(n, need * sizeof(float), n.allocs);
+    n.wpartCap = need;
+}
+
+template <int MODE>
+static void wgradLaunch(Net& n, hipStream_t s, WgArgs a, float* dst) {
+    int TX = a.nx >= 16 ? 16 : (a.nx >= 8 ? 8 : 4);
+    const int nt = ((a.nx + TX - 1) / TX) * ((a.ny + 3) / 4) * ((a.nz + 3) / 4);
+    const int pairs = (a.cout / 16) * (a.cin / 16);
+    int nChunk = std::max(1, std::min(nt, 512 / pairs));
+    a.tilesPerChunk = (nt + nChunk - 1) / nChunk;
+    nChunk = (nt + a.tilesPerChunk - 1) / a.tilesPerChunk;
+    a.ntiles = nt;
+    const long L = 27L * a.cin * a.cout;
+    ensurePart(n, L * nChunk);
+    a.part = n.wpart;
+    if (TX == 16) hipLaunchKernelGGL((k_conv_wgrad<16, 4, 4, MODE>), dim3(nChunk, pairs), dim3(NB), 0, s, a);
+    else if (TX == 8) hipLaunchKernelGGL((k_conv_wgrad<8, 4, 4, MODE>), dim3(nChunk, pairs), dim3(NB), 0, s, a);
+    else hipLaunchKernelGGL((k_conv_wgrad<4, 4, 4, MODE>), dim3(nChunk, pairs), dim3(NB), 0, s, a);
+    hipLaunchKernelGGL(k_reduce_parts, dim3((L + 255) / 256), dim3(256), 0, s, n.wpart, nChunk, (int)L, dst, 1);
+}
+
+static void edgeWgrad(Net& n, hipStream_t s, bool in, const void* x, const float* g, float* dst) {
+    const int nv = n.nx * n.ny * n.nz;
+    const int vpc = 256;
+    const int nChunk = (nv + vpc - 1) / vpc;
+    ensurePart(n, 432L * nChunk);
+    if (in) hipLaunchKernelGGL(k_edge_wgrad<true>, dim3(nChunk), dim3(448), 0, s, x, g, n.wpart, n.nx, n.ny, n.nz, vpc);
+    else hipLaunchKernelGGL(k_edge_wgrad<false>, dim3(nChunk), dim3(448), 0, s, x, g, n.wpart, n.nx, n.ny, n.nz, vpc);
+    hipLaunchKernelGGL(k_reduce_parts, dim3((432 + 255) / 256), dim3(256), 0, s, n.wpart, nChunk, 432, dst, 1);
+}
+
+void net_backward(Net& n, hipStream_t s, const float* vin, const float* gout) {
+    // zero the activation gradients
+    for (int l = 0; l < NET_OUT; ++l) {
+        const LayerDef& d = kNet[l];
+        const long e = (long)lvlDim(n.nx, d.level) * lvlDim(n.ny, d.level) * lvlDim(n.nz, d.level) * d.cout;
+        hipLaunchKernelGGL(k_zerof, dim3((e + 255) / 256), dim3(256), 0, s, n.gact[l], e);
+    }
+    const int nv0 = n.nx * n.ny * n.nz;
+    // out layer
+    edgeWgrad(n, s, false, n.act[NET_OUT - 1], gout, n.grad + n.off[NET_OUT]);
+    hipLaunchKernelGGL(k_out_ginput, dim3((nv0 + 255) / 256), dim3(256), 0, s, gout, n.theta + n.off[NET_OUT], n.gact[NET_OUT - 1], n.nx, n.ny, n.nz);
+    for (int l = NET_OUT - 1; l >= 1; --l) {
+        const LayerDef& d = kNet[l];
+        const int lx = lvlDim(n.nx, d.level), ly = lvlDim(n.ny, d.level), lz = lvlDim(n.nz, d.level);
+        WgArgs w;
+        w.x = n.act[d.src];
+        w.skip = d.skip >= 0 ? n.act[d.skip] : nullptr;
+        w.g = n.gact[l];
+        w.nx = lx; w.ny = ly; w.nz = lz; w.cin = d.cin; w.cout = d.cout;
+        float* dst = n.grad + n.off[l];
+        if (d.mode == MODE_PLAIN) wgradLaunch<MODE_PLAIN>(n, s, w, dst);
+        else if (d.mode == MODE_POOL) wgradLaunch<MODE_POOL>(n, s, w, dst);
+        else wgradLaunch<MODE_UPADD>(n, s, w, dst);
+        // gradient of the staged input: the conv with flipped taps and swapped channels
+        ConvArgs a;
+        a.in = n.gact[l];
+        a.skip = nullptr;
+        a.w = n.wb + n.poff[l];
+        a.nx = lx; a.ny = ly; a.nz = lz; a.cin = d.cout; a.cout = d.cin; a.done = nullptr;
+        if (d.mode == MODE_PLAIN) {
+            a.out = n.gact[d.src];
+            a.accum = 1;
+            convLaunch<MODE_PLAIN, float, float>(s, a);
+        } else {
+            a.out = n.gstage;
+            a.accum = 0;
+            convLaunch<MODE_PLAIN, float, float>(s, a);
+            const long e = (long)lx * ly * lz * d.cin;
+            if (d.mode == MODE_POOL) {
+                hipLaunchKernelGGL(k_pool_adj, dim3((8 * e + 255) / 256), dim3(256), 0, s, n.gstage, n.gact[d.src], lx, ly, lz, d.cin);
+            } else {
+                hipLaunchKernelGGL(k_up_adj, dim3((e / 8 + 255) / 256), dim3(256), 0, s, n.gstage, n.gact[d.src], lx / 2, ly / 2, lz / 2, d.cin);
+                hipLaunchKernelGGL(k_addf, dim3((e + 255) / 256), dim3(256), 0, s, n.gstage, n.gact[d.skip], e);
+            }
+        }
+    }
+    edgeWgrad(n, s, true, vin, n.gact[NET_IN], n.grad + n.off[NET_IN]);
+}
+
+void net_adam(Net& n, hipStream_t s, float lr, float gscale) {
+    const float b1 = 0.9f, b2 = 0.999f;
+    ++n.adamT;
+    const float c1 = 1.f - std::pow(b1, (float)n.adamT), c2 = 1.f - std::pow(b2, (float)n.adamT);
+    hipLaunchKernelGGL(k_adam, dim3((n.nParams + 255) / 256), dim3(256), 0, s, n.theta, n.grad, n.m1, n.m2, n.nParams, lr,
+                       gscale, b1, b2, c1, c2, 1e-8f);
+    net_pack(n, s);
+}
+
+}  // namespace caffe

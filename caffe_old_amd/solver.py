@@ -69,6 +69,9 @@ class Solver:
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
+            if getattr(self, "_net", None) is not None:
+                self.lib.caffe_ctx_attach_net(self._h, None)
+                self._net = None
             self.lib.caffe_ctx_destroy(self._h)
             self._h = C.c_void_p()
 
@@ -77,6 +80,12 @@ class Solver:
             self.close()
         except Exception:
             pass
+
+    def attach_net(self, net):
+        """Use a trained ``caffe_old_amd.learned.PressureNet`` in SOLVE_LEARNED_FCG solves
+        (None detaches).  The solver keeps a reference so the network outlives it."""
+        self._check(self.lib.caffe_ctx_attach_net(self._h, net._h if net is not None else None), "caffe_ctx_attach_net")
+        self._net = net
 
     def set_solve_opts(self, which: str, mode: int = L.SOLVE_PLAIN_FP64, rtol: float = 1e-8, maxit: int = 50000,
                        inner_rtol: float = 1e-4):

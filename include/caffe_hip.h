@@ -58,7 +58,9 @@ enum { CAFFE_SIMPLE = 0, CAFFE_PISO = 1, CAFFE_IBPISO = 2 };
 enum {
     CAFFE_SOLVE_PLAIN_FP64 = 0,   /* preconditioned Krylov in FP64 (the judge and fallback) */
     CAFFE_SOLVE_IR_FP32 = 1,      /* FP32 inner Krylov inside FP64 iterative refinement */
-    CAFFE_SOLVE_LEARNED_FCG = 2   /* learned initial guess + IR_FP32; accepted only at rtol */
+    CAFFE_SOLVE_LEARNED_FCG = 2   /* flexible CG with the learned two-grid cycle (FP32 inside
+                                     FP64 refinement); accepted only at rtol, else the plain
+                                     solve runs (fell_back) */
 };
 
 /* The keys of case/case.info and case/initialConditions.info the hot path reads
@@ -146,6 +148,51 @@ int caffe_ghost_rows(caffe_ctx* ctx, int* n, long* cells, long* cols, double* va
 int caffe_time_kernel(caffe_ctx* ctx, const char* which, int reps, double* ms, double* bytes);
 int caffe_sync(caffe_ctx* ctx);
 long caffe_padded_size(caffe_ctx* ctx);
+
+/* ---------------------------------------------------------------------------------
+ * Learned multilevel preconditioner of the pressure-correction solve.  No counterpart in
+ * the reference: there the p' system goes to PETSc BiCGStab + SOR (SparseMatrix.cpp:86-106)
+ * and the learned stage replaces that preconditioner; the plain FP64 solve stays the
+ * fallback and judge.
+ *
+ * Cycle for a residual r (fine grid, dimensions divisible by 16):
+ *   z1 = w D^-1 r;  c = net(P^T D^-1 (r - A z1));  y = z1 + P c;  z = y + w D^-1 (r - A y)
+ * with w = 0.8, P the cell-centred trilinear prolongation to the fine grid, and net a
+ * linear network of twelve 3x3x3 convolutions over four levels (coarse grid, /2, /4, /8;
+ * 16/32/64/64 channels), BF16 on MFMA with FP32 accumulation.
+ * Parameters: FP32, layers in order, each [27][cout][cin] (tap (dz+1)*9+(dy+1)*3+(dx+1)).
+ * Coarse fields are [K][J][I] (I fastest); fine fields and matrices use the padded layout
+ * of the context (matrix = 4*P floats: aP, aI, aJ, aK as caffe_matrix_download which 1).
+ * ------------------------------------------------------------------------------- */
+typedef struct caffe_net caffe_net;
+int caffe_net_create(int device, unsigned long long seed, caffe_net** out);
+int caffe_net_destroy(caffe_net* net);
+const char* caffe_net_last_error(caffe_net* net);
+long caffe_net_param_count(void);
+int caffe_net_get_params(caffe_net* net, float* host, long n);
+int caffe_net_set_params(caffe_net* net, const float* host, long n);
+/* the network alone on a coarse grid (dimensions multiples of 8): out = net(in);
+ * backward: gparams = d<gout, net(in)>/dtheta */
+int caffe_net_forward(caffe_net* net, int nx, int ny, int nz, const float* in, float* out);
+int caffe_net_backward(caffe_net* net, int nx, int ny, int nz, const float* in, const float* gout, float* gparams);
+/* z = cycle(r) on the fine grid for the FP32 matrix A */
+int caffe_net_precond(caffe_net* net, int nI, int nJ, int nK, const float* A, const float* r, float* z);
+/* training loss of one pair: L = e^T A e / x^T A x, e = x - cycle(b); gparams may be NULL */
+int caffe_net_loss_grad(caffe_net* net, int nI, int nJ, int nK, const float* A, const float* b, const float* x,
+                        double* loss, float* gparams);
+/* append the context's current p' system (matrix, right-hand side, converged correction;
+ * call after caffe_pcorr_solve) to the device-resident training set */
+int caffe_net_collect(caffe_net* net, caffe_ctx* ctx, long* count);
+int caffe_net_clear(caffe_net* net);
+long caffe_net_dataset_size(caffe_net* net);
+/* `steps` Adam steps over random mini-batches of `batch` pairs, cosine-decayed rate from lr;
+ * losses[steps] (may be NULL) receives the mean batch loss of each step */
+int caffe_net_train(caffe_net* net, int steps, int batch, float lr, unsigned long long seed, float* losses);
+/* loss of every stored pair under the current parameters */
+int caffe_net_eval(caffe_net* net, float* losses, long n);
+/* use the network in CAFFE_SOLVE_LEARNED_FCG solves of ctx (NULL detaches); the network
+ * must outlive the attachment */
+int caffe_ctx_attach_net(caffe_ctx* ctx, caffe_net* net);
 
 #ifdef __cplusplus
 }
