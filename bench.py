@@ -69,13 +69,21 @@ def main():
 
     from caffe_old_amd import _lib as L
     from caffe_old_amd import case as cs
+    from caffe_old_amd.learned import PressureNet, trained_weights_path
     from caffe_old_amd.solver import make_solver
 
     mode = {"ir": L.SOLVE_IR_FP32, "plain": L.SOLVE_PLAIN_FP64, "learned": L.SOLVE_LEARNED_FCG}[a.mode]
+    inner = {"ir": 1e-2, "plain": 1e-2, "learned": 1e-4}
     case = cs.cavity_piso_128(a.n, steps=a.steps + a.warmup)
     s = make_solver(case)
     s.set_solve_opts("momentum", rtol=1e-8, maxit=5000)
-    s.set_solve_opts("pcorr", mode=mode, rtol=1e-8, maxit=100000, inner_rtol=1e-2)
+    s.set_solve_opts("pcorr", mode=mode, rtol=1e-8, maxit=100000, inner_rtol=inner[a.mode])
+    # the trained learned preconditioner for this grid (trained by caffe_old_amd.train_pcorr_net)
+    net = None
+    wpath = trained_weights_path(a.n)
+    if os.path.exists(wpath):
+        net = PressureNet(seed=0).load(wpath)
+        s.attach_net(net)
 
     def log(*x):
         if not a.quiet:
@@ -107,9 +115,10 @@ def main():
 
     # pressure-correction solve: plain FP64 vs FP32-inner / FP64-refinement, same system
     psolve = {}
-    for name, md in (("plain_fp64", L.SOLVE_PLAIN_FP64), ("ir_fp32", L.SOLVE_IR_FP32), ("learned_fcg", L.SOLVE_LEARNED_FCG)):
+    for name, md, inr in (("plain_fp64", L.SOLVE_PLAIN_FP64, 1e-2), ("ir_fp32", L.SOLVE_IR_FP32, 1e-2),
+                          ("learned_fcg", L.SOLVE_LEARNED_FCG, 1e-4)):
         s.pcorr_assemble()
-        r = s.pcorr_solve(mode=md, rtol=1e-8, maxit=100000, inner_rtol=1e-2)
+        r = s.pcorr_solve(mode=md, rtol=1e-8, maxit=100000, inner_rtol=inr)
         psolve[name] = {"ms": round(r["ms"], 3), "iters": r["iters"], "refinements": r["refinements"],
                         "relres": r["relres"], "fell_back": r["fell_back"]}
     log("p' solves:", psolve)

@@ -121,7 +121,7 @@ def load(path: str = LIB_PATH):
         "caffe_net_collect": ([vp, vp, lp], C.c_int),
         "caffe_net_clear": ([vp], C.c_int),
         "caffe_net_dataset_size": ([vp], C.c_long),
-        "caffe_net_train": ([vp, C.c_int, C.c_int, C.c_float, C.c_ulonglong, fp], C.c_int),
+        "caffe_net_train": ([vp, C.c_int, C.c_int, C.c_float, C.c_double, C.c_ulonglong, fp], C.c_int),
         "caffe_net_eval": ([vp, fp, C.c_long], C.c_int),
         "caffe_ctx_attach_net": ([vp, vp], C.c_int),
     }

@@ -148,10 +148,10 @@ __global__ void k_l_spmv(Dims D, SymF S, const float* e, float* Ae, double* part
     }
     writePartials<FB, 1>(v, part);
 }
-__global__ void k_l_loss(const double* part, int nb, double nrmInv, float* loss) {
+__global__ void k_l_loss(const double* part, int nb, const double* nrm, float* loss) {
     double t[1];
     readPartials<FB, 1>(part, nb, t);
-    if (threadIdx.x == 0) *loss += (float)(t[0] * nrmInv);
+    if (threadIdx.x == 0) *loss += (float)(t[0] / *nrm);
 }
 __global__ void k_l_final(const double* part, int nb, double* out) {
     double t[1];
@@ -159,13 +159,38 @@ __global__ void k_l_final(const double* part, int nb, double* out) {
     if (threadIdx.x == 0) *out = t[0];
 }
 // g = s Ae (s = -2 / x^T A x);  h = g - w A (D^-1 g)   (dL/dy = (I - w A D^-1) dL/dz)
-__global__ void k_l_h(Dims D, SymF S, const float* Ae, float s, float* h) {
+__global__ void k_l_h(Dims D, SymF S, const float* Ae, const double* nrm, float* h) {
+    const float s = (float)(-2.0 / *nrm);
     auto gi = [&](int c) { return S.inv(c) * s * Ae[c]; };
     FOR_FINE(D, q, p) { h[p] = s * Ae[p] - NET_OMEGA * S.apply(gi, p); }
 }
 
+// Synthetic exact pair: x = sum of Neumann cosine modes (eigenvectors of the constant-
+// coefficient Laplacian on the box), b = A x computed by the caller.
+constexpr int N_MODES = 8;
+struct Modes {
+    int k[N_MODES][3];
+    float a[N_MODES];
+};
+__global__ void k_modes(Dims D, Modes m, float* x) {
+    const float pi = 3.14159265358979f;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < D.N; q += gridDim.x * blockDim.x) {
+        int i, j, k;
+        const int p = D.cell(q, i, j, k);
+        float s = 0.f;
+        for (int t = 0; t < N_MODES; ++t)
+            s += m.a[t] * cosf(pi * m.k[t][0] * (i + 0.5f) / D.nI) * cosf(pi * m.k[t][1] * (j + 0.5f) / D.nJ)
+                 * cosf(pi * m.k[t][2] * (k + 0.5f) / D.nK);
+        x[p] = s;
+    }
+}
+__global__ void k_spmv_f(Dims D, SymF S, const float* x, float* y) {
+    auto xf = [&](int c) { return x[c]; };
+    FOR_FINE(D, q, p) { y[p] = S.apply(xf, p); }
+}
+
 // ---------------------------------------------------------------- host: cycle and loss
-enum { FV_R = 0, FV_E, FV_W, FV_Y, FV_Z, FV_P0, FV_P1, FV_Q0, FV_Q1, FV_H, FV_AE, N_FV = 12 };
+enum { FV_R = 0, FV_E, FV_W, FV_Y, FV_Z, FV_P0, FV_P1, FV_Q0, FV_Q1, FV_H, FV_AE, FV_XS, FV_BS, N_FV };
 
 void tg_ensure_fine(Net& n, const Dims& D) {
     if (n.fineP == D.P && n.fw) return;
@@ -195,26 +220,46 @@ void tg_apply(Net& n, hipStream_t s, const Dims& D, const float* A, const float*
     hipLaunchKernelGGL(k_tg_post2, dim3(g), dim3(FB), 0, s, D, S, r, y, z, (const float*)nullptr);
 }
 
-void tg_loss(Net& n, hipStream_t s, const Sample& smp, bool withGrad, float* lossDev) {
-    const Dims& D = smp.D;
+// loss of the pair (A, b, x): x^T A x first (device scalar), then the cycle on b, the
+// error e = x - cycle(b), e^T A e; with the gradient, dL/dc = P^T (I - w A D^-1)(-2 A e / x^T A x)
+static void lossCore(Net& n, hipStream_t s, const Dims& D, const float* A, const float* b, const float* x, bool withGrad,
+                     float* lossDev) {
     float* w = n.fw + (size_t)FV_W * D.P;
     float* y = n.fw + (size_t)FV_Y * D.P;
     float* e = n.fw + (size_t)FV_E * D.P;
     float* Ae = n.fw + (size_t)FV_AE * D.P;
     float* h = n.fw + (size_t)FV_H * D.P;
-    const SymF S = symF(smp.A, D);
+    double* nrm = n.scal + 16;
+    const SymF S = symF(A, D);
     const int g = fgrid(D);
-    hipLaunchKernelGGL(k_tg_pre, dim3(g), dim3(FB), 0, s, D, S, smp.b, w, (const double*)nullptr);
+    hipLaunchKernelGGL(k_l_spmv, dim3(g), dim3(FB), 0, s, D, S, x, Ae, n.part);
+    hipLaunchKernelGGL(k_l_final, dim3(1), dim3(FB), 0, s, n.part, g, nrm);
+    hipLaunchKernelGGL(k_tg_pre, dim3(g), dim3(FB), 0, s, D, S, b, w, (const double*)nullptr);
     restrictLaunch(s, D, w, n.vin, nullptr);
     net_forward(n, s, n.vin, n.vout, nullptr);
-    hipLaunchKernelGGL(k_tg_post1, dim3(g), dim3(FB), 0, s, D, S, smp.b, n.vout, y, (const double*)nullptr);
-    hipLaunchKernelGGL(k_tg_post2, dim3(g), dim3(FB), 0, s, D, S, smp.b, y, e, (const float*)smp.x);
+    hipLaunchKernelGGL(k_tg_post1, dim3(g), dim3(FB), 0, s, D, S, b, n.vout, y, (const double*)nullptr);
+    hipLaunchKernelGGL(k_tg_post2, dim3(g), dim3(FB), 0, s, D, S, b, y, e, x);
     hipLaunchKernelGGL(k_l_spmv, dim3(g), dim3(FB), 0, s, D, S, e, Ae, n.part);
-    hipLaunchKernelGGL(k_l_loss, dim3(1), dim3(FB), 0, s, n.part, g, 1.0 / smp.nrm, lossDev);
+    hipLaunchKernelGGL(k_l_loss, dim3(1), dim3(FB), 0, s, n.part, g, (const double*)nrm, lossDev);
     if (!withGrad) return;
-    hipLaunchKernelGGL(k_l_h, dim3(g), dim3(FB), 0, s, D, S, Ae, (float)(-2.0 / smp.nrm), h);
+    hipLaunchKernelGGL(k_l_h, dim3(g), dim3(FB), 0, s, D, S, Ae, (const double*)nrm, h);
     restrictLaunch(s, D, h, n.gout, nullptr);
     net_backward(n, s, n.vin, n.gout);
+}
+
+void tg_loss(Net& n, hipStream_t s, const Sample& smp, bool withGrad, float* lossDev) {
+    lossCore(n, s, smp.D, smp.A, smp.b, smp.x, withGrad, lossDev);
+}
+
+// loss of a synthetic exact pair on the matrix of `smp`: x from `m`, b = A x
+void tg_loss_synthetic(Net& n, hipStream_t s, const Sample& smp, const Modes& m, bool withGrad, float* lossDev) {
+    const Dims& D = smp.D;
+    float* x = n.fw + (size_t)FV_XS * D.P;
+    float* b = n.fw + (size_t)FV_BS * D.P;
+    const int g = fgrid(D);
+    hipLaunchKernelGGL(k_modes, dim3(g), dim3(FB), 0, s, D, m, x);
+    hipLaunchKernelGGL(k_spmv_f, dim3(g), dim3(FB), 0, s, D, symF(smp.A, D), (const float*)x, b);
+    lossCore(n, s, D, smp.A, b, x, withGrad, lossDev);
 }
 
 // x^T A x of a sample
@@ -374,9 +419,14 @@ int learned_fcg(Ctx& c, double shift, double rtol, int maxit, double delta, int*
     int its = 0;
     FcgCtl lc{delta * delta, maxit, c.singularP ? (double)D.N : 0.};
     while (rel > rtol && its < maxit && *refs < 40) {
+        // the inner solve gains `delta`, or only what the last round still needs
+        const double need = std::max(delta, 0.5 * rtol / rel);
+        lc.tol2 = need * need;
         const double scale = 1.0 / rr;
         hipLaunchKernelGGL(k_f_init, dim3(g), dim3(FB), 0, s, D, (const double*)(c.wp + 6 * P), scale, r, e, partC, st);
-        lc.maxit = maxit - its;
+        // a refinement round that needs more than 300 iterations means the cycle does not
+        // fit this system: the round ends there and the progress test below decides
+        lc.maxit = std::min(maxit - its, 300);
         int chunk = 4, it = 0;
         while (true) {
             for (int k = 0; k < chunk; ++k) {
@@ -400,7 +450,7 @@ int learned_fcg(Ctx& c, double shift, double rtol, int maxit, double delta, int*
         pcorr_true_resid(c, shift, &rr, &bb);
         rel = rr / bb;
         ++*refs;
-        if (!(rel < prev) || it == 0) break;
+        if (!(rel < 0.5 * prev) || it == 0) break;
     }
     *relOut = rel;
     return its;
@@ -441,6 +491,24 @@ struct Rng {
     double normal() { return std::sqrt(-2.0 * std::log(uni())) * std::cos(6.283185307179586 * uni()); }
 };
 
+// A synthetic pair's modes: one band per pair, kmax = 2^B with B uniform in
+// [0, log2(n/2)] (up to the coarse grid's highest mode), wave numbers uniform in
+// [0, kmax] per direction, N(0,1) amplitudes.  Each band weighs the same in the loss.
+Modes drawModes(Rng& r, const Dims& D) {
+    const int nmin = std::min(D.nI, std::min(D.nJ, D.nK));
+    int lmax = 0;
+    while ((2 << lmax) <= nmin / 2) ++lmax;
+    const int B = (int)(r.next() % (uint64_t)(lmax + 1));
+    const int kmax = 1 << B;
+    Modes m;
+    for (int t = 0; t < N_MODES; ++t) {
+        for (int d = 0; d < 3; ++d) m.k[t][d] = (int)(r.next() % (uint64_t)(kmax + 1));
+        if (m.k[t][0] == 0 && m.k[t][1] == 0 && m.k[t][2] == 0) m.k[t][r.next() % 3] = 1;
+        m.a[t] = (float)r.normal();
+    }
+    return m;
+}
+
 void freeNet(Net& n) {
     (void)hipDeviceSynchronize();
     for (void* p : n.dimAllocs) (void)hipFree(p);
@@ -470,6 +538,11 @@ __global__ void k_collect(Dims D, const double* Ap, const double* bp, const doub
     b[p] = live ? (float)((bp[p] - shift) * s) : 0.f;
     x[p] = live ? (float)(xp[p] * s) : 0.f;
 }
+__global__ void k_negate(float* a, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = -a[i];
+}
+
 // sum and count of b over the cells that carry a row
 __global__ void k_sum_live(Dims D, const double* b, const int8_t* st, double* part) {
     double v[2] = {0., 0.};
@@ -735,7 +808,16 @@ int caffe_net_collect(caffe_net* x, caffe_ctx* cx, long* count) {
         HIPCHECK(hipStreamSynchronize(s));
         tg_ensure_fine(n, D);
         smp.nrm = sampleNorm(n, n.stream, smp);
-        if (!(smp.nrm > 0.) || !std::isfinite(smp.nrm)) { freeSample(smp); n.err = "x^T A x is not positive"; return CAFFE_ERR_STATE; }
+        // the assembled p' matrix has aP = -(sum of the face coefficients) (negative
+        // semi-definite); the pair is stored as (-A, -b), which has the same solution and
+        // a positive energy norm.  The cycle is unchanged by negating A and r together,
+        // so the solver applies it to the assembled system as it stands.
+        if (smp.nrm < 0.) {
+            hipLaunchKernelGGL(k_negate, dim3((4 * D.P + 255) / 256), dim3(256), 0, n.stream, smp.A, 4 * D.P);
+            hipLaunchKernelGGL(k_negate, dim3((D.P + 255) / 256), dim3(256), 0, n.stream, smp.b, D.P);
+            smp.nrm = -smp.nrm;
+        }
+        if (!(smp.nrm > 0.) || !std::isfinite(smp.nrm)) { freeSample(smp); n.err = "x^T A x vanishes"; return CAFFE_ERR_STATE; }
         n.data.push_back(smp);
         if (count) *count = (long)n.data.size();
         return CAFFE_OK;
@@ -753,9 +835,13 @@ int caffe_net_clear(caffe_net* x) {
 
 long caffe_net_dataset_size(caffe_net* x) { return x ? (long)x->n.data.size() : -1; }
 
-int caffe_net_train(caffe_net* x, int steps, int batch, float lr, unsigned long long seed, float* losses) {
+int caffe_net_train(caffe_net* x, int steps, int batch, float lr, double synthetic, unsigned long long seed,
+                    float* losses) {
     return nguard(x, [&](Net& n) {
-        if (steps < 1 || batch < 1 || !(lr > 0.f)) { n.err = "bad training arguments"; return CAFFE_ERR_ARG; }
+        if (steps < 1 || batch < 1 || !(lr > 0.f) || !(synthetic >= 0. && synthetic <= 1.)) {
+            n.err = "bad training arguments";
+            return CAFFE_ERR_ARG;
+        }
         if (n.data.empty()) { n.err = "empty training set"; return CAFFE_ERR_STATE; }
         const Dims D = n.data[0].D;
         for (auto& smp : n.data)
@@ -771,13 +857,13 @@ int caffe_net_train(caffe_net* x, int steps, int batch, float lr, unsigned long 
         Rng rng{seed + 0x51ed2701ULL};
         const size_t ns = n.data.size();
         for (int t = 0; t < steps; ++t) {
-            const float lrt = (float)(lr * 0.5 * (1.0 + std::cos(3.141592653589793 * t / steps)));
             HIPCHECK(hipMemsetAsync(n.grad, 0, n.nParams * 4, s));
             for (int b = 0; b < batch; ++b) {
                 const Sample& smp = n.data[rng.next() % ns];
-                tg_loss(n, s, smp, true, n.losses + t);
+                if (rng.uni() < synthetic) tg_loss_synthetic(n, s, smp, drawModes(rng, D), true, n.losses + t);
+                else tg_loss(n, s, smp, true, n.losses + t);
             }
-            net_adam(n, s, lrt, 1.0f / batch);
+            net_adam(n, s, lr, 1.0f / batch);
         }
         std::vector<float> h(steps);
         HIPCHECK(hipMemcpyAsync(h.data(), n.losses, (size_t)steps * 4, hipMemcpyDeviceToHost, s));

@@ -127,9 +127,11 @@ class PressureNet:
     def dataset_size(self) -> int:
         return int(self.lib.caffe_net_dataset_size(self._h))
 
-    def train(self, steps: int, batch: int = 4, lr: float = 1e-3, seed: int = 0) -> np.ndarray:
+    def train(self, steps: int, batch: int = 4, lr: float = 1e-3, seed: int = 0, synthetic: float = 0.5) -> np.ndarray:
+        """`steps` Adam steps at rate lr; each batch entry is a stored pair or, with
+        probability `synthetic`, an exact synthetic pair (cosine modes, b = A x)."""
         losses = np.zeros(steps, dtype=np.float32)
-        self._check(self.lib.caffe_net_train(self._h, steps, batch, lr, seed, _fp(losses)), "caffe_net_train")
+        self._check(self.lib.caffe_net_train(self._h, steps, batch, lr, synthetic, seed, _fp(losses)), "caffe_net_train")
         return losses
 
     def eval(self) -> np.ndarray:

@@ -185,9 +185,15 @@ int caffe_net_loss_grad(caffe_net* net, int nI, int nJ, int nK, const float* A, 
 int caffe_net_collect(caffe_net* net, caffe_ctx* ctx, long* count);
 int caffe_net_clear(caffe_net* net);
 long caffe_net_dataset_size(caffe_net* net);
-/* `steps` Adam steps over random mini-batches of `batch` pairs, cosine-decayed rate from lr;
- * losses[steps] (may be NULL) receives the mean batch loss of each step */
-int caffe_net_train(caffe_net* net, int steps, int batch, float lr, unsigned long long seed, float* losses);
+/* `steps` Adam steps at rate lr over random mini-batches of `batch` pairs (Adam moments
+ * persist across calls, so a caller can schedule the rate call by call).  Each batch
+ * entry is, with probability `synthetic`, an exact synthetic pair on the matrix of a
+ * random stored system: x = a random sum of Neumann cosine modes from one frequency band,
+ * b = A x (the stored right-hand sides cover only the bands the flow excites, while a
+ * preconditioner meets every band in the Krylov residuals).  losses[steps] (may be NULL)
+ * receives the mean batch loss of each step. */
+int caffe_net_train(caffe_net* net, int steps, int batch, float lr, double synthetic, unsigned long long seed,
+                    float* losses);
 /* loss of every stored pair under the current parameters */
 int caffe_net_eval(caffe_net* net, float* losses, long n);
 /* use the network in CAFFE_SOLVE_LEARNED_FCG solves of ctx (NULL detaches); the network
