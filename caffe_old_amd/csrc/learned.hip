@@ -62,19 +62,19 @@ __global__ void k_tg_pre(Dims D, SymF S, const float* r, float* w, const double*
     FOR_FINE(D, q, p) { w[p] = S.inv(p) * (r[p] - S.apply(z1, p)); }
 }
 
-// v = P^T w on the coarse grid (nI/2 x nJ/2 x nK/2), fine w padded
+// v = P^T w on the coarse grid (nI/2 x nJ/2 x nK/2), fine w padded.  Coarse cell C
+// gathers fine cells 2C-1 .. 2C+2 in each direction; the weight of fine f is 3/4 when
+// f>>1 == C plus 1/4 when its other neighbour is C.  The halo of w is zero, so fine
+// cells just outside the grid (weight 0) are read through the padded layout unmasked.
 __global__ void k_tg_restrict(Dims D, const float* w, float* v, const double* done) {
     if (fdone(done)) return;
     const int cx = D.nI >> 1, cy = D.nJ >> 1, cz = D.nK >> 1;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= cx * cy * cz) return;
     const int I = e % cx, J = (e / cx) % cy, K = e / (cx * cy);
-    float wx[4], wy[4], wz[4];
-    int fx[4], fy[4], fz[4];
-    auto weights = [](int C, int nc, int nf, int* f, float* wt) {
+    auto weights = [](int C, int nc, int nf, float* wt) {
         for (int m = 0; m < 4; ++m) {
             const int ff = 2 * C - 1 + m;
-            f[m] = ff;
             float s = 0.f;
             if (ff >= 0 && ff < nf) {
                 int c0, c1;
@@ -85,20 +85,22 @@ __global__ void k_tg_restrict(Dims D, const float* w, float* v, const double* do
             wt[m] = s;
         }
     };
-    weights(I, cx, D.nI, fx, wx);
-    weights(J, cy, D.nJ, fy, wy);
-    weights(K, cz, D.nK, fz, wz);
+    float wx[4], wy[4], wz[4];
+    weights(I, cx, D.nI, wx);
+    weights(J, cy, D.nJ, wy);
+    weights(K, cz, D.nK, wz);
+    // padded index of fine cell (2I-1, 2J-1, 2K-1): lies in the halo at the low faces
+    const int p0 = (2 * I) + (2 * J) * D.sJ + (2 * K) * D.sK;
     float s = 0.f;
+#pragma unroll
     for (int c = 0; c < 4; ++c) {
-        if (wz[c] == 0.f) continue;
+        float sy = 0.f;
+#pragma unroll
         for (int b = 0; b < 4; ++b) {
-            if (wy[b] == 0.f) continue;
-            const float wzy = wz[c] * wy[b];
-            for (int a = 0; a < 4; ++a) {
-                if (wx[a] == 0.f) continue;
-                s += wzy * wx[a] * w[D.pidx(fx[a], fy[b], fz[c])];
-            }
+            const float* row = w + p0 + c * D.sK + b * D.sJ;
+            sy += wy[b] * (wx[0] * row[0] + wx[1] * row[1] + wx[2] * row[2] + wx[3] * row[3]);
         }
+        s += wz[c] * sy;
     }
     v[e] = s;
 }
@@ -418,6 +420,7 @@ int learned_fcg(Ctx& c, double shift, double rtol, int maxit, double delta, int*
     double rel = bb > 0. ? rr / bb : 0.;
     int its = 0;
     FcgCtl lc{delta * delta, maxit, c.singularP ? (double)D.N : 0.};
+    int lastRound = 0;
     while (rel > rtol && its < maxit && *refs < 40) {
         // the inner solve gains `delta`, or only what the last round still needs
         const double need = std::max(delta, 0.5 * rtol / rel);
@@ -427,7 +430,11 @@ int learned_fcg(Ctx& c, double shift, double rtol, int maxit, double delta, int*
         // a refinement round that needs more than 300 iterations means the cycle does not
         // fit this system: the round ends there and the progress test below decides
         lc.maxit = std::min(maxit - its, 300);
-        int chunk = 4, it = 0;
+        // iterations are queued in chunks between convergence checks: the first chunk of a
+        // round is the previous round's count, later ones the count the observed rate of
+        // decrease still needs (a queued iteration past convergence costs 18 launches
+        // that return at once)
+        int chunk = lastRound > 0 ? lastRound : 6, it = 0;
         while (true) {
             for (int k = 0; k < chunk; ++k) {
                 hipLaunchKernelGGL(k_f_pre, dim3(g), dim3(FB), 0, s, D, S, r, w, partC, g, st, lc);
@@ -442,8 +449,16 @@ int learned_fcg(Ctx& c, double shift, double rtol, int maxit, double delta, int*
             HIPCHECK(hipStreamSynchronize(s));
             it = (int)c.hscal[SC_CG + F_IT];
             if (c.hscal[SC_CG + F_DONE] != 0. || it >= lc.maxit) break;
-            if (chunk < 16) chunk *= 2;
+            // F_RR is the residual checked before the last queued iteration (it - 1 done)
+            const double rr0 = c.hscal[SC_CG + F_RR], bb0 = c.hscal[SC_CG + F_BB];
+            chunk = 4;
+            if (it > 1 && rr0 > 0. && bb0 > 0. && rr0 < bb0) {
+                const double rate = std::log(rr0 / bb0) / (it - 1);          // log decrease per iteration
+                const double need = std::log(lc.tol2 * bb0 / rr0) / rate;   // iterations still needed from it - 1
+                chunk = std::max(1, std::min(32, (int)std::ceil(need)));
+            }
         }
+        lastRound = it;
         its += it;
         hipLaunchKernelGGL(k_f_axpy, dim3(g), dim3(FB), 0, s, D, c.xp, (const float*)e, 1.0 / scale);
         const double prev = rel;

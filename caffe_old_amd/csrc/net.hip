@@ -5,10 +5,13 @@
 // Kernels
 //   k_conv         3x3x3 convolution on MFMA (v_mfma_f32_16x16x16_bf16): BF16 operands,
 //                  FP32 accumulation.  One workgroup computes a TXxTYxTZ output tile for
-//                  16 output channels; the K loop runs over 16-channel input blocks, each
-//                  staged with its one-voxel halo in LDS together with that block's
-//                  27x16x16 weights.  A wave owns TX*TY*TZ/64 groups of 16 voxels; per tap
-//                  it reads one B fragment (weights) and one A fragment per group.
+//                  16 output channels.  A single staging pass puts every input channel of
+//                  the tile and its one-voxel halo in LDS together with the 27xCINx16
+//                  weights (up to 147 KB of the 160 KB a gfx950 workgroup may hold), so
+//                  the MFMA loop over taps and 16-channel input blocks runs without a
+//                  barrier.  A wave owns TX*TY*TZ/64 groups of 16 voxels; per tap and input
+//                  block it reads one B fragment (weights) and one A fragment per group.
+//                  Smaller tiles on the coarse levels give those small grids more workgroups.
 //                  The staging applies the layer's input operator on the fly: plain,
 //                  2x2x2 average pooling of the finer level, or nearest up-sampling of the
 //                  coarser level plus the skip features.  The same kernel computes the
@@ -54,7 +57,7 @@ __device__ __forceinline__ void ld4(const float* p, float (&v)[4]) {
 __device__ __forceinline__ bool isDone(const double* done) { return done && *done != 0.; }
 
 constexpr int NB = 256;   // threads per block of the network kernels
-constexpr int LS = 20;    // LDS row: 16 BF16 channels + 4 padding (conflict-free 8-byte reads)
+constexpr int LS = 20;    // LDS row of the weight-gradient kernel: 16 BF16 channels + 4 padding
 
 // Channels [c, c+4) of the staged input at voxel (gx,gy,gz) of a level with dims
 // (nx,ny,nz), C channels, as 4 packed BF16.
@@ -91,6 +94,15 @@ __device__ __forceinline__ uint2 stage4(const TIN* in, const TIN* skip, int C, i
     }
 }
 
+// Channels [c, c+8) of the staged input, as 8 packed BF16 (16-byte loads).
+template <int MODE, class TIN>
+__device__ __forceinline__ uint4 stage8(const TIN* in, const TIN* skip, int C, int nx, int ny, int nz, int gx, int gy,
+                                        int gz, int c) {
+    const uint2 lo = stage4<MODE, TIN>(in, skip, C, nx, ny, nz, gx, gy, gz, c);
+    const uint2 hi = stage4<MODE, TIN>(in, skip, C, nx, ny, nz, gx, gy, gz, c + 4);
+    return make_uint4(lo.x, lo.y, hi.x, hi.y);
+}
+
 struct ConvArgs {
     const void* in;
     const void* skip;
@@ -101,14 +113,15 @@ struct ConvArgs {
     const double* done;
 };
 
-template <int TX, int TY, int TZ, int MODE, class TIN, class TOUT>
+template <int TX, int TY, int TZ, int CIN, int MODE, class TIN, class TOUT>
 __global__ __launch_bounds__(NB) void k_conv(ConvArgs a) {
     if (isDone(a.done)) return;
     constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2, HV = HX * HY * HZ;
     constexpr int TV = TX * TY * TZ, G = TV / 64;
+    constexpr int NCB = CIN / 16, LSI = CIN + 4, C4 = CIN / 4;  // LDS row: CIN channels + 4 padding
     static_assert(G >= 1 && TV % 64 == 0, "tile must hold a multiple of 64 voxels");
-    __shared__ __align__(16) uint16_t sIn[HV * LS];
-    __shared__ __align__(16) uint16_t sW[27 * 16 * LS];
+    __shared__ __align__(16) uint16_t sIn[HV * LSI];
+    __shared__ __align__(16) uint16_t sW[27 * 16 * LSI];   // [t][co][ci] of this block of 16 outputs
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int n = lane & 15, kq = lane >> 4;
     const int ntx = (a.nx + TX - 1) / TX, nty = (a.ny + TY - 1) / TY;
@@ -117,9 +130,59 @@ __global__ __launch_bounds__(NB) void k_conv(ConvArgs a) {
     b /= ntx;
     const int by = b % nty, bz = b / nty;
     const int ox = bx * TX, oy = by * TY, oz = bz * TZ;
-    const int cob = blockIdx.y, ncb = a.cin >> 4;
+    const int cob = blockIdx.y;
     const TIN* in = (const TIN*)a.in;
     const TIN* sk = (const TIN*)a.skip;
+
+    // one staging pass: every input channel of the halo tile, and the weights.  All
+    // global loads of a thread are issued before its LDS stores, so their latencies
+    // overlap instead of adding up.
+    constexpr int C8 = CIN / 8, NI = HV * C8, NIT = (NI + NB - 1) / NB;
+    constexpr int NWI = NCB * 27 * 16 * 2, NWT = (NWI + NB - 1) / NB;
+    const uint16_t* ws = a.w + (size_t)cob * NCB * (27 * 256);
+    uint4 vin[NIT], vw[NWT];
+#pragma unroll
+    for (int u = 0; u < NIT; ++u) {
+        const int it = tid + u * NB;
+        vin[u] = make_uint4(0u, 0u, 0u, 0u);
+        if (it < NI) {
+            const int hv = it / C8, c8 = it - hv * C8;
+            const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+            const int gx = ox + hx - 1, gy = oy + hy - 1, gz = oz + hz - 1;
+            if (gx >= 0 && gx < a.nx && gy >= 0 && gy < a.ny && gz >= 0 && gz < a.nz)
+                vin[u] = stage8<MODE, TIN>(in, sk, CIN, a.nx, a.ny, a.nz, gx, gy, gz, c8 * 8);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < NWT; ++u) {
+        const int it = tid + u * NB;
+        if (it < NWI) vw[u] = *(const uint4*)(ws + (size_t)it * 8);   // packed [cb][t][co][ci]
+    }
+#pragma unroll
+    for (int u = 0; u < NIT; ++u) {
+        const int it = tid + u * NB;
+        if (it < NI) {
+            const int hv = it / C8, c8 = it - hv * C8;
+            uint16_t* d = &sIn[hv * LSI + c8 * 8];
+            *(uint2*)d = make_uint2(vin[u].x, vin[u].y);
+            *(uint2*)(d + 4) = make_uint2(vin[u].z, vin[u].w);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < NWT; ++u) {
+        const int it = tid + u * NB;
+        if (it < NWI) {
+            const int h = it & 1;
+            int r = it >> 1;
+            const int co = r % 16;
+            r /= 16;
+            const int t = r % 27, cb = r / 27;
+            uint16_t* d = &sW[(t * 16 + co) * LSI + cb * 16 + h * 8];
+            *(uint2*)d = make_uint2(vw[u].x, vw[u].y);
+            *(uint2*)(d + 4) = make_uint2(vw[u].z, vw[u].w);
+        }
+    }
+    __syncthreads();
 
     int base[G];
 #pragma unroll
@@ -131,32 +194,16 @@ __global__ __launch_bounds__(NB) void k_conv(ConvArgs a) {
     f32x4 acc[G];
 #pragma unroll
     for (int g = 0; g < G; ++g) acc[g] = f32x4{0.f, 0.f, 0.f, 0.f};
-
-    for (int cb = 0; cb < ncb; ++cb) {
-        __syncthreads();
-        for (int it = tid; it < HV * 4; it += NB) {
-            const int hv = it >> 2, c4 = it & 3;
-            const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
-            const int gx = ox + hx - 1, gy = oy + hy - 1, gz = oz + hz - 1;
-            uint2 v = make_uint2(0u, 0u);
-            if (gx >= 0 && gx < a.nx && gy >= 0 && gy < a.ny && gz >= 0 && gz < a.nz)
-                v = stage4<MODE, TIN>(in, sk, a.cin, a.nx, a.ny, a.nz, gx, gy, gz, cb * 16 + c4 * 4);
-            *(uint2*)&sIn[hv * LS + c4 * 4] = v;
-        }
-        const uint16_t* ws = a.w + (size_t)(cob * ncb + cb) * (27 * 256);
-        for (int it = tid; it < 27 * 16 * 4; it += NB) {
-            const int row = it >> 2, c4 = it & 3;
-            *(uint2*)&sW[row * LS + c4 * 4] = *(const uint2*)(ws + row * 16 + c4 * 4);
-        }
-        __syncthreads();
 #pragma unroll 3
-        for (int t = 0; t < 27; ++t) {
-            const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
-            const int toff = (dz * HY + dy) * HX + dx;
-            const bf16x4 bw = *(const bf16x4*)&sW[(t * 16 + n) * LS + kq * 4];
+    for (int t = 0; t < 27; ++t) {
+        const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+        const int toff = (dz * HY + dy) * HX + dx;
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) {
+            const bf16x4 bw = *(const bf16x4*)&sW[(t * 16 + n) * LSI + cb * 16 + kq * 4];
 #pragma unroll
             for (int g = 0; g < G; ++g) {
-                const bf16x4 av = *(const bf16x4*)&sIn[(base[g] + toff) * LS + kq * 4];
+                const bf16x4 av = *(const bf16x4*)&sIn[(base[g] + toff) * LSI + cb * 16 + kq * 4];
                 acc[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bw, acc[g], 0, 0, 0);
             }
         }
@@ -297,53 +344,130 @@ __device__ __forceinline__ void vox(int v, int nx, int ny, int& x, int& y, int& 
     z = r / ny;
 }
 
+// Neighbour (i+dx, j+dy, k+dz) of a voxel, clamped into the grid, and whether it lies
+// inside: loads through a clamped index and a mask have no branch, so the compiler issues
+// all 27 of them before the first use.
+__device__ __forceinline__ int nbr(int i, int j, int k, int t, int nx, int ny, int nz, bool& in) {
+    const int a = i + t % 3 - 1, b = j + (t / 3) % 3 - 1, c = k + t / 9 - 1;
+    in = a >= 0 && a < nx && b >= 0 && b < ny && c >= 0 && c < nz;
+    const int ac = min(max(a, 0), nx - 1), bc = min(max(b, 0), ny - 1), cc = min(max(c, 0), nz - 1);
+    return (cc * ny + bc) * nx + ac;
+}
+
+// The edge layers stage a 16x4x4 output tile with its one-voxel halo in LDS (clamped
+// index and mask: no branch around the loads, which therefore issue together), then one
+// thread computes one output voxel from LDS.
+constexpr int ETX = 16, ETY = 4, ETZ = 4, EHX = ETX + 2, EHY = ETY + 2, EHV = EHX * EHY * (ETZ + 2);
+__device__ __forceinline__ void edgeTile(int nx, int ny, int& ox, int& oy, int& oz) {
+    const int ntx = (nx + ETX - 1) / ETX, nty = (ny + ETY - 1) / ETY;
+    int b = blockIdx.x;
+    ox = (b % ntx) * ETX;
+    b /= ntx;
+    oy = (b % nty) * ETY;
+    oz = (b / nty) * ETZ;
+}
+// halo voxel hv of the tile at (ox,oy,oz): clamped global index and whether it is inside
+__device__ __forceinline__ int haloVox(int hv, int ox, int oy, int oz, int nx, int ny, int nz, bool& in) {
+    const int hx = hv % EHX, hy = (hv / EHX) % EHY, hz = hv / (EHX * EHY);
+    const int gx = ox + hx - 1, gy = oy + hy - 1, gz = oz + hz - 1;
+    in = gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz;
+    return (min(max(gz, 0), nz - 1) * ny + min(max(gy, 0), ny - 1)) * nx + min(max(gx, 0), nx - 1);
+}
+static int edgeBlocks(int nx, int ny, int nz) {
+    return ((nx + ETX - 1) / ETX) * ((ny + ETY - 1) / ETY) * ((nz + ETZ - 1) / ETZ);
+}
+
 // in: 1 -> 16 channels; W [27][16][1]
-__global__ void k_in_fwd(const float* x, const float* W, uint16_t* out, int nx, int ny, int nz, const double* done) {
+__global__ __launch_bounds__(256) void k_in_fwd(const float* x, const float* W, uint16_t* out, int nx, int ny, int nz,
+                                                const double* done) {
     if (isDone(done)) return;
-    __shared__ float sW[27 * 16];
-    for (int i = threadIdx.x; i < 27 * 16; i += blockDim.x) sW[i] = W[i];
-    __syncthreads();
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nx * ny * nz) return;
-    int i, j, k;
-    vox(v, nx, ny, i, j, k);
-    float acc[16];
-    for (int c = 0; c < 16; ++c) acc[c] = 0.f;
-    for (int t = 0; t < 27; ++t) {
-        const int a = i + t % 3 - 1, b = j + (t / 3) % 3 - 1, cz = k + t / 9 - 1;
-        if (a < 0 || a >= nx || b < 0 || b >= ny || cz < 0 || cz >= nz) continue;
-        const float xv = x[(cz * ny + b) * nx + a];
-        for (int c = 0; c < 16; ++c) acc[c] += sW[t * 16 + c] * xv;
+    __shared__ __align__(16) float sW[27 * 16];
+    __shared__ float sx[EHV];
+    int ox, oy, oz;
+    edgeTile(nx, ny, ox, oy, oz);
+    const int tid = threadIdx.x;
+    constexpr int NIT = (EHV + 255) / 256;
+    float tv[NIT];
+#pragma unroll
+    for (int u = 0; u < NIT; ++u) {
+        const int hv = tid + u * 256;
+        bool in = false;
+        const int g = haloVox(hv < EHV ? hv : 0, ox, oy, oz, nx, ny, nz, in);
+        const float val = x[g];
+        tv[u] = (in && hv < EHV) ? val : 0.f;
     }
-    uint32_t w[8];
-    for (int c = 0; c < 8; ++c) w[c] = f2bf(acc[2 * c]) | (f2bf(acc[2 * c + 1]) << 16);
-    uint4* o = (uint4*)(out + (size_t)v * 16);
-    o[0] = make_uint4(w[0], w[1], w[2], w[3]);
-    o[1] = make_uint4(w[4], w[5], w[6], w[7]);
+    for (int i = tid; i < 27 * 16; i += 256) sW[i] = W[i];
+#pragma unroll
+    for (int u = 0; u < NIT; ++u)
+        if (tid + u * 256 < EHV) sx[tid + u * 256] = tv[u];
+    __syncthreads();
+    const int tx = tid % ETX, ty = (tid / ETX) % ETY, tz = tid / (ETX * ETY);
+    const int gx = ox + tx, gy = oy + ty, gz = oz + tz;
+    const int hb = ((tz + 1) * EHY + (ty + 1)) * EHX + (tx + 1);
+    float acc[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) acc[c] = 0.f;
+#pragma unroll 3
+    for (int t = 0; t < 27; ++t) {
+        const float xv = sx[hb + ((t / 9 - 1) * EHY + ((t / 3) % 3 - 1)) * EHX + (t % 3 - 1)];
+#pragma unroll
+        for (int c4 = 0; c4 < 4; ++c4) {
+            const float4 w4 = *(const float4*)&sW[t * 16 + 4 * c4];
+            acc[4 * c4] += w4.x * xv;
+            acc[4 * c4 + 1] += w4.y * xv;
+            acc[4 * c4 + 2] += w4.z * xv;
+            acc[4 * c4 + 3] += w4.w * xv;
+        }
+    }
+    if (gx < nx && gy < ny && gz < nz) {
+        uint32_t w[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) w[c] = f2bf(acc[2 * c]) | (f2bf(acc[2 * c + 1]) << 16);
+        uint4* o = (uint4*)(out + ((size_t)(gz * ny + gy) * nx + gx) * 16);
+        o[0] = make_uint4(w[0], w[1], w[2], w[3]);
+        o[1] = make_uint4(w[4], w[5], w[6], w[7]);
+    }
 }
 
 // out: 16 -> 1 channel; W [27][1][16]
-__global__ void k_out_fwd(const uint16_t* a, const float* W, float* y, int nx, int ny, int nz, const double* done) {
+__global__ __launch_bounds__(256) void k_out_fwd(const uint16_t* a, const float* W, float* y, int nx, int ny, int nz,
+                                                 const double* done) {
     if (isDone(done)) return;
-    __shared__ float sW[27 * 16];
-    for (int i = threadIdx.x; i < 27 * 16; i += blockDim.x) sW[i] = W[i];
-    __syncthreads();
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nx * ny * nz) return;
-    int i, j, k;
-    vox(v, nx, ny, i, j, k);
-    float s = 0.f;
-    for (int t = 0; t < 27; ++t) {
-        const int ai = i + t % 3 - 1, bj = j + (t / 3) % 3 - 1, ck = k + t / 9 - 1;
-        if (ai < 0 || ai >= nx || bj < 0 || bj >= ny || ck < 0 || ck >= nz) continue;
-        const uint16_t* p = a + ((size_t)(ck * ny + bj) * nx + ai) * 16;
-        for (int c4 = 0; c4 < 4; ++c4) {
-            float v4[4];
-            ld4(p + 4 * c4, v4);
-            for (int q = 0; q < 4; ++q) s += sW[t * 16 + 4 * c4 + q] * v4[q];
-        }
+    __shared__ __align__(16) float sW[27 * 16];
+    __shared__ __align__(16) uint4 sa[EHV * 2];   // 16 BF16 channels per halo voxel
+    int ox, oy, oz;
+    edgeTile(nx, ny, ox, oy, oz);
+    const int tid = threadIdx.x;
+    constexpr int NI = EHV * 2, NIT = (NI + 255) / 256;
+    uint4 tv[NIT];
+#pragma unroll
+    for (int u = 0; u < NIT; ++u) {
+        const int it = tid + u * 256;
+        const int hv = it < NI ? (it >> 1) : 0, h = it & 1;
+        bool in = false;
+        const int g = haloVox(hv, ox, oy, oz, nx, ny, nz, in);
+        const uint4 val = ((const uint4*)(a + (size_t)g * 16))[h];
+        tv[u] = (in && it < NI) ? val : make_uint4(0u, 0u, 0u, 0u);
     }
-    y[v] = s;
+    for (int i = tid; i < 27 * 16; i += 256) sW[i] = W[i];
+#pragma unroll
+    for (int u = 0; u < NIT; ++u)
+        if (tid + u * 256 < NI) sa[tid + u * 256] = tv[u];
+    __syncthreads();
+    const int tx = tid % ETX, ty = (tid / ETX) % ETY, tz = tid / (ETX * ETY);
+    const int gx = ox + tx, gy = oy + ty, gz = oz + tz;
+    const int hb = ((tz + 1) * EHY + (ty + 1)) * EHX + (tx + 1);
+    float s = 0.f;
+#pragma unroll 3
+    for (int t = 0; t < 27; ++t) {
+        const int hv = hb + ((t / 9 - 1) * EHY + ((t / 3) % 3 - 1)) * EHX + (t % 3 - 1);
+        const uint4 h0 = sa[2 * hv], h1 = sa[2 * hv + 1];
+        const uint32_t hw[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            s += sW[t * 16 + 2 * q] * bf2f(hw[q] & 0xffffu) + sW[t * 16 + 2 * q + 1] * bf2f(hw[q] >> 16);
+    }
+    if (gx < nx && gy < ny && gz < nz) y[(gz * ny + gy) * nx + gx] = s;
 }
 
 // gradient of the out layer's input: ga[v][ci] += sum_t W[t][0][ci] g[v - off(t)]
@@ -355,14 +479,21 @@ __global__ void k_out_ginput(const float* g, const float* W, float* ga, int nx, 
     if (v >= nx * ny * nz) return;
     int i, j, k;
     vox(v, nx, ny, i, j, k);
-    float acc[16];
-    for (int c = 0; c < 16; ++c) acc[c] = 0.f;
+    float gv[27];
+#pragma unroll
     for (int t = 0; t < 27; ++t) {
-        const int ai = i - (t % 3 - 1), bj = j - ((t / 3) % 3 - 1), ck = k - (t / 9 - 1);
-        if (ai < 0 || ai >= nx || bj < 0 || bj >= ny || ck < 0 || ck >= nz) continue;
-        const float gv = g[(ck * ny + bj) * nx + ai];
-        for (int c = 0; c < 16; ++c) acc[c] += sW[t * 16 + c] * gv;
+        bool in;
+        const int u = nbr(i, j, k, 26 - t, nx, ny, nz, in);   // v - off(t) = v + off(26 - t)
+        const float val = g[u];
+        gv[t] = in ? val : 0.f;
     }
+    float acc[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) acc[c] = 0.f;
+#pragma unroll
+    for (int t = 0; t < 27; ++t)
+#pragma unroll
+        for (int c = 0; c < 16; ++c) acc[c] += sW[t * 16 + c] * gv[t];
     float* o = ga + (size_t)v * 16;
     for (int c = 0; c < 16; ++c) o[c] += acc[c];
 }
@@ -511,23 +642,31 @@ void net_pack(Net& n, hipStream_t s) {
     n.packed = true;
 }
 
-template <int MODE, class TIN, class TOUT>
-static void convLaunch(hipStream_t s, const ConvArgs& a) {
-    if (a.nx >= 16) {
+// Tile by level size: 16x4x4 on grids of 32 and more, 8x4x4 on 16, 4x4x4 below (more
+// workgroups where the grid is small).
+template <int CIN, int MODE, class TIN, class TOUT>
+static void convLaunchC(hipStream_t s, const ConvArgs& a) {
+    if (a.nx >= 32) {
         const int nt = ((a.nx + 15) / 16) * ((a.ny + 3) / 4) * ((a.nz + 3) / 4);
-        hipLaunchKernelGGL((k_conv<16, 4, 4, MODE, TIN, TOUT>), dim3(nt, a.cout / 16), dim3(NB), 0, s, a);
-    } else if (a.nx >= 8) {
+        hipLaunchKernelGGL((k_conv<16, 4, 4, CIN, MODE, TIN, TOUT>), dim3(nt, a.cout / 16), dim3(NB), 0, s, a);
+    } else if (a.nx >= 16) {
         const int nt = ((a.nx + 7) / 8) * ((a.ny + 3) / 4) * ((a.nz + 3) / 4);
-        hipLaunchKernelGGL((k_conv<8, 4, 4, MODE, TIN, TOUT>), dim3(nt, a.cout / 16), dim3(NB), 0, s, a);
+        hipLaunchKernelGGL((k_conv<8, 4, 4, CIN, MODE, TIN, TOUT>), dim3(nt, a.cout / 16), dim3(NB), 0, s, a);
     } else {
         const int nt = ((a.nx + 3) / 4) * ((a.ny + 3) / 4) * ((a.nz + 3) / 4);
-        hipLaunchKernelGGL((k_conv<4, 4, 4, MODE, TIN, TOUT>), dim3(nt, a.cout / 16), dim3(NB), 0, s, a);
+        hipLaunchKernelGGL((k_conv<4, 4, 4, CIN, MODE, TIN, TOUT>), dim3(nt, a.cout / 16), dim3(NB), 0, s, a);
     }
+}
+template <int MODE, class TIN, class TOUT>
+static void convLaunch(hipStream_t s, const ConvArgs& a) {
+    if (a.cin == 16) convLaunchC<16, MODE, TIN, TOUT>(s, a);
+    else if (a.cin == 32) convLaunchC<32, MODE, TIN, TOUT>(s, a);
+    else if (a.cin == 64) convLaunchC<64, MODE, TIN, TOUT>(s, a);
+    else throw std::string("convolution input channels must be 16, 32 or 64");
 }
 
 void net_forward(Net& n, hipStream_t s, const float* vin, float* vout, const double* done) {
-    const int nv0 = n.nx * n.ny * n.nz;
-    hipLaunchKernelGGL(k_in_fwd, dim3((nv0 + 255) / 256), dim3(256), 0, s, vin, n.theta + n.off[0], n.act[0], n.nx, n.ny, n.nz, done);
+    hipLaunchKernelGGL(k_in_fwd, dim3(edgeBlocks(n.nx, n.ny, n.nz)), dim3(256), 0, s, vin, n.theta + n.off[0], n.act[0], n.nx, n.ny, n.nz, done);
     for (int l = 1; l < NET_OUT; ++l) {
         const LayerDef& d = kNet[l];
         ConvArgs a;
@@ -541,7 +680,7 @@ void net_forward(Net& n, hipStream_t s, const float* vin, float* vout, const dou
         else if (d.mode == MODE_POOL) convLaunch<MODE_POOL, uint16_t, uint16_t>(s, a);
         else convLaunch<MODE_UPADD, uint16_t, uint16_t>(s, a);
     }
-    hipLaunchKernelGGL(k_out_fwd, dim3((nv0 + 255) / 256), dim3(256), 0, s, n.act[NET_OUT - 1], n.theta + n.off[NET_OUT], vout, n.nx, n.ny, n.nz, done);
+    hipLaunchKernelGGL(k_out_fwd, dim3(edgeBlocks(n.nx, n.ny, n.nz)), dim3(256), 0, s, n.act[NET_OUT - 1], n.theta + n.off[NET_OUT], vout, n.nx, n.ny, n.nz, done);
 }
 
 static void ensurePart(Net& n, long need) {
