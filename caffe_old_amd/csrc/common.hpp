@@ -132,38 +132,64 @@ __device__ inline double blockMax(double v, double* lds) {
 }
 
 // Deterministic grid reductions without device-scope fences.  A producing kernel
-// writes one row of up to 8 partial sums per block; a consuming kernel (or a
+// writes one row of up to RED_STRIDE partial sums per block; a consuming kernel (or a
 // one-block finalize kernel) adds the rows in block order.  Kernel boundaries give
 // the visibility, so no block ever waits for, or fences against, another.
 constexpr int MAX_RED_BLOCKS = 1024;
-constexpr int RED_STRIDE = 8;
+constexpr int RED_STRIDE = 16;   // partial sums per row (at most)
 
 template <int BS, int K>
 __device__ inline void writePartials(const double (&v)[K], double* buf) {
-    __shared__ double lds[BS / 64];
+    static_assert(K <= RED_STRIDE, "more partial sums than a row holds");
+    __shared__ double lds[BS / 64][K];
+    double w[K];
 #pragma unroll
     for (int q = 0; q < K; ++q) {
-        double s = blockSum<BS>(v[q], lds);
-        if (threadIdx.x == 0) buf[blockIdx.x * RED_STRIDE + q] = s;
+        w[q] = v[q];
+        for (int o = 32; o > 0; o >>= 1) w[q] += __shfl_xor(w[q], o, 64);
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < K; ++q) lds[wid][q] = w[q];
+    __syncthreads();
+    if ((int)threadIdx.x < K) {
+        double s = 0.;
+        for (int wv = 0; wv < BS / 64; ++wv) s += lds[wv][threadIdx.x];
+        buf[blockIdx.x * RED_STRIDE + threadIdx.x] = s;
     }
 }
 
-// every thread of the block receives the K totals of nb rows
+// every thread of the block receives the K totals of nb rows: the first wave adds
+// rows lane, lane+64, ... in order and folds them by a butterfly; one barrier
 template <int BS, int K>
 __device__ inline void readPartials(const double* buf, int nb, double (&out)[K]) {
-    __shared__ double lds[BS / 64];
+    static_assert(K <= RED_STRIDE, "more partial sums than a row holds");
     __shared__ double res[K];
+    if (threadIdx.x < 64) {
+        double a[K];
 #pragma unroll
-    for (int q = 0; q < K; ++q) {
-        double a = 0.;
-        for (int b = threadIdx.x; b < nb; b += BS) a += buf[b * RED_STRIDE + q];
-        double s = blockSum<BS>(a, lds);
-        if (threadIdx.x == 0) res[q] = s;
+        for (int q = 0; q < K; ++q) a[q] = 0.;
+        for (int b = threadIdx.x; b < nb; b += 64)
+#pragma unroll
+            for (int q = 0; q < K; ++q) a[q] += buf[b * RED_STRIDE + q];
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            for (int o = 32; o > 0; o >>= 1) a[q] += __shfl_xor(a[q], o, 64);
+            if (threadIdx.x == 0) res[q] = a[q];
+        }
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < K; ++q) out[q] = res[q];
-    __syncthreads();
+}
+
+// Block b of a launch whose grid is a multiple of 8 takes chunk (b % 8) * (nb / 8) + b / 8:
+// blocks are dealt to the 8 XCDs round-robin, so each XCD then works on one contiguous
+// eighth of the cells and the stencil neighbours it reads stay in its own L2.
+__device__ inline int xcdChunk() {
+    const int nb = gridDim.x, b = blockIdx.x;
+    return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
 }
 
 }  // namespace caffe

@@ -476,54 +476,65 @@ void launch_correct(Ctx& c) {
 }
 
 // ------------------------------------------------------------------ continuity and Courant
-// Simple::computeContinuityError (Simple.cpp:358-380) scans ACTIVE cells in k,j,i order:
-// if |m| > e then e = |m|/V.  The scan is order dependent; it is evaluated exactly as a
-// sequence of "first later cell with |m| > e" searches, each a parallel min-index
-// reduction inside one workgroup, starting after the previous hit.
+// Simple::computeContinuityError (Simple.cpp:358-380) in the reference's order: scan the
+// ACTIVE cells (i fastest, then j, then k) with e = 0; whenever |m| > e, e = |m| / V.
+// The scan is a sequence of rounds, each finding the first ACTIVE cell at or after
+// `start` with |m| > e (a grid-wide first-index search) and then moving e and start on;
+// the number of rounds is the number of updates the sequential scan makes, which is
+// small (after the first hit e = |m| / V exceeds |m| by the inverse cell volume).
+// State cs: [0] e, [1] start, [2] done.
 template <bool U>
-__global__ void __launch_bounds__(1024) k_continuity_quirk(Geo g, const int8_t* st, const double* mflow, double* out) {
+__global__ void k_cq_scan(Geo g, const int8_t* st, const double* mflow, const double* cs, double* part) {
     const Dims& D = g.D;
-    __shared__ long smin[16];
-    __shared__ double se;
-    __shared__ long sstart;
-    __shared__ int sdone;
-    if (threadIdx.x == 0) { se = 0.; sstart = 0; sdone = 0; }
-    __syncthreads();
-    const long CH = 1024L * 8;
-    while (true) {
-        const double e = se;
-        long found = D.N;
-        for (long base = sstart; base < D.N && found == D.N; base += CH) {
-            long best = D.N;
-            for (int r = 0; r < 8; ++r) {
-                long qq = base + (long)r * 1024 + threadIdx.x;
-                if (qq < D.N) {
-                    int i, j, k;
-                    long p = D.cell(qq, i, j, k);
-                    if (st[p] == ST_ACTIVE && fabs(mflow[p]) > e && qq < best) best = qq;
-                }
-            }
-            for (int o = 32; o > 0; o >>= 1) { long t = __shfl_xor(best, o, 64); best = t < best ? t : best; }
-            if ((threadIdx.x & 63) == 0) smin[threadIdx.x >> 6] = best;
-            __syncthreads();
-            long b2 = D.N;
-            for (int w = 0; w < 16; ++w) b2 = smin[w] < b2 ? smin[w] : b2;
-            found = b2;
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            if (found >= D.N) sdone = 1;
-            else {
-                int i, j, k;
-                long p = D.cell(found, i, j, k);
-                se = fabs(mflow[p]) / G<U>::vol(g, p);
-                sstart = found + 1;
-            }
-        }
-        __syncthreads();
-        if (sdone) break;
+    if (cs[2] != 0.) return;   // not written by this kernel: every thread of a block agrees
+    const double e = cs[0];
+    const long start = (long)cs[1];
+    const long chunk = ((long)D.N + gridDim.x - 1) / gridDim.x;
+    const long b0 = (long)blockIdx.x * chunk, b1 = min((long)D.N, b0 + chunk);
+    long best = D.N;
+    for (long q = max(b0, start) + threadIdx.x; q < b1; q += blockDim.x) {
+        int i, j, k;
+        const long p = D.cell((int)q, i, j, k);
+        if (st[p] == ST_ACTIVE && fabs(mflow[p]) > e) { best = q; break; }
     }
-    if (threadIdx.x == 0) *out = se;
+    for (int o = 32; o > 0; o >>= 1) { const long t = __shfl_xor(best, o, 64); best = t < best ? t : best; }
+    __shared__ long sm[BS / 64];
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long m = D.N;
+        for (int w = 0; w < BS / 64; ++w) m = sm[w] < m ? sm[w] : m;
+        part[blockIdx.x * RED_STRIDE] = (double)m;
+    }
+}
+template <bool U>
+__global__ void k_cq_step(Geo g, const double* mflow, const double* part, int nb, double* cs) {
+    const Dims& D = g.D;
+    if (cs[2] != 0.) return;
+    long best = D.N;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        const long v = (long)part[b * RED_STRIDE];
+        best = v < best ? v : best;
+    }
+    for (int o = 32; o > 0; o >>= 1) { const long t = __shfl_xor(best, o, 64); best = t < best ? t : best; }
+    __shared__ long sm[BS / 64];
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long m = D.N;
+        for (int w = 0; w < BS / 64; ++w) m = sm[w] < m ? sm[w] : m;
+        if (m >= D.N) {
+            cs[2] = 1.;
+        } else {
+            int i, j, k;
+            const long p = D.cell((int)m, i, j, k);
+            cs[0] = fabs(mflow[p]) / G<U>::vol(g, p);
+            cs[1] = (double)(m + 1);
+        }
+    }
+}
+__global__ void k_cq_init(double* cs) {
+    if (threadIdx.x == 0) { cs[0] = 0.; cs[1] = 0.; cs[2] = 0.; }
 }
 
 template <bool U, int MODE>
@@ -569,9 +580,24 @@ static double readScalar(Ctx& c, int slot) {
 }
 
 double continuity_quirk(Ctx& c) {
-    if (c.uniform) hipLaunchKernelGGL(k_continuity_quirk<true>, dim3(1), dim3(1024), 0, c.stream, c.geo, c.st, c.mflow, c.scal + SC_MISC);
-    else hipLaunchKernelGGL(k_continuity_quirk<false>, dim3(1), dim3(1024), 0, c.stream, c.geo, c.st, c.mflow, c.scal + SC_MISC);
-    return readScalar(c, SC_MISC);
+    double* cs = c.scal + SC_MISC;
+    const int nb = blocksFor(c.D.N, BS, 1024);
+    hipLaunchKernelGGL(k_cq_init, dim3(1), dim3(64), 0, c.stream, cs);
+    for (long round = 0; round <= (long)c.D.N; round += 4) {
+        for (int r = 0; r < 4; ++r) {
+            if (c.uniform) {
+                hipLaunchKernelGGL(k_cq_scan<true>, dim3(nb), dim3(BS), 0, c.stream, c.geo, c.st, c.mflow, (const double*)cs, c.red.part);
+                hipLaunchKernelGGL(k_cq_step<true>, dim3(1), dim3(BS), 0, c.stream, c.geo, c.mflow, (const double*)c.red.part, nb, cs);
+            } else {
+                hipLaunchKernelGGL(k_cq_scan<false>, dim3(nb), dim3(BS), 0, c.stream, c.geo, c.st, c.mflow, (const double*)cs, c.red.part);
+                hipLaunchKernelGGL(k_cq_step<false>, dim3(1), dim3(BS), 0, c.stream, c.geo, c.mflow, (const double*)c.red.part, nb, cs);
+            }
+        }
+        HIPCHECK(hipMemcpyAsync(c.hscal + SC_MISC, cs, 3 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        HIPCHECK(hipStreamSynchronize(c.stream));
+        if (c.hscal[SC_MISC + 2] != 0.) break;
+    }
+    return c.hscal[SC_MISC];
 }
 
 static double maxReduce(Ctx& c, int mode, double dt) {

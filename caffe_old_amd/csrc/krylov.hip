@@ -81,11 +81,13 @@ struct Sym4 {              // pressure-correction matrix
     __device__ double diag(int p) const { return (double)A[p]; }
 };
 
-// grid-stride over interior cells: q = interior id, p = padded index
+// interior cells in one contiguous chunk per block (XCD-aware chunk order): q = interior
+// id, p = padded index
 __device__ inline int cellIdx(const Dims& D, int q) { int i, j, k; return D.cell(q, i, j, k); }
 #define FOR_CELLS(D, q, p)                                                                 \
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x, p = 0;                            \
-         q < (D).N && ((p = cellIdx((D), q)), true); q += gridDim.x * blockDim.x)
+    for (int q##c_ = ((D).N + (int)gridDim.x - 1) / (int)gridDim.x, q = xcdChunk() * q##c_ + (int)threadIdx.x, \
+             q##e_ = min((D).N, q - (int)threadIdx.x + q##c_), p = 0;                      \
+         q < q##e_ && ((p = cellIdx((D), q)), true); q += blockDim.x)
 
 #define BLOCK0 (blockIdx.x == 0 && threadIdx.x == 0)
 

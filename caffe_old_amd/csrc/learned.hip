@@ -39,9 +39,11 @@ struct SymF {  // symmetric p' matrix: aP, and the coefficient of each cell's lo
 };
 static SymF symF(const float* A, const Dims& D) { return SymF{A, D.P, D.sJ, D.sK}; }
 
+// interior cells in one contiguous chunk per block (XCD-aware chunk order)
 #define FOR_FINE(D, q, p)                                                                  \
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x, p = 0;                            \
-         q < (D).N && ((p = fineIdx((D), q)), true); q += gridDim.x * blockDim.x)
+    for (int q##c_ = ((D).N + (int)gridDim.x - 1) / (int)gridDim.x, q = xcdChunk() * q##c_ + (int)threadIdx.x, \
+             q##e_ = min((D).N, q - (int)threadIdx.x + q##c_), p = 0;                      \
+         q < q##e_ && ((p = fineIdx((D), q)), true); q += blockDim.x)
 __device__ inline int fineIdx(const Dims& D, int q) { int i, j, k; return D.cell(q, i, j, k); }
 __device__ __forceinline__ bool fdone(const double* done) { return done && *done != 0.; }
 static int fgrid(const Dims& D) { return blocksFor(D.N, FB, FGRID); }

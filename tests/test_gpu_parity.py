@@ -177,6 +177,26 @@ def test_piso_courant_and_continuity():
     assert rep["continuity_max"] == pytest.approx(o.continuity_max(), rel=1e-12, abs=0)
 
 
+def test_continuity_scan_many_updates():
+    """The order-dependent continuity scan with 70 updates (70 search rounds on the GPU):
+    mass imbalances 8192**t at increasing scan positions, each above 1/V = 4096 times the
+    previous hit, over a background of tiny values."""
+    case = cs.lid_cavity(16, 0.01, 1.0, "simple", max_iterations=2)
+    g, o = _pair(case, uniform=True)
+    n = 16
+    rng = np.random.default_rng(3)
+    vals = rng.uniform(-1.0, 1.0, n ** 3) * 1e-300
+    hits = np.linspace(0, n ** 3 - 1, 70).astype(int)
+    vals[hits] = np.where(rng.random(70) < 0.5, -1.0, 1.0) * 8192.0 ** np.arange(70)
+    m = np.zeros((n + 2) ** 3)
+    m.reshape(n + 2, n + 2, n + 2)[1:-1, 1:-1, 1:-1] = vals.reshape(n, n, n)
+    g.set("massFlow", m)
+    o.set("massFlow", m)
+    rep = g.step_report()
+    assert rep["continuity"] == o.continuity()
+    assert rep["continuity"] == 8192.0 ** 69 * 4096.0
+
+
 def test_ib_channel_small():
     # the channel with an immersed sphere, scaled down: 4 x 1 x 1 on 64 x 16 x 16 (h = 1/16),
     # sphere r = 0.2 at (1, .5, .5) so that it spans several cells
