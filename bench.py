@@ -1,12 +1,20 @@
 #!/usr/bin/env python3
 """Benchmark: PISO time steps per second on the 128^3 lid-driven cavity (Re 1000, two
 pressure correctors, dt = 0.5 h / U), plus the pressure-correction solve time to a true
-FP64 relative residual of 1e-8: FP32-inner / FP64-refinement (and learned) vs plain FP64.
+FP64 relative residual of 1e-8: learned flexible CG (BF16 network inside FP32 iterations,
+FP64 refinement) and FP32-inner / FP64-refinement CG vs plain FP64 CG, same system.
 
 One "step" = Piso::solve(dt) (Piso.cpp:12-29): momentum assembly + 3 BiCGStab solves,
 Rhie-Chow, then 2 x (p' assembly + p' Krylov solve + correct), continuity and Courant.
 Inputs are resident in HBM before the timed region; the timed region contains K full
-steps bracketed by stream synchronisation.
+steps bracketed by stream synchronisation.  The p' solves of the timed steps use the
+learned FCG when trained weights for the grid ship with the package (--mode auto).
+
+roofline: the pressure-correction stencil kernel of the timed steps (k_f_k1 of the
+learned FCG; k_cg_k1 for --mode ir/plain), its duration averaged over its launches in
+the timed steps from HIP event pairs recorded around them on the solver's stream, its
+algorithmic bytes per launch (interior cells x bytes per cell), and the HBM bytes per
+launch counted by rocprofv3 (profiles/traffic.json, calibrated against a copy kernel).
 
 Prints one JSON line.  --gpus N > 1 is not supported: the whole domain lives on one GPU.
 """
@@ -56,9 +64,12 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--mode", choices=["ir", "plain", "learned"], default="ir")
+    ap.add_argument("--mode", choices=["auto", "ir", "plain", "learned"], default="auto",
+                    help="p' solve: learned FCG (auto when trained weights exist for the grid), FP32 IR, plain FP64")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after the timed steps, 5 launches of a plain copy of 8 P doubles (counter calibration)")
     ap.add_argument("--kernel-only", action="store_true",
                     help="one warm-up step, then only the dominant kernel and the copy kernel (for counter runs)")
     a = ap.parse_args()
@@ -69,21 +80,25 @@ def main():
 
     from caffe_old_amd import _lib as L
     from caffe_old_amd import case as cs
-    from caffe_old_amd.learned import PressureNet, trained_weights_path
     from caffe_old_amd.solver import make_solver
 
+    from caffe_old_amd.learned import PressureNet, trained_weights_path
+    wpath = trained_weights_path(a.n)
+    if a.mode == "auto":
+        a.mode = "learned" if os.path.exists(wpath) else "ir"
     mode = {"ir": L.SOLVE_IR_FP32, "plain": L.SOLVE_PLAIN_FP64, "learned": L.SOLVE_LEARNED_FCG}[a.mode]
     inner = {"ir": 1e-2, "plain": 1e-2, "learned": 1e-4}
     case = cs.cavity_piso_128(a.n, steps=a.steps + a.warmup)
     s = make_solver(case)
     s.set_solve_opts("momentum", rtol=1e-8, maxit=5000)
     s.set_solve_opts("pcorr", mode=mode, rtol=1e-8, maxit=100000, inner_rtol=inner[a.mode])
-    # the trained learned preconditioner for this grid (trained by caffe_old_amd.train_pcorr_net)
+    # the learned preconditioner trained for this grid (caffe_old_amd.train_pcorr_net)
     net = None
-    wpath = trained_weights_path(a.n)
     if os.path.exists(wpath):
         net = PressureNet(seed=0).load(wpath)
         s.attach_net(net)
+    # the dominant kernel of the timed steps: the pressure-correction stencil kernel
+    kname = {"learned": "fcg_k1", "ir": "pcorr_cg_k1_fp32", "plain": "pcorr_cg_k1_fp64"}[a.mode]
 
     def log(*x):
         if not a.quiet:
@@ -100,6 +115,7 @@ def main():
     for w in range(a.warmup):
         s.solve()
         log(f"warmup step {w}: continuity {s.continuityError:.3e}")
+    s.kernel_probe(kname, True)   # HIP event pairs around that kernel's launches, on its stream
     s.sync()
     t0 = time.perf_counter()
     iters_p = 0
@@ -109,6 +125,10 @@ def main():
         iters_p += p["iters"]
     s.sync()
     elapsed = time.perf_counter() - t0
+    kms, klaunches, kbytes = s.kernel_probe_read()
+    s.kernel_probe(kname, False)
+    if a.calibrate_copy:
+        s.time_kernel("copy_fp64", reps=5)
     ms_per_step = 1000.0 * elapsed / a.steps
     rep = s.step_report()
     log(f"timed {a.steps} steps: {ms_per_step:.2f} ms/step, last p' iters {p['iters']}, courant {rep['courant']:.3f}")
@@ -123,11 +143,8 @@ def main():
                         "relres": r["relres"], "fell_back": r["fell_back"]}
     log("p' solves:", psolve)
 
-    # roofline of the dominant kernel (pressure-correction CG stencil kernel)
-    kname = "pcorr_cg_k1_fp64" if a.mode == "plain" else "pcorr_cg_k1_fp32"
-    s.pcorr_assemble()
-    kms, kbytes = s.time_kernel(kname, reps=200)
-    achieved = kbytes / (kms * 1e-3) / 1e9
+    # roofline of the dominant kernel, timed live inside the timed steps (probe above)
+    achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tfile):
@@ -147,14 +164,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if a.mode == "plain" else "f64+f32",
+        "dtype": {"plain": "f64", "ir": "f64+f32", "learned": "f64+f32+bf16"}[a.mode],
         "data": "synthetic: lid-driven cavity from rest (no external data)",
         "config": {"workload": f"3D lid-driven cavity {a.n}^3 uniform hex, Re 1000 (mu 1e-3, lid 1), PISO 2 correctors, "
                                f"dt {case.time_step:g} (CFL 0.5), p' solve to true relres 1e-8 ({a.mode})",
                    "cells": case.n_cells, "p_solve_ms": psolve},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "ms_per_launch": round(kms, 5), "bytes_per_launch": kbytes},
+                     "ms_per_launch": round(kms, 5), "bytes_per_launch": kbytes, "launches_timed": klaunches},
     }
     if not a.no_cpu:
         log("cpu baseline ...")

@@ -28,6 +28,7 @@ EXPORTS = [
     "caffe_pcorr_assemble", "caffe_pcorr_solve", "caffe_correct", "caffe_ib_field", "caffe_ib_coeffs",
     "caffe_grad_u", "caffe_step_report", "caffe_last_reports", "caffe_field_download", "caffe_field_upload",
     "caffe_metric_download", "caffe_matrix_download", "caffe_rhs_download", "caffe_ghost_rows", "caffe_time_kernel",
+    "caffe_kernel_probe", "caffe_kernel_probe_read",
     "caffe_sync",
     "caffe_padded_size",
     # learned pressure-correction preconditioner
@@ -106,6 +107,8 @@ def load(path: str = LIB_PATH):
         "caffe_rhs_download": ([vp, C.c_int, dp, C.c_long], C.c_int),
         "caffe_ghost_rows": ([vp, ip, lp, lp, dp, dp], C.c_int),
         "caffe_time_kernel": ([vp, C.c_char_p, C.c_int, dp, dp], C.c_int),
+        "caffe_kernel_probe": ([vp, C.c_char_p, C.c_int], C.c_int),
+        "caffe_kernel_probe_read": ([vp, dp, lp, dp], C.c_int),
         "caffe_sync": ([vp], C.c_int),
         "caffe_padded_size": ([vp], C.c_long),
         "caffe_net_create": ([C.c_int, C.c_ulonglong, C.POINTER(C.c_void_p)], C.c_int),

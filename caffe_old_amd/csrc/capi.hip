@@ -1,5 +1,6 @@
 // capi.hip -- the C ABI (include/caffe_hip.h): context, case setup, the
 // Solver::solve orchestration of SIMPLE / PISO / IbPiso, and field exchange.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -351,6 +352,7 @@ int caffe_ctx_destroy(caffe_ctx* x) {
     freeAll(c);
     if (c.hscal) (void)hipHostFree(c.hscal);
     if (c.hflags) (void)hipHostFree(c.hflags);
+    for (auto e : c.probe.ev) (void)hipEventDestroy(e);
     if (c.ev0) (void)hipEventDestroy(c.ev0);
     if (c.ev1) (void)hipEventDestroy(c.ev1);
     if (c.stream) (void)hipStreamDestroy(c.stream);
@@ -545,6 +547,55 @@ int caffe_time_kernel(caffe_ctx* x, const char* which, int reps, double* ms, dou
         *ms = time_cg_k1(c, f32, reps);
         // algorithmic bytes per interior cell: aP, aI, aJ, aK + z + p_old read, p_new + Ap written
         if (bytes) *bytes = (double)c.D.N * (f32 ? 4.0 : 8.0) * 8.0;
+        return CAFFE_OK;
+    });
+}
+
+int caffe_kernel_probe(caffe_ctx* x, const char* which, int enable) {
+    return guard(x, [&](Ctx& c) {
+        READY(c);
+        std::string w(which ? which : "");
+        int kind = PROBE_NONE;
+        if (w == "fcg_k1") kind = PROBE_FCG_K1;
+        else if (w == "pcorr_cg_k1_fp32") kind = PROBE_CG_K1_FP32;
+        else if (w == "pcorr_cg_k1_fp64") kind = PROBE_CG_K1_FP64;
+        else { c.err = "unknown probe kernel " + w; return CAFFE_ERR_ARG; }
+        HIPCHECK(hipStreamSynchronize(c.stream));
+        if (enable && c.probe.ev.empty()) {
+            c.probe.ev.resize(2 * PROBE_MAX);
+            for (auto& e : c.probe.ev) HIPCHECK(hipEventCreate(&e));
+        }
+        c.probe.which = enable ? kind : PROBE_NONE;
+        c.probe.used = 0;
+        return CAFFE_OK;
+    });
+}
+
+int caffe_kernel_probe_read(caffe_ctx* x, double* avg_ms, long* launches, double* bytes) {
+    return guard(x, [&](Ctx& c) {
+        READY(c);
+        if (!avg_ms || !launches) { c.err = "null argument"; return CAFFE_ERR_ARG; }
+        HIPCHECK(hipStreamSynchronize(c.stream));
+        std::vector<double> d;
+        for (int i = 0; i < c.probe.used; ++i) {
+            float ms = 0.f;
+            HIPCHECK(hipEventElapsedTime(&ms, c.probe.ev[2 * i], c.probe.ev[2 * i + 1]));
+            d.push_back(ms);
+        }
+        // launches queued past convergence return at once: count those that ran
+        std::vector<double> srt = d;
+        std::sort(srt.begin(), srt.end());
+        const double med = srt.empty() ? 0. : srt[srt.size() / 2];
+        double sum = 0.;
+        long n = 0;
+        for (double v : d)
+            if (v >= 0.5 * med) { sum += v; ++n; }
+        *avg_ms = n ? sum / n : 0.;
+        *launches = n;
+        // algorithmic bytes per launch per interior cell: FCG k1 reads z, p_old, r and the
+        // four matrix arrays, writes p and q (FP32: 9 x 4 B); CG k1 the same without r
+        const double per = c.probe.which == PROBE_FCG_K1 ? 36. : (c.probe.which == PROBE_CG_K1_FP32 ? 32. : 64.);
+        if (bytes) *bytes = per * (double)c.D.N;
         return CAFFE_OK;
     });
 }

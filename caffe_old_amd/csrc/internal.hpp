@@ -20,6 +20,9 @@ enum {
 };
 enum { FL_MDONE = 0, FL_PDONE = 1, FL_MITER = 2, FL_PITER = 3, N_FLAGS = 8 };
 
+enum { PROBE_NONE = 0, PROBE_FCG_K1 = 1, PROBE_CG_K1_FP32 = 2, PROBE_CG_K1_FP64 = 3 };
+constexpr int PROBE_MAX = 4096;   // launches recorded per probe window
+
 struct Overlay {          // ghost-cell rows (IbPiso.cpp:148-185), 9 entries each
     int n = 0;
     int* cell = nullptr;      // padded index of the ghost cell
@@ -82,6 +85,21 @@ struct Ctx {
     bool ibReady = false;     // ghost rows computed for the current case
     double courant = 0.;      // last Courant number (Simple.cpp:88)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    // live timing of one kernel inside the solves (event pairs around its launches)
+    struct Probe {
+        int which = 0;                    // PROBE_*
+        std::vector<hipEvent_t> ev;       // 2 per launch
+        int used = 0;
+        // the event pair for the next launch of `kind`, or false when not recording
+        bool slot(int kind, hipEvent_t& start, hipEvent_t& stop) {
+            if (which != kind || used >= PROBE_MAX) return false;
+            start = ev[2 * used];
+            stop = ev[2 * used + 1];
+            ++used;
+            return true;
+        }
+    } probe;
 
     // learned preconditioner (optional)
     void* net = nullptr;

@@ -23,6 +23,8 @@
 //             or FP32 iterations with FP64 reliable updates (the true residual is
 //             recomputed in FP64 and the FP32 recurrence restarted from it, keeping the
 //             search direction).  With ghost rows (nonsymmetric) BiCGStab is used.
+#include <hip/hip_ext.h>
+
 #include "internal.hpp"
 
 #include <cmath>
@@ -581,7 +583,12 @@ static int cgSolve(Ctx& c, double shift, double rtol, int maxit, double delta, i
     int chunk = 8;
     while (true) {
         for (int q = 0; q < chunk; ++q) {
-            hipLaunchKernelGGL((k_cg_k1<T>), dim3(g), dim3(KBS), 0, c.stream, D, A, z, p0, p1, qv, partB, g, partA, st, lc);
+            hipEvent_t e0, e1;
+            if (c.probe.slot(mixed ? PROBE_CG_K1_FP32 : PROBE_CG_K1_FP64, e0, e1))
+                hipExtLaunchKernelGGL((k_cg_k1<T>), dim3(g), dim3(KBS), 0, c.stream, e0, e1, 0, D, A, (const T*)z, p0, p1, qv,
+                                      (const double*)partB, g, partA, st, lc);
+            else
+                hipLaunchKernelGGL((k_cg_k1<T>), dim3(g), dim3(KBS), 0, c.stream, D, A, z, p0, p1, qv, partB, g, partA, st, lc);
             hipLaunchKernelGGL((k_cg_k2<T>), dim3(g), dim3(KBS), 0, c.stream, D, A, x, r, z, p0, p1, qv, partA, g, partB, st);
         }
         readBack(c, st, SC_CG, 8);

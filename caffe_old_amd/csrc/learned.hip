@@ -17,6 +17,8 @@
 #include <cmath>
 #include <cstring>
 
+#include <hip/hip_ext.h>
+
 #include "net.hpp"
 
 namespace caffe {
@@ -444,7 +446,12 @@ int learned_fcg(Ctx& c, double shift, double rtol, int maxit, double delta, int*
                 net_forward(n, s, n.vin, n.vout, done);
                 hipLaunchKernelGGL(k_tg_post1, dim3(g), dim3(FB), 0, s, D, S, r, n.vout, y, done);
                 hipLaunchKernelGGL(k_f_post2, dim3(g), dim3(FB), 0, s, D, S, r, y, z, q0, q1, partA, st);
-                hipLaunchKernelGGL(k_f_k1, dim3(g), dim3(FB), 0, s, D, S, z, r, p0, p1, q0, q1, partA, g, partB, st, lc);
+                hipEvent_t e0, e1;
+                if (c.probe.slot(PROBE_FCG_K1, e0, e1))   // dispatch-bracketing events: the kernel's own time
+                    hipExtLaunchKernelGGL(k_f_k1, dim3(g), dim3(FB), 0, s, e0, e1, 0, D, S, (const float*)z, (const float*)r,
+                                          p0, p1, q0, q1, (const double*)partA, g, partB, st, lc);
+                else
+                    hipLaunchKernelGGL(k_f_k1, dim3(g), dim3(FB), 0, s, D, S, z, r, p0, p1, q0, q1, partA, g, partB, st, lc);
                 hipLaunchKernelGGL(k_f_k2, dim3(g), dim3(FB), 0, s, D, e, r, p0, p1, q0, q1, partB, g, partC, st);
             }
             HIPCHECK(hipMemcpyAsync(c.hscal + SC_CG, st, 8 * sizeof(double), hipMemcpyDeviceToHost, s));

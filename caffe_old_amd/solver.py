@@ -165,6 +165,17 @@ class Solver:
         self._check(self.lib.caffe_time_kernel(self._h, which.encode(), reps, C.byref(ms), C.byref(by)), "caffe_time_kernel")
         return ms.value, by.value
 
+    def kernel_probe(self, which: str, enable: bool = True):
+        """Start (or stop) live event timing of one kernel inside the following solves."""
+        self._check(self.lib.caffe_kernel_probe(self._h, which.encode(), 1 if enable else 0), "caffe_kernel_probe")
+
+    def kernel_probe_read(self):
+        """(average ms per launch that ran, number of such launches, algorithmic bytes per launch)"""
+        ms, n, by = C.c_double(), C.c_long(), C.c_double()
+        self._check(self.lib.caffe_kernel_probe_read(self._h, C.byref(ms), C.byref(n), C.byref(by)),
+                    "caffe_kernel_probe_read")
+        return ms.value, n.value, by.value
+
     def sync(self):
         self._check(self.lib.caffe_sync(self._h), "caffe_sync")
 

@@ -146,6 +146,15 @@ int caffe_ghost_rows(caffe_ctx* ctx, int* n, long* cells, long* cols, double* va
  * "pcorr_cg_k1_fp64" / "pcorr_cg_k1_fp32" (pressure-correction CG stencil kernel on the
  * current p' system), "copy_fp64" (a plain copy of 8*P doubles, for counter calibration).  No counterpart in the reference (its timing is Time::tic/toc). */
 int caffe_time_kernel(caffe_ctx* ctx, const char* which, int reps, double* ms, double* bytes);
+/* Live timing of one kernel inside the solves that follow (HIP event pairs around each of
+ * its launches on the context's stream, up to 4096): which = "fcg_k1" (stencil kernel of the
+ * learned flexible CG), "pcorr_cg_k1_fp32" / "pcorr_cg_k1_fp64" (stencil kernel of the
+ * pressure CG); enable 0 stops recording.  _read returns the average duration of the
+ * launches that ran (launches queued past convergence return at once and are left out),
+ * their number, and the kernel's algorithmic HBM bytes per launch.  No counterpart in the
+ * reference. */
+int caffe_kernel_probe(caffe_ctx* ctx, const char* which, int enable);
+int caffe_kernel_probe_read(caffe_ctx* ctx, double* avg_ms, long* launches, double* bytes);
 int caffe_sync(caffe_ctx* ctx);
 long caffe_padded_size(caffe_ctx* ctx);
 
