@@ -177,6 +177,7 @@ int setup(Ctx& c, const caffe_case* cs, const double* nodes) {
     c.wm = dalloc(c, 12 * 3 * D.P);
     c.wp = dalloc(c, 8 * D.P);
     c.wp32 = talloc<float>(c, 8 * D.P);
+    if (c.hasIB) c.wb32 = talloc<float>(c, 11 * D.P);
     // identity rows everywhere until assembled (keeps the Jacobi diagonal finite)
     launch_fill(c, c.Am, 1.0, D.P);
     launch_fill(c, c.Ap, 1.0, D.P);

@@ -68,6 +68,11 @@ struct Ctx {
     double* wm = nullptr;     // momentum BiCGStab: 12 vectors * 3 * P
     double* wp = nullptr;     // pressure: 8 * P
     float* wp32 = nullptr;    // FP32 inner: 8 * P
+    // FP32 BiCGStab work of the ghost-row p' system (11 * P).  Every work buffer is viewed
+    // with one element type only: kernels write interior cells, and the stencils read the
+    // zero halo through boundary-face coefficients (nonzero at a fixed-pressure outlet),
+    // so a buffer shared between FP32 and FP64 views would carry stale bytes there.
+    float* wb32 = nullptr;
     struct { double* part = nullptr; } red;  // 3 partial-sum buffers of MAX_RED_BLOCKS rows
     double* scal = nullptr;   // N_SCAL device scalars
     int* flags = nullptr;     // N_FLAGS device ints

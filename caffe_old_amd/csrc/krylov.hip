@@ -623,8 +623,10 @@ static int bicgPressure(Ctx& c, double shift, double rtol, int maxit, double inn
     const int P = D.P;
     Sym4<T> A = symP<T>(c);
     Sym4<double> A64 = symP<double>(c);
-    T* w = (T*)c.wm;
-    T* e = (T*)(c.wm + 12 * P);
+    // FP64: the momentum work block (double views only); FP32: its own block
+    constexpr bool f32 = std::is_same<T, float>::value;
+    T* w = f32 ? (T*)c.wb32 : (T*)c.wm;
+    T* e = f32 ? (T*)(c.wb32 + 10 * (size_t)P) : (T*)(c.wm + 12 * (size_t)P);
     double* r64 = c.wp + 6 * P;
     double rr, bb;
     trueResid<1>(c, A64, c.bp, shift, c.xp, r64, &rr, &bb);
@@ -679,7 +681,7 @@ int pcorr_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep) {
             if (sym) iters += cgSolve<float>(c, shift, o.rtol, o.maxit, delta, &refs, &rel);
             else iters += bicgPressure<float>(c, shift, o.rtol, o.maxit, delta, &refs, &rel);
         }
-        if (rel > o.rtol) {  // the plain FP64 solve stays as fallback and judge, from x = 0
+        if (!(rel <= o.rtol)) {  // the plain FP64 solve stays as fallback and judge, from x = 0 (also for NaN)
             fell = 1;
             int r2 = 0;
             hipLaunchKernelGGL(k_zero_d, dim3(blocksFor(P, KBS)), dim3(KBS), 0, c.stream, c.xp, P);
