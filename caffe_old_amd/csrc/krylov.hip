@@ -33,7 +33,8 @@
 namespace caffe {
 
 constexpr int KBS = 1024;
-constexpr int KGRID = 512;   // at most 2 blocks of 16 waves per CU: the partial rows stay few
+constexpr int KGRID = 256;   // one block of 16 waves per CU: the prologue that adds the partial rows
+                             // is paid once per CU, and the rows are few
 
 // ------------------------------------------------------------------ operators
 struct OvView {
@@ -678,8 +679,15 @@ int pcorr_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep) {
         }
         if (!learnedDone) {
             const double delta = o.inner_rtol > 0. ? o.inner_rtol : 1e-2;
-            if (sym) iters += cgSolve<float>(c, shift, o.rtol, o.maxit, delta, &refs, &rel);
-            else iters += bicgPressure<float>(c, shift, o.rtol, o.maxit, delta, &refs, &rel);
+            if (sym) {
+                iters += cgSolve<float>(c, shift, o.rtol, o.maxit, delta, &refs, &rel);
+            } else {
+                // ghost-row systems: FP32 BiCGStab does not reach 1e-8 on them (measured on
+                // the 256x64x64 channel: every inner tolerance from 1e-2 to 1e-5 fell back),
+                // so they go to the FP64 solve directly and the report says so
+                fell = 1;
+                iters += bicgPressure<double>(c, shift, o.rtol, o.maxit, 0., &refs, &rel);
+            }
         }
         if (!(rel <= o.rtol)) {  // the plain FP64 solve stays as fallback and judge, from x = 0 (also for NaN)
             fell = 1;
