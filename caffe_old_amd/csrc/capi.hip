@@ -593,9 +593,10 @@ int caffe_kernel_probe_read(caffe_ctx* x, double* avg_ms, long* launches, double
             if (v >= 0.5 * med) { sum += v; ++n; }
         *avg_ms = n ? sum / n : 0.;
         *launches = n;
-        // algorithmic bytes per launch per interior cell: FCG k1 reads z, p_old, r and the
-        // four matrix arrays, writes p and q (FP32: 9 x 4 B); CG k1 the same without r
-        const double per = c.probe.which == PROBE_FCG_K1 ? 36. : (c.probe.which == PROBE_CG_K1_FP32 ? 32. : 64.);
+        // algorithmic bytes per launch per interior cell: the stencil kernels read the
+        // search direction's two inputs (z, p_old) and the four matrix arrays and write p
+        // and q: 8 x 4 B in FP32, 8 x 8 B in FP64
+        const double per = c.probe.which == PROBE_CG_K1_FP64 ? 64. : 32.;
         if (bytes) *bytes = per * (double)c.D.N;
         return CAFFE_OK;
     });

@@ -283,7 +283,7 @@ static double sampleNorm(Net& n, hipStream_t s, const Sample& smp) {
 }
 
 // ---------------------------------------------------------------- flexible CG
-enum { F_RR = 0, F_BB, F_IT, F_DONE, F_PQ, F_SQ, F_PAR, F_FIRSTRR };
+enum { F_RR = 0, F_BB, F_IT, F_DONE, F_PQ, F_SQ, F_PAR, F_FIRSTRR, F_SR, F_PR };
 struct FcgCtl { double tol2; int maxit; double nLive; };
 
 // r = s r64 (live cells), e = 0; partials (r,r), sum r
@@ -312,7 +312,7 @@ __global__ void k_f_pre(Dims D, SymF S, const float* r, float* w, const double* 
     const double bb = st[F_BB] < 0. ? rr : st[F_BB];
     const bool done = st[F_DONE] != 0. || rr <= lc.tol2 * bb || st[F_IT] >= lc.maxit || rr == 0.;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        st[F_BB] = bb; st[F_RR] = rr;
+        st[F_BB] = bb; st[F_RR] = rr; st[F_SR] = t[1];
         if (st[F_IT] == 1.) st[F_FIRSTRR] = rr;
         if (done) st[F_DONE] = 1.;
     }
@@ -321,30 +321,34 @@ __global__ void k_f_pre(Dims D, SymF S, const float* r, float* w, const double* 
     FOR_FINE(D, q, p) { w[p] = S.inv(p) * (r[p] - S.apply(z1, p)); }
 }
 
-// z = y + w D^-1 (r - A y); partials (z, q_old), sum z
+// z = y + w D^-1 (r - A y); partials (z, q_old), sum z, (z, r)
 __global__ void k_f_post2(Dims D, SymF S, const float* r, const float* y, float* z, const float* q0, const float* q1,
                           double* part, const double* st) {
     if (st[F_DONE] != 0.) return;
     const int it = (int)st[F_IT];
     const float* qold = (it & 1) ? q0 : q1;
-    double v[2] = {0., 0.};
+    double v[3] = {0., 0., 0.};
     auto yf = [&](int c) { return y[c]; };
     FOR_FINE(D, q, p) {
-        const float zz = y[p] + NET_OMEGA * S.inv(p) * (r[p] - S.apply(yf, p));
+        const float rp = r[p];
+        const float zz = y[p] + NET_OMEGA * S.inv(p) * (rp - S.apply(yf, p));
         z[p] = zz;
         v[0] += (double)zz * (double)qold[p];
         v[1] += (double)zz;
+        v[2] += (double)zz * (double)rp;
     }
-    writePartials<FB, 2>(v, part);
+    writePartials<FB, 3>(v, part);
 }
 
 // beta = -(z - zbar, q_old) / (p_old, q_old); p = (z - zbar) + beta p_old formed at the
-// stencil points; q = A p; partials (p,q), (p,r), sum q
-__global__ void k_f_k1(Dims D, SymF S, const float* z, const float* r, float* p0, float* p1, float* q0, float* q1,
+// stencil points; q = A p; partials (p,q), sum q.  (p, r) needs no pass of its own:
+// alpha of the previous step makes (p_old, r) = 0, so (p, r) = (z - zbar, r), from the
+// partials of k_f_post2; block 0 records it for k_f_k2.
+__global__ void k_f_k1(Dims D, SymF S, const float* z, float* p0, float* p1, float* q0, float* q1,
                        const double* partIn, int nb, double* partOut, double* st, FcgCtl lc) {
     if (st[F_DONE] != 0.) return;
-    double t[2];
-    readPartials<FB, 2>(partIn, nb, t);
+    double t[3];
+    readPartials<FB, 3>(partIn, nb, t);
     const int it = (int)st[F_IT];
     const double zbar = lc.nLive > 0. ? t[1] / lc.nLive : 0.;
     const double zq = t[0] - zbar * st[F_SQ];
@@ -352,33 +356,32 @@ __global__ void k_f_k1(Dims D, SymF S, const float* z, const float* r, float* p0
     const float* pold = (it & 1) ? p0 : p1;
     float* pnew = (it & 1) ? p1 : p0;
     float* qnew = (it & 1) ? q1 : q0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) st[F_PAR] = (double)(it & 1);
+    if (blockIdx.x == 0 && threadIdx.x == 0) { st[F_PAR] = (double)(it & 1); st[F_PR] = t[2] - zbar * st[F_SR]; }
     const float zb = (float)zbar, bt = (float)beta;
     auto pf = [&](int c) { return (z[c] - zb) + bt * pold[c]; };
-    double v[3] = {0., 0., 0.};
+    double v[2] = {0., 0.};
     FOR_FINE(D, q, p) {
         const float pn = pf(p);
         const float qq = S.apply(pf, p);
         pnew[p] = pn;
         qnew[p] = qq;
         v[0] += (double)pn * (double)qq;
-        v[1] += (double)pn * (double)r[p];
-        v[2] += (double)qq;
+        v[1] += (double)qq;
     }
-    writePartials<FB, 3>(v, partOut);
+    writePartials<FB, 2>(v, partOut);
 }
 
 // alpha = (p,r)/(p,q); e += alpha p; r -= alpha q; partials (r,r), sum r
 __global__ void k_f_k2(Dims D, float* e, float* r, const float* p0, const float* p1, const float* q0, const float* q1,
                        const double* partIn, int nb, double* partOut, double* st) {
     if (st[F_DONE] != 0.) return;
-    double t[3];
-    readPartials<FB, 3>(partIn, nb, t);
+    double t[2];
+    readPartials<FB, 2>(partIn, nb, t);
     const int par = (int)st[F_PAR];
     const float* pv = par ? p1 : p0;
     const float* qv = par ? q1 : q0;
-    const double alpha = t[0] != 0. ? t[1] / t[0] : 0.;
-    if (blockIdx.x == 0 && threadIdx.x == 0) { st[F_PQ] = t[0]; st[F_SQ] = t[2]; st[F_IT] += 1.; }
+    const double alpha = t[0] != 0. ? st[F_PR] / t[0] : 0.;
+    if (blockIdx.x == 0 && threadIdx.x == 0) { st[F_PQ] = t[0]; st[F_SQ] = t[1]; st[F_IT] += 1.; }
     const float al = (float)alpha;
     double v[2] = {0., 0.};
     FOR_FINE(D, q, p) {
@@ -451,10 +454,10 @@ int learned_fcg(Ctx& c, double shift, double rtol, int maxit, double delta, int*
                 hipLaunchKernelGGL(k_f_post2, dim3(g), dim3(FB), 0, s, D, S, r, y, z, q0, q1, partA, st);
                 hipEvent_t e0, e1;
                 if (c.probe.slot(PROBE_FCG_K1, e0, e1))   // dispatch-bracketing events: the kernel's own time
-                    hipExtLaunchKernelGGL(k_f_k1, dim3(g), dim3(FB), 0, s, e0, e1, 0, D, S, (const float*)z, (const float*)r,
+                    hipExtLaunchKernelGGL(k_f_k1, dim3(g), dim3(FB), 0, s, e0, e1, 0, D, S, (const float*)z,
                                           p0, p1, q0, q1, (const double*)partA, g, partB, st, lc);
                 else
-                    hipLaunchKernelGGL(k_f_k1, dim3(g), dim3(FB), 0, s, D, S, z, r, p0, p1, q0, q1, partA, g, partB, st, lc);
+                    hipLaunchKernelGGL(k_f_k1, dim3(g), dim3(FB), 0, s, D, S, z, p0, p1, q0, q1, partA, g, partB, st, lc);
                 hipLaunchKernelGGL(k_f_k2, dim3(g), dim3(FB), 0, s, D, e, r, p0, p1, q0, q1, partB, g, partC, st);
             }
             HIPCHECK(hipMemcpyAsync(c.hscal + SC_CG, st, 8 * sizeof(double), hipMemcpyDeviceToHost, s));
