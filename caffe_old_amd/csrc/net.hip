@@ -584,11 +584,36 @@ __global__ void k_pack(const float* W, uint16_t* dst, int cin, int cout, int fli
     dst[e] = (uint16_t)f2bf(v);
 }
 
-__global__ void k_adam(float* th, const float* g, float* m1, float* m2, long n, float lr, float gscale, float b1,
-                       float b2, float c1, float c2, float eps) {
+// Gradient clipping: the global norm of the (batch-averaged) gradient is limited to
+// 5x a running average of the previous steps' norms.  A deep linear network trained by
+// Adam otherwise meets rare gradient spikes while the second moments are still small,
+// and one such step can throw the loss up by orders of magnitude.
+// clip[0] = running norm (0 before the first step), clip[1] = factor for this step.
+__global__ void k_gnorm_part(const float* g, long n, float gscale, double* part) {
+    double v[1] = {0.};
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double x = (double)g[i] * gscale;
+        v[0] += x * x;
+    }
+    writePartials<256, 1>(v, part);
+}
+__global__ void k_gnorm_final(const double* part, int nb, double* clip) {
+    double t[1];
+    readPartials<256, 1>(part, nb, t);
+    if (threadIdx.x == 0) {
+        const double nrm = sqrt(t[0]);
+        const double lim = clip[0] > 0. ? 5. * clip[0] : nrm;
+        const double used = nrm > lim ? lim : nrm;
+        clip[1] = (nrm > 0. && std::isfinite(nrm)) ? used / nrm : 0.;   // a non-finite gradient is dropped
+        if (std::isfinite(used)) clip[0] = clip[0] > 0. ? 0.9 * clip[0] + 0.1 * used : used;
+    }
+}
+
+__global__ void k_adam(float* th, const float* g, float* m1, float* m2, long n, float lr, float gscale, const double* clip,
+                       float b1, float b2, float c1, float c2, float eps) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float gi = g[i] * gscale;
+    const float gi = g[i] * gscale * (float)clip[1];
     const float a = b1 * m1[i] + (1.f - b1) * gi;
     const float b = b2 * m2[i] + (1.f - b2) * gi * gi;
     m1[i] = a;
@@ -776,8 +801,13 @@ void net_adam(Net& n, hipStream_t s, float lr, float gscale) {
     const float b1 = 0.9f, b2 = 0.999f;
     ++n.adamT;
     const float c1 = 1.f - std::pow(b1, (float)n.adamT), c2 = 1.f - std::pow(b2, (float)n.adamT);
+    double* clip = n.scal + 24;
+    if (n.adamT == 1) HIPCHECK(hipMemsetAsync(clip, 0, 2 * sizeof(double), s));
+    const int nb = 512;
+    hipLaunchKernelGGL(k_gnorm_part, dim3(nb), dim3(256), 0, s, (const float*)n.grad, n.nParams, gscale, n.part);
+    hipLaunchKernelGGL(k_gnorm_final, dim3(1), dim3(256), 0, s, (const double*)n.part, nb, clip);
     hipLaunchKernelGGL(k_adam, dim3((n.nParams + 255) / 256), dim3(256), 0, s, n.theta, n.grad, n.m1, n.m2, n.nParams, lr,
-                       gscale, b1, b2, c1, c2, 1e-8f);
+                       gscale, (const double*)clip, b1, b2, c1, c2, 1e-8f);
     net_pack(n, s);
 }
 
