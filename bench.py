@@ -66,6 +66,8 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--mode", choices=["auto", "ir", "plain", "learned"], default="auto",
                     help="p' solve: learned FCG (auto when trained weights exist for the grid), FP32 IR, plain FP64")
+    ap.add_argument("--mom-mode", choices=["ir", "plain"], default="ir",
+                    help="momentum solves: FP32 BiCGStab inside FP64 refinement, or plain FP64")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -90,7 +92,8 @@ def main():
     inner = {"ir": 1e-2, "plain": 1e-2, "learned": 1e-4}
     case = cs.cavity_piso_128(a.n, steps=a.steps + a.warmup)
     s = make_solver(case)
-    s.set_solve_opts("momentum", rtol=1e-8, maxit=5000)
+    mom_mode = {"ir": L.SOLVE_IR_FP32, "plain": L.SOLVE_PLAIN_FP64}[a.mom_mode]
+    s.set_solve_opts("momentum", mode=mom_mode, rtol=1e-8, maxit=5000)
     s.set_solve_opts("pcorr", mode=mode, rtol=1e-8, maxit=100000, inner_rtol=inner[a.mode])
     # the learned preconditioner trained for this grid (caffe_old_amd.train_pcorr_net)
     net = None
@@ -142,6 +145,16 @@ def main():
         psolve[name] = {"ms": round(r["ms"], 3), "iters": r["iters"], "refinements": r["refinements"],
                         "relres": r["relres"], "fell_back": r["fell_back"]}
     log("p' solves:", psolve)
+    # momentum solve (3 components): plain FP64 vs FP32-inner / FP64-refinement, same system
+    msolve = {}
+    s.save_u0()
+    s.momentum_assemble(case.time_step)
+    for name, md in (("plain_fp64", L.SOLVE_PLAIN_FP64), ("ir_fp32", L.SOLVE_IR_FP32)):
+        s.set_solve_opts("momentum", mode=md, rtol=1e-8, maxit=5000)
+        r = s.momentum_solve()
+        msolve[name] = {"ms": round(r["ms"], 3), "iters": r["iters"], "refinements": r["refinements"],
+                        "relres": r["relres"], "fell_back": r["fell_back"]}
+    log("momentum solves:", msolve)
 
     # roofline of the dominant kernel, timed live inside the timed steps (probe above)
     achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
@@ -168,7 +181,7 @@ def main():
         "data": "synthetic: lid-driven cavity from rest (no external data)",
         "config": {"workload": f"3D lid-driven cavity {a.n}^3 uniform hex, Re 1000 (mu 1e-3, lid 1), PISO 2 correctors, "
                                f"dt {case.time_step:g} (CFL 0.5), p' solve to true relres 1e-8 ({a.mode})",
-                   "cells": case.n_cells, "p_solve_ms": psolve},
+                   "momentum": a.mom_mode, "cells": case.n_cells, "p_solve_ms": psolve, "momentum_solve_ms": msolve},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "ms_per_launch": round(kms, 5), "bytes_per_launch": kbytes, "launches_timed": klaunches},

@@ -46,6 +46,7 @@ void freeAll(Ctx& c) {
     if (c.stream) (void)hipStreamSynchronize(c.stream);
     for (void* p : c.allocs) (void)hipFree(p);
     c.allocs.clear();
+    c.wb32 = nullptr; c.Am32 = nullptr; c.wm32 = nullptr;
     c.ov = Overlay();
     c.ready = false;
 }
@@ -324,6 +325,10 @@ bool fieldByName(Ctx& c, const std::string& n, FieldRef& f) {
 
 }  // namespace
 
+namespace caffe {
+float* falloc(Ctx& c, size_t n) { return talloc<float>(c, n); }
+}  // namespace caffe
+
 extern "C" {
 
 int caffe_ctx_create(int device, caffe_ctx** out) {
@@ -370,7 +375,10 @@ int caffe_case_setup(caffe_ctx* x, const caffe_case* cs, const double* nodes) {
 int caffe_set_solve_opts(caffe_ctx* x, int which, const caffe_solve_opts* o) {
     return guard(x, [&](Ctx& c) {
         if (!o || which < 0 || which > 1 || !(o->rtol > 0.) || o->maxit < 1) { c.err = "invalid solve options"; return CAFFE_ERR_ARG; }
-        if (which == 0 && o->mode != CAFFE_SOLVE_PLAIN_FP64) { c.err = "momentum solves are FP64"; return CAFFE_ERR_ARG; }
+        if (which == 0 && o->mode != CAFFE_SOLVE_PLAIN_FP64 && o->mode != CAFFE_SOLVE_IR_FP32) {
+            c.err = "momentum solves are FP64 or FP32 inside FP64 refinement";
+            return CAFFE_ERR_ARG;
+        }
         (which == 0 ? c.momOpts : c.pcOpts) = *o;
         return CAFFE_OK;
     });

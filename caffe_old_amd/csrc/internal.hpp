@@ -73,6 +73,10 @@ struct Ctx {
     // zero halo through boundary-face coefficients (nonzero at a fixed-pressure outlet),
     // so a buffer shared between FP32 and FP64 views would carry stale bytes there.
     float* wb32 = nullptr;
+    // FP32 momentum solves inside FP64 refinement (allocated at first use): the band in
+    // FP32 (7 * P) and the BiCGStab work block plus the correction (33 * P)
+    float* Am32 = nullptr;
+    float* wm32 = nullptr;
     struct { double* part = nullptr; } red;  // 3 partial-sum buffers of MAX_RED_BLOCKS rows
     double* scal = nullptr;   // N_SCAL device scalars
     int* flags = nullptr;     // N_FLAGS device ints
@@ -140,6 +144,9 @@ double time_cg_k1(Ctx& c, bool fp32, int reps);
 // ---- immersed boundary (ib.hip)
 int ib_field(Ctx& c);
 int ib_coeffs(Ctx& c);
+
+// zero-filled device allocation owned by the context (freed by freeAll)
+float* falloc(Ctx& c, size_t n);
 
 // FP64 true residual of the p' system (r = b - shift - A x into wp + 6P): ||r||, ||b - shift||
 void pcorr_true_resid(Ctx& c, double shift, double* rr, double* bb);

@@ -43,8 +43,9 @@ struct OvView {
     const void* vals;      // double or float, 9 per row
 };
 
-struct Band7 {             // momentum matrix (FP64)
-    const double* A; int P, sJ, sK; OvView ov; const int8_t* st;
+template <class T>
+struct BandT {             // momentum matrix; T = float only without ghost rows
+    const T* A; int P, sJ, sK; OvView ov; const int8_t* st;
     template <class F>
     __device__ double apply(F x, int p) const {
         if (ov.rowOf) {
@@ -56,11 +57,13 @@ struct Band7 {             // momentum matrix (FP64)
                 return s;
             }
         }
-        return A[p] * x(p) + A[P + p] * x(p + 1) + A[2 * P + p] * x(p - 1) + A[3 * P + p] * x(p + sJ)
-             + A[4 * P + p] * x(p - sJ) + A[5 * P + p] * x(p + sK) + A[6 * P + p] * x(p - sK);
+        return (double)A[p] * x(p) + (double)A[P + p] * x(p + 1) + (double)A[2 * P + p] * x(p - 1)
+             + (double)A[3 * P + p] * x(p + sJ) + (double)A[4 * P + p] * x(p - sJ) + (double)A[5 * P + p] * x(p + sK)
+             + (double)A[6 * P + p] * x(p - sK);
     }
-    __device__ double diag(int p) const { return A[p]; }
+    __device__ double diag(int p) const { return (double)A[p]; }
 };
+using Band7 = BandT<double>;
 
 template <class T>
 struct Sym4 {              // pressure-correction matrix
@@ -504,10 +507,87 @@ static int runBicgstab(Ctx& c, Op A, const double* b, double shift, T* x, T* w, 
 }
 
 // ---------------------------------------------------------------- momentum
+__global__ void k_add_f32_3(Dims D, double* x, const float* e) {
+    FOR_CELLS(D, q, p) {
+        x[p] += (double)e[p];
+        x[D.P + p] += (double)e[D.P + p];
+        x[2 * D.P + p] += (double)e[2 * D.P + p];
+    }
+}
+
+// FP32 BiCGStab rounds inside FP64 refinement (systems without ghost rows).  x starts at
+// zero; each round solves A dx = r64 for the three components in FP32 from dx = 0 to a
+// relative tolerance of at most 1e-5 (or only what is still needed), adds dx in FP64
+// and recomputes the true FP64 residual, until the worst component meets rtol.
+static int momentumIR(Ctx& c, const caffe_solve_opts& o, int* rounds, double* relOut) {
+    const Dims& D = c.D;
+    const int P = D.P;
+    if (!c.Am32) {
+        c.Am32 = falloc(c, 7 * (size_t)P);
+        c.wm32 = falloc(c, 33 * (size_t)P);
+    }
+    hipLaunchKernelGGL(k_cast_f32, dim3(blocksFor(7 * P, KBS)), dim3(KBS), 0, c.stream, c.Am, c.Am32, 7 * P);
+    Band7 A = bandM(c);
+    BandT<float> A32{c.Am32, P, D.sJ, D.sK, OvView{nullptr, nullptr, nullptr}, c.st};
+    float* w = c.wm32;
+    float* dx = c.wm32 + 30 * (size_t)P;
+    double* rt = c.wm + 33 * (size_t)P;
+    const int g = gridFor(D);
+    hipLaunchKernelGGL(k_zero_d, dim3(blocksFor(3 * P, KBS)), dim3(KBS), 0, c.stream, c.xm, 3 * P);
+    double rr[3], bb[3];
+    trueResid<3>(c, A, c.bm, 0., c.xm, rt, rr, bb);
+    auto worst = [&]() { double m = 0.; for (int s = 0; s < 3; ++s) m = std::max(m, bb[s] > 0. ? rr[s] / bb[s] : 0.); return m; };
+    double rel = worst();
+    int iters = 0, done[3];
+    *rounds = 0;
+    while (rel > o.rtol && iters < o.maxit && *rounds < 20) {
+        const double inner = std::min(1e-5 > 0.5 * o.rtol / rel ? 1e-5 : 0.5 * o.rtol / rel, 0.5);
+        iters += runBicgstab<3, float>(c, A32, rt, 0., dx, w, inner, o.maxit - iters, 4, done);
+        hipLaunchKernelGGL(k_add_f32_3, dim3(g), dim3(KBS), 0, c.stream, D, c.xm, (const float*)dx);
+        trueResid<3>(c, A, c.bm, 0., c.xm, rt, rr, bb);
+        const double prev = rel;
+        rel = worst();
+        ++*rounds;
+        if (!(rel < 0.5 * prev)) break;   // no progress in FP32: finish in FP64
+    }
+    *relOut = rel;
+    return iters;
+}
+
 int momentum_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep) {
     HIPCHECK(hipEventRecord(c.ev0, c.stream));
     const int P = c.D.P;
     Band7 A = bandM(c);
+    if (o.mode == CAFFE_SOLVE_IR_FP32 && !c.hasIB) {
+        int rounds = 0, fell = 0;
+        double rel = 0.;
+        int iters = momentumIR(c, o, &rounds, &rel);
+        if (!(rel <= o.rtol)) {   // finish from the current x in FP64
+            int done[3];
+            double* dx = c.wm + 30 * P;
+            double* rt = c.wm + 33 * P;
+            double rr[3], bb[3];
+            for (int r = 0; r < 10 && !(rel <= o.rtol) && iters < o.maxit; ++r) {
+                const double sub = std::min(0.5, 0.5 * o.rtol / rel);
+                iters += runBicgstab<3, double>(c, A, rt, 0., dx, c.wm, sub, o.maxit - iters, 4, done);
+                add_vec(c, c.xm, dx, 3 * P);
+                trueResid<3>(c, A, c.bm, 0., c.xm, rt, rr, bb);
+                rel = 0.;
+                for (int s = 0; s < 3; ++s) rel = std::max(rel, bb[s] > 0. ? rr[s] / bb[s] : 0.);
+                ++rounds;
+            }
+            fell = 1;
+        }
+        HIPCHECK(hipEventRecord(c.ev1, c.stream));
+        HIPCHECK(hipEventSynchronize(c.ev1));
+        float ms = 0.f;
+        HIPCHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+        if (rep) {
+            rep->iters = iters; rep->refinements = rounds; rep->relres = rel; rep->converged = rel <= o.rtol;
+            rep->fell_back = fell; rep->ms = ms;
+        }
+        return rel <= o.rtol ? CAFFE_OK : CAFFE_ERR_NOT_CONVERGED;
+    }
     int done[3];
     double* w = c.wm;              // 10 * 3 * P
     double* dx = c.wm + 30 * P;    // 3 * P

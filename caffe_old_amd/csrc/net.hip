@@ -228,6 +228,357 @@ __global__ __launch_bounds__(NB) void k_conv(ConvArgs a) {
     }
 }
 
+// ---------------------------------------------------------------- forward convolution
+// k_convf: the forward pass of the ten MFMA layers (BF16 in, BF16 out), on
+// v_mfma_f32_16x16x32_bf16.  The reduction index runs over (tap, input channel) in steps
+// of 32: two taps per step at 16 input channels (a zero 28th tap completes the last
+// pair), one tap per step at 32, half a tap at 64.  LDS rows hold a voxel's channels (or
+// a weight row's input channels) without padding; their 16-byte chunks are XOR-swizzled
+// by row, so the eight lanes that read one chunk index of eight consecutive rows hit
+// eight distinct bank groups.  Without padding four level-0 workgroups fit in a CU and a
+// 64^3 layer runs in one round.  The output tile goes through LDS for 16-byte stores;
+// with POOLOUT the same pass writes the 2x2x2 average of the tile's BF16 outputs (summed
+// in dz, dy, dx order and scaled by 1/8, the order the POOL staging of the backward
+// pass uses), which the next level's first layer reads as its plain input.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+struct ConvFArgs {
+    const uint16_t* in;     // PLAIN: input at this level; UPADD: the coarser level's output
+    const uint16_t* skip;   // UPADD: skip features at this level
+    const uint16_t* w;      // packed [cout/16][cin/16][27][16 co][16 ci]
+    uint16_t* out;          // [v][cout]
+    uint16_t* pooled;       // POOLOUT: [v/8][cout] at the next level
+    int nx, ny, nz, cout;
+    const double* done;
+};
+
+template <int C8>
+__device__ __forceinline__ int swz(int r, int c) {
+    // 64 banks of 4 bytes: sixteen consecutive rows read at one chunk index must cover
+    // 256 distinct bytes
+    constexpr int S = C8 == 2 ? 3 : (C8 == 4 ? 2 : 1);
+    return c ^ ((r >> S) & (C8 - 1));
+}
+__device__ __forceinline__ uint4 addbf8(uint4 x, uint4 y) {
+    const uint32_t xs[4] = {x.x, x.y, x.z, x.w}, ys[4] = {y.x, y.y, y.z, y.w};
+    uint32_t r[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float lo = bf2f(xs[i] & 0xffffu) + bf2f(ys[i] & 0xffffu);
+        const float hi = bf2f(xs[i] >> 16) + bf2f(ys[i] >> 16);
+        r[i] = f2bf(lo) | (f2bf(hi) << 16);
+    }
+    return make_uint4(r[0], r[1], r[2], r[3]);
+}
+__device__ __forceinline__ int tapOff(int t, int HX, int HY) {
+    if (t >= 27) return 0;   // the zero tap: any row in bounds
+    const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+    return (dz * HY + dy) * HX + dx;
+}
+
+template <int TX, int TY, int TZ, int CIN, int MODE, bool POOLOUT>
+__global__ __launch_bounds__(NB) void k_convf(ConvFArgs a) {
+    if (isDone(a.done)) return;
+    constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2, HV = HX * HY * HZ;
+    constexpr int TV = TX * TY * TZ, G = TV / 64;
+    constexpr int C8 = CIN / 8;                               // 16-byte chunks per row
+    constexpr int NT = CIN == 16 ? 28 : 27;                   // taps held in LDS
+    constexpr int NK = CIN == 16 ? 14 : 27 * (CIN / 32);      // K = 32 steps
+    static_assert(G >= 1 && TV % 64 == 0, "tile must hold a multiple of 64 voxels");
+    static_assert(TV * 16 <= HV * CIN, "output tile must fit in the input tile's LDS");
+    __shared__ __align__(16) uint16_t sIn[HV * CIN];
+    __shared__ __align__(16) uint16_t sW[NT * 16 * CIN];    // [t][co][ci] of this block's 16 outputs
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int n = lane & 15, kq = lane >> 4;
+    const int ntx = (a.nx + TX - 1) / TX, nty = (a.ny + TY - 1) / TY;
+    int b = blockIdx.x;
+    const int bx = b % ntx;
+    b /= ntx;
+    const int by = b % nty, bz = b / nty;
+    const int ox = bx * TX, oy = by * TY, oz = bz * TZ;
+    const int cob = blockIdx.y;
+
+    // staging: all global loads of a thread are issued before its LDS stores
+    constexpr int NI = HV * C8, NIT = (NI + NB - 1) / NB;
+    constexpr int NWI = (CIN / 16) * 27 * 16 * 2, NWT = (NWI + NB - 1) / NB;
+    const uint16_t* ws = a.w + (size_t)cob * (CIN / 16) * (27 * 256);
+    uint4 vin[NIT], vsk[MODE == MODE_UPADD ? NIT : 1], vw[NWT];
+#pragma unroll
+    for (int u = 0; u < NIT; ++u) {
+        const int it = tid + u * NB;
+        vin[u] = make_uint4(0u, 0u, 0u, 0u);
+        if (MODE == MODE_UPADD) vsk[u] = make_uint4(0u, 0u, 0u, 0u);
+        if (it < NI) {
+            const int hv = it / C8, c = it - hv * C8;
+            const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+            const int gx = ox + hx - 1, gy = oy + hy - 1, gz = oz + hz - 1;
+            if (gx >= 0 && gx < a.nx && gy >= 0 && gy < a.ny && gz >= 0 && gz < a.nz) {
+                if (MODE == MODE_PLAIN) {
+                    vin[u] = *(const uint4*)(a.in + ((size_t)(gz * a.ny + gy) * a.nx + gx) * CIN + 8 * c);
+                } else {
+                    const int cx = a.nx >> 1, cy = a.ny >> 1;
+                    vin[u] = *(const uint4*)(a.in + ((size_t)((gz >> 1) * cy + (gy >> 1)) * cx + (gx >> 1)) * CIN + 8 * c);
+                    vsk[u] = *(const uint4*)(a.skip + ((size_t)(gz * a.ny + gy) * a.nx + gx) * CIN + 8 * c);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < NWT; ++u) {
+        const int it = tid + u * NB;
+        vw[u] = make_uint4(0u, 0u, 0u, 0u);
+        if (it < NWI) vw[u] = *(const uint4*)(ws + (size_t)it * 8);   // packed [cb][t][co][ci]
+    }
+#pragma unroll
+    for (int u = 0; u < NIT; ++u) {
+        const int it = tid + u * NB;
+        if (it < NI) {
+            const int hv = it / C8, c = it - hv * C8;
+            const uint4 v = MODE == MODE_UPADD ? addbf8(vin[u], vsk[u]) : vin[u];
+            *(uint4*)&sIn[hv * CIN + 8 * swz<C8>(hv, c)] = v;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < NWT; ++u) {
+        const int it = tid + u * NB;
+        if (it < NWI) {
+            const int h = it & 1;
+            int r = it >> 1;
+            const int co = r % 16;
+            r /= 16;
+            const int t = r % 27, cb = r / 27;
+            const int row = t * 16 + co;
+            *(uint4*)&sW[row * CIN + 8 * swz<C8>(row, cb * 2 + h)] = vw[u];
+        }
+    }
+    if (CIN == 16 && tid < 32) {   // the zero 28th tap
+        const int row = 27 * 16 + (tid >> 1);
+        *(uint4*)&sW[row * CIN + 8 * (tid & 1)] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    __syncthreads();
+
+    int base[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int vi = (wave * G + g) * 16 + n;
+        const int tx = vi % TX, ty = (vi / TX) % TY, tz = vi / (TX * TY);
+        base[g] = ((tz + 1) * HY + (ty + 1)) * HX + (tx + 1);
+    }
+    f32x4 acc[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) acc[g] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+    for (int kc = 0; kc < NK; ++kc) {
+        int t, c;
+        if (CIN == 16) { t = 2 * kc + (kq >> 1); c = kq & 1; }
+        else if (CIN == 32) { t = kc; c = kq; }
+        else { t = kc >> 1; c = (kc & 1) * 4 + kq; }
+        const int wrow = t * 16 + n;
+        const bf16x8 bw = *(const bf16x8*)&sW[wrow * CIN + 8 * swz<C8>(wrow, c)];
+        const int toff = tapOff(t, HX, HY);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int row = base[g] + toff;
+            const bf16x8 av = *(const bf16x8*)&sIn[row * CIN + 8 * swz<C8>(row, c)];
+            acc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bw, acc[g], 0, 0, 0);
+        }
+    }
+
+    // epilogue: the BF16 tile in LDS ([TV][16], two swizzled chunks per voxel)
+    __syncthreads();
+    uint16_t* sOut = sIn;
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int vi = (wave * G + g) * 16 + kq * 4 + q;
+            sOut[vi * 16 + 8 * swz<2>(vi, n >> 3) + (n & 7)] = (uint16_t)f2bf(acc[g][q]);
+        }
+    __syncthreads();
+    for (int it = tid; it < TV * 2; it += NB) {
+        const int vi = it >> 1, h = it & 1;
+        const int tx = vi % TX, ty = (vi / TX) % TY, tz = vi / (TX * TY);
+        const int gx = ox + tx, gy = oy + ty, gz = oz + tz;
+        if (gx < a.nx && gy < a.ny && gz < a.nz)
+            *(uint4*)(a.out + ((size_t)(gz * a.ny + gy) * a.nx + gx) * a.cout + cob * 16 + 8 * h) =
+                *(const uint4*)&sOut[vi * 16 + 8 * swz<2>(vi, h)];
+    }
+    if constexpr (POOLOUT) {
+        constexpr int PX = TX / 2, PY = TY / 2, PZ = TZ / 2, PV = PX * PY * PZ;
+        const int px0 = ox >> 1, py0 = oy >> 1, pz0 = oz >> 1;
+        const int pnx = a.nx >> 1, pny = a.ny >> 1, pnz = a.nz >> 1;
+        for (int it = tid; it < PV * 16; it += NB) {
+            const int pv = it >> 4, ch = it & 15;
+            const int px = pv % PX, py = (pv / PX) % PY, pz = pv / (PX * PY);
+            float s = 0.f;
+#pragma unroll
+            for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+                for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+                    for (int dx = 0; dx < 2; ++dx) {
+                        const int vi = ((2 * pz + dz) * TY + 2 * py + dy) * TX + 2 * px + dx;
+                        s += bf2f(sOut[vi * 16 + 8 * swz<2>(vi, ch >> 3) + (ch & 7)]);
+                    }
+            s *= 0.125f;
+            const int gx = px0 + px, gy = py0 + py, gz = pz0 + pz;
+            if (gx < pnx && gy < pny && gz < pnz)
+                a.pooled[((size_t)(gz * pny + gy) * pnx + gx) * a.cout + cob * 16 + ch] = (uint16_t)f2bf(s);
+        }
+    }
+}
+
+// k_convf16: the 16-input-channel layers on 16-voxel x rows (tile 16x4x4), on
+// v_mfma_f32_16x16x16_bf16 with one tap per MFMA.  The 27 weight fragments of the
+// block's 16 outputs stay in registers for the whole tile (54 VGPRs), so the loop reads
+// only activations from LDS: per (dz, dy) and 16-voxel row one 8-byte fragment (lane n
+// holds voxel n), and the dx = -1 and dx = +1 fragments are that row shifted by one lane
+// (DPP row_shr:1 / row_shl:1 within the 16-lane rows), with the two halo voxels from a
+// second read of the row.  This reads two thirds of the LDS bytes of one read per tap.
+__device__ __forceinline__ uint32_t shr1(uint32_t old, uint32_t v) {   // lane n <- lane n-1; lane 0 keeps old
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, 0x111, 0xf, 0xf, false);
+}
+__device__ __forceinline__ uint32_t shl1(uint32_t old, uint32_t v) {   // lane n <- lane n+1; lane 15 keeps old
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, 0x101, 0xf, 0xf, false);
+}
+__device__ __forceinline__ bf16x4 asbf4(uint32_t x, uint32_t y) {
+    const uint2 u = make_uint2(x, y);
+    return *(const bf16x4*)&u;
+}
+
+template <int MODE, bool POOLOUT>
+__global__ __launch_bounds__(NB) void k_convf16(ConvFArgs a) {
+    if (isDone(a.done)) return;
+    constexpr int TX = 16, TY = 4, TZ = 4, CIN = 16, C8 = 2;
+    constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2, HV = HX * HY * HZ;
+    constexpr int TV = TX * TY * TZ, G = TV / 64;
+    static_assert(TV * 16 <= HV * CIN, "output tile must fit in the input tile's LDS");
+    __shared__ __align__(16) uint16_t sIn[HV * CIN];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int n = lane & 15, kq = lane >> 4;
+    const int ntx = (a.nx + TX - 1) / TX, nty = (a.ny + TY - 1) / TY;
+    int b = blockIdx.x;
+    const int bx = b % ntx;
+    b /= ntx;
+    const int by = b % nty, bz = b / nty;
+    const int ox = bx * TX, oy = by * TY, oz = bz * TZ;
+    const int cob = blockIdx.y;
+
+    constexpr int NI = HV * C8, NIT = (NI + NB - 1) / NB;
+    uint4 vin[NIT], vsk[MODE == MODE_UPADD ? NIT : 1];
+#pragma unroll
+    for (int u = 0; u < NIT; ++u) {
+        const int it = tid + u * NB;
+        vin[u] = make_uint4(0u, 0u, 0u, 0u);
+        if (MODE == MODE_UPADD) vsk[u] = make_uint4(0u, 0u, 0u, 0u);
+        if (it < NI) {
+            const int hv = it / C8, c = it - hv * C8;
+            const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+            const int gx = ox + hx - 1, gy = oy + hy - 1, gz = oz + hz - 1;
+            if (gx >= 0 && gx < a.nx && gy >= 0 && gy < a.ny && gz >= 0 && gz < a.nz) {
+                if (MODE == MODE_PLAIN) {
+                    vin[u] = *(const uint4*)(a.in + ((size_t)(gz * a.ny + gy) * a.nx + gx) * CIN + 8 * c);
+                } else {
+                    const int cx = a.nx >> 1, cy = a.ny >> 1;
+                    vin[u] = *(const uint4*)(a.in + ((size_t)((gz >> 1) * cy + (gy >> 1)) * cx + (gx >> 1)) * CIN + 8 * c);
+                    vsk[u] = *(const uint4*)(a.skip + ((size_t)(gz * a.ny + gy) * a.nx + gx) * CIN + 8 * c);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < NIT; ++u) {
+        const int it = tid + u * NB;
+        if (it < NI) {
+            const int hv = it / C8, c = it - hv * C8;
+            const uint4 v = MODE == MODE_UPADD ? addbf8(vin[u], vsk[u]) : vin[u];
+            *(uint4*)&sIn[hv * CIN + 8 * swz<C8>(hv, c)] = v;
+        }
+    }
+    // weights (after the staging, so that their registers and the staging's do not
+    // overlap): lane (n, kq) holds w[t][ci = 4kq..4kq+3][co = n] for all 27 taps
+    const uint16_t* ws = a.w + (size_t)cob * (27 * 256) + n * 16 + 4 * kq;
+    uint2 wr[27];
+#pragma unroll
+    for (int t = 0; t < 27; ++t) wr[t] = *(const uint2*)(ws + t * 256);
+    __syncthreads();
+
+    // lane (n, kq): channels 4kq..4kq+3 (8 bytes) of an LDS row
+    auto frag = [&](int row) { return *(const uint2*)&sIn[row * CIN + 8 * swz<C8>(row, kq >> 1) + 4 * (kq & 1)]; };
+    // the second read of a row: lanes 0-7 read voxel n-1, lanes 8-15 voxel n+1 (always
+    // inside the halo row), so that lane 0 holds voxel -1 and lane 15 voxel 16; reading
+    // it in every lane keeps the loop free of branches
+    const int hoff = n < 8 ? -1 : 1;
+    int base[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int vi = (wave * G + g) * 16 + n;   // a group is one x row: tx = n
+        const int ty = (vi / TX) % TY, tz = vi / (TX * TY);
+        base[g] = ((tz + 1) * HY + (ty + 1)) * HX + (n + 1);
+    }
+    f32x4 acc[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) acc[g] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int p = 0; p < 9; ++p) {
+        const int dz = p / 3 - 1, dy = p % 3 - 1, t0 = 3 * p;   // taps t0 (dx -1), t0+1, t0+2
+        const int poff = (dz * HY + dy) * HX;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int row = base[g] + poff;
+            const uint2 c0 = frag(row);
+            const uint2 h = frag(row + hoff);
+            const bf16x4 am = asbf4(shr1(h.x, c0.x), shr1(h.y, c0.y));
+            const bf16x4 ap = asbf4(shl1(h.x, c0.x), shl1(h.y, c0.y));
+            acc[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am, asbf4(wr[t0].x, wr[t0].y), acc[g], 0, 0, 0);
+            acc[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(asbf4(c0.x, c0.y), asbf4(wr[t0 + 1].x, wr[t0 + 1].y), acc[g], 0, 0, 0);
+            acc[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ap, asbf4(wr[t0 + 2].x, wr[t0 + 2].y), acc[g], 0, 0, 0);
+        }
+    }
+
+    __syncthreads();
+    uint16_t* sOut = sIn;
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int vi = (wave * G + g) * 16 + kq * 4 + q;
+            sOut[vi * 16 + 8 * swz<2>(vi, n >> 3) + (n & 7)] = (uint16_t)f2bf(acc[g][q]);
+        }
+    __syncthreads();
+    for (int it = tid; it < TV * 2; it += NB) {
+        const int vi = it >> 1, hh = it & 1;
+        const int tx = vi % TX, ty = (vi / TX) % TY, tz = vi / (TX * TY);
+        const int gx = ox + tx, gy = oy + ty, gz = oz + tz;
+        if (gx < a.nx && gy < a.ny && gz < a.nz)
+            *(uint4*)(a.out + ((size_t)(gz * a.ny + gy) * a.nx + gx) * a.cout + cob * 16 + 8 * hh) =
+                *(const uint4*)&sOut[vi * 16 + 8 * swz<2>(vi, hh)];
+    }
+    if constexpr (POOLOUT) {
+        constexpr int PX = TX / 2, PY = TY / 2, PZ = TZ / 2, PV = PX * PY * PZ;
+        const int px0 = ox >> 1, py0 = oy >> 1, pz0 = oz >> 1;
+        const int pnx = a.nx >> 1, pny = a.ny >> 1, pnz = a.nz >> 1;
+        for (int it = tid; it < PV * 16; it += NB) {
+            const int pv = it >> 4, ch = it & 15;
+            const int px = pv % PX, py = (pv / PX) % PY, pz = pv / (PX * PY);
+            float sum = 0.f;
+#pragma unroll
+            for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+                for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+                    for (int dx = 0; dx < 2; ++dx) {
+                        const int vi = ((2 * pz + dz) * TY + 2 * py + dy) * TX + 2 * px + dx;
+                        sum += bf2f(sOut[vi * 16 + 8 * swz<2>(vi, ch >> 3) + (ch & 7)]);
+                    }
+            sum *= 0.125f;
+            const int gx = px0 + px, gy = py0 + py, gz = pz0 + pz;
+            if (gx < pnx && gy < pny && gz < pnz)
+                a.pooled[((size_t)(gz * pny + gy) * pnx + gx) * a.cout + cob * 16 + ch] = (uint16_t)f2bf(sum);
+        }
+    }
+}
+
 struct WgArgs {
     const uint16_t* x;     // source activation (BF16) of the staged input
     const uint16_t* skip;
@@ -637,7 +988,7 @@ void net_set_dims(Net& n, int nx, int ny, int nz, bool train) {
     HIPCHECK(hipDeviceSynchronize());
     for (void* p : n.dimAllocs) (void)hipFree(p);
     n.dimAllocs.clear();
-    for (int l = 0; l < NET_LAYERS; ++l) { n.act[l] = nullptr; n.gact[l] = nullptr; }
+    for (int l = 0; l < NET_LAYERS; ++l) { n.act[l] = nullptr; n.gact[l] = nullptr; n.pooled[l] = nullptr; }
     n.nx = nx; n.ny = ny; n.nz = nz;
     n.trainBufs = train;
     const size_t v0 = (size_t)nx * ny * nz;
@@ -650,6 +1001,8 @@ void net_set_dims(Net& n, int nx, int ny, int nz, bool train) {
             if (train) n.gact[l] = (float*)net_alloc(n, vl * d.cout * 4, n.dimAllocs);
         }
         gst = std::max(gst, vl * d.cin);
+        if (d.mode == MODE_POOL)   // the producer's epilogue writes the pooled input of this layer
+            n.pooled[d.src] = (uint16_t*)net_alloc(n, vl * d.cin * 2, n.dimAllocs);
     }
     n.vin = (float*)net_alloc(n, v0 * 4, n.dimAllocs);
     n.vout = (float*)net_alloc(n, v0 * 4, n.dimAllocs);
@@ -690,20 +1043,50 @@ static void convLaunch(hipStream_t s, const ConvArgs& a) {
     else throw std::string("convolution input channels must be 16, 32 or 64");
 }
 
+template <int CIN, int MODE, bool POOLOUT>
+static void convFLaunchC(hipStream_t s, const ConvFArgs& a) {
+    if (CIN == 16 && a.nx >= 32) {
+        const int nt = ((a.nx + 15) / 16) * ((a.ny + 3) / 4) * ((a.nz + 3) / 4);
+        hipLaunchKernelGGL((k_convf16<MODE, POOLOUT>), dim3(nt, a.cout / 16), dim3(NB), 0, s, a);
+    } else if (a.nx >= 32) {
+        const int nt = ((a.nx + 15) / 16) * ((a.ny + 3) / 4) * ((a.nz + 3) / 4);
+        hipLaunchKernelGGL((k_convf<16, 4, 4, CIN, MODE, POOLOUT>), dim3(nt, a.cout / 16), dim3(NB), 0, s, a);
+    } else if (a.nx >= 16) {
+        const int nt = ((a.nx + 7) / 8) * ((a.ny + 3) / 4) * ((a.nz + 3) / 4);
+        hipLaunchKernelGGL((k_convf<8, 4, 4, CIN, MODE, POOLOUT>), dim3(nt, a.cout / 16), dim3(NB), 0, s, a);
+    } else {
+        const int nt = ((a.nx + 3) / 4) * ((a.ny + 3) / 4) * ((a.nz + 3) / 4);
+        hipLaunchKernelGGL((k_convf<4, 4, 4, CIN, MODE, POOLOUT>), dim3(nt, a.cout / 16), dim3(NB), 0, s, a);
+    }
+}
+template <int MODE, bool POOLOUT>
+static void convFLaunch(hipStream_t s, const ConvFArgs& a, int cin) {
+    if (cin == 16) convFLaunchC<16, MODE, POOLOUT>(s, a);
+    else if (cin == 32) convFLaunchC<32, MODE, POOLOUT>(s, a);
+    else if (cin == 64) convFLaunchC<64, MODE, POOLOUT>(s, a);
+    else throw std::string("convolution input channels must be 16, 32 or 64");
+}
+
 void net_forward(Net& n, hipStream_t s, const float* vin, float* vout, const double* done) {
     hipLaunchKernelGGL(k_in_fwd, dim3(edgeBlocks(n.nx, n.ny, n.nz)), dim3(256), 0, s, vin, n.theta + n.off[0], n.act[0], n.nx, n.ny, n.nz, done);
     for (int l = 1; l < NET_OUT; ++l) {
         const LayerDef& d = kNet[l];
-        ConvArgs a;
-        a.in = n.act[d.src];
+        ConvFArgs a;
+        // a POOL layer reads the average its source layer's epilogue wrote
+        a.in = d.mode == MODE_POOL ? n.pooled[d.src] : n.act[d.src];
         a.skip = d.skip >= 0 ? n.act[d.skip] : nullptr;
         a.w = n.wf + n.poff[l];
         a.out = n.act[l];
+        a.pooled = n.pooled[l];
         a.nx = lvlDim(n.nx, d.level); a.ny = lvlDim(n.ny, d.level); a.nz = lvlDim(n.nz, d.level);
-        a.cin = d.cin; a.cout = d.cout; a.accum = 0; a.done = done;
-        if (d.mode == MODE_PLAIN) convLaunch<MODE_PLAIN, uint16_t, uint16_t>(s, a);
-        else if (d.mode == MODE_POOL) convLaunch<MODE_POOL, uint16_t, uint16_t>(s, a);
-        else convLaunch<MODE_UPADD, uint16_t, uint16_t>(s, a);
+        a.cout = d.cout; a.done = done;
+        if (d.mode == MODE_UPADD) {
+            if (a.pooled) convFLaunch<MODE_UPADD, true>(s, a, d.cin);
+            else convFLaunch<MODE_UPADD, false>(s, a, d.cin);
+        } else {
+            if (a.pooled) convFLaunch<MODE_PLAIN, true>(s, a, d.cin);
+            else convFLaunch<MODE_PLAIN, false>(s, a, d.cin);
+        }
     }
     hipLaunchKernelGGL(k_out_fwd, dim3(edgeBlocks(n.nx, n.ny, n.nz)), dim3(256), 0, s, n.act[NET_OUT - 1], n.theta + n.off[NET_OUT], vout, n.nx, n.ny, n.nz, done);
 }

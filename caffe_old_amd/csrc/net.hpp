@@ -74,6 +74,7 @@ struct Net {
     int nx = 0, ny = 0, nz = 0;
     bool trainBufs = false;
     uint16_t* act[NET_LAYERS] = {};  // BF16 layer outputs (channels last); NET_OUT unused
+    uint16_t* pooled[NET_LAYERS] = {};  // 2x2x2 averages of a layer output that a POOL layer reads (BF16)
     float* gact[NET_LAYERS] = {};    // FP32 gradients of the layer outputs
     float* gstage = nullptr;         // FP32 gradient of a layer's staged input
     float* vin = nullptr;            // coarse input (FP32)

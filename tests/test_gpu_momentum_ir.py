@@ -1,0 +1,77 @@
+"""Momentum solves as FP32 BiCGStab rounds inside FP64 refinement (CAFFE_SOLVE_IR_FP32
+for the momentum system), against the plain FP64 BiCGStab and against the oracle.
+
+Both modes converge the three momentum systems to the same true FP64 relative residual,
+so the fields after PISO steps agree to the size of that tolerance.  Ghost-row (IB)
+momentum systems take the FP64 path in either mode.
+"""
+import numpy as np
+import pytest
+
+from caffe_old_amd import _lib as L
+from caffe_old_amd import case as cs
+from tests.helpers import interior
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _piso(case, mom_mode, steps, rtol):
+    from caffe_old_amd.solver import make_solver
+    s = make_solver(case)
+    s.set_solve_opts("momentum", mode=mom_mode, rtol=rtol, maxit=5000)
+    s.set_solve_opts("pcorr", mode=L.SOLVE_PLAIN_FP64, rtol=rtol, maxit=100000)
+    reps = []
+    for _ in range(steps):
+        s.solve()
+        reps.append(s.reports()[0])
+    out = {f: s.get(f).copy() for f in ("u", "p")}
+    s.close()
+    return out, reps
+
+
+def test_momentum_ir_matches_plain_64():
+    c = cs.lid_cavity(64, 1e-3, 1.0, "piso", alpha_u=1.0, alpha_p=1.0, time_accurate=True, dt=0.5 / 64,
+                      n_pcorr=2, max_iterations=4)
+    a, ra = _piso(c, L.SOLVE_IR_FP32, 3, 1e-10)
+    b, rb = _piso(c, L.SOLVE_PLAIN_FP64, 3, 1e-10)
+    for r in ra:
+        print(f"IR momentum: iters {r['iters']} rounds {r['refinements']} relres {r['relres']:.2e} {r['ms']:.2f} ms")
+        assert r["converged"] and r["relres"] <= 1e-10 and r["fell_back"] == 0 and r["refinements"] >= 2
+    print("plain momentum ms", [round(r["ms"], 2) for r in rb])
+    for f in ("u", "p"):
+        e = _rel(a[f], b[f])
+        print(f"{f}: IR vs plain rel L2 {e:.2e}")
+        assert e <= 1e-8
+
+
+def test_momentum_ir_matches_oracle_12():
+    from oracle.pyoracle import Oracle
+    from caffe_old_amd.solver import make_solver
+    c = cs.lid_cavity(12, 1e-3, 1.0, "piso", alpha_u=1.0, alpha_p=1.0, time_accurate=True, dt=0.5 / 12,
+                      n_pcorr=2, max_iterations=3)
+    g = make_solver(c)
+    g.set_solve_opts("momentum", mode=L.SOLVE_IR_FP32, rtol=1e-12, maxit=5000)
+    g.set_solve_opts("pcorr", rtol=1e-12, maxit=50000)
+    o = Oracle(c, rtol=1e-13)
+    for _ in range(c.n_solves):
+        g.solve()
+        o.solve()
+    for f in ("u", "p"):
+        e = _rel(interior(g.get(f), g.shape), interior(o.get(f), g.shape))
+        print(f"{f}: IR momentum vs oracle rel L2 {e:.2e}")
+        assert e <= 1e-6
+    g.close()
+
+
+def test_momentum_ir_ghost_rows_take_fp64():
+    case = cs.channel_ib_sphere(64, 16, 16, steps=2)
+    a, ra = _piso(case, L.SOLVE_IR_FP32, 2, 1e-8)
+    b, rb = _piso(case, L.SOLVE_PLAIN_FP64, 2, 1e-8)
+    for x, y in zip(ra, rb):
+        assert x["converged"] and x["iters"] == y["iters"]
+    for f in ("u", "p"):
+        assert np.array_equal(a[f], b[f])
