@@ -1,0 +1,560 @@
+"""The reference's case files, mesh files and Tecplot solution output.
+
+What a user of the reference has on disk for a case directory (``ExampleCases/*``):
+
+* ``case/case.info`` and ``case/initialConditions.info``: Boost property-tree INFO
+  files (Input.cpp:29-43, InitialConditions.cpp:5-74);
+* ``mesh/structuredMesh.info``: the eight vertices and the node resolution that the
+  ``hexaMesh`` module turns into ``mesh/structuredMesh.dat`` (HexaMeshGen.cpp:14-124,
+  Modules/hexaMesh.cpp:8-28);
+* ``mesh/structuredMesh.dat``: a Tecplot ASCII grid the solvers read
+  (StructuredMesh.cpp:27-67, 215-280);
+* ``solution/<mesh name>.dat``: the Tecplot ASCII solution, one zone per write
+  (HexaFvmMesh::writeTec360, HexaFvmMesh.cpp:479-611).
+
+This module reads and writes all of them, byte for byte where the reference writes,
+and builds a :class:`~caffe_old_amd.case.Case` from a case directory, so that the
+example cases run unchanged through the GPU solver (``python -m caffe_old_amd.run``).
+"""
+from __future__ import annotations
+
+import math
+import os
+import re
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from caffe_old_amd.case import FACES, Boundary, Case, transfinite_nodes
+
+
+# ------------------------------------------------------------------ INFO files
+class InfoError(Exception):
+    """A missing node or a value that does not convert (Boost's ptree_bad_path /
+    ptree_bad_data, which the reference lets escape)."""
+
+
+class InfoNode:
+    """One node of an INFO tree: a key, its data string and its children in file order."""
+
+    __slots__ = ("key", "data", "children")
+
+    def __init__(self, key: str = "", data: str = ""):
+        self.key = key
+        self.data = data
+        self.children: List["InfoNode"] = []
+
+    def child(self, key: str) -> Optional["InfoNode"]:
+        for c in self.children:
+            if c.key == key:
+                return c
+        return None
+
+    def get_child(self, path: str) -> "InfoNode":
+        node = self
+        for part in path.split("."):
+            nxt = node.child(part)
+            if nxt is None:
+                raise InfoError(f"No such node ({path})")
+            node = nxt
+        return node
+
+    def has(self, path: str) -> bool:
+        try:
+            self.get_child(path)
+            return True
+        except InfoError:
+            return False
+
+    def get(self, path: str, kind=str):
+        """The data at a dotted path converted like ptree::get<T>: the whole string must
+        convert (surrounding white space aside)."""
+        s = self.get_child(path).data
+        if kind is str:
+            return s
+        t = s.strip()
+        if kind is int:
+            if re.fullmatch(r"[+-]?\d+", t):
+                return int(t)
+        elif kind is float:
+            if re.fullmatch(r"[+-]?(\d+\.?\d*|\.\d+)([eE][+-]?\d+)?", t):
+                return float(t)
+        else:
+            raise TypeError(f"unsupported conversion {kind}")
+        raise InfoError(f'conversion of data "{s}" to type "{kind.__name__}" failed ({path})')
+
+
+_ESC = {"0": "\0", "a": "\a", "b": "\b", "f": "\f", "n": "\n", "r": "\r", "t": "\t", "v": "\v",
+        '"': '"', "'": "'", "\\": "\\"}
+
+
+def _tokens(line: str, where: str):
+    """Split one line into tokens: '{', '}', bare words and quoted strings; ';' starts a
+    comment.  A quoted string followed by '\\' at the end of the line continues on the
+    next line (returned as the flag)."""
+    out = []
+    i, n = 0, len(line)
+    cont = False
+    while i < n:
+        ch = line[i]
+        if ch.isspace():
+            i += 1
+        elif ch == ";":
+            break
+        elif ch in "{}":
+            out.append((ch, False))
+            i += 1
+        elif ch == '"':
+            j = i + 1
+            buf = []
+            while j < n and line[j] != '"':
+                if line[j] == "\\" and j + 1 < n:
+                    e = line[j + 1]
+                    if e not in _ESC:
+                        raise InfoError(f"invalid escape sequence in {where}")
+                    buf.append(_ESC[e])
+                    j += 2
+                else:
+                    buf.append(line[j])
+                    j += 1
+            if j >= n:
+                raise InfoError(f"unterminated string in {where}")
+            out.append(("".join(buf), True))
+            i = j + 1
+            rest = line[i:].strip()
+            if rest.startswith("\\") and (len(rest) == 1 or rest[1:].lstrip().startswith(";")):
+                cont = True
+                break
+        else:
+            j = i
+            while j < n and not line[j].isspace() and line[j] not in '{};"':
+                j += 1
+            out.append((line[i:j], False))
+            i = j
+    return out, cont
+
+
+def parse_info(text: str, source: str = "<string>", base_dir: Optional[str] = None) -> InfoNode:
+    """Parse INFO text (the grammar of boost/property_tree/info_parser): on each line
+    tokens alternate key, data, key, ...; '{' opens the children of the last key, '}'
+    closes them, at any point of a line; a new line starts with a key; ';' comments;
+    '#include "file"' reads a file relative to ``base_dir``."""
+    root = InfoNode()
+    stack = [root]
+    last: Optional[InfoNode] = None
+    pending: Optional[InfoNode] = None   # a data string continued on the next line
+    for lineno, raw in enumerate(text.splitlines(), 1):
+        where = f"{source}:{lineno}"
+        stripped = raw.strip()
+        if pending is None and stripped.startswith("#include"):
+            m = re.match(r'#include\s+"([^"]*)"', stripped)
+            if not m:
+                raise InfoError(f"invalid #include in {where}")
+            fn = m.group(1)
+            if base_dir is not None and not os.path.isabs(fn):
+                fn = os.path.join(base_dir, fn)
+            inc = read_info(fn)
+            stack[-1].children.extend(inc.children)
+            last = stack[-1].children[-1] if stack[-1].children else last
+            continue
+        toks, cont = _tokens(raw, where)
+        expect_key = True
+        if pending is not None:
+            if not toks or not toks[0][1]:
+                raise InfoError(f"expected a string continuation in {where}")
+            pending.data += toks[0][0]
+            toks = toks[1:]
+            expect_key = True
+            pending = pending if cont else None
+            if pending is not None:
+                continue
+        for idx, (tok, quoted) in enumerate(toks):
+            if not quoted and tok == "{":
+                if last is None:
+                    raise InfoError(f"'{{' without a key in {where}")
+                stack.append(last)
+                last = None
+                expect_key = True
+            elif not quoted and tok == "}":
+                if len(stack) == 1:
+                    raise InfoError(f"unmatched '}}' in {where}")
+                last = stack.pop()
+                expect_key = True
+            elif expect_key:
+                node = InfoNode(tok)
+                stack[-1].children.append(node)
+                last = node
+                expect_key = False
+            else:
+                last.data = tok
+                expect_key = True
+                if cont and idx == len(toks) - 1 and quoted:
+                    pending = last
+    if len(stack) != 1:
+        raise InfoError(f"unmatched '{{' in {source}")
+    return root
+
+
+def read_info(path: str) -> InfoNode:
+    with open(path, "r") as f:
+        text = f.read()
+    return parse_info(text, source=path, base_dir=os.path.dirname(os.path.abspath(path)))
+
+
+_NUM_PREFIX = re.compile(r"\s*[+-]?(\d+\.?\d*|\.\d+)([eE][+-]?\d+)?")
+
+
+def _stod(s: str) -> float:
+    """std::stod: the longest numeric prefix after leading white space."""
+    m = _NUM_PREFIX.match(s)
+    if not m:
+        raise ValueError(f'stod: no conversion of "{s}"')
+    return float(m.group(0))
+
+
+def stov(s: str) -> Tuple[float, float, float]:
+    """Vector3D.cpp:215-240: the text between '(' and ')' split at any of ',' and ' '.
+    As there, the substring starts after the first '(' and has length
+    index(')') - 1."""
+    a = s.find("(")
+    b = s.find(")")
+    if a < 0 or b < 0:
+        raise ValueError(f'stov: "{s}" is not a vector "(x,y,z)"')
+    body = s[a + 1:a + 1 + (b - 1)]
+    parts = [p for p in re.split(r"[, ]+", body) if p != ""]
+    if len(parts) < 3:
+        raise ValueError(f'stov: "{s}" has fewer than three components')
+    return _stod(parts[0]), _stod(parts[1]), _stod(parts[2])
+
+
+# ------------------------------------------------------------------ meshes
+def read_mesh_info(path: str):
+    """HexaMeshGen::readFile (HexaMeshGen.cpp:14-30): mesh name, the eight vertices
+    scaled by MetricConversion, and the node resolution (cast to int as there)."""
+    t = read_info(path)
+    name = t.get("MeshName")
+    conv = t.get("MetricConversion", float)
+    verts = []
+    for v in range(1, 9):
+        x, y, z = stov(t.get(f"Vertices.v{v}"))
+        verts.append((conv * x, conv * y, conv * z))
+    rx, ry, rz = stov(t.get("Resolution"))
+    return name, np.array(verts, dtype=np.float64), (int(rx), int(ry), int(rz))
+
+
+def generate_mesh(vertices, resolution) -> np.ndarray:
+    """HexaMeshGen::generateMesh for nodes (nI, nJ, nK) = resolution: (nK, nJ, nI, 3)."""
+    ni, nj, nk = resolution
+    if min(ni, nj, nk) < 2:
+        raise ValueError("a resolution needs at least two nodes in each direction")
+    return transfinite_nodes(vertices, ni - 1, nj - 1, nk - 1)
+
+
+def _unit(v):
+    return v / math.sqrt(float(v @ v))
+
+
+def quad_is_planar(p) -> bool:
+    """Geometry::checkQuadrilateralIsPlanar (Geometry.cpp:27-38): the unit normals of the
+    two triangles that share the diagonal p0-p2 agree to 1e-12."""
+    p = np.asarray(p, dtype=np.float64)
+    n1 = _unit(np.cross(p[1] - p[0], p[2] - p[0]))
+    n2 = _unit(np.cross(p[2] - p[0], p[3] - p[0]))
+    d = n1 - n2
+    return math.sqrt(float(d @ d)) <= 1e-12
+
+
+def check_mesh(vertices) -> None:
+    """HexaMeshGen::checkMesh (HexaMeshGen.cpp:118-124) with the face quadrilaterals of
+    Geometry::checkHexahedronSurfacesIsPlanar (Geometry.cpp:111-180)."""
+    v = np.asarray(vertices, dtype=np.float64)
+    faces = ((1, 2, 6, 5), (0, 4, 7, 3), (2, 3, 7, 6), (0, 1, 5, 4), (4, 5, 6, 7), (0, 3, 2, 1))
+    for f in faces:
+        if not quad_is_planar(v[list(f)]):
+            raise ValueError("one or more surfaces of the mesh geometry is not planar, which is not currently supported.")
+
+
+def _g(x: float, prec: int) -> str:
+    """An ostream's default floating format at the given precision (%g)."""
+    return "%.*g" % (prec, x)
+
+
+def mesh_file_text(name: str, nodes: np.ndarray) -> str:
+    """The bytes HexaMeshGen::writeMeshFile (HexaMeshGen.cpp:81-115) writes: a Tecplot
+    grid header, then x, y, z in BLOCK order, one i row per line, each value followed by
+    a space, with the default six significant digits."""
+    nk, nj, ni, _ = nodes.shape
+    out = [f'TITLE = "{name}"\n', 'VARIABLES = "x", "y", "z"\n', "FILETYPE = GRID\n",
+           f"ZONE I = {ni}, J = {nj}, K = {nk}\n", "DATAPACKING = BLOCK\n"]
+    for c in range(3):
+        comp = nodes[..., c]
+        for k in range(nk):
+            for j in range(nj):
+                out.append("".join(_g(float(x), 6) + " " for x in comp[k, j]) + "\n")
+    return "".join(out)
+
+
+def write_mesh_file(path: str, name: str, nodes: np.ndarray) -> None:
+    with open(path, "w") as f:
+        f.write(mesh_file_text(name, nodes))
+
+
+def _read_header(lines, path):
+    """StructuredMesh::readTecplotMeshHeader (StructuredMesh.cpp:215-280)."""
+    name = ""
+    dims = {}
+    n_read = 0
+    for raw in lines:
+        n_read += 1
+        toks = re.split(r'[ ,="]+', raw.rstrip("\n"))
+        if not toks or toks[0] == "":
+            continue
+        head = toks[0].upper()
+        if head == "TITLE":
+            name = toks[1] if len(toks) > 1 else ""
+        elif head in ("VARIABLES", "FILETYPE", "STRANDID", "SOLUTIONTIME", "ZONETYPE"):
+            continue
+        elif head == "ZONE":
+            for i in range(1, 7, 2):
+                if i + 1 >= len(toks):
+                    raise ValueError(f"{path}: incomplete zone specifier")
+                key = toks[i].upper()
+                if key not in ("I", "J", "K"):
+                    raise ValueError(f'{path}: invalid zone specifier "{toks[i]}".')
+                dims[key] = int(toks[i + 1])
+        elif head == "DATAPACKING":
+            break
+        else:
+            raise ValueError(f'{path}: invalid mesh header "{toks[0]}".')
+    if set(dims) != {"I", "J", "K"}:
+        raise ValueError(f"{path}: one or more mesh dimensions has not been set.")
+    return name, dims["I"], dims["J"], dims["K"], n_read
+
+
+def read_structured_mesh(path: str):
+    """StructuredMesh::initialize(filename) (StructuredMesh.cpp:27-67): the mesh name and
+    the nodes as (nK, nJ, nI, 3)."""
+    with open(path, "r") as f:
+        lines = f.readlines()
+    name, ni, nj, nk, n_hdr = _read_header(lines, path)
+    vals = np.array(" ".join(lines[n_hdr:]).split(), dtype=np.float64)
+    n = ni * nj * nk
+    if vals.size < 3 * n:
+        raise ValueError(f"{path}: {vals.size} node values for {3 * n}")
+    nodes = vals[:3 * n].reshape(3, nk, nj, ni).transpose(1, 2, 3, 0)
+    return name, np.ascontiguousarray(nodes)
+
+
+# ------------------------------------------------------------------ cases
+class RunControl:
+    """RunControl (RunControl.cpp:38-109): the termination test and the write interval.
+    ``continue_run`` increments the iteration count and the simulation time, then
+    tests, so M iterations run M - 1 solves."""
+
+    def __init__(self, t: InfoNode):
+        self.termination = t.get("RunControl.terminationCondition")
+        self.max_iters = t.get("RunControl.maxNumberOfIterations", int)
+        self.time_step = t.get("RunControl.timeStep", float)
+        self.max_sim_time = t.get("RunControl.maxSimTime", float)
+        self.file_write_interval = t.get("RunControl.fileWriteInterval", int)
+        if self.termination not in ("iterations", "simTime"):
+            # "realTime" is a warning in the reference and never terminates
+            raise ValueError(f'invalid termination condition "{self.termination}" selected.')
+        self.itrs = 0
+        self.sim_time = 0.0
+        self.residual_norm = 0.0
+
+    def continue_run(self) -> bool:
+        self.itrs += 1
+        self.sim_time += self.time_step
+        if self.termination == "iterations":
+            return self.itrs < self.max_iters
+        return self.sim_time < self.max_sim_time
+
+    def write_to_file(self) -> bool:
+        return self.itrs % self.file_write_interval == 0
+
+    def n_solves(self) -> int:
+        """The number of solves continue_run allows, counted without running."""
+        itrs, t, n = 0, 0.0, 0
+        while True:
+            itrs += 1
+            t += self.time_step
+            ok = itrs < self.max_iters if self.termination == "iterations" else t < self.max_sim_time
+            if not ok:
+                return n
+            n += 1
+
+    def update_message(self) -> str:
+        pct = (self.itrs / self.max_iters * 100.0 if self.termination == "iterations"
+               else self.sim_time / self.max_sim_time * 100.0)
+        return (f"Simulation completion (%) |      {_g(pct, 6)}\n"
+                f"Iterations completed      |      {self.itrs}\n"
+                f"Simulation time (sec)     |      {_g(self.sim_time, 6)}\n"
+                f"Residual norm             |      {_g(self.residual_norm, 6)}")
+
+
+def _initial(ic: InfoNode, field: str, vector: bool):
+    """InitialConditions::setInitialConditions (InitialConditions.cpp:12-74), uniform
+    type: the value of "<field>.value"."""
+    kind = ic.get(f"{field}.type")
+    if kind == "uniform":
+        s = ic.get(f"{field}.value")
+        return stov(s) if vector else ic.get(f"{field}.value", float)
+    if kind in ("sphere", "box"):
+        raise NotImplementedError(f'initial condition type "{kind}" for field "{field}" is not supported '
+                                  "(the solver takes uniform initial fields)")
+    raise ValueError(f'Unrecognized initial condition type "{kind}".')
+
+
+def case_from_info(params: InfoNode, ics: InfoNode, nodes: np.ndarray, algo: str) -> Case:
+    """A Case from the parsed case.info and initialConditions.info.  The keys are read in
+    the order the reference reads them, so a deck that lacks several fails on the same
+    node: RunControl::initialize (RunControl.cpp:38-51), Solver.cpp:8 (timeAccurate "ON"
+    means unsteady), FlowBoundaryConditions.cpp:25-29 (member initialisation of the
+    boundaries), Simple.cpp:61-67 (inner iterations only when unsteady; relaxation
+    factors), Simple.cpp:104-108 (rho, mu), InitialConditions.cpp:12-74 (u, then p),
+    Piso.cpp:9 (pressure corrections), IbPiso.cpp:37-38 (the sphere)."""
+    rc = RunControl(params)
+    unsteady = params.get("Solver.timeAccurate") == "ON"
+    bounds = []
+    for f in FACES:
+        typ = params.get(f"Boundaries.{f}.type")
+        ref_p = params.get(f"Boundaries.{f}.refValue", float)
+        ref_u = stov(params.get(f"Boundaries.{f}.refVector"))
+        bounds.append(Boundary(typ, ref_u, ref_p))
+    n_inner = params.get("Solver.numberOfInnerIterations", int) if unsteady else 1
+    alpha_u = params.get("Solver.relaxationFactorMomentum", float)
+    alpha_p = params.get("Solver.relaxationFactorPCorr", float)
+    rho = params.get("Solver.rho", float)
+    mu = params.get("Solver.mu", float)
+    u0 = _initial(ics, "u", True)
+    p0 = _initial(ics, "p", False)
+    n_pcorr = params.get("Solver.numberOfPressureCorrections", int) if algo in ("piso", "ibpiso") else 1
+    sphere = None
+    if algo == "ibpiso":
+        sphere = (params.get("ImmersedBoundaries.Sphere.radius", float),
+                  stov(params.get("ImmersedBoundaries.Sphere.center")))
+    return Case(nodes=nodes, algo=algo, time_accurate=unsteady, rho=rho, mu=mu, alpha_u=alpha_u, alpha_p=alpha_p,
+                n_inner=n_inner, n_pcorr=n_pcorr, boundaries=bounds, u_init=tuple(u0), p_init=float(p0),
+                sphere=sphere, time_step=rc.time_step, max_iterations=rc.n_solves() + 1)
+
+
+def load_case(case_dir: str, algo: str):
+    """Read a case directory as a solver module does (Input(), StructuredMesh::initialize
+    on mesh/structuredMesh.dat): returns (mesh name, Case, RunControl)."""
+    params = read_info(os.path.join(case_dir, "case", "case.info"))
+    ics = read_info(os.path.join(case_dir, "case", "initialConditions.info"))
+    mesh_dat = os.path.join(case_dir, "mesh", "structuredMesh.dat")
+    if not os.path.exists(mesh_dat):
+        raise FileNotFoundError(f"{mesh_dat} not found: generate it first (python -m caffe_old_amd.run hexamesh {case_dir})")
+    name, nodes = read_structured_mesh(mesh_dat)
+    return name, case_from_info(params, ics, nodes, algo), RunControl(params)
+
+
+# ------------------------------------------------------------------ solution output
+class Tec360Writer:
+    """HexaFvmMesh::writeTec360 (HexaFvmMesh.cpp:479-611): an ASCII Tecplot file of
+    strand 1 whose first zone carries the nodes and whose later zones share them; cell
+    values in BLOCK order, one i row per line, each value followed by a space, 16
+    significant digits.  Scalars come before vectors; a vector v writes v_x, v_y, v_z.
+    As in the reference, the VARIABLES list and the first ZONE title share a line."""
+
+    PRECISION = 16
+
+    def __init__(self, path: str, name: str, nodes: np.ndarray, scalars: Sequence[str], vectors: Sequence[str]):
+        self.path = path
+        self.name = name
+        self.nodes = np.asarray(nodes, dtype=np.float64)
+        self.scalars = list(scalars)
+        self.vectors = list(vectors)
+        self.started = False
+
+    def _zone_header(self, time: float, first: bool) -> str:
+        g = lambda x: _g(x, self.PRECISION)
+        nk, nj, ni, _ = self.nodes.shape
+        nvar = 3 + len(self.scalars) + 3 * len(self.vectors)
+        s = (f'ZONE T = "{self.name}Time:{g(time)}s"\n'
+             f"STRANDID = 1, SOLUTIONTIME = {g(time)}\n"
+             f"I = {ni}, J = {nj}, K = {nk}\n"
+             "DATAPACKING = BLOCK\n")
+        if not first:
+            s += "VARSHARELIST = ([1-3] = 1)\n"
+        if nvar == 4:
+            s += "VARLOCATION = ([4] = CELLCENTERED)\n"
+        elif nvar > 4:
+            s += f"VARLOCATION = ([4-{nvar}] = CELLCENTERED)\n"
+        return s
+
+    @staticmethod
+    def _rows(a: np.ndarray, prec: int) -> str:
+        out = []
+        for k in range(a.shape[0]):
+            for j in range(a.shape[1]):
+                out.append("".join(_g(float(x), prec) + " " for x in a[k, j]) + "\n")
+        return "".join(out)
+
+    def zone_text(self, time: float, scalars: Dict[str, np.ndarray], vectors: Dict[str, np.ndarray],
+                  first: bool) -> str:
+        """scalars: name -> (nK, nJ, nI) cell values; vectors: name -> (3, nK, nJ, nI)."""
+        parts = []
+        if first:
+            var = 'VARIABLES = "x", "y", "z", ' + "".join(f'"{s}", ' for s in self.scalars)
+            var += "".join(f'"{v}_{c}", ' for v in self.vectors for c in "xyz")
+            parts.append(f'TITLE = "{self.name}"\n' + var)
+        parts.append(self._zone_header(time, first))
+        if first:
+            for c in range(3):
+                parts.append(self._rows(self.nodes[..., c], self.PRECISION))
+        for s in self.scalars:
+            parts.append(self._rows(np.asarray(scalars[s]), self.PRECISION))
+        for v in self.vectors:
+            arr = np.asarray(vectors[v])
+            for c in range(3):
+                parts.append(self._rows(arr[c], self.PRECISION))
+        return "".join(parts)
+
+    def write(self, time: float, scalars: Dict[str, np.ndarray], vectors: Dict[str, np.ndarray]) -> None:
+        first = not self.started
+        text = self.zone_text(time, scalars, vectors, first)
+        with open(self.path, "w" if first else "a") as f:
+            f.write(text)
+        self.started = True
+
+
+def read_tec360(path: str):
+    """Read a file written by HexaFvmMesh::writeTec360 (or Tec360Writer): returns
+    (nodes (nK, nJ, nI, 3), zones), each zone a dict with "time" and, per variable
+    name after x, y, z, its cell values as (nK-1, nJ-1, nI-1)."""
+    with open(path, "r") as f:
+        text = f.read()
+    m = re.search(r"VARIABLES = (.*?)ZONE T", text, re.S)
+    if not m:
+        raise ValueError(f"{path}: no VARIABLES line")
+    names = re.findall(r'"([^"]*)"', m.group(1))[3:]
+    zones = []
+    nodes = None
+    pos = m.end() - len("ZONE T")
+    heads = [h.start() for h in re.finditer(r"ZONE T = ", text)]
+    for zi, start in enumerate(heads):
+        end = heads[zi + 1] if zi + 1 < len(heads) else len(text)
+        block = text[start:end]
+        t = float(re.search(r"SOLUTIONTIME = (\S+)", block).group(1))
+        dims = re.search(r"I = (\d+), J = (\d+), K = (\d+)", block)
+        ni, nj, nk = (int(dims.group(q)) for q in (1, 2, 3))
+        body_start = re.search(r"CELLCENTERED\)\n", block)
+        body = block[body_start.end() if body_start else block.index("DATAPACKING = BLOCK\n") + 20:]
+        vals = np.array(body.split(), dtype=np.float64)
+        nc = (ni - 1) * (nj - 1) * (nk - 1)
+        o = 0
+        if zi == 0:
+            nn = ni * nj * nk
+            nodes = vals[:3 * nn].reshape(3, nk, nj, ni).transpose(1, 2, 3, 0).copy()
+            o = 3 * nn
+        zone = {"time": t}
+        for nm in names:
+            zone[nm] = vals[o:o + nc].reshape(nk - 1, nj - 1, ni - 1)
+            o += nc
+        zones.append(zone)
+    del pos
+    return nodes, zones
