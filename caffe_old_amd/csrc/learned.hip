@@ -73,10 +73,8 @@ __global__ void k_tg_pre(Dims D, SymF S, const float* r, float* w, const double*
 // gathers fine cells 2C-1 .. 2C+2 in each direction; the weight of fine f is 3/4 when
 // f>>1 == C plus 1/4 when its other neighbour is C.  The halo of w is zero, so fine
 // cells just outside the grid (weight 0) are read through the padded layout unmasked.
-__global__ void k_tg_restrict(Dims D, const float* w, float* v, const double* done) {
-    if (fdone(done)) return;
+__device__ __forceinline__ void restrictAt(const Dims& D, const float* w, float* v, int e) {
     const int cx = D.nI >> 1, cy = D.nJ >> 1, cz = D.nK >> 1;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= cx * cy * cz) return;
     const int I = e % cx, J = (e / cx) % cy, K = e / (cx * cy);
     auto weights = [](int C, int nc, int nf, float* wt) {
@@ -110,6 +108,10 @@ __global__ void k_tg_restrict(Dims D, const float* w, float* v, const double* do
         s += wz[c] * sy;
     }
     v[e] = s;
+}
+__global__ void k_tg_restrict(Dims D, const float* w, float* v, const double* done) {
+    if (fdone(done)) return;
+    restrictAt(D, w, v, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // (P c) at fine cell (i,j,k)
@@ -199,7 +201,7 @@ __global__ void k_spmv_f(Dims D, SymF S, const float* x, float* y) {
 }
 
 // ---------------------------------------------------------------- host: cycle and loss
-enum { FV_R = 0, FV_E, FV_W, FV_Y, FV_Z, FV_P0, FV_P1, FV_Q0, FV_Q1, FV_H, FV_AE, FV_XS, FV_BS, N_FV };
+enum { FV_R = 0, FV_E, FV_W, FV_Y, FV_Z, FV_P0, FV_P1, FV_Q0, FV_Q1, FV_H, FV_AE, FV_XS, FV_BS, FV_R1, N_FV };
 
 void tg_ensure_fine(Net& n, const Dims& D) {
     if (n.fineP == D.P && n.fw) return;
@@ -300,14 +302,25 @@ __global__ void k_f_init(Dims D, const double* r64, double s, float* r, float* e
     if (blockIdx.x == 0 && threadIdx.x == 0) { st[F_IT] = 0.; st[F_DONE] = 0.; st[F_BB] = -1.; }
 }
 
-// convergence check on the partials of (r,r), sum r; then w = D^-1 (r - A w D^-1 r).
-// Block 0 raises F_DONE in this kernel, so no block returns on it before the block-wide
-// reduction: a block whose waves read the flag on either side of that write would
-// otherwise split at the barrier.  When the flag is read as set, the partials already
-// meet the criterion, so every wave reaches the same decision.
-__global__ void k_f_pre(Dims D, SymF S, const float* r, float* w, const double* partIn, int nb, double* st, FcgCtl lc) {
+// The residual alternates between two buffers by iteration parity (r[F_IT & 1] is the
+// current one), so that k_f_k2pre can write the next residual while its neighbours
+// still read the current one.
+
+// w = D^-1 (r - A w D^-1 r) for the first iteration of a round (r = r0)
+__global__ void k_f_pre0(Dims D, SymF S, const float* r, float* w, const double* st) {
+    if (st[F_DONE] != 0.) return;
+    auto z1 = [&](int c) { return NET_OMEGA * S.inv(c) * r[c]; };
+    FOR_FINE(D, q, p) { w[p] = S.inv(p) * (r[p] - S.apply(z1, p)); }
+}
+
+// Convergence check on the partials of (r,r), sum r (from k_f_init or k_f_k2pre), then
+// v = P^T w.  Block 0 raises F_DONE in this kernel, so no block returns on it before the
+// block-wide reduction: a block whose waves read the flag on either side of that write
+// would otherwise split at the barrier.  When the flag is read as set, the partials
+// already meet the criterion, so every wave reaches the same decision.
+__global__ void k_f_restrict(Dims D, const float* w, float* v, const double* partIn, int nb, double* st, FcgCtl lc) {
     double t[2];
-    readPartials<FB, 2>(partIn, nb, t);
+    readPartials<256, 2>(partIn, nb, t);
     const double rr = lc.nLive > 0. ? fmax(t[0] - t[1] * t[1] / lc.nLive, 0.) : t[0];
     const double bb = st[F_BB] < 0. ? rr : st[F_BB];
     const bool done = st[F_DONE] != 0. || rr <= lc.tol2 * bb || st[F_IT] >= lc.maxit || rr == 0.;
@@ -317,16 +330,28 @@ __global__ void k_f_pre(Dims D, SymF S, const float* r, float* w, const double* 
         if (done) st[F_DONE] = 1.;
     }
     if (done) return;
-    auto z1 = [&](int c) { return NET_OMEGA * S.inv(c) * r[c]; };
-    FOR_FINE(D, q, p) { w[p] = S.inv(p) * (r[p] - S.apply(z1, p)); }
+    restrictAt(D, w, v, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// y = w D^-1 r + P c
+__global__ void k_f_post1(Dims D, SymF S, const float* r0, const float* r1, const float* c, float* y, const double* st) {
+    if (st[F_DONE] != 0.) return;
+    const float* r = ((int)st[F_IT] & 1) ? r1 : r0;
+    const int cx = D.nI >> 1, cy = D.nJ >> 1, cz = D.nK >> 1;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < D.N; q += gridDim.x * blockDim.x) {
+        int i, j, k;
+        const int p = D.cell(q, i, j, k);
+        y[p] = NET_OMEGA * S.inv(p) * r[p] + prolongAt(c, cx, cy, cz, i, j, k);
+    }
 }
 
 // z = y + w D^-1 (r - A y); partials (z, q_old), sum z, (z, r)
-__global__ void k_f_post2(Dims D, SymF S, const float* r, const float* y, float* z, const float* q0, const float* q1,
-                          double* part, const double* st) {
+__global__ void k_f_post2(Dims D, SymF S, const float* r0, const float* r1, const float* y, float* z, const float* q0,
+                          const float* q1, double* part, const double* st) {
     if (st[F_DONE] != 0.) return;
     const int it = (int)st[F_IT];
     const float* qold = (it & 1) ? q0 : q1;
+    const float* r = (it & 1) ? r1 : r0;
     double v[3] = {0., 0., 0.};
     auto yf = [&](int c) { return y[c]; };
     FOR_FINE(D, q, p) {
@@ -343,7 +368,7 @@ __global__ void k_f_post2(Dims D, SymF S, const float* r, const float* y, float*
 // beta = -(z - zbar, q_old) / (p_old, q_old); p = (z - zbar) + beta p_old formed at the
 // stencil points; q = A p; partials (p,q), sum q.  (p, r) needs no pass of its own:
 // alpha of the previous step makes (p_old, r) = 0, so (p, r) = (z - zbar, r), from the
-// partials of k_f_post2; block 0 records it for k_f_k2.
+// partials of k_f_post2; block 0 records it for k_f_k2pre.
 __global__ void k_f_k1(Dims D, SymF S, const float* z, float* p0, float* p1, float* q0, float* q1,
                        const double* partIn, int nb, double* partOut, double* st, FcgCtl lc) {
     if (st[F_DONE] != 0.) return;
@@ -371,25 +396,34 @@ __global__ void k_f_k1(Dims D, SymF S, const float* z, float* p0, float* p1, flo
     writePartials<FB, 2>(v, partOut);
 }
 
-// alpha = (p,r)/(p,q); e += alpha p; r -= alpha q; partials (r,r), sum r
-__global__ void k_f_k2(Dims D, float* e, float* r, const float* p0, const float* p1, const float* q0, const float* q1,
-                       const double* partIn, int nb, double* partOut, double* st) {
+// alpha = (p,r)/(p,q); e += alpha p; r' = r - alpha q into the other residual buffer;
+// partials (r',r'), sum r'; and the next iteration's pre-smoothing from r' formed at the
+// stencil points: w = D^-1 (r' - A w D^-1 r').  The convergence test on these partials
+// is made by the next k_f_restrict, so a w computed after the last iteration is unused.
+__global__ void k_f_k2pre(Dims D, SymF S, float* e, const float* r0, const float* r1, float* w, const float* p0,
+                          const float* p1, const float* q0, const float* q1, const double* partIn, int nb,
+                          double* partOut, double* st) {
     if (st[F_DONE] != 0.) return;
     double t[2];
     readPartials<FB, 2>(partIn, nb, t);
-    const int par = (int)st[F_PAR];
+    const int par = (int)st[F_PAR];          // this iteration's parity, set by k_f_k1
     const float* pv = par ? p1 : p0;
     const float* qv = par ? q1 : q0;
+    const float* rc = par ? r1 : r0;
+    float* rn = (float*)(par ? r0 : r1);
     const double alpha = t[0] != 0. ? st[F_PR] / t[0] : 0.;
     if (blockIdx.x == 0 && threadIdx.x == 0) { st[F_PQ] = t[0]; st[F_SQ] = t[1]; st[F_IT] += 1.; }
     const float al = (float)alpha;
+    auto rnew = [&](int c) { return rc[c] - al * qv[c]; };
+    auto z1 = [&](int c) { return NET_OMEGA * S.inv(c) * rnew(c); };
     double v[2] = {0., 0.};
     FOR_FINE(D, q, p) {
         e[p] += al * pv[p];
-        const float rn = r[p] - al * qv[p];
-        r[p] = rn;
-        v[0] += (double)rn * (double)rn;
-        v[1] += (double)rn;
+        const float rp = rnew(p);
+        rn[p] = rp;
+        w[p] = S.inv(p) * (rp - S.apply(z1, p));
+        v[0] += (double)rp * (double)rp;
+        v[1] += (double)rp;
     }
     writePartials<FB, 2>(v, partOut);
 }
@@ -411,6 +445,7 @@ int learned_fcg(Ctx& c, double shift, double rtol, int maxit, double delta, int*
     const SymF S = symF(c.Ap32, D);
     const int P = D.P;
     float* r = n.fw + (size_t)FV_R * P;
+    float* r1 = n.fw + (size_t)FV_R1 * P;
     float* e = n.fw + (size_t)FV_E * P;
     float* w = n.fw + (size_t)FV_W * P;
     float* y = n.fw + (size_t)FV_Y * P;
@@ -437,28 +472,34 @@ int learned_fcg(Ctx& c, double shift, double rtol, int maxit, double delta, int*
         lc.tol2 = need * need;
         const double scale = 1.0 / rr;
         hipLaunchKernelGGL(k_f_init, dim3(g), dim3(FB), 0, s, D, (const double*)(c.wp + 6 * P), scale, r, e, partC, st);
+        hipLaunchKernelGGL(k_f_pre0, dim3(g), dim3(FB), 0, s, D, S, (const float*)r, w, (const double*)st);
         // a refinement round that needs more than 300 iterations means the cycle does not
         // fit this system: the round ends there and the progress test below decides
         lc.maxit = std::min(maxit - its, 300);
         // iterations are queued in chunks between convergence checks: the first chunk of a
         // round is the previous round's count, later ones the count the observed rate of
-        // decrease still needs (a queued iteration past convergence costs 18 launches
+        // decrease still needs (a queued iteration past convergence costs 17 launches
         // that return at once)
         int chunk = lastRound > 0 ? lastRound : 6, it = 0;
         while (true) {
             for (int k = 0; k < chunk; ++k) {
-                hipLaunchKernelGGL(k_f_pre, dim3(g), dim3(FB), 0, s, D, S, r, w, partC, g, st, lc);
-                restrictLaunch(s, D, w, n.vin, done);
+                const int nc = (D.nI / 2) * (D.nJ / 2) * (D.nK / 2);
+                hipLaunchKernelGGL(k_f_restrict, dim3((nc + 255) / 256), dim3(256), 0, s, D, (const float*)w, n.vin,
+                                   (const double*)partC, g, st, lc);
                 net_forward(n, s, n.vin, n.vout, done);
-                hipLaunchKernelGGL(k_tg_post1, dim3(g), dim3(FB), 0, s, D, S, r, n.vout, y, done);
-                hipLaunchKernelGGL(k_f_post2, dim3(g), dim3(FB), 0, s, D, S, r, y, z, q0, q1, partA, st);
+                hipLaunchKernelGGL(k_f_post1, dim3(g), dim3(FB), 0, s, D, S, (const float*)r, (const float*)r1,
+                                   (const float*)n.vout, y, (const double*)st);
+                hipLaunchKernelGGL(k_f_post2, dim3(g), dim3(FB), 0, s, D, S, (const float*)r, (const float*)r1, (const float*)y,
+                                   z, (const float*)q0, (const float*)q1, partA, (const double*)st);
                 hipEvent_t e0, e1;
                 if (c.probe.slot(PROBE_FCG_K1, e0, e1))   // dispatch-bracketing events: the kernel's own time
                     hipExtLaunchKernelGGL(k_f_k1, dim3(g), dim3(FB), 0, s, e0, e1, 0, D, S, (const float*)z,
                                           p0, p1, q0, q1, (const double*)partA, g, partB, st, lc);
                 else
                     hipLaunchKernelGGL(k_f_k1, dim3(g), dim3(FB), 0, s, D, S, z, p0, p1, q0, q1, partA, g, partB, st, lc);
-                hipLaunchKernelGGL(k_f_k2, dim3(g), dim3(FB), 0, s, D, e, r, p0, p1, q0, q1, partB, g, partC, st);
+                hipLaunchKernelGGL(k_f_k2pre, dim3(g), dim3(FB), 0, s, D, S, e, (const float*)r, (const float*)r1, w,
+                                   (const float*)p0, (const float*)p1, (const float*)q0, (const float*)q1,
+                                   (const double*)partB, g, partC, st);
             }
             HIPCHECK(hipMemcpyAsync(c.hscal + SC_CG, st, 8 * sizeof(double), hipMemcpyDeviceToHost, s));
             HIPCHECK(hipStreamSynchronize(s));

@@ -780,11 +780,18 @@ __global__ __launch_bounds__(256) void k_in_fwd(const float* x, const float* W, 
     }
 }
 
-// out: 16 -> 1 channel; W [27][1][16]
+// out: 16 -> 1 channel; W [27][1][16].  The BF16 activations meet the FP32 weights
+// through v_dot2_f32_bf16: each weight is split into a BF16 high part and a BF16
+// remainder (w = hi + lo to 2^-17 relative), so a tap is 16 dot2 operations instead
+// of 16 FMAs and 16 BF16 unpacks.
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float dot2(uint32_t a, uint32_t b, float c) {
+    return __builtin_amdgcn_fdot2_f32_bf16(*(const bf16x2*)&a, *(const bf16x2*)&b, c, false);
+}
 __global__ __launch_bounds__(256) void k_out_fwd(const uint16_t* a, const float* W, float* y, int nx, int ny, int nz,
                                                  const double* done) {
     if (isDone(done)) return;
-    __shared__ __align__(16) float sW[27 * 16];
+    __shared__ __align__(16) uint32_t sWh[27 * 8], sWl[27 * 8];   // [t][channel pair]: hi and lo parts
     __shared__ __align__(16) uint4 sa[EHV * 2];   // 16 BF16 channels per halo voxel
     int ox, oy, oz;
     edgeTile(nx, ny, ox, oy, oz);
@@ -800,7 +807,12 @@ __global__ __launch_bounds__(256) void k_out_fwd(const uint16_t* a, const float*
         const uint4 val = ((const uint4*)(a + (size_t)g * 16))[h];
         tv[u] = (in && it < NI) ? val : make_uint4(0u, 0u, 0u, 0u);
     }
-    for (int i = tid; i < 27 * 16; i += 256) sW[i] = W[i];
+    if (tid < 27 * 8) {
+        const float w0 = W[2 * tid], w1 = W[2 * tid + 1];
+        const uint32_t h0 = f2bf(w0), h1 = f2bf(w1);
+        sWh[tid] = h0 | (h1 << 16);
+        sWl[tid] = f2bf(w0 - bf2f(h0)) | (f2bf(w1 - bf2f(h1)) << 16);
+    }
 #pragma unroll
     for (int u = 0; u < NIT; ++u)
         if (tid + u * 256 < NI) sa[tid + u * 256] = tv[u];
@@ -808,17 +820,23 @@ __global__ __launch_bounds__(256) void k_out_fwd(const uint16_t* a, const float*
     const int tx = tid % ETX, ty = (tid / ETX) % ETY, tz = tid / (ETX * ETY);
     const int gx = ox + tx, gy = oy + ty, gz = oz + tz;
     const int hb = ((tz + 1) * EHY + (ty + 1)) * EHX + (tx + 1);
-    float s = 0.f;
+    float s[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll 3
     for (int t = 0; t < 27; ++t) {
         const int hv = hb + ((t / 9 - 1) * EHY + ((t / 3) % 3 - 1)) * EHX + (t % 3 - 1);
         const uint4 h0 = sa[2 * hv], h1 = sa[2 * hv + 1];
         const uint32_t hw[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
+        const uint4 wh0 = *(const uint4*)&sWh[t * 8], wh1 = *(const uint4*)&sWh[t * 8 + 4];
+        const uint4 wl0 = *(const uint4*)&sWl[t * 8], wl1 = *(const uint4*)&sWl[t * 8 + 4];
+        const uint32_t wh[8] = {wh0.x, wh0.y, wh0.z, wh0.w, wh1.x, wh1.y, wh1.z, wh1.w};
+        const uint32_t wl[8] = {wl0.x, wl0.y, wl0.z, wl0.w, wl1.x, wl1.y, wl1.z, wl1.w};
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
-            s += sW[t * 16 + 2 * q] * bf2f(hw[q] & 0xffffu) + sW[t * 16 + 2 * q + 1] * bf2f(hw[q] >> 16);
+        for (int q = 0; q < 8; ++q) {
+            s[q & 3] = dot2(hw[q], wh[q], s[q & 3]);
+            s[q & 3] = dot2(hw[q], wl[q], s[q & 3]);
+        }
     }
-    if (gx < nx && gy < ny && gz < nz) y[(gz * ny + gy) * nx + gx] = s;
+    if (gx < nx && gy < ny && gz < nz) y[(gz * ny + gy) * nx + gx] = (s[0] + s[1]) + (s[2] + s[3]);
 }
 
 // gradient of the out layer's input: ga[v][ci] += sum_t W[t][0][ci] g[v - off(t)]
