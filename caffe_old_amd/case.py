@@ -10,7 +10,9 @@ A :class:`Case` carries what the reference reads from ``case/case.info``,
 * ``Boundaries.{east..bottom}.{type, refVector, refValue}``
   (FlowBoundaryConditions.cpp:25-42),
 * ``ImmersedBoundaries.Sphere.{radius, center}`` (IbPiso.cpp:37-38),
-* uniform initial ``u`` and ``p`` (InitialConditions.cpp:12-74, uniform only),
+* uniform initial ``u`` and ``p`` (InitialConditions.cpp:12-74, uniform only); for the
+  Diffusion solver ``p_init`` is the initial ``phi`` and the boundaries are fixed /
+  zeroGradient / empty with refValue,
 * the node array of the structured mesh, (nK+1, nJ+1, nI+1, 3) in C order so that
   i runs fastest in memory, as ``Array3D`` stores it (Array3D.tpp:40-58).
 
@@ -29,9 +31,13 @@ from typing import List, Optional, Sequence, Tuple
 import numpy as np
 
 FACES = ("east", "west", "north", "south", "top", "bottom")
-# FlowBoundaryConditions::Type (FlowBoundaryConditions.h:10), same strings as case.info
-BC_TYPES = {"inlet": 0, "outlet": 1, "wall": 2, "empty": 3, "zeroGradient": 4}
-ALGOS = {"simple": 0, "piso": 1, "ibpiso": 2}
+# FlowBoundaryConditions::Type (FlowBoundaryConditions.h:10), same strings as case.info;
+# "fixed" is the PrimitiveBoundaryCondition type the Diffusion solver reads
+# (PrimitiveBoundaryCondition.cpp:33-58), where zeroGradient and empty also apply
+BC_TYPES = {"inlet": 0, "outlet": 1, "wall": 2, "empty": 3, "zeroGradient": 4, "fixed": 5}
+FLOW_BC = ("inlet", "outlet", "wall", "empty", "zeroGradient")
+SCALAR_BC = ("fixed", "zeroGradient", "empty")
+ALGOS = {"simple": 0, "piso": 1, "ibpiso": 2, "diffusion": 3}
 
 
 @dataclass
@@ -74,6 +80,11 @@ class Case:
             raise ValueError("six boundaries (east, west, north, south, top, bottom) are required")
         if self.algo == "ibpiso" and self.sphere is None:
             raise ValueError("ibpiso needs an immersed sphere")
+        allowed = SCALAR_BC if self.algo == "diffusion" else FLOW_BC
+        for b in self.boundaries:
+            if b.type not in allowed:
+                # FlowBoundaryConditions.cpp:41-42 / PrimitiveBoundaryCondition.cpp:49-50
+                raise ValueError(f'invalid boundary type "{b.type}" for the {self.algo} solver')
 
     @property
     def shape(self) -> Tuple[int, int, int]:

@@ -417,6 +417,8 @@ def case_from_info(params: InfoNode, ics: InfoNode, nodes: np.ndarray, algo: str
     Piso.cpp:9 (pressure corrections), IbPiso.cpp:37-38 (the sphere)."""
     rc = RunControl(params)
     unsteady = params.get("Solver.timeAccurate") == "ON"
+    if algo == "diffusion":
+        return _diffusion_case(params, ics, nodes, rc, unsteady)
     bounds = []
     for f in FACES:
         typ = params.get(f"Boundaries.{f}.type")
@@ -438,6 +440,21 @@ def case_from_info(params: InfoNode, ics: InfoNode, nodes: np.ndarray, algo: str
     return Case(nodes=nodes, algo=algo, time_accurate=unsteady, rho=rho, mu=mu, alpha_u=alpha_u, alpha_p=alpha_p,
                 n_inner=n_inner, n_pcorr=n_pcorr, boundaries=bounds, u_init=tuple(u0), p_init=float(p0),
                 sphere=sphere, time_step=rc.time_step, max_iterations=rc.n_solves() + 1)
+
+
+def _diffusion_case(params: InfoNode, ics: InfoNode, nodes: np.ndarray, rc: RunControl, unsteady: bool) -> Case:
+    """The Diffusion solver's reads (Diffusion.cpp:8-27): the boundary condition of phi
+    takes each face's type (fixed / zeroGradient / empty) then its refValue
+    (PrimitiveBoundaryCondition.tpp:13-27); then the initial phi."""
+    bounds = []
+    for f in FACES:
+        typ = params.get(f"Boundaries.{f}.type")
+        if typ not in ("fixed", "zeroGradient", "empty"):
+            raise ValueError(f'invalid boundary type "{typ}".')
+        bounds.append(Boundary(typ, (0.0, 0.0, 0.0), params.get(f"Boundaries.{f}.refValue", float)))
+    phi0 = _initial(ics, "phi", False)
+    return Case(nodes=nodes, algo="diffusion", time_accurate=unsteady, boundaries=bounds, p_init=float(phi0),
+                time_step=rc.time_step, max_iterations=rc.n_solves() + 1)
 
 
 def load_case(case_dir: str, algo: str):

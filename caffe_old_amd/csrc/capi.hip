@@ -102,9 +102,14 @@ void upload(Ctx& c, double* dst, const std::vector<double>& src) {
 int setup(Ctx& c, const caffe_case* cs, const double* nodes) {
     if (!cs || !nodes) { c.err = "null case or nodes"; return CAFFE_ERR_ARG; }
     if (cs->nI < 1 || cs->nJ < 1 || cs->nK < 1) { c.err = "cell counts must be positive"; return CAFFE_ERR_ARG; }
-    if (cs->algo < 0 || cs->algo > 2) { c.err = "unknown algorithm"; return CAFFE_ERR_ARG; }
-    for (int f = 0; f < 6; ++f)
-        if (cs->bcType[f] < 0 || cs->bcType[f] > 4) { c.err = "invalid boundary type"; return CAFFE_ERR_ARG; }
+    if (cs->algo < 0 || cs->algo > 3) { c.err = "unknown algorithm"; return CAFFE_ERR_ARG; }
+    for (int f = 0; f < 6; ++f) {
+        const int t = cs->bcType[f];
+        const bool ok = cs->algo == CAFFE_DIFFUSION
+            ? (t == CAFFE_BC_FIXED || t == CAFFE_BC_ZERO_GRADIENT || t == CAFFE_BC_EMPTY)   // PrimitiveBoundaryCondition.tpp:17-24
+            : (t >= 0 && t <= 4);                                                          // FlowBoundaryConditions.cpp:31-42
+        if (!ok) { c.err = "invalid boundary type"; return CAFFE_ERR_ARG; }
+    }
     if (cs->alphaU == 0. || (!cs->steady && false)) { c.err = "relaxationFactorMomentum must be nonzero"; return CAFFE_ERR_ARG; }
     freeAll(c);
     c.cs = *cs;
@@ -117,6 +122,19 @@ int setup(Ctx& c, const caffe_case* cs, const double* nodes) {
     mapTypes(c);
     c.hasIB = (cs->algo == CAFFE_IBPISO);
     c.singularP = !c.hasIB;
+    c.indefiniteP = false;
+    if (cs->algo == CAFFE_DIFFUSION) {
+        // phi uses the p' storage and solver: singular when steady without a fixed face,
+        // indefinite when unsteady (a0 = -sum a_nb + V/dt, Diffusion.cpp:60)
+        for (int f = 0; f < 6; ++f) {
+            const int t = cs->bcType[f];
+            c.pT[f] = t == CAFFE_BC_FIXED ? BC_FIXED : (t == CAFFE_BC_EMPTY ? BC_EMPTY : BC_ZERO_GRADIENT);
+            c.uT[f] = BC_FIXED;
+            c.massCorr[f] = false;
+        }
+        c.singularP = cs->steady != 0;
+        c.indefiniteP = cs->steady == 0;
+    }
     for (int f = 0; f < 6; ++f) if (c.pT[f] == BC_FIXED) c.singularP = false;
 
     // ---- mesh
@@ -189,6 +207,9 @@ int setup(Ctx& c, const caffe_case* cs, const double* nodes) {
     double up[18], pp[6];
     initialPatch(c.uT, uref, 3, up);
     initialPatch(c.pT, pref, 1, pp);
+    // the Diffusion solver's boundary condition sets all six patches to their refValue once
+    // (PrimitiveBoundaryCondition.tpp:3-30)
+    if (cs->algo == CAFFE_DIFFUSION) for (int f = 0; f < 6; ++f) pp[f] = pref[f];
     std::vector<double> host(D.P);
     for (int q = 0; q < 3; ++q) {
         std::fill(host.begin(), host.end(), 0.);
@@ -277,8 +298,27 @@ int ibPrepare(Ctx& c) {
     return CAFFE_OK;
 }
 
+// Diffusion::solve (Diffusion.cpp:33-131): assemble, solve (from x = 0), phi = x, the
+// boundary patches, the extrapolated-face gradient; returns ||A x - b||_inf
+int diffusionStep(Ctx& c, double dt, double* resInf) {
+    launch_diff_assemble(c, dt);
+    int rc = pcorr_solve(c, c.pcOpts, &c.lastPc);
+    if (rc != CAFFE_OK && rc != CAFFE_ERR_NOT_CONVERGED) return rc;
+    launch_copy_cells(c, c.xp, c.p);
+    double* f[1] = {c.p};
+    launch_halo(c, f, 1, c.pT);
+    double* g1[3] = {c.gpc[0], c.gpc[1], c.gpc[2]};
+    double* g[3] = {c.gp[0], c.gp[1], c.gp[2]};
+    launch_grad_scalar(c, c.p, g1);
+    launch_grad_extrap(c, c.p, g1, g);
+    const double r = diff_resid_inf(c, c.p);
+    if (resInf) *resInf = r;
+    return CAFFE_OK;
+}
+
 int solveStep(Ctx& c, double dt, double* continuity) {
     if (!c.ready) { c.err = "caffe_case_setup has not been called"; return CAFFE_ERR_STATE; }
+    if (c.cs.algo == CAFFE_DIFFUSION) return diffusionStep(c, dt, continuity);
     saveU0(c);
     int rc;
     if (c.hasIB && (rc = ibPrepare(c)) != CAFFE_OK) return rc;
@@ -311,7 +351,8 @@ bool fieldByName(Ctx& c, const std::string& n, FieldRef& f) {
     else if (n == "gradP") set3(c.gp);
     else if (n == "gradPCorr") set3(c.gpc);
     else if (n == "mI" || n == "mJ" || n == "mK") { f.ptr[0] = c.mF[n == "mI" ? 0 : (n == "mJ" ? 1 : 2)]; f.nc = 1; }
-    else if (n == "p") { f.ptr[0] = c.p; f.nc = 1; }
+    else if (n == "p" || n == "phi") { f.ptr[0] = c.p; f.nc = 1; }
+    else if (n == "gradPhi") set3(c.gp);
     else if (n == "pCorr") { f.ptr[0] = c.pc; f.nc = 1; }
     else if (n == "d") { f.ptr[0] = c.d; f.nc = 1; }
     else if (n == "massFlow") { f.ptr[0] = c.mflow; f.nc = 1; }

@@ -312,6 +312,9 @@ __global__ void k_copy_noninactive(Dims D, const int8_t* st, const double* src, 
     const long p = D.cell(q, i, j, k);
     if (st[p] != ST_INACTIVE) dst[p] = src[p];
 }
+void launch_copy_cells(Ctx& c, const double* src, double* dst) {
+    hipLaunchKernelGGL(k_copy_noninactive, dim3(blocksFor(c.D.N, BS)), dim3(BS), 0, c.stream, c.D, c.st, src, dst);
+}
 void launch_pc_copyback(Ctx& c) {
     hipLaunchKernelGGL(k_copy_noninactive, dim3(blocksFor(c.D.N, BS)), dim3(BS), 0, c.stream, c.D, c.st, c.xp, c.pc);
 }
@@ -615,6 +618,148 @@ static double maxReduce(Ctx& c, int mode, double dt) {
 }
 double continuity_max(Ctx& c) { return maxReduce(c, 0, 0.); }
 double courant(Ctx& c, double dt) { return maxReduce(c, 1, dt); }
+
+// ------------------------------------------------------------------ Diffusion
+// Diffusion::solve assembly loop (Diffusion.cpp:40-75): a_nb = d_f (the face metric),
+// a0 = -(sum a_nb) + a0P with a0P = V/dt when unsteady (the sign as in the reference),
+// b = a0P phi; the fold of PrimitiveBoundaryCondition::setImplicitBoundaryCoefficients
+// (PrimitiveBoundaryCondition.tpp:144-255) in east..bottom order.  Stored like the p'
+// system: aP and each cell's low-face coefficient, zero on domain faces.
+struct DiffArgs { const double* phi; double* A; double* b; double dt; int unsteady; int pT[6]; };
+
+template <bool U>
+__global__ void k_diff_assemble(Geo g, DiffArgs a) {
+    const Dims& D = g.D;
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= D.N) return;
+    int i, j, k;
+    const long p = D.cell(q, i, j, k);
+    const long P = D.P, sJ = D.sJ, sK = D.sK;
+    const bool atB[7] = {false, i == D.nI - 1, i == 0, j == D.nJ - 1, j == 0, k == D.nK - 1, k == 0};
+    double A[7];
+    A[1] = G<U>::d(g, 0, p + 1, atB[1]);
+    A[2] = G<U>::d(g, 0, p, atB[2]);
+    A[3] = G<U>::d(g, 1, p + sJ, atB[3]);
+    A[4] = G<U>::d(g, 1, p, atB[4]);
+    A[5] = G<U>::d(g, 2, p + sK, atB[5]);
+    A[6] = G<U>::d(g, 2, p, atB[6]);
+    const double a0P = a.unsteady ? G<U>::vol(g, p) / a.dt : 0.;
+    A[0] = -(A[1] + A[2] + A[3] + A[4] + A[5] + A[6]) + a0P;
+    double b = a0P * a.phi[p];
+    const long nb[7] = {0, p + 1, p - 1, p + sJ, p - sJ, p + sK, p - sK};
+#pragma unroll
+    for (int f = 1; f < 7; ++f) {
+        if (!atB[f]) continue;
+        if (a.pT[f - 1] == BC_FIXED) b -= A[f] * a.phi[nb[f]];
+        else { A[0] += A[f]; A[f] = 0.; }
+    }
+    a.A[p] = A[0];
+    a.b[p] = b;
+    a.A[P + p] = atB[2] ? 0. : A[2];
+    a.A[2 * P + p] = atB[4] ? 0. : A[4];
+    a.A[3 * P + p] = atB[6] ? 0. : A[6];
+    if (atB[1]) a.A[P + p + 1] = 0.;
+    if (atB[3]) a.A[2 * P + p + sJ] = 0.;
+    if (atB[5]) a.A[3 * P + p + sK] = 0.;
+}
+
+void launch_diff_assemble(Ctx& c, double dt) {
+    DiffArgs a;
+    a.phi = c.p; a.A = c.Ap; a.b = c.bp; a.dt = dt; a.unsteady = c.cs.steady ? 0 : 1;
+    for (int f = 0; f < 6; ++f) a.pT[f] = c.pT[f];
+    const int nb = blocksFor(c.D.N, BS);
+    if (c.uniform) hipLaunchKernelGGL(k_diff_assemble<true>, dim3(nb), dim3(BS), 0, c.stream, c.geo, a);
+    else hipLaunchKernelGGL(k_diff_assemble<false>, dim3(nb), dim3(BS), 0, c.stream, c.geo, a);
+}
+
+// FvScalarScheme::extrapolateInteriorFaces (FvScalarScheme.cpp:70-160) followed by the
+// divergence-theorem gradient (FvScalarScheme.cpp:48-66), one domain.  An interior face
+// takes 0.5 (phi_L + grad_L . r_L + phi_H + grad_H . r_H) from the gradient of the
+// interpolated faces (g1), clamped to [min, max] of its low-side cell and that cell's six
+// neighbours (the reference sets it while visiting the low-side cell); domain faces keep
+// the patch value.
+template <bool U>
+__device__ __forceinline__ double cellMax(const double* f, long p, const Dims& D, bool mx) {
+    const long o[6] = {1, -1, D.sJ, -D.sJ, D.sK, -D.sK};
+    double v = f[p];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) v = mx ? fmax(v, f[p + o[q]]) : fmin(v, f[p + o[q]]);
+    return v;
+}
+template <bool U>
+__device__ __forceinline__ double extrapFace(const Geo& g, const double* f, const double* const* g1, int dir, long lo,
+                                             int il, int jl, int kl) {
+    // the face between cell lo (indices il, jl, kl) and its high-side neighbour
+    const Dims& D = g.D;
+    const long s = dir == 0 ? 1 : (dir == 1 ? D.sJ : D.sK);
+    const long hi = lo + s;
+    const int ih = il + (dir == 0), jh = jl + (dir == 1), kh = kl + (dir == 2);
+    const V3 fc = G<U>::fc(g, dir, ih, jh, kh, hi);
+    const V3 rl = fc - G<U>::xc(g, il, jl, kl, lo), rh = fc - G<U>::xc(g, ih, jh, kh, hi);
+    const double dl = g1[0][lo] * rl.x + g1[1][lo] * rl.y + g1[2][lo] * rl.z;
+    const double dh = g1[0][hi] * rh.x + g1[1][hi] * rh.y + g1[2][hi] * rh.z;
+    const double v = 0.5 * (f[lo] + dl + f[hi] + dh);
+    const double up = cellMax<U>(f, lo, D, true), dn = cellMax<U>(f, lo, D, false);
+    return fmax(fmin(up, v), dn);
+}
+struct ExtrapArgs { const double* phi; const double* g1[3]; double* o[3]; };
+template <bool U>
+__global__ void k_grad_extrap(Geo g, ExtrapArgs a) {
+    const Dims& D = g.D;
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= D.N) return;
+    int ijk[3];
+    const long p = D.cell(q, ijk[0], ijk[1], ijk[2]);
+    const long str[3] = {1, D.sJ, D.sK};
+    const int nn[3] = {D.nI, D.nJ, D.nK};
+    V3 acc = mk(0., 0., 0.);
+    bool first = true;
+#pragma unroll
+    for (int dir = 0; dir < 3; ++dir) {
+        const long s = str[dir];
+        const double fe = ijk[dir] < nn[dir] - 1 ? extrapFace<U>(g, a.phi, a.g1, dir, p, ijk[0], ijk[1], ijk[2]) : a.phi[p + s];
+        const double fw = ijk[dir] > 0
+            ? extrapFace<U>(g, a.phi, a.g1, dir, p - s, ijk[0] - (dir == 0), ijk[1] - (dir == 1), ijk[2] - (dir == 2))
+            : a.phi[p - s];
+        V3 te = G<U>::S(g, dir, p + s) * fe;
+        V3 tw = (-G<U>::S(g, dir, p)) * fw;
+        if (first) { acc = te + tw; first = false; }
+        else { acc = acc + te; acc = acc + tw; }
+    }
+    acc = acc / G<U>::vol(g, p);
+    a.o[0][p] = acc.x; a.o[1][p] = acc.y; a.o[2][p] = acc.z;
+}
+void launch_grad_extrap(Ctx& c, const double* phi, double* const* g1, double* const* out) {
+    ExtrapArgs a;
+    a.phi = phi;
+    for (int q = 0; q < 3; ++q) { a.g1[q] = g1[q]; a.o[q] = out[q]; }
+    const int nb = blocksFor(c.D.N, BS);
+    if (c.uniform) hipLaunchKernelGGL(k_grad_extrap<true>, dim3(nb), dim3(BS), 0, c.stream, c.geo, a);
+    else hipLaunchKernelGGL(k_grad_extrap<false>, dim3(nb), dim3(BS), 0, c.stream, c.geo, a);
+}
+
+// ||A x - b||_inf over the rows (Diffusion.cpp:127-130: scale(-1, b); res = A x + b)
+__global__ void k_diff_resid(Dims D, const double* A, const double* b, const double* x, double* part) {
+    __shared__ double lds[BS / 64];
+    const long P = D.P, sJ = D.sJ, sK = D.sK;
+    double mx = 0.;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < D.N; q += (long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        const long p = D.cell(q, i, j, k);
+        const double ax = A[p] * x[p] + A[P + p] * x[p - 1] + A[P + p + 1] * x[p + 1] + A[2 * P + p] * x[p - sJ]
+                        + A[2 * P + p + sJ] * x[p + sJ] + A[3 * P + p] * x[p - sK] + A[3 * P + p + sK] * x[p + sK];
+        const double r = fabs(ax + (-b[p]));
+        mx = r > mx ? r : mx;
+    }
+    const double r = blockMax<BS>(mx, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+double diff_resid_inf(Ctx& c, const double* x) {
+    const int nb = std::min(blocksFor(c.D.N, BS), 1024);
+    hipLaunchKernelGGL(k_diff_resid, dim3(nb), dim3(BS), 0, c.stream, c.D, (const double*)c.Ap, (const double*)c.bp, x, c.red.part);
+    hipLaunchKernelGGL(k_max_final, dim3(1), dim3(BS), 0, c.stream, c.red.part, nb, c.scal + SC_MISC + 1);
+    return readScalar(c, SC_MISC + 1);
+}
 
 // plain copy of n doubles: the reference kernel that calibrates HBM byte counters
 __global__ void k_copy(const double* __restrict__ a, double* __restrict__ b, int n) {

@@ -45,6 +45,7 @@ struct Ctx {
     bool uniform = true;
     bool hasIB = false;
     bool singularP = false;   // pure Neumann p' system (null space = constants)
+    bool indefiniteP = false; // the Sym4 system is symmetric but indefinite (unsteady Diffusion): BiCGStab, not CG
     int uT[6], pT[6], flowT[6];
     bool massCorr[6];
 
@@ -126,6 +127,12 @@ void launch_grad_vector(Ctx& c);
 void launch_correct(Ctx& c);
 double continuity_quirk(Ctx& c);
 double continuity_max(Ctx& c);
+// Diffusion (Diffusion.cpp:33-131): assembly into the p' storage, the extrapolated-face
+// gradient of phi, and ||A phi - b||_inf
+void launch_diff_assemble(Ctx& c, double dt);
+void launch_copy_cells(Ctx& c, const double* src, double* dst);   // non-INACTIVE cells
+void launch_grad_extrap(Ctx& c, const double* phi, double* const* g1, double* const* out);
+double diff_resid_inf(Ctx& c, const double* x);
 double courant(Ctx& c, double dt);
 void launch_fill(Ctx& c, double* a, double v, long n);
 double time_copy(Ctx& c, double* a, double* b, int n, int reps);

@@ -742,7 +742,7 @@ int pcorr_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep) {
     const double shift = c.singularP ? meanOf(c, c.bp) : 0.;
     int iters = 0, refs = 0, fell = 0;
     double rel = 1.;
-    const bool sym = !c.hasIB;
+    const bool sym = !c.hasIB && !c.indefiniteP;
     hipLaunchKernelGGL(k_zero_d, dim3(blocksFor(P, KBS)), dim3(KBS), 0, c.stream, c.xp, P);
     if (o.mode == CAFFE_SOLVE_PLAIN_FP64) {
         if (sym) iters += cgSolve<double>(c, shift, o.rtol, o.maxit, 0., &refs, &rel);

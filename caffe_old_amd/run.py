@@ -3,15 +3,17 @@
 
     python -m caffe_old_amd.run hexamesh CASE_DIR
     python -m caffe_old_amd.run {simple,piso,ibpiso} CASE_DIR [--pcorr plain|ir|learned] [--momentum plain|ir]
+    python -m caffe_old_amd.run diffusion CASE_DIR
 
 ``hexamesh`` does what Modules/hexaMesh.cpp:8-28 does: read mesh/structuredMesh.info,
 generate the nodes, check the faces are planar, and write mesh/structuredMesh.dat.
-The solver commands follow Modules/caffeSimple.cpp, caffePiso.cpp and
-caffeIbPiso.cpp: read case/case.info, case/initialConditions.info and
+The solver commands follow Modules/caffeSimple.cpp, caffePiso.cpp, caffeIbPiso.cpp
+and caffeDiffusion.cpp: read case/case.info, case/initialConditions.info and
 mesh/structuredMesh.dat, run the RunControl loop (RunControl.cpp:53-84) with one
 solver call per iteration, and write solution/<mesh name>.dat every
 fileWriteInterval iterations and once more at the end, with the fields the
-reference's solvers register for output: p and massFlow (scalars) and u (vector).
+reference's solvers register for output: p and massFlow (scalars) and u (vector), or
+for the Diffusion solver phi and gradPhi (Diffusion.cpp:21-22).
 
 The solves are converged to a true FP64 relative residual (--rtol, default 1e-8); the
 reference's PETSc solves stop at rtol 1e-5 on the preconditioned norm.
@@ -41,12 +43,15 @@ def hexamesh(case_dir: str) -> str:
 
 
 def cell_fields(solver):
-    """Interior cell values of p, massFlow and u, (nK, nJ, nI) per component."""
+    """Interior cell values of the output fields, (nK, nJ, nI) per component."""
     nI, nJ, nK = solver.shape
 
     def inner(a):
         return a.reshape(nK + 2, nJ + 2, nI + 2)[1:-1, 1:-1, 1:-1]
 
+    if solver.case.algo == "diffusion":
+        g = solver.get("gradPhi")
+        return {"phi": inner(solver.get("phi"))}, {"gradPhi": [inner(g[c]) for c in range(3)]}
     u = solver.get("u")
     scal = {"p": inner(solver.get("p")), "massFlow": inner(solver.get("massFlow"))}
     vec = {"u": [inner(u[c]) for c in range(3)]}
@@ -76,7 +81,8 @@ def run(algo: str, case_dir: str, pcorr: str = "plain", momentum: str = "plain",
             print(f"no trained network for {nI}x{nJ}x{nK}: learned solves fall back to plain FP64")
     sol_dir = os.path.join(case_dir, "solution")
     os.makedirs(sol_dir, exist_ok=True)
-    writer = caseio.Tec360Writer(os.path.join(sol_dir, name + ".dat"), name, case.nodes, ["p", "massFlow"], ["u"])
+    out_vars = (["phi"], ["gradPhi"]) if algo == "diffusion" else (["p", "massFlow"], ["u"])
+    writer = caseio.Tec360Writer(os.path.join(sol_dir, name + ".dat"), name, case.nodes, *out_vars)
     t0 = time.perf_counter()
     n = 0
     while rc.continue_run():
@@ -98,7 +104,7 @@ def run(algo: str, case_dir: str, pcorr: str = "plain", momentum: str = "plain",
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("module", choices=["hexamesh", "simple", "piso", "ibpiso"])
+    ap.add_argument("module", choices=["hexamesh", "simple", "piso", "ibpiso", "diffusion"])
     ap.add_argument("case_dir", nargs="?", default=".")
     ap.add_argument("--pcorr", choices=["plain", "ir", "learned"], default="plain")
     ap.add_argument("--momentum", choices=["plain", "ir"], default="plain")

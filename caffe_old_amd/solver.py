@@ -24,7 +24,7 @@ def _dp(a):
 class Solver:
     """One GPU-resident solver instance (the reference's Solver subclasses)."""
 
-    NCOMP = {"u": 3, "u0": 3, "h": 3, "gradP": 3, "gradPCorr": 3, "gradU": 9}
+    NCOMP = {"u": 3, "u0": 3, "h": 3, "gradP": 3, "gradPCorr": 3, "gradU": 9, "gradPhi": 3}
 
     def __init__(self, case: Case, device: int = 0, uniform: bool | None = None):
         self.lib = L.load()
@@ -235,8 +235,22 @@ class IbPiso(Solver):
     """PISO with a ghost-cell immersed sphere (IbPiso.cpp)."""
 
 
+class Diffusion(Solver):
+    """Diffusion (Diffusion.cpp:33-131): one steady or unsteady diffusion solve of phi per
+    iteration; ``solve`` returns ||A x - b||_inf.  Fields "phi" and "gradPhi"; the solve
+    options are set as those of the pressure correction (``set_solve_opts("pcorr", ...)``)."""
+
+    def displayUpdateMessage(self) -> str:
+        m, p = self.reports()
+        msg = (f"Diffusion: completed iteration.\n"
+               f"Diffusion: solver iterations : {p['iters']}\n"
+               f"Diffusion: residual (inf)    : {self.continuityError:.6e}")
+        print(msg)
+        return msg
+
+
 def make_solver(case: Case, device: int = 0, uniform: bool | None = None) -> Solver:
-    cls = {"simple": Simple, "piso": Piso, "ibpiso": IbPiso}[case.algo]
+    cls = {"simple": Simple, "piso": Piso, "ibpiso": IbPiso, "diffusion": Diffusion}[case.algo]
     return cls(case, device=device, uniform=uniform)
 
 

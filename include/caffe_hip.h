@@ -51,9 +51,14 @@ extern "C" {
 typedef struct caffe_ctx caffe_ctx;
 
 /* FlowBoundaryConditions::Type (src/Domains/Field/FlowBoundaryConditions.h:10) */
-enum { CAFFE_BC_INLET = 0, CAFFE_BC_OUTLET = 1, CAFFE_BC_WALL = 2, CAFFE_BC_EMPTY = 3, CAFFE_BC_ZERO_GRADIENT = 4 };
+enum { CAFFE_BC_INLET = 0, CAFFE_BC_OUTLET = 1, CAFFE_BC_WALL = 2, CAFFE_BC_EMPTY = 3, CAFFE_BC_ZERO_GRADIENT = 4,
+       CAFFE_BC_FIXED = 5 /* PrimitiveBoundaryCondition FIXED: the Diffusion solver's boundaries
+                             (PrimitiveBoundaryCondition.tpp:3-30), with EMPTY and ZERO_GRADIENT */ };
 /* Solver family */
-enum { CAFFE_SIMPLE = 0, CAFFE_PISO = 1, CAFFE_IBPISO = 2 };
+enum { CAFFE_SIMPLE = 0, CAFFE_PISO = 1, CAFFE_IBPISO = 2,
+       CAFFE_DIFFUSION = 3 /* Diffusion (Diffusion.cpp:8-131): the scalar phi with pInit as its
+                              uniform initial value and bcVal as the refValues; caffe_solve returns
+                              ||A x - b||_inf; solve options are those of the p' system */ };
 /* Pressure-correction / momentum solve modes */
 enum {
     CAFFE_SOLVE_PLAIN_FP64 = 0,   /* preconditioned Krylov in FP64 (the judge and fallback) */

@@ -665,6 +665,9 @@ struct Oracle {
     Field<V3> u, u0, uStar, h, gradP, gradPC;
     Field<double> p, rho, mu, mflow, pC, d;
     Field<T3> gradU;
+    Field<double> phi;      // Diffusion: phiField_ (CONSERVED)
+    Field<V3> gradPhi;      // Diffusion: gradPhiField_ (AUXILLARY)
+    PrimBC<double> phiBC;
     PrimBC<V3> uBC, hBC;
     PrimBC<double> pBC, rhoBC, muBC, dBC, pCBC;
     int flowType[6];
@@ -687,6 +690,7 @@ struct Oracle {
         cs = *c;
         mesh.init(c->nI, c->nJ, c->nK, nodes);
         iMap.init(c->nI, c->nJ, c->nK);
+        if (c->algo == OC_DIFFUSION) { initDiffusion(c); return; }
         u.init(mesh, true); p.init(mesh, true); rho.init(mesh, true); mu.init(mesh, true);
         mflow.init(mesh, true); pC.init(mesh, true); gradU.init(mesh, true); gradP.init(mesh, true);
         gradPC.init(mesh, true); u0.init(mesh, false); uStar.init(mesh, false); d.init(mesh, true); h.init(mesh, true);
@@ -1012,7 +1016,88 @@ struct Oracle {
         }
     }
 
+    // ---- Diffusion (Diffusion.cpp:8-131)
+    // Diffusion::Diffusion: phi CONSERVED, gradPhi AUXILLARY, PrimitiveBoundaryCondition<double>
+    // (PrimitiveBoundaryCondition.tpp:3-30): every face's type (fixed / zeroGradient /
+    // empty) and refValue, then setFixedBoundaries once, so that all six patches and
+    // boundary faces start at their refValue; one 7-point matrix; uniform initial phi on
+    // the cells.
+    void initDiffusion(const oc_case* c) {
+        phi.init(mesh, true);
+        gradPhi.init(mesh, false);
+        phiBC.init(phi);
+        pcSingular = c->steady != 0;
+        for (int i = 0; i < 6; ++i) {
+            const int t = c->bcType[i];
+            phiBC.types[i] = t == OC_FIXED ? P_FIXED : (t == OC_EMPTY ? P_EMPTY : P_ZERO_GRADIENT);
+            phiBC.ref[i] = c->bcVal[i];
+            if (phiBC.types[i] == P_FIXED) pcSingular = false;
+        }
+        phiBC.setFixed();
+        A[0].allocate(iMap.nActive);
+        bv[0].assign(iMap.nActive, 0.);
+        xv[0].assign(iMap.nActive, 0.);
+        phi.cells.assign(c->pInit);
+    }
+    // FvScalarScheme::extrapolateInteriorFaces (FvScalarScheme.cpp:70-160), one domain:
+    // interpolate, gradient, then each interior E/N/T face from both cells' extrapolations,
+    // clamped to the range of the low-side cell and its six neighbours
+    void extrapolateFaces(Field<double>& F, Field<V3>& G) {
+        const Mesh& M = mesh;
+        interpolate(F);
+        gradScalar(F, G);
+        for (int k = 0; k < M.nK; ++k) for (int j = 0; j < M.nJ; ++j) for (int i = 0; i < M.nI; ++i) {
+            double up = F(i, j, k), lo = F(i, j, k);
+            const double nb[6] = {F(i + 1, j, k), F(i - 1, j, k), F(i, j + 1, k), F(i, j - 1, k), F(i, j, k + 1), F(i, j, k - 1)};
+            for (int q = 0; q < 6; ++q) { up = std::max(up, nb[q]); lo = std::min(lo, nb[q]); }
+            if (i < M.nI - 1) {
+                double v = 0.5 * (F(i, j, k) + dot(G(i, j, k), M.rfE(i, j, k)) + F(i + 1, j, k) + dot(G(i + 1, j, k), M.rfW(i + 1, j, k)));
+                F.faceE(i, j, k) = std::max(std::min(up, v), lo);
+            }
+            if (j < M.nJ - 1) {
+                double v = 0.5 * (F(i, j, k) + dot(G(i, j, k), M.rfN(i, j, k)) + F(i, j + 1, k) + dot(G(i, j + 1, k), M.rfS(i, j + 1, k)));
+                F.faceN(i, j, k) = std::max(std::min(up, v), lo);
+            }
+            if (k < M.nK - 1) {
+                double v = 0.5 * (F(i, j, k) + dot(G(i, j, k), M.rfT(i, j, k)) + F(i, j, k + 1) + dot(G(i, j, k + 1), M.rfB(i, j, k + 1)));
+                F.faceT(i, j, k) = std::max(std::min(up, v), lo);
+            }
+        }
+    }
+    // Diffusion::solve (Diffusion.cpp:33-131); returns ||A x - b||_inf
+    double solveDiffusion(double dt) {
+        const Mesh& M = mesh;
+        const bool unsteady = cs.steady == 0;
+        for (int k = 0; k < M.nK; ++k) for (int j = 0; j < M.nJ; ++j) for (int i = 0; i < M.nI; ++i) {
+            const double a0P = unsteady ? M.vol(i, j, k) / dt : 0.;
+            double a[7];
+            a[1] = M.dE(i, j, k); a[2] = M.dW(i, j, k); a[3] = M.dN(i, j, k);
+            a[4] = M.dS(i, j, k); a[5] = M.dT(i, j, k); a[6] = M.dB(i, j, k);
+            a[0] = -(a[1] + a[2] + a[3] + a[4] + a[5] + a[6]) + a0P;
+            double b = a0P * phi(i, j, k);
+            phiBC.fold(i, j, k, a, b);
+            const int cols[7] = {iMap(i, j, k), iMap(i + 1, j, k), iMap(i - 1, j, k), iMap(i, j + 1, k),
+                                 iMap(i, j - 1, k), iMap(i, j, k + 1), iMap(i, j, k - 1)};
+            A[0].setRow(iMap(i, j, k), 7, cols, a);
+            bv[0][iMap(i, j, k)] = b;
+        }
+        std::fill(xv[0].begin(), xv[0].end(), 0.);
+        itersPC = kspSolve(0, 0, pcSingular);
+        for (int k = 0; k < M.nK; ++k) for (int j = 0; j < M.nJ; ++j) for (int i = 0; i < M.nI; ++i)
+            phi(i, j, k) = xv[0][iMap(i, j, k)];
+        phiBC.setBoundaries();
+        extrapolateFaces(phi, gradPhi);
+        gradScalar(phi, gradPhi);
+        // scale(-1, b); multiplyAdd(A, x, b, res): res = A x - b
+        std::vector<double> ax(bv[0].size());
+        spmv(A[0], xv[0], ax, nt());
+        double r = 0.;
+        for (size_t q = 0; q < ax.size(); ++q) r = std::max(r, std::fabs(ax[q] + (-bv[0][q])));
+        return r;
+    }
+
     double solve(double dt) {
+        if (cs.algo == OC_DIFFUSION) return solveDiffusion(dt);
         u0.copyFrom(u);
         if (cs.algo == OC_IBPISO) computeIbField();
         int nInner = cs.steady ? 1 : cs.nInner;  // Simple.cpp:61-64
@@ -1112,6 +1197,11 @@ long oc_get(void* h, const char* name, double* out) {
         Field<double>& F = n == "p" ? o.p : n == "pCorr" ? o.pC : n == "d" ? o.d : o.mflow;
         o.exportCells(F, out, sc);
         return (long)P;
+    }
+    if (n == "phi") { o.exportCells(o.phi, out, sc); return (long)P; }
+    if (n == "gradPhi") {
+        for (int c = 0; c < 3; ++c) o.exportCells(o.gradPhi, out + c * P, comp(c));
+        return (long)(3 * P);
     }
     if (n == "gradU") {
         for (int c = 0; c < 9; ++c) o.exportCells(o.gradU, out + c * P, [c](const T3& t) { return t.a[c]; });

@@ -22,9 +22,10 @@ extern "C" {
 #endif
 
 /* FlowBoundaryConditions::Type (FlowBoundaryConditions.h:10) */
-enum { OC_INLET = 0, OC_OUTLET = 1, OC_WALL = 2, OC_EMPTY = 3, OC_ZERO_GRADIENT = 4 };
+enum { OC_INLET = 0, OC_OUTLET = 1, OC_WALL = 2, OC_EMPTY = 3, OC_ZERO_GRADIENT = 4,
+       OC_FIXED = 5 /* PrimitiveBoundaryCondition FIXED, the Diffusion solver's boundaries */ };
 /* Solver family (Simple.h, Piso.h, IbPiso.h) */
-enum { OC_SIMPLE = 0, OC_PISO = 1, OC_IBPISO = 2 };
+enum { OC_SIMPLE = 0, OC_PISO = 1, OC_IBPISO = 2, OC_DIFFUSION = 3 /* Diffusion.h */ };
 /* Linear solver used in place of PETSc KSPBCGS + PCSOR (SparseMatrix.cpp:86-106) */
 enum { OC_KSP_PETSC_LIKE = 0, OC_KSP_TIGHT = 1 };
 

@@ -167,7 +167,7 @@ class Oracle:
         return lambda *a: fn(self.h, *a)
 
     # -- state in the padded device layout
-    _NC = {"u": 3, "u0": 3, "h": 3, "gradP": 3, "gradPCorr": 3, "gradU": 9}
+    _NC = {"u": 3, "u0": 3, "h": 3, "gradP": 3, "gradPCorr": 3, "gradU": 9, "gradPhi": 3}
 
     def get(self, name):
         nc = self._NC.get(name, 1)

@@ -294,3 +294,21 @@ def test_tec360_read_back(tmp_path):
         assert np.array_equal(z["p"], r16(sc["p"])) and np.array_equal(z["massFlow"], r16(sc["massFlow"]))
         for c, nm in enumerate(("u_x", "u_y", "u_z")):
             assert np.array_equal(z[nm], r16(ve["u"][c]))
+
+
+def test_diffusion_deck():
+    # FlatPlateDiffusion2D: the Diffusion solver's boundaries are fixed / empty with a
+    # refValue each (PrimitiveBoundaryCondition.tpp:13-27), the initial field is phi
+    d = _case("FlatPlateDiffusion2D")
+    p = io.read_info(os.path.join(d, "case", "case.info"))
+    ics = io.read_info(os.path.join(d, "case", "initialConditions.info"))
+    name, v, res = io.read_mesh_info(os.path.join(d, "mesh", "structuredMesh.info"))
+    c = io.case_from_info(p, ics, io.generate_mesh(v, (9, 5, 2)), "diffusion")
+    assert c.algo == "diffusion" and not c.time_accurate
+    assert [(b.type, b.refValue) for b in c.boundaries] == [("fixed", 0.0), ("fixed", 1.0), ("fixed", 0.0),
+                                                            ("fixed", 0.0), ("empty", 0.0), ("empty", 0.0)]
+    assert c.p_init == 0.0 and c.n_solves == 1
+    # a flow solver fails on the deck where FlowBoundaryConditions does: it reads
+    # refVector after refValue (FlowBoundaryConditions.cpp:27-29)
+    with pytest.raises(io.InfoError, match=r"Boundaries\.east\.refVector"):
+        io.case_from_info(p, ics, io.generate_mesh(v, (9, 5, 2)), "simple")
