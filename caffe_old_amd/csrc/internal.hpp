@@ -77,6 +77,7 @@ struct Ctx {
     // FP32 momentum solves inside FP64 refinement (allocated at first use): the band in
     // FP32 (7 * P) and the BiCGStab work block plus the correction (33 * P)
     float* Am32 = nullptr;
+    int momChunk[4] = {4, 4, 4, 4};   // BiCGStab iterations per momentum FP32 round in the last solve
     float* wm32 = nullptr;
     struct { double* part = nullptr; } red;  // 3 partial-sum buffers of MAX_RED_BLOCKS rows
     double* scal = nullptr;   // N_SCAL device scalars

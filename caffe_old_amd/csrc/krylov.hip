@@ -226,18 +226,13 @@ __global__ void k_bi_init(Dims D, Op A, const double* b, double shift, T* r, T* 
     }
 }
 
-// KA: [rho, rr] convergence; beta; p_new = r + beta (p - omega v) at the stencil points;
-// v_new = A D^-1 p_new; partial (rh, v_new)
-template <int NV, class T, class Op>
-__global__ void k_bi_a(Dims D, Op A, const T* r, const T* rh, const T* pold, T* pnew, const T* vold, T* vnew,
-                       const T* dinv, const double* partIn, int nb, double* partOut, double* st, LoopCtl lc) {
-    bool any = false;
-    for (int s = 0; s < NV; ++s) any |= st[B_STRIDE * s + B_DONE] == 0.;
-    if (!any) return;
+// KA head, every block: the convergence test on the partials (rh, r), (r, r) of KC,
+// beta and the omega of the previous step; block 0 records them
+template <int NV>
+__device__ __forceinline__ void biaHead(const double* partIn, int nb, double* st, const LoopCtl& lc, double* beta,
+                                        double* om, bool* live) {
     double t[2 * NV];
     readPartials<KBS, 2 * NV>(partIn, nb, t);
-    double beta[NV], om[NV];
-    bool live[NV];
     for (int s = 0; s < NV; ++s) {
         const double* c = st + B_STRIDE * s;
         const double rho = t[2 * s], rr = t[2 * s + 1];
@@ -255,6 +250,35 @@ __global__ void k_bi_a(Dims D, Op A, const T* r, const T* rh, const T* pold, T* 
             w[B_BB] = bb; w[B_RR] = rr; w[B_RHOCUR] = rho; w[B_DONE] = done;
         }
     }
+}
+// KB head: alpha = rho / (rh, v)
+template <int NV>
+__device__ __forceinline__ void bibHead(const double* partIn, int nb, double* st, double* al, bool* live) {
+    double t[NV];
+    readPartials<KBS, NV>(partIn, nb, t);
+    for (int s = 0; s < NV; ++s) {
+        const double* c = st + B_STRIDE * s;
+        live[s] = c[B_DONE] == 0. && t[s] != 0.;
+        al[s] = live[s] ? c[B_RHOCUR] / t[s] : 0.;
+        if (BLOCK0) {
+            double* w = st + B_STRIDE * s;
+            w[B_ALPHA] = al[s];
+            if (c[B_DONE] == 0. && t[s] == 0.) w[B_BRK] = 1.;
+        }
+    }
+}
+
+// KA: [rho, rr] convergence; beta; p_new = r + beta (p - omega v) at the stencil points;
+// v_new = A D^-1 p_new; partial (rh, v_new)
+template <int NV, class T, class Op>
+__global__ void k_bi_a(Dims D, Op A, const T* r, const T* rh, const T* pold, T* pnew, const T* vold, T* vnew,
+                       const T* dinv, const double* partIn, int nb, double* partOut, double* st, LoopCtl lc) {
+    bool any = false;
+    for (int s = 0; s < NV; ++s) any |= st[B_STRIDE * s + B_DONE] == 0.;
+    if (!any) return;
+    double beta[NV], om[NV];
+    bool live[NV];
+    biaHead<NV>(partIn, nb, st, lc, beta, om, live);
     double acc[NV];
     for (int s = 0; s < NV; ++s) acc[s] = 0.;
     const int P = D.P;
@@ -281,20 +305,9 @@ __global__ void k_bi_b(Dims D, Op A, const T* r, const T* v, T* sv, T* tv, const
     bool any = false;
     for (int s = 0; s < NV; ++s) any |= st[B_STRIDE * s + B_DONE] == 0.;
     if (!any) return;
-    double t[NV];
-    readPartials<KBS, NV>(partIn, nb, t);
     double al[NV];
     bool live[NV];
-    for (int s = 0; s < NV; ++s) {
-        const double* c = st + B_STRIDE * s;
-        live[s] = c[B_DONE] == 0. && t[s] != 0.;
-        al[s] = live[s] ? c[B_RHOCUR] / t[s] : 0.;
-        if (BLOCK0) {
-            double* w = st + B_STRIDE * s;
-            w[B_ALPHA] = al[s];
-            if (c[B_DONE] == 0. && t[s] == 0.) w[B_BRK] = 1.;
-        }
-    }
+    bibHead<NV>(partIn, nb, st, al, live);
     double acc[2 * NV];
     for (int s = 0; s < 2 * NV; ++s) acc[s] = 0.;
     const int P = D.P;
@@ -312,6 +325,85 @@ __global__ void k_bi_b(Dims D, Op A, const T* r, const T* v, T* sv, T* tv, const
         }
     }
     writePartials<KBS, 2 * NV>(acc, partOut);
+}
+
+// KA and KB for the three momentum components on the band without ghost rows: the
+// seven coefficients and D^-1 at the seven stencil points are loaded once per cell for
+// the three components (the generic kernels reload them per component, as their stores
+// may alias them).  Same arithmetic as Band7::apply.
+template <class T>
+__global__ void k_bi_a_band(Dims D, BandT<T> A, const T* __restrict__ r, const T* __restrict__ rh,
+                            const T* __restrict__ pold, T* __restrict__ pnew, const T* __restrict__ vold,
+                            T* __restrict__ vnew, const T* __restrict__ dinv, const double* partIn, int nb,
+                            double* partOut, double* st, LoopCtl lc) {
+    bool any = false;
+    for (int s = 0; s < 3; ++s) any |= st[B_STRIDE * s + B_DONE] == 0.;
+    if (!any) return;
+    double beta[3], om[3];
+    bool live[3];
+    biaHead<3>(partIn, nb, st, lc, beta, om, live);
+    double acc[3] = {0., 0., 0.};
+    const int P = D.P;
+    const int o[7] = {0, 1, -1, D.sJ, -D.sJ, D.sK, -D.sK};
+    FOR_CELLS(D, q, p) {
+        double a[7], di[7];
+#pragma unroll
+        for (int m = 0; m < 7; ++m) { a[m] = (double)A.A[m * P + p]; di[m] = (double)dinv[p + o[m]]; }
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            if (!live[s]) continue;
+            const int b = s * P + p;
+            double ph[7], pn = 0.;
+#pragma unroll
+            for (int m = 0; m < 7; ++m) {
+                const double pf = (double)r[b + o[m]] + beta[s] * ((double)pold[b + o[m]] - om[s] * (double)vold[b + o[m]]);
+                if (m == 0) pn = pf;
+                ph[m] = pf * di[m];
+            }
+            const double vn = a[0] * ph[0] + a[1] * ph[1] + a[2] * ph[2] + a[3] * ph[3] + a[4] * ph[4] + a[5] * ph[5]
+                            + a[6] * ph[6];
+            pnew[b] = (T)pn; vnew[b] = (T)vn;
+            acc[s] += (double)rh[b] * vn;
+        }
+    }
+    writePartials<KBS, 3>(acc, partOut);
+}
+template <class T>
+__global__ void k_bi_b_band(Dims D, BandT<T> A, const T* __restrict__ r, const T* __restrict__ v, T* __restrict__ sv,
+                            T* __restrict__ tv, const T* __restrict__ dinv, const double* partIn, int nb,
+                            double* partOut, double* st) {
+    bool any = false;
+    for (int s = 0; s < 3; ++s) any |= st[B_STRIDE * s + B_DONE] == 0.;
+    if (!any) return;
+    double al[3];
+    bool live[3];
+    bibHead<3>(partIn, nb, st, al, live);
+    double acc[6] = {0., 0., 0., 0., 0., 0.};
+    const int P = D.P;
+    const int o[7] = {0, 1, -1, D.sJ, -D.sJ, D.sK, -D.sK};
+    FOR_CELLS(D, q, p) {
+        double a[7], di[7];
+#pragma unroll
+        for (int m = 0; m < 7; ++m) { a[m] = (double)A.A[m * P + p]; di[m] = (double)dinv[p + o[m]]; }
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            if (!live[s]) continue;
+            const int b = s * P + p;
+            double sh[7], ss = 0.;
+#pragma unroll
+            for (int m = 0; m < 7; ++m) {
+                const double sf = (double)r[b + o[m]] - al[s] * (double)v[b + o[m]];
+                if (m == 0) ss = sf;
+                sh[m] = sf * di[m];
+            }
+            const double tt = a[0] * sh[0] + a[1] * sh[1] + a[2] * sh[2] + a[3] * sh[3] + a[4] * sh[4] + a[5] * sh[5]
+                            + a[6] * sh[6];
+            sv[b] = (T)ss; tv[b] = (T)tt;
+            acc[2 * s] += tt * ss;
+            acc[2 * s + 1] += tt * tt;
+        }
+    }
+    writePartials<KBS, 6>(acc, partOut);
 }
 
 // KC: [omega = (t,s)/(t,t)] x += alpha D^-1 p + omega D^-1 s; r = s - omega t;
@@ -487,8 +579,20 @@ static int runBicgstab(Ctx& c, Op A, const double* b, double shift, T* x, T* w, 
     int it = 0;
     while (true) {
         for (int q = 0; q < chunk; ++q) {
-            hipLaunchKernelGGL((k_bi_a<NV, T, Op>), dim3(g), dim3(KBS), 0, c.stream, D, A, r, rh, pc, pn, vc, vn, dinv, partC, g, partA, st, lc);
-            hipLaunchKernelGGL((k_bi_b<NV, T, Op>), dim3(g), dim3(KBS), 0, c.stream, D, A, r, vn, s, t, dinv, partA, g, partB, st);
+            if constexpr (NV == 3 && std::is_same<Op, BandT<T>>::value) {
+                if (!A.ov.rowOf) {
+                    hipLaunchKernelGGL((k_bi_a_band<T>), dim3(g), dim3(KBS), 0, c.stream, D, A, (const T*)r, (const T*)rh,
+                                       (const T*)pc, pn, (const T*)vc, vn, (const T*)dinv, (const double*)partC, g, partA, st, lc);
+                    hipLaunchKernelGGL((k_bi_b_band<T>), dim3(g), dim3(KBS), 0, c.stream, D, A, (const T*)r, (const T*)vn, s, t,
+                                       (const T*)dinv, (const double*)partA, g, partB, st);
+                } else {
+                    hipLaunchKernelGGL((k_bi_a<NV, T, Op>), dim3(g), dim3(KBS), 0, c.stream, D, A, r, rh, pc, pn, vc, vn, dinv, partC, g, partA, st, lc);
+                    hipLaunchKernelGGL((k_bi_b<NV, T, Op>), dim3(g), dim3(KBS), 0, c.stream, D, A, r, vn, s, t, dinv, partA, g, partB, st);
+                }
+            } else {
+                hipLaunchKernelGGL((k_bi_a<NV, T, Op>), dim3(g), dim3(KBS), 0, c.stream, D, A, r, rh, pc, pn, vc, vn, dinv, partC, g, partA, st, lc);
+                hipLaunchKernelGGL((k_bi_b<NV, T, Op>), dim3(g), dim3(KBS), 0, c.stream, D, A, r, vn, s, t, dinv, partA, g, partB, st);
+            }
             hipLaunchKernelGGL((k_bi_c<NV, T, Op>), dim3(g), dim3(KBS), 0, c.stream, D, A, x, r, rh, pn, s, t, dinv, partB, g, partC, st);
             std::swap(pc, pn); std::swap(vc, vn);
         }
@@ -535,16 +639,24 @@ static int momentumIR(Ctx& c, const caffe_solve_opts& o, int* rounds, double* re
     const int g = gridFor(D);
     hipLaunchKernelGGL(k_zero_d, dim3(blocksFor(3 * P, KBS)), dim3(KBS), 0, c.stream, c.xm, 3 * P);
     double rr[3], bb[3];
-    trueResid<3>(c, A, c.bm, 0., c.xm, rt, rr, bb);
     auto worst = [&]() { double m = 0.; for (int s = 0; s < 3; ++s) m = std::max(m, bb[s] > 0. ? rr[s] / bb[s] : 0.); return m; };
-    double rel = worst();
+    // x starts at zero, so the first round's right-hand side is b itself and its
+    // relative residual is 1: no residual pass before it
+    double rel = 1.;
+    const double* rhs = c.bm;
     int iters = 0, done[3];
     *rounds = 0;
     while (rel > o.rtol && iters < o.maxit && *rounds < 20) {
         const double inner = std::min(1e-5 > 0.5 * o.rtol / rel ? 1e-5 : 0.5 * o.rtol / rel, 0.5);
-        iters += runBicgstab<3, float>(c, A32, rt, 0., dx, w, inner, o.maxit - iters, 4, done);
+        // the round queues as many iterations between read-backs as the same round of the
+        // previous solve needed (4 at first)
+        const int slot = std::min(*rounds, 3);
+        const int it = runBicgstab<3, float>(c, A32, rhs, 0., dx, w, inner, o.maxit - iters, c.momChunk[slot], done);
+        c.momChunk[slot] = std::max(1, std::min(32, it));
+        iters += it;
         hipLaunchKernelGGL(k_add_f32_3, dim3(g), dim3(KBS), 0, c.stream, D, c.xm, (const float*)dx);
         trueResid<3>(c, A, c.bm, 0., c.xm, rt, rr, bb);
+        rhs = rt;
         const double prev = rel;
         rel = worst();
         ++*rounds;
