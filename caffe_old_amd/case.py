@@ -37,7 +37,7 @@ FACES = ("east", "west", "north", "south", "top", "bottom")
 BC_TYPES = {"inlet": 0, "outlet": 1, "wall": 2, "empty": 3, "zeroGradient": 4, "fixed": 5}
 FLOW_BC = ("inlet", "outlet", "wall", "empty", "zeroGradient")
 SCALAR_BC = ("fixed", "zeroGradient", "empty")
-ALGOS = {"simple": 0, "piso": 1, "ibpiso": 2, "diffusion": 3}
+ALGOS = {"simple": 0, "piso": 1, "ibpiso": 2, "diffusion": 3, "coupled": 4}
 
 
 @dataclass

@@ -414,7 +414,9 @@ def case_from_info(params: InfoNode, ics: InfoNode, nodes: np.ndarray, algo: str
     means unsteady), FlowBoundaryConditions.cpp:25-29 (member initialisation of the
     boundaries), Simple.cpp:61-67 (inner iterations only when unsteady; relaxation
     factors), Simple.cpp:104-108 (rho, mu), InitialConditions.cpp:12-74 (u, then p),
-    Piso.cpp:9 (pressure corrections), IbPiso.cpp:37-38 (the sphere)."""
+    Piso.cpp:9 (pressure corrections), IbPiso.cpp:37-38 (the sphere); the Coupled solver
+    reads the boundaries, its inner iterations, relaxationFactorMomentum, rho and mu
+    (Coupled.cpp:18-33)."""
     rc = RunControl(params)
     unsteady = params.get("Solver.timeAccurate") == "ON"
     if algo == "diffusion":
@@ -427,6 +429,11 @@ def case_from_info(params: InfoNode, ics: InfoNode, nodes: np.ndarray, algo: str
         bounds.append(Boundary(typ, ref_u, ref_p))
     n_inner = params.get("Solver.numberOfInnerIterations", int) if unsteady else 1
     alpha_u = params.get("Solver.relaxationFactorMomentum", float)
+    if algo == "coupled":
+        # Coupled.cpp:25-33: no pressure-correction keys and no initial conditions
+        return Case(nodes=nodes, algo=algo, time_accurate=unsteady, rho=params.get("Solver.rho", float),
+                    mu=params.get("Solver.mu", float), alpha_u=alpha_u, n_inner=n_inner, boundaries=bounds,
+                    time_step=rc.time_step, max_iterations=rc.n_solves() + 1)
     alpha_p = params.get("Solver.relaxationFactorPCorr", float)
     rho = params.get("Solver.rho", float)
     mu = params.get("Solver.mu", float)

@@ -102,7 +102,7 @@ void upload(Ctx& c, double* dst, const std::vector<double>& src) {
 int setup(Ctx& c, const caffe_case* cs, const double* nodes) {
     if (!cs || !nodes) { c.err = "null case or nodes"; return CAFFE_ERR_ARG; }
     if (cs->nI < 1 || cs->nJ < 1 || cs->nK < 1) { c.err = "cell counts must be positive"; return CAFFE_ERR_ARG; }
-    if (cs->algo < 0 || cs->algo > 3) { c.err = "unknown algorithm"; return CAFFE_ERR_ARG; }
+    if (cs->algo < 0 || cs->algo > 4) { c.err = "unknown algorithm"; return CAFFE_ERR_ARG; }
     for (int f = 0; f < 6; ++f) {
         const int t = cs->bcType[f];
         const bool ok = cs->algo == CAFFE_DIFFUSION
@@ -113,6 +113,11 @@ int setup(Ctx& c, const caffe_case* cs, const double* nodes) {
     if (cs->alphaU == 0. || (!cs->steady && false)) { c.err = "relaxationFactorMomentum must be nonzero"; return CAFFE_ERR_ARG; }
     freeAll(c);
     c.cs = *cs;
+    if (cs->algo == CAFFE_COUPLED) {
+        // the Coupled constructor reads no initial conditions (Coupled.cpp:5-34)
+        c.cs.uInit[0] = c.cs.uInit[1] = c.cs.uInit[2] = 0.;
+        c.cs.pInit = 0.;
+    }
     Dims& D = c.D;
     D.nI = cs->nI; D.nJ = cs->nJ; D.nK = cs->nK;
     D.sJ = D.nI + 2; D.sK = (long)(D.nI + 2) * (D.nJ + 2);
@@ -197,6 +202,15 @@ int setup(Ctx& c, const caffe_case* cs, const double* nodes) {
     c.wp = dalloc(c, 8 * D.P);
     c.wp32 = talloc<float>(c, 8 * D.P);
     if (c.hasIB) c.wb32 = talloc<float>(c, 11 * D.P);
+    if (cs->algo == CAFFE_COUPLED) {
+        c.cA = dalloc(c, (size_t)CP_PLANES * D.P);
+        c.cb = dalloc(c, 4 * (size_t)D.P);
+        c.cx = dalloc(c, 4 * (size_t)D.P);
+        c.cw = dalloc(c, 52 * (size_t)D.P);
+        // identity diagonals until assembled
+        launch_fill(c, c.cA + (size_t)CP_M * D.P, 1.0, D.P);
+        launch_fill(c, c.cA + (size_t)CP_L * D.P, 1.0, D.P);
+    }
     // identity rows everywhere until assembled (keeps the Jacobi diagonal finite)
     launch_fill(c, c.Am, 1.0, D.P);
     launch_fill(c, c.Ap, 1.0, D.P);
@@ -213,7 +227,7 @@ int setup(Ctx& c, const caffe_case* cs, const double* nodes) {
     std::vector<double> host(D.P);
     for (int q = 0; q < 3; ++q) {
         std::fill(host.begin(), host.end(), 0.);
-        for (int k = 0; k < D.nK; ++k) for (int j = 0; j < D.nJ; ++j) for (int i = 0; i < D.nI; ++i) host[D.pidx(i, j, k)] = cs->uInit[q];
+        for (int k = 0; k < D.nK; ++k) for (int j = 0; j < D.nJ; ++j) for (int i = 0; i < D.nI; ++i) host[D.pidx(i, j, k)] = c.cs.uInit[q];
         for (int f = 0; f < 6; ++f) setHostPatch(D, host, f, up[3 * f + q]);
         upload(c, c.u[q], host);
         // h: same types and references as u, interior zero
@@ -221,7 +235,7 @@ int setup(Ctx& c, const caffe_case* cs, const double* nodes) {
         upload(c, c.h[q], host);
     }
     std::fill(host.begin(), host.end(), 0.);
-    for (int k = 0; k < D.nK; ++k) for (int j = 0; j < D.nJ; ++j) for (int i = 0; i < D.nI; ++i) host[D.pidx(i, j, k)] = cs->pInit;
+    for (int k = 0; k < D.nK; ++k) for (int j = 0; j < D.nJ; ++j) for (int i = 0; i < D.nI; ++i) host[D.pidx(i, j, k)] = c.cs.pInit;
     for (int f = 0; f < 6; ++f) setHostPatch(D, host, f, pp[f]);
     upload(c, c.p, host);
     (void)zero6;
@@ -316,9 +330,28 @@ int diffusionStep(Ctx& c, double dt, double* resInf) {
     return CAFFE_OK;
 }
 
+// Coupled::solve (Coupled.cpp:38-50): u0 = u, then numberOfInnerIterations (1 when steady)
+// passes of computeMomentumAndPressure (Coupled.cpp:54-280: assemble the 4-variable system,
+// solve it from x = 0, update u, p, grad p, h and the Rhie-Chow faces); returns the
+// continuity error of Coupled.cpp:316-338
+int coupledStep(Ctx& c, double dt, double* continuity) {
+    saveU0(c);
+    const int nInner = c.cs.steady ? 1 : c.cs.nInner;
+    for (int it = 0; it < nInner; ++it) {
+        launch_cp_assemble(c, dt);
+        const int rc = coupled_solve(c, c.pcOpts, &c.lastPc);
+        if (rc != CAFFE_OK && rc != CAFFE_ERR_NOT_CONVERGED) return rc;
+        launch_cp_update(c, c.cx);
+    }
+    const double e = cp_continuity(c);
+    if (continuity) *continuity = e;
+    return CAFFE_OK;
+}
+
 int solveStep(Ctx& c, double dt, double* continuity) {
     if (!c.ready) { c.err = "caffe_case_setup has not been called"; return CAFFE_ERR_STATE; }
     if (c.cs.algo == CAFFE_DIFFUSION) return diffusionStep(c, dt, continuity);
+    if (c.cs.algo == CAFFE_COUPLED) return coupledStep(c, dt, continuity);
     saveU0(c);
     int rc;
     if (c.hasIB && (rc = ibPrepare(c)) != CAFFE_OK) return rc;
@@ -538,9 +571,12 @@ int caffe_matrix_download(caffe_ctx* x, int which, double* host, long n) {
     return guard(x, [&](Ctx& c) {
         READY(c);
         const long P = c.D.P;
-        const double* src = which == 0 ? c.Am : c.Ap;
-        long need = which == 0 ? 7 * P : 4 * P;
-        if (which < 0 || which > 1 || !host || n != need) { c.err = "bad matrix download arguments"; return CAFFE_ERR_ARG; }
+        const double* src = which == 0 ? c.Am : (which == 1 ? c.Ap : c.cA);
+        long need = which == 0 ? 7 * P : (which == 1 ? 4 * P : (long)CP_PLANES * P);
+        if (which < 0 || which > 2 || (which == 2 && !c.cA) || !host || n != need) {
+            c.err = "bad matrix download arguments";
+            return CAFFE_ERR_ARG;
+        }
         HIPCHECK(hipMemcpyAsync(host, src, need * sizeof(double), hipMemcpyDeviceToHost, c.stream));
         HIPCHECK(hipStreamSynchronize(c.stream));
         return CAFFE_OK;
@@ -551,9 +587,12 @@ int caffe_rhs_download(caffe_ctx* x, int which, double* host, long n) {
     return guard(x, [&](Ctx& c) {
         READY(c);
         const long P = c.D.P;
-        const double* src = which == 0 ? c.bm : c.bp;
-        long need = which == 0 ? 3 * P : P;
-        if (which < 0 || which > 1 || !host || n != need) { c.err = "bad rhs download arguments"; return CAFFE_ERR_ARG; }
+        const double* src = which == 0 ? c.bm : (which == 1 ? c.bp : c.cb);
+        long need = which == 0 ? 3 * P : (which == 1 ? P : 4 * P);
+        if (which < 0 || which > 2 || (which == 2 && !c.cb) || !host || n != need) {
+            c.err = "bad rhs download arguments";
+            return CAFFE_ERR_ARG;
+        }
         HIPCHECK(hipMemcpyAsync(host, src, need * sizeof(double), hipMemcpyDeviceToHost, c.stream));
         HIPCHECK(hipStreamSynchronize(c.stream));
         return CAFFE_OK;
@@ -609,7 +648,9 @@ int caffe_kernel_probe(caffe_ctx* x, const char* which, int enable) {
         if (w == "fcg_k1") kind = PROBE_FCG_K1;
         else if (w == "pcorr_cg_k1_fp32") kind = PROBE_CG_K1_FP32;
         else if (w == "pcorr_cg_k1_fp64") kind = PROBE_CG_K1_FP64;
+        else if (w == "coupled_bicg_a") kind = PROBE_CP_A;
         else { c.err = "unknown probe kernel " + w; return CAFFE_ERR_ARG; }
+        if (kind == PROBE_CP_A && !c.cA) { c.err = "not a Coupled case"; return CAFFE_ERR_STATE; }
         HIPCHECK(hipStreamSynchronize(c.stream));
         if (enable && c.probe.ev.empty()) {
             c.probe.ev.resize(2 * PROBE_MAX);
@@ -647,7 +688,9 @@ int caffe_kernel_probe_read(caffe_ctx* x, double* avg_ms, long* launches, double
         // algorithmic bytes per launch per interior cell: the stencil kernels read the
         // search direction's two inputs (z, p_old) and the four matrix arrays and write p
         // and q: 8 x 4 B in FP32, 8 x 8 B in FP64
-        const double per = c.probe.which == PROBE_CG_K1_FP64 ? 64. : 32.;
+        // The Coupled BiCGStab kernel KA reads the 56 coefficient planes, r, p_old, v_old, rh
+        // and D^-1 of the four variables and writes p and v: 84 doubles per cell
+        const double per = c.probe.which == PROBE_CP_A ? 672. : (c.probe.which == PROBE_CG_K1_FP64 ? 64. : 32.);
         if (bytes) *bytes = per * (double)c.D.N;
         return CAFFE_OK;
     });

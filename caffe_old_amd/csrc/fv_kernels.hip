@@ -619,6 +619,330 @@ static double maxReduce(Ctx& c, int mode, double dt) {
 double continuity_max(Ctx& c) { return maxReduce(c, 0, 0.); }
 double courant(Ctx& c, double dt) { return maxReduce(c, 1, dt); }
 
+// ------------------------------------------------------------------ Coupled
+// Coupled::computeMomentumAndPressure (Coupled.cpp:54-280): the four rows of a cell (u, v,
+// w momentum and continuity) in planes of P doubles, neighbour order own, E, W, N, S, T, B
+// (the column order of Coupled.cpp:118-132 and 179-214).  A coefficient whose column is an
+// off-domain neighbour (IndexMap gives -1, and PETSc drops it) is stored as zero.
+// Interpolation weights are the reference's: gE = V_E / (V_E + V_P), gW = V_W / (V_W + V_P)
+// in the cell's own interior, 0 on a domain face (HexaFvmMesh.cpp:884-907).
+template <bool U>
+__device__ __forceinline__ void cpWeights(const Geo& g, long p, const long* nb, const bool* atB, double* w) {
+    const double v = G<U>::vol(g, p);
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        if (atB[f + 1]) { w[f] = 0.; continue; }
+        const double vn = G<U>::vol(g, nb[f + 1]);
+        w[f] = vn / (vn + v);
+    }
+}
+// outward area normals of the six faces, E W N S T B
+template <bool U>
+__device__ __forceinline__ void cpNormals(const Geo& g, long p, const Dims& D, V3* S) {
+    S[0] = G<U>::S(g, 0, p + 1); S[1] = -G<U>::S(g, 0, p);
+    S[2] = G<U>::S(g, 1, p + D.sJ); S[3] = -G<U>::S(g, 1, p);
+    S[4] = G<U>::S(g, 2, p + D.sK); S[5] = -G<U>::S(g, 2, p);
+}
+__device__ __forceinline__ double comp(const V3& v, int c) { return c == 0 ? v.x : (c == 1 ? v.y : v.z); }
+
+struct CpArgs {
+    const double *mI, *mJ, *mK;
+    const double *u[3], *u0[3], *p, *d, *gp[3];
+    double *A, *b, *dOut;
+    double rho, mu, om, dt;
+    int steady;
+    int uT[6], pT[6];
+};
+
+// Coupled.cpp:66-144: momentum rows (14 coefficients: a_P / omega and a_nb shared by the
+// components, then the pressure coefficients of component c) and d = V / a_P
+template <bool U>
+__global__ void k_cp_momentum(Geo g, CpArgs a) {
+    const Dims& D = g.D;
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= D.N) return;
+    int i, j, k;
+    const long p = D.cell(q, i, j, k);
+    const long P = D.P, sJ = D.sJ, sK = D.sK;
+    const double vol = G<U>::vol(g, p);
+    const double a0P = a.steady ? 0. : a.rho * vol / a.dt;
+    const double mE = a.mI[p + 1], mW = a.mI[p], mN = a.mJ[p + sJ], mS = a.mJ[p], mT = a.mK[p + sK], mB = a.mK[p];
+    const double dE = G<U>::d(g, 0, p + 1, i == D.nI - 1), dW = G<U>::d(g, 0, p, i == 0);
+    const double dN = G<U>::d(g, 1, p + sJ, j == D.nJ - 1), dS = G<U>::d(g, 1, p, j == 0);
+    const double dT = G<U>::d(g, 2, p + sK, k == D.nK - 1), dB = G<U>::d(g, 2, p, k == 0);
+    const double mu = a.mu;
+    double A[7];
+    A[0] = (dmax(mE, 0.) + mu * dE - dmin(mW, 0.) + mu * dW + dmax(mN, 0.) + mu * dN - dmin(mS, 0.) + mu * dS
+            + dmax(mT, 0.) + mu * dT - dmin(mB, 0.) + mu * dB + a0P) / a.om;
+    A[1] = dmin(mE, 0.) - mu * dE;
+    A[2] = -dmax(mW, 0.) - mu * dW;
+    A[3] = dmin(mN, 0.) - mu * dN;
+    A[4] = -dmax(mS, 0.) - mu * dS;
+    A[5] = dmin(mT, 0.) - mu * dT;
+    A[6] = -dmax(mB, 0.) - mu * dB;
+    double bu[3];
+    const double r = (1. - a.om) * A[0];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { bu[c] = a0P * a.u0[c][p]; bu[c] += r * a.u[c][p]; }
+    const long nb[7] = {p, p + 1, p - 1, p + sJ, p - sJ, p + sK, p - sK};
+    const bool atB[7] = {false, i == D.nI - 1, i == 0, j == D.nJ - 1, j == 0, k == D.nK - 1, k == 0};
+#pragma unroll
+    for (int f = 1; f < 7; ++f) {
+        if (!atB[f]) continue;
+        if (a.uT[f - 1] == BC_FIXED) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) bu[c] -= A[f] * a.u[c][nb[f]];
+        } else {
+            A[0] += A[f]; A[f] = 0.;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 7; ++m) a.A[(CP_M + m) * P + p] = atB[m] ? 0. : A[m];
+    V3 S[6];
+    cpNormals<U>(g, p, D, S);
+    double w[6];
+    cpWeights<U>(g, p, nb, atB, w);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double G7[7];
+        G7[0] = w[0] * comp(S[0], c) + w[1] * comp(S[1], c) + w[2] * comp(S[2], c) + w[3] * comp(S[3], c)
+              + w[4] * comp(S[4], c) + w[5] * comp(S[5], c);
+#pragma unroll
+        for (int f = 0; f < 6; ++f) G7[f + 1] = (1. - w[f]) * comp(S[f], c);
+#pragma unroll
+        for (int f = 1; f < 7; ++f) {
+            if (!atB[f]) continue;
+            if (a.pT[f - 1] == BC_FIXED) bu[c] -= G7[f] * a.p[nb[f]];
+            else { G7[0] += G7[f]; G7[f] = 0.; }
+        }
+#pragma unroll
+        for (int m = 0; m < 7; ++m) a.A[(CP_G + 7 * c + m) * P + p] = atB[m] ? 0. : G7[m];
+        a.b[c * P + p] = bu[c];
+    }
+    a.dOut[p] = vol / A[0];
+}
+
+// Coupled.cpp:150-226: the continuity row.  Velocity coefficients rho g S(c) with the
+// velocity fold (its right-hand side keeps component c of the vector fold), the d-Laplacian
+// of p, and the right-hand side -sum rho d (grad p)_f . S_f with the pressure fold.  d faces:
+// interpolated inside, the patch on a domain face; (grad p)_f: interpolated inside, zero on a
+// domain face (never assigned there).
+template <bool U>
+__global__ void k_cp_continuity(Geo g, CpArgs a) {
+    const Dims& D = g.D;
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= D.N) return;
+    int i, j, k;
+    const long p = D.cell(q, i, j, k);
+    const long P = D.P, sJ = D.sJ, sK = D.sK;
+    const long nb[7] = {p, p + 1, p - 1, p + sJ, p - sJ, p + sK, p - sK};
+    const bool atB[7] = {false, i == D.nI - 1, i == 0, j == D.nJ - 1, j == 0, k == D.nK - 1, k == 0};
+    const double rho = a.rho;
+    V3 S[6];
+    cpNormals<U>(g, p, D, S);
+    double w[6];
+    cpWeights<U>(g, p, nb, atB, w);
+    double bp = 0.;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double Dc[7];
+        Dc[0] = rho * w[0] * comp(S[0], c) + rho * w[1] * comp(S[1], c) + rho * w[2] * comp(S[2], c)
+              + rho * w[3] * comp(S[3], c) + rho * w[4] * comp(S[4], c) + rho * w[5] * comp(S[5], c);
+#pragma unroll
+        for (int f = 0; f < 6; ++f) Dc[f + 1] = rho * (1. - w[f]) * comp(S[f], c);
+        double bu = 0.;
+#pragma unroll
+        for (int f = 1; f < 7; ++f) {
+            if (!atB[f]) continue;
+            if (a.uT[f - 1] == BC_FIXED) bu -= Dc[f] * a.u[c][nb[f]];
+            else { Dc[0] += Dc[f]; Dc[f] = 0.; }
+        }
+        bp += bu;
+#pragma unroll
+        for (int m = 0; m < 7; ++m) a.A[(CP_D + 7 * c + m) * P + p] = atB[m] ? 0. : Dc[m];
+    }
+    const long str[3] = {1, sJ, sK};
+    double df[6], dd[6];
+    V3 gf[6];
+#pragma unroll
+    for (int dir = 0; dir < 3; ++dir) {
+        const long s = str[dir];
+        const bool bh = atB[2 * dir + 1], bl = atB[2 * dir + 2];
+        df[2 * dir] = faceHi<U>(g, a.d, p, s, dir, bh);
+        df[2 * dir + 1] = faceLo<U>(g, a.d, p, s, dir, bl);
+        dd[2 * dir] = G<U>::d(g, dir, p + s, bh);
+        dd[2 * dir + 1] = G<U>::d(g, dir, p, bl);
+        gf[2 * dir] = bh ? mk(0., 0., 0.)
+                         : mk(faceHi<U>(g, a.gp[0], p, s, dir, false), faceHi<U>(g, a.gp[1], p, s, dir, false),
+                              faceHi<U>(g, a.gp[2], p, s, dir, false));
+        gf[2 * dir + 1] = bl ? mk(0., 0., 0.)
+                             : mk(faceLo<U>(g, a.gp[0], p, s, dir, false), faceLo<U>(g, a.gp[1], p, s, dir, false),
+                                  faceLo<U>(g, a.gp[2], p, s, dir, false));
+    }
+    double L[7];
+    L[0] = rho * df[0] * dd[0] + rho * df[1] * dd[1] + rho * df[2] * dd[2] + rho * df[3] * dd[3] + rho * df[4] * dd[4]
+         + rho * df[5] * dd[5];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) L[f + 1] = -rho * df[f] * dd[f];
+    bp -= rho * df[0] * dot(gf[0], S[0]) + rho * df[1] * dot(gf[1], S[1]) + rho * df[2] * dot(gf[2], S[2])
+        + rho * df[3] * dot(gf[3], S[3]) + rho * df[4] * dot(gf[4], S[4]) + rho * df[5] * dot(gf[5], S[5]);
+#pragma unroll
+    for (int f = 1; f < 7; ++f) {
+        if (!atB[f]) continue;
+        if (a.pT[f - 1] == BC_FIXED) bp -= L[f] * a.p[nb[f]];
+        else { L[0] += L[f]; L[f] = 0.; }
+    }
+#pragma unroll
+    for (int m = 0; m < 7; ++m) a.A[(CP_L + m) * P + p] = atB[m] ? 0. : L[m];
+    a.b[3 * P + p] = bp;
+}
+
+static CpArgs cpArgs(Ctx& c, double dt) {
+    CpArgs a;
+    a.mI = c.mF[0]; a.mJ = c.mF[1]; a.mK = c.mF[2];
+    for (int q = 0; q < 3; ++q) { a.u[q] = c.u[q]; a.u0[q] = c.u0[q]; a.gp[q] = c.gp[q]; }
+    a.p = c.p; a.d = c.d; a.A = c.cA; a.b = c.cb; a.dOut = c.d;
+    a.rho = c.cs.rho; a.mu = c.cs.mu; a.om = c.cs.alphaU; a.dt = dt; a.steady = c.cs.steady;
+    for (int q = 0; q < 6; ++q) { a.uT[q] = c.uT[q]; a.pT[q] = c.pT[q]; }
+    return a;
+}
+
+// assembly of the whole system: momentum rows and d, the d patches (zero gradient), the
+// continuity rows
+void launch_cp_assemble(Ctx& c, double dt) {
+    const CpArgs a = cpArgs(c, dt);
+    const int nb = blocksFor(c.D.N, BS);
+    if (c.uniform) hipLaunchKernelGGL(k_cp_momentum<true>, dim3(nb), dim3(BS), 0, c.stream, c.geo, a);
+    else hipLaunchKernelGGL(k_cp_momentum<false>, dim3(nb), dim3(BS), 0, c.stream, c.geo, a);
+    int zg[6] = {BC_ZERO_GRADIENT, BC_ZERO_GRADIENT, BC_ZERO_GRADIENT, BC_ZERO_GRADIENT, BC_ZERO_GRADIENT, BC_ZERO_GRADIENT};
+    double* dd[1] = {c.d};
+    launch_halo(c, dd, 1, zg);
+    if (c.uniform) hipLaunchKernelGGL(k_cp_continuity<true>, dim3(nb), dim3(BS), 0, c.stream, c.geo, a);
+    else hipLaunchKernelGGL(k_cp_continuity<false>, dim3(nb), dim3(BS), 0, c.stream, c.geo, a);
+}
+
+// Coupled.cpp:239-255: u and p from the solution
+__global__ void k_cp_copyback(Dims D, const double* x, double* u0, double* u1, double* u2, double* p) {
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= D.N) return;
+    int i, j, k;
+    const long c = D.cell(q, i, j, k);
+    const long P = D.P;
+    u0[c] = x[c]; u1[c] = x[P + c]; u2[c] = x[2 * P + c]; p[c] = x[3 * P + c];
+}
+
+// Coupled.cpp:258: the divergence-theorem gradient of p from its face values, which this
+// solver never interpolates: zero on interior faces, the patch on domain faces
+template <bool U>
+__global__ void k_cp_gradp(Geo g, const double* phi, double* o0, double* o1, double* o2) {
+    const Dims& D = g.D;
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= D.N) return;
+    int ijk[3];
+    const long p = D.cell(q, ijk[0], ijk[1], ijk[2]);
+    const long str[3] = {1, D.sJ, D.sK};
+    const int nn[3] = {D.nI, D.nJ, D.nK};
+    V3 acc = mk(0., 0., 0.);
+    bool first = true;
+#pragma unroll
+    for (int dir = 0; dir < 3; ++dir) {
+        const long s = str[dir];
+        const double fe = ijk[dir] == nn[dir] - 1 ? phi[p + s] : 0.;
+        const double fw = ijk[dir] == 0 ? phi[p - s] : 0.;
+        V3 te = G<U>::S(g, dir, p + s) * fe;
+        V3 tw = (-G<U>::S(g, dir, p)) * fw;
+        if (first) { acc = te + tw; first = false; }
+        else { acc = acc + te; acc = acc + tw; }
+    }
+    acc = acc / G<U>::vol(g, p);
+    o0[p] = acc.x; o1[p] = acc.y; o2[p] = acc.z;
+}
+
+// Coupled.cpp:261-273: h = u + d grad p
+__global__ void k_cp_h(Dims D, const double* u0, const double* u1, const double* u2, const double* d,
+                       const double* g0, const double* g1, const double* g2, double* h0, double* h1, double* h2) {
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= D.N) return;
+    int i, j, k;
+    const long p = D.cell(q, i, j, k);
+    const double dd = d[p];
+    h0[p] = u0[p] + dd * g0[p]; h1[p] = u1[p] + dd * g1[p]; h2[p] = u2[p] + dd * g2[p];
+}
+
+// Coupled::rhieChowInterpolateFaces (Coupled.cpp:282-314): as Simple's without the density
+// factor, and on a W/S/B domain face the pressure term has the sign of the E/N/T faces
+template <bool U>
+__global__ void k_cp_rhie_chow(Geo g, RcArgs a) {
+    const Dims& D = g.D;
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= D.N) return;
+    int ijk[3];
+    const long p = D.cell(q, ijk[0], ijk[1], ijk[2]);
+    const long str[3] = {1, D.sJ, D.sK};
+    const int nn[3] = {D.nI, D.nJ, D.nK};
+    const double pP = a.p[p];
+#pragma unroll
+    for (int dir = 0; dir < 3; ++dir) {
+        const long s = str[dir];
+        if (ijk[dir] == 0) {
+            V3 hf = mk(a.h[0][p - s], a.h[1][p - s], a.h[2][p - s]);
+            V3 Sw = -G<U>::S(g, dir, p);
+            a.mF[dir][p] = -dot(hf, Sw) - a.d[p - s] * (a.p[p - s] - pP) * G<U>::d(g, dir, p, true);
+        }
+        const bool bnd = ijk[dir] == nn[dir] - 1;
+        V3 hf = mk(faceHi<U>(g, a.h[0], p, s, dir, bnd), faceHi<U>(g, a.h[1], p, s, dir, bnd), faceHi<U>(g, a.h[2], p, s, dir, bnd));
+        const double df = faceHi<U>(g, a.d, p, s, dir, bnd);
+        a.mF[dir][p + s] = dot(hf, G<U>::S(g, dir, p + s)) - df * (a.p[p + s] - pP) * G<U>::d(g, dir, p + s, bnd);
+    }
+}
+
+// Coupled.cpp:239-279: u and p from the solution, the p patches, grad p and its faces, h and
+// its patches, the u patches, and the Rhie-Chow faces
+void launch_cp_update(Ctx& c, const double* x) {
+    const Dims& D = c.D;
+    const int nb = blocksFor(D.N, BS);
+    hipLaunchKernelGGL(k_cp_copyback, dim3(nb), dim3(BS), 0, c.stream, D, x, c.u[0], c.u[1], c.u[2], c.p);
+    double* pf[1] = {c.p};
+    launch_halo(c, pf, 1, c.pT);
+    if (c.uniform) hipLaunchKernelGGL(k_cp_gradp<true>, dim3(nb), dim3(BS), 0, c.stream, c.geo, c.p, c.gp[0], c.gp[1], c.gp[2]);
+    else hipLaunchKernelGGL(k_cp_gradp<false>, dim3(nb), dim3(BS), 0, c.stream, c.geo, c.p, c.gp[0], c.gp[1], c.gp[2]);
+    hipLaunchKernelGGL(k_cp_h, dim3(nb), dim3(BS), 0, c.stream, D, c.u[0], c.u[1], c.u[2], c.d, c.gp[0], c.gp[1], c.gp[2],
+                       c.h[0], c.h[1], c.h[2]);
+    double* uh[6] = {c.h[0], c.h[1], c.h[2], c.u[0], c.u[1], c.u[2]};
+    launch_halo(c, uh, 6, c.uT);
+    RcArgs a;
+    a.st = c.st;
+    for (int q = 0; q < 3; ++q) { a.h[q] = c.h[q]; a.mF[q] = c.mF[q]; }
+    a.d = c.d; a.p = c.p; a.rho = 1.;
+    if (c.uniform) hipLaunchKernelGGL(k_cp_rhie_chow<true>, dim3(nb), dim3(BS), 0, c.stream, c.geo, a);
+    else hipLaunchKernelGGL(k_cp_rhie_chow<false>, dim3(nb), dim3(BS), 0, c.stream, c.geo, a);
+}
+
+// Coupled::computeContinuityError (Coupled.cpp:316-338): max over every cell of
+// |mE - mW + mN - mS + mT - mB| / V
+template <bool U>
+__global__ void k_cp_continuity_max(Geo g, const double* mI, const double* mJ, const double* mK, double* part) {
+    const Dims& D = g.D;
+    __shared__ double lds[BS / 64];
+    double mx = 0.;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < D.N; q += (long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        const long p = D.cell(q, i, j, k);
+        const double v = fabs(mI[p + 1] - mI[p] + mJ[p + D.sJ] - mJ[p] + mK[p + D.sK] - mK[p]) / G<U>::vol(g, p);
+        mx = v > mx ? v : mx;
+    }
+    double r = blockMax<BS>(mx, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+double cp_continuity(Ctx& c) {
+    const int nb = blocksFor(c.D.N, BS, 2048);
+    if (c.uniform) hipLaunchKernelGGL(k_cp_continuity_max<true>, dim3(nb), dim3(BS), 0, c.stream, c.geo, c.mF[0], c.mF[1], c.mF[2], c.red.part);
+    else hipLaunchKernelGGL(k_cp_continuity_max<false>, dim3(nb), dim3(BS), 0, c.stream, c.geo, c.mF[0], c.mF[1], c.mF[2], c.red.part);
+    hipLaunchKernelGGL(k_max_final, dim3(1), dim3(BS), 0, c.stream, c.red.part, nb, c.scal + SC_MISC + 1);
+    return readScalar(c, SC_MISC + 1);
+}
+
 // ------------------------------------------------------------------ Diffusion
 // Diffusion::solve assembly loop (Diffusion.cpp:40-75): a_nb = d_f (the face metric),
 // a0 = -(sum a_nb) + a0P with a0P = V/dt when unsteady (the sign as in the reference),

@@ -20,7 +20,17 @@ enum {
 };
 enum { FL_MDONE = 0, FL_PDONE = 1, FL_MITER = 2, FL_PITER = 3, N_FLAGS = 8 };
 
-enum { PROBE_NONE = 0, PROBE_FCG_K1 = 1, PROBE_CG_K1_FP32 = 2, PROBE_CG_K1_FP64 = 3 };
+enum { PROBE_NONE = 0, PROBE_FCG_K1 = 1, PROBE_CG_K1_FP32 = 2, PROBE_CG_K1_FP64 = 3, PROBE_CP_A = 4 };
+
+// Coupled system (Coupled.cpp:54-228): coefficient planes of P doubles per cell, each group
+// of seven in the neighbour order own, E, W, N, S, T, B
+enum {
+    CP_M = 0,        // momentum a_P / omega and a_nb, shared by u, v, w
+    CP_G = 7,        // + 7c: pressure coefficients of the momentum row of component c
+    CP_D = 28,       // + 7c: velocity-c coefficients of the continuity row
+    CP_L = 49,       // pressure coefficients of the continuity row
+    CP_PLANES = 56
+};
 constexpr int PROBE_MAX = 4096;   // launches recorded per probe window
 
 struct Overlay {          // ghost-cell rows (IbPiso.cpp:148-185), 9 entries each
@@ -112,6 +122,13 @@ struct Ctx {
         }
     } probe;
 
+    // Coupled solver: coefficient planes (CP_PLANES * P), right-hand side and solution
+    // (u, v, w, p blocks of P), BiCGStab work and the null-vector solve (52 * P)
+    double* cA = nullptr;
+    double* cb = nullptr;
+    double* cx = nullptr;
+    double* cw = nullptr;
+
     // learned preconditioner (optional)
     void* net = nullptr;
 };
@@ -135,6 +152,11 @@ void launch_copy_cells(Ctx& c, const double* src, double* dst);   // non-INACTIV
 void launch_grad_extrap(Ctx& c, const double* phi, double* const* g1, double* const* out);
 double diff_resid_inf(Ctx& c, const double* x);
 double courant(Ctx& c, double dt);
+// Coupled (Coupled.cpp:54-338): assembly of the 4-variable system, the field update after its
+// solve (u, p, grad p, h, Rhie-Chow faces), the continuity error
+void launch_cp_assemble(Ctx& c, double dt);
+void launch_cp_update(Ctx& c, const double* x);
+double cp_continuity(Ctx& c);
 void launch_fill(Ctx& c, double* a, double v, long n);
 double time_copy(Ctx& c, double* a, double* b, int n, int reps);
 void launch_set_inactive_fields(Ctx& c);
@@ -145,6 +167,9 @@ void compute_metrics(Ctx& c, const double* dNodes);
 // ---- Krylov (krylov.hip)
 int momentum_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep);
 int pcorr_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep);
+// the Coupled system: Jacobi BiCGStab over the joint (u, v, w, p) vector in FP64, to a true
+// relative residual rtol; p is returned with zero mean when no face fixes it
+int coupled_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep);
 void pcorr_make_fp32(Ctx& c);
 void add_vec(Ctx& c, double* x, const double* d, int n);
 double time_cg_k1(Ctx& c, bool fp32, int reps);

@@ -730,6 +730,360 @@ int momentum_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep) {
     return rel <= o.rtol ? CAFFE_OK : CAFFE_ERR_NOT_CONVERGED;
 }
 
+// ---------------------------------------------------------------- Coupled
+// The 4-variable system of Coupled.cpp:54-234 (replacing A_[0].solve on it).  One BiCGStab
+// over the joint vector (u, v, w, p blocks of P): its dot products add up all four blocks.
+// Jacobi preconditioning (right): the diagonal is a_P / omega in the momentum rows and the
+// d-Laplacian diagonal in the continuity rows.
+struct CpOp {
+    const double* A; int P, sJ, sK;
+    // y = the four rows of cell p applied to f(v, c): variable v at padded cell c
+    template <class F>
+    __device__ __forceinline__ void apply(F f, int p, double* y) const {
+        const int o[7] = {0, 1, -1, sJ, -sJ, sK, -sK};
+        double xp[7];
+#pragma unroll
+        for (int m = 0; m < 7; ++m) xp[m] = f(3, p + o[m]);
+        double cont = 0.;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double mom = 0.;
+#pragma unroll
+            for (int m = 0; m < 7; ++m) {
+                const double xv = f(c, p + o[m]);
+                mom += A[(CP_M + m) * P + p] * xv + A[(CP_G + 7 * c + m) * P + p] * xp[m];
+                cont += A[(CP_D + 7 * c + m) * P + p] * xv;
+            }
+            y[c] = mom;
+        }
+#pragma unroll
+        for (int m = 0; m < 7; ++m) cont += A[(CP_L + m) * P + p] * xp[m];
+        y[3] = cont;
+    }
+};
+
+// r = b (minus shift on the continuity rows), rh = r, p = v = x = 0, D^-1; partials (rh, r), (r, r)
+__global__ void k_cp_init(Dims D, CpOp A, const double* b, double shift, double* r, double* rh, double* p0, double* v0,
+                          double* x, double* dinv, double* part, double* st) {
+    double acc[2] = {0., 0.};
+    const int P = D.P;
+    FOR_CELLS(D, q, p) {
+        const double dm = 1. / A.A[CP_M * P + p], dp = 1. / A.A[CP_L * P + p];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int o = v * P + p;
+            const double bv = v == 3 ? b[o] - shift : b[o];
+            dinv[o] = v == 3 ? dp : dm;
+            r[o] = bv; rh[o] = bv; p0[o] = 0.; v0[o] = 0.; x[o] = 0.;
+            acc[0] += bv * bv;
+            acc[1] += bv * bv;
+        }
+    }
+    writePartials<KBS, 2>(acc, part);
+    if (BLOCK0) {
+        st[B_IT] = 0.; st[B_DONE] = 0.; st[B_BB] = -1.; st[B_RHOPREV] = 1.; st[B_ALPHA] = 1.; st[B_OMEGA] = 1.; st[B_BRK] = 0.;
+    }
+}
+
+// KA: p_new = r + beta (p - omega v) at the stencil points; v = A D^-1 p_new; partial (rh, v)
+__global__ void k_cp_a(Dims D, CpOp A, const double* __restrict__ r, const double* __restrict__ rh,
+                       const double* __restrict__ pold, double* __restrict__ pnew, const double* __restrict__ vold,
+                       double* __restrict__ vnew, const double* __restrict__ dinv, const double* partIn, int nb,
+                       double* partOut, double* st, LoopCtl lc) {
+    if (st[B_DONE] != 0.) return;
+    double beta, om;
+    bool live;
+    biaHead<1>(partIn, nb, st, lc, &beta, &om, &live);
+    if (!live) return;
+    double acc[1] = {0.};
+    const int P = D.P;
+    FOR_CELLS(D, q, p) {
+        auto pf = [&](int v, int c) { const int o = v * P + c; return r[o] + beta * (pold[o] - om * vold[o]); };
+        auto ph = [&](int v, int c) { return pf(v, c) * dinv[v * P + c]; };
+        double y[4];
+        A.apply(ph, p, y);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int o = v * P + p;
+            pnew[o] = pf(v, p); vnew[o] = y[v];
+            acc[0] += rh[o] * y[v];
+        }
+    }
+    writePartials<KBS, 1>(acc, partOut);
+}
+
+// KB: s = r - alpha v at the stencil points; t = A D^-1 s; partials (t, s), (t, t)
+__global__ void k_cp_b(Dims D, CpOp A, const double* __restrict__ r, const double* __restrict__ v,
+                       double* __restrict__ sv, double* __restrict__ tv, const double* __restrict__ dinv,
+                       const double* partIn, int nb, double* partOut, double* st) {
+    if (st[B_DONE] != 0.) return;
+    double al;
+    bool live;
+    bibHead<1>(partIn, nb, st, &al, &live);
+    if (!live) return;
+    double acc[2] = {0., 0.};
+    const int P = D.P;
+    FOR_CELLS(D, q, p) {
+        auto sf = [&](int w, int c) { const int o = w * P + c; return r[o] - al * v[o]; };
+        auto sh = [&](int w, int c) { return sf(w, c) * dinv[w * P + c]; };
+        double y[4];
+        A.apply(sh, p, y);
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            const int o = w * P + p;
+            const double ss = sf(w, p);
+            sv[o] = ss; tv[o] = y[w];
+            acc[0] += y[w] * ss;
+            acc[1] += y[w] * y[w];
+        }
+    }
+    writePartials<KBS, 2>(acc, partOut);
+}
+
+// KC: omega = (t, s) / (t, t); x += alpha D^-1 p + omega D^-1 s; r = s - omega t; partials (rh, r), (r, r)
+__global__ void k_cp_c(Dims D, double* x, double* r, const double* rh, const double* pv, const double* sv,
+                       const double* tv, const double* dinv, const double* partIn, int nb, double* partOut, double* st) {
+    if (st[B_DONE] != 0.) return;
+    double t[2];
+    readPartials<KBS, 2>(partIn, nb, t);
+    // B_BRK is 1 when KB found (rh, v) = 0 (it then wrote no s, t); this kernel marks a zero
+    // omega with 2, which the other blocks may already see: they still update, as block 0 does
+    const bool live = st[B_BRK] != 1.;
+    const double al = st[B_ALPHA];
+    const double om = t[1] > 0. ? t[0] / t[1] : 0.;
+    if (BLOCK0 && live) {
+        st[B_OMEGA] = om; st[B_RHOPREV] = st[B_RHOCUR]; st[B_IT] = st[B_IT] + 1.;
+        if (om == 0.) st[B_BRK] = 2.;
+    }
+    double acc[2] = {0., 0.};
+    const int P = D.P;
+    FOR_CELLS(D, q, p) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int o = v * P + p;
+            double rn = r[o];
+            if (live) {
+                const double di = dinv[o];
+                x[o] = x[o] + al * (pv[o] * di) + om * (sv[o] * di);
+                rn = sv[o] - om * tv[o];
+                r[o] = rn;
+            }
+            acc[0] += rh[o] * rn;
+            acc[1] += rn * rn;
+        }
+    }
+    writePartials<KBS, 2>(acc, partOut);
+}
+
+// r = b (minus shift on the continuity rows) - A x in FP64; partials (r, r), (b, b), sum of
+// the continuity block of r, sum of the p block of x
+__global__ void k_cp_resid(Dims D, CpOp A, const double* b, double shift, const double* x, double* r, double* part) {
+    double acc[4] = {0., 0., 0., 0.};
+    const int P = D.P;
+    FOR_CELLS(D, q, p) {
+        auto xf = [&](int v, int c) { return x[v * P + c]; };
+        double y[4];
+        A.apply(xf, p, y);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int o = v * P + p;
+            const double bv = v == 3 ? b[o] - shift : b[o];
+            const double rv = bv - y[v];
+            r[o] = rv;
+            acc[0] += rv * rv;
+            acc[1] += bv * bv;
+        }
+        acc[2] += r[3 * P + p];
+        acc[3] += x[3 * P + p];
+    }
+    writePartials<KBS, 4>(acc, part);
+}
+
+__global__ void k_cp_sum_p(Dims D, const double* b, double* part) {
+    double acc[1] = {0.};
+    FOR_CELLS(D, q, p) { acc[0] += b[3 * D.P + p]; }
+    writePartials<KBS, 1>(acc, part);
+}
+__global__ void k_cp_add(Dims D, double* x, const double* d) {
+    FOR_CELLS(D, q, p) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) x[v * D.P + p] += d[v * D.P + p];
+    }
+}
+
+// Jacobi BiCGStab on A x = b from x = 0, to the recurrence tolerance rtol
+static int runCoupled(Ctx& c, CpOp A, const double* b, double shift, double* x, double rtol, int maxit, int chunk,
+                      int* doneOut) {
+    const Dims& D = c.D;
+    const size_t V = 4 * (size_t)D.P;
+    double* r = c.cw; double* rh = r + V; double* p0 = rh + V; double* p1 = p0 + V; double* v0 = p1 + V;
+    double* v1 = v0 + V; double* s = v1 + V; double* t = s + V; double* dinv = t + V;
+    double* st = c.scal + SC_BI;
+    double* partC = c.red.part;
+    double* partA = c.red.part + MAX_RED_BLOCKS * RED_STRIDE;
+    double* partB = c.red.part + 2 * MAX_RED_BLOCKS * RED_STRIDE;
+    const int g = gridFor(D);
+    LoopCtl lc{rtol * rtol, maxit, 0.};
+    hipLaunchKernelGGL(k_cp_init, dim3(g), dim3(KBS), 0, c.stream, D, A, b, shift, r, rh, p0, v0, x, dinv, partC, st);
+    double *pc = p0, *pn = p1, *vc = v0, *vn = v1;
+    int it = 0;
+    while (true) {
+        for (int q = 0; q < chunk; ++q) {
+            hipEvent_t e0, e1;
+            if (c.probe.slot(PROBE_CP_A, e0, e1))
+                hipExtLaunchKernelGGL(k_cp_a, dim3(g), dim3(KBS), 0, c.stream, e0, e1, 0, D, A, (const double*)r,
+                                      (const double*)rh, (const double*)pc, pn, (const double*)vc, vn, (const double*)dinv,
+                                      (const double*)partC, g, partA, st, lc);
+            else
+                hipLaunchKernelGGL(k_cp_a, dim3(g), dim3(KBS), 0, c.stream, D, A, (const double*)r, (const double*)rh,
+                                   (const double*)pc, pn, (const double*)vc, vn, (const double*)dinv, (const double*)partC, g,
+                                   partA, st, lc);
+            hipLaunchKernelGGL(k_cp_b, dim3(g), dim3(KBS), 0, c.stream, D, A, (const double*)r, (const double*)vn, s, t,
+                               (const double*)dinv, (const double*)partA, g, partB, st);
+            hipLaunchKernelGGL(k_cp_c, dim3(g), dim3(KBS), 0, c.stream, D, x, r, (const double*)rh, (const double*)pn,
+                               (const double*)s, (const double*)t, (const double*)dinv, (const double*)partB, g, partC, st);
+            std::swap(pc, pn); std::swap(vc, vn);
+        }
+        readBack(c, st, SC_BI, B_STRIDE);
+        it = (int)c.hscal[SC_BI + B_IT];
+        if (c.hscal[SC_BI + B_DONE] != 0. || it >= maxit) break;
+        if (chunk < 32) chunk *= 2;
+    }
+    *doneOut = (int)c.hscal[SC_BI + B_DONE];
+    return it;
+}
+
+// g = -A e_p (e_p: p = 1, u = 0): minus the row sums of the pressure coefficients;
+// partial sum of g . g
+__global__ void k_cp_nullrhs(Dims D, CpOp A, double* g, double* part) {
+    double acc[1] = {0.};
+    const int P = D.P;
+    FOR_CELLS(D, q, p) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int base = v == 3 ? CP_L : CP_G + 7 * v;
+            double sum = 0.;
+#pragma unroll
+            for (int m = 0; m < 7; ++m) sum += A.A[(base + m) * P + p];
+            g[v * P + p] = -sum;
+            acc[0] += sum * sum;
+        }
+    }
+    writePartials<KBS, 1>(acc, part);
+}
+// x -= beta (e_p + z)
+__global__ void k_cp_gauge(Dims D, double* x, const double* z, double beta) {
+    FOR_CELLS(D, q, p) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) x[v * D.P + p] -= beta * ((v == 3 ? 1. : 0.) + z[v * D.P + p]);
+    }
+}
+
+namespace {
+struct CpSolver {
+    Ctx& c;
+    CpOp A;
+    int g;
+    double* rt;
+    double* dx;
+    // r = b - shift - A x: returns ||r|| (without the component along the continuity rows'
+    // zero sum when singular), ||b - shift|| and the sum of the p block of x
+    void resid(const double* b, double shift, const double* x, double* rr, double* bb, double* xsum) {
+        const Dims& D = c.D;
+        hipLaunchKernelGGL(k_cp_resid, dim3(g), dim3(KBS), 0, c.stream, D, A, b, shift, x, rt, c.red.part);
+        hipLaunchKernelGGL(k_finalize<4>, dim3(1), dim3(KBS), 0, c.stream, c.red.part, g, c.scal + SC_MISC + 8);
+        readBack(c, c.scal + SC_MISC + 8, SC_MISC + 8, 4);
+        const double r2 = c.hscal[SC_MISC + 8], rs = c.hscal[SC_MISC + 10];
+        *rr = std::sqrt(c.singularP ? std::max(r2 - rs * rs / (double)D.N, 0.) : r2);
+        *bb = std::sqrt(c.hscal[SC_MISC + 9]);
+        *xsum = c.hscal[SC_MISC + 11];
+    }
+    double meanP(const double* v) {
+        const Dims& D = c.D;
+        hipLaunchKernelGGL(k_cp_sum_p, dim3(g), dim3(KBS), 0, c.stream, D, v, c.red.part);
+        hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(KBS), 0, c.stream, c.red.part, g, c.scal + SC_MISC + 4);
+        readBack(c, c.scal + SC_MISC + 4, SC_MISC + 4, 1);
+        return c.hscal[SC_MISC + 4] / (double)D.N;
+    }
+    // A x = b - shift from x = 0 to a true relative residual rtol: BiCGStab, restarted from the
+    // true residual while that gains; returns the relative residual
+    double solve(const double* b, double shift, double* x, double rtol, int maxit, int* iters, int* restarts,
+                 double* xsum) {
+        int done = 0;
+        *iters += runCoupled(c, A, b, shift, x, rtol, maxit - *iters, 8, &done);
+        double rr, bb;
+        resid(b, shift, x, &rr, &bb, xsum);
+        double rel = bb > 0. ? rr / bb : 0.;
+        for (int k = 0; rel > rtol && k < 20 && *iters < maxit; ++k) {
+            const double sub = std::min(0.5, 0.5 * rtol / rel);
+            *iters += runCoupled(c, A, rt, 0., dx, sub, maxit - *iters, 8, &done);
+            hipLaunchKernelGGL(k_cp_add, dim3(g), dim3(KBS), 0, c.stream, c.D, x, (const double*)dx);
+            const double prev = rel;
+            resid(b, shift, x, &rr, &bb, xsum);
+            rel = bb > 0. ? rr / bb : 0.;
+            ++*restarts;
+            if (!(rel < prev)) break;
+        }
+        return rel;
+    }
+};
+}  // namespace
+
+int coupled_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep) {
+    HIPCHECK(hipEventRecord(c.ev0, c.stream));
+    const Dims& D = c.D;
+    const size_t V = 4 * (size_t)D.P;
+    CpSolver S{c, CpOp{c.cA, D.P, D.sJ, D.sK}, gridFor(D), c.cw + 36 * (size_t)D.P, c.cw + 40 * (size_t)D.P};
+    double* gv = c.cw + 44 * (size_t)D.P;   // -A e_p
+    double* z = gv + V;                     // A z = -A e_p
+    // Without a fixed-pressure face the continuity rows add up to zero: the system is
+    // singular, its right-hand side loses its (rounding) mean on those rows, and the solution
+    // is moved along the null vector n = e_p + z (A z = -A e_p) to zero-mean p.  With planar
+    // faces A e_p = 0 and n is the constant pressure; on cells with non-planar faces (normals
+    // from one triangle of each face) n carries velocity components, and z costs one more
+    // solve, made only when a plain shift of p would move the residual by more than
+    // 1e-3 rtol.
+    const double shift = c.singularP ? S.meanP(c.cb) : 0.;
+    int iters = 0, restarts = 0;
+    double xs = 0.;
+    double rel = S.solve(c.cb, shift, c.cx, o.rtol, o.maxit, &iters, &restarts, &xs);
+    if (c.singularP) {
+        const double xbar = xs / (double)D.N;
+        const int g = S.g;
+        hipLaunchKernelGGL(k_cp_nullrhs, dim3(g), dim3(KBS), 0, c.stream, D, S.A, gv, c.red.part);
+        hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(KBS), 0, c.stream, c.red.part, g, c.scal + SC_MISC + 4);
+        readBack(c, c.scal + SC_MISC + 4, SC_MISC + 4, 1);
+        const double gnorm = std::sqrt(c.hscal[SC_MISC + 4]);
+        double rr, bb, tmp;
+        S.resid(c.cb, shift, c.cx, &rr, &bb, &tmp);
+        double zbar = 0.;
+        bool plain = true;
+        if (std::fabs(xbar) * gnorm > 1e-3 * o.rtol * bb) {
+            double zs = 0.;
+            const double zsh = S.meanP(gv);
+            const double zrel = S.solve(gv, zsh, z, o.rtol, o.maxit, &iters, &restarts, &zs);
+            zbar = zs / (double)D.N;
+            plain = !(zrel <= o.rtol) || !std::isfinite(zbar);   // no null vector: shift p only
+        }
+        if (plain) {
+            zbar = 0.;
+            hipLaunchKernelGGL(k_zero_d, dim3(blocksFor((int)V, KBS)), dim3(KBS), 0, c.stream, z, (int)V);
+        }
+        hipLaunchKernelGGL(k_cp_gauge, dim3(g), dim3(KBS), 0, c.stream, D, c.cx, (const double*)z, xbar / (1. + zbar));
+        S.resid(c.cb, shift, c.cx, &rr, &bb, &tmp);
+        rel = bb > 0. ? rr / bb : 0.;
+    }
+    HIPCHECK(hipEventRecord(c.ev1, c.stream));
+    HIPCHECK(hipEventSynchronize(c.ev1));
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    if (rep) {
+        rep->iters = iters; rep->refinements = restarts; rep->relres = rel; rep->converged = rel <= o.rtol;
+        rep->fell_back = 0; rep->ms = ms;
+    }
+    return rel <= o.rtol ? CAFFE_OK : CAFFE_ERR_NOT_CONVERGED;
+}
+
 void pcorr_make_fp32(Ctx& c) {
     const int n = 4 * c.D.P;
     hipLaunchKernelGGL(k_cast_f32, dim3(blocksFor(n, KBS)), dim3(KBS), 0, c.stream, c.Ap, c.Ap32, n);

@@ -4,11 +4,12 @@
     python -m caffe_old_amd.run hexamesh CASE_DIR
     python -m caffe_old_amd.run {simple,piso,ibpiso} CASE_DIR [--pcorr plain|ir|learned] [--momentum plain|ir]
     python -m caffe_old_amd.run diffusion CASE_DIR
+    python -m caffe_old_amd.run coupled CASE_DIR
 
 ``hexamesh`` does what Modules/hexaMesh.cpp:8-28 does: read mesh/structuredMesh.info,
 generate the nodes, check the faces are planar, and write mesh/structuredMesh.dat.
-The solver commands follow Modules/caffeSimple.cpp, caffePiso.cpp, caffeIbPiso.cpp
-and caffeDiffusion.cpp: read case/case.info, case/initialConditions.info and
+The solver commands follow Modules/caffeSimple.cpp, caffePiso.cpp, caffeIbPiso.cpp,
+caffeDiffusion.cpp and caffeCoupled.cpp: read case/case.info, case/initialConditions.info and
 mesh/structuredMesh.dat, run the RunControl loop (RunControl.cpp:53-84) with one
 solver call per iteration, and write solution/<mesh name>.dat every
 fileWriteInterval iterations and once more at the end, with the fields the
@@ -104,7 +105,7 @@ def run(algo: str, case_dir: str, pcorr: str = "plain", momentum: str = "plain",
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("module", choices=["hexamesh", "simple", "piso", "ibpiso", "diffusion"])
+    ap.add_argument("module", choices=["hexamesh", "simple", "piso", "ibpiso", "diffusion", "coupled"])
     ap.add_argument("case_dir", nargs="?", default=".")
     ap.add_argument("--pcorr", choices=["plain", "ir", "learned"], default="plain")
     ap.add_argument("--momentum", choices=["plain", "ir"], default="plain")

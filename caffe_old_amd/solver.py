@@ -197,16 +197,18 @@ class Solver:
         return out.reshape(nc, self.P) if nc > 1 else out
 
     def matrix(self, which: int) -> np.ndarray:
-        n = (7 if which == 0 else 4) * self.P
+        """which 0: momentum band (7, P); 1: p' storage (4, P); 2: Coupled planes (56, P)."""
+        n = {0: 7, 1: 4, 2: 56}[which] * self.P
         out = np.zeros(n)
         self._check(self.lib.caffe_matrix_download(self._h, which, _dp(out), n), "caffe_matrix_download")
         return out.reshape(-1, self.P)
 
     def rhs(self, which: int) -> np.ndarray:
-        n = (3 if which == 0 else 1) * self.P
+        """which 0: momentum (3, P); 1: p' (P,); 2: Coupled (4, P)."""
+        n = {0: 3, 1: 1, 2: 4}[which] * self.P
         out = np.zeros(n)
         self._check(self.lib.caffe_rhs_download(self._h, which, _dp(out), n), "caffe_rhs_download")
-        return out.reshape(-1, self.P) if which == 0 else out
+        return out.reshape(-1, self.P) if which != 1 else out
 
     def ghost_rows(self):
         n = C.c_int()
@@ -249,8 +251,24 @@ class Diffusion(Solver):
         return msg
 
 
+class Coupled(Solver):
+    """Coupled (Coupled.cpp:38-338): u, v, w and p in one block system per inner iteration,
+    solved in FP64 (Jacobi BiCGStab over the joint vector); ``solve`` returns the max over
+    cells of |sum of face mass flows| / V.  The solve options are set as those of the
+    pressure correction (``set_solve_opts("pcorr", rtol=..., maxit=...)``); the mode is not
+    used.  Initial u and p are zero whatever the case holds (the reference reads no initial
+    conditions for this solver)."""
+
+    def displayUpdateMessage(self) -> str:
+        m, p = self.reports()
+        msg = ("Solver: completed iteration.\n"
+               f"Solver: BiCGStab iterations: {p['iters']}")
+        print(msg)
+        return msg
+
+
 def make_solver(case: Case, device: int = 0, uniform: bool | None = None) -> Solver:
-    cls = {"simple": Simple, "piso": Piso, "ibpiso": IbPiso, "diffusion": Diffusion}[case.algo]
+    cls = {"simple": Simple, "piso": Piso, "ibpiso": IbPiso, "diffusion": Diffusion, "coupled": Coupled}[case.algo]
     return cls(case, device=device, uniform=uniform)
 
 

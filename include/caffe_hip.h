@@ -30,6 +30,7 @@
  *   caffe_correct             Simple::correct (Simple.cpp:301-356)
  *   caffe_ib_field            IbPiso::computeIbField (IbPiso.cpp:65-117)
  *   caffe_ib_coeffs           IbPiso::computeIbCoeffs (IbPiso.cpp:119-190)
+ *                             and Coupled::solve (src/Solvers/Coupled.cpp:38-50)
  *   caffe_step_report         Simple::computeContinuityError / computeCourantNumber
  *                             (Simple.cpp:358-408)
  *   caffe_field_download      Field<T>::cellData() / patches / faces
@@ -56,9 +57,14 @@ enum { CAFFE_BC_INLET = 0, CAFFE_BC_OUTLET = 1, CAFFE_BC_WALL = 2, CAFFE_BC_EMPT
                              (PrimitiveBoundaryCondition.tpp:3-30), with EMPTY and ZERO_GRADIENT */ };
 /* Solver family */
 enum { CAFFE_SIMPLE = 0, CAFFE_PISO = 1, CAFFE_IBPISO = 2,
-       CAFFE_DIFFUSION = 3 /* Diffusion (Diffusion.cpp:8-131): the scalar phi with pInit as its
+       CAFFE_DIFFUSION = 3, /* Diffusion (Diffusion.cpp:8-131): the scalar phi with pInit as its
                               uniform initial value and bcVal as the refValues; caffe_solve returns
-                              ||A x - b||_inf; solve options are those of the p' system */ };
+                              ||A x - b||_inf; solve options are those of the p' system */
+       CAFFE_COUPLED = 4    /* Coupled (Coupled.cpp:5-338): u, v, w and p in one block system
+                              (Jacobi BiCGStab in FP64 whatever the mode; the options of the p'
+                              system give rtol and maxit, its report the result); uInit and pInit
+                              are ignored (the reference reads no initial conditions);
+                              caffe_solve returns the max over cells of |sum of face mass flows|/V */ };
 /* Pressure-correction / momentum solve modes */
 enum {
     CAFFE_SOLVE_PLAIN_FP64 = 0,   /* preconditioned Krylov in FP64 (the judge and fallback) */
@@ -139,8 +145,11 @@ int caffe_field_upload(caffe_ctx* ctx, const char* name, const double* host, lon
 /* Metric arrays (general meshes): vol xc SI SJ SK fcI fcJ fcK dI dJ dK gI gJ gK */
 int caffe_metric_download(caffe_ctx* ctx, const char* name, double* host, long n);
 /* Assembled systems.  which 0: band 7*P (aP aE aW aN aS aT aB); which 1: 4*P
- * (aP, aI, aJ, aK: coefficient of the west/south/bottom face of each cell).
- * rhs: 3*P (momentum) or P (pressure correction). */
+ * (aP, aI, aJ, aK: coefficient of the west/south/bottom face of each cell); which 2 (Coupled
+ * cases): 56*P, eight groups of seven planes in the order own, E, W, N, S, T, B: the momentum
+ * band, the pressure coefficients of the u, v, w momentum rows, the u, v, w coefficients of
+ * the continuity row, and its pressure coefficients (Coupled.cpp:80-206).
+ * rhs: 3*P (momentum), P (pressure correction) or 4*P (Coupled: u, v, w, continuity). */
 int caffe_matrix_download(caffe_ctx* ctx, int which, double* host, long n);
 int caffe_rhs_download(caffe_ctx* ctx, int which, double* host, long n);
 /* Ghost rows: n = number of ghost cells; cells[n], cols[9n] (padded indices, -1 dropped),
@@ -154,7 +163,8 @@ int caffe_time_kernel(caffe_ctx* ctx, const char* which, int reps, double* ms, d
 /* Live timing of one kernel inside the solves that follow (HIP event pairs around each of
  * its launches on the context's stream, up to 4096): which = "fcg_k1" (stencil kernel of the
  * learned flexible CG), "pcorr_cg_k1_fp32" / "pcorr_cg_k1_fp64" (stencil kernel of the
- * pressure CG); enable 0 stops recording.  _read returns the average duration of the
+ * pressure CG), "coupled_bicg_a" (first BiCGStab kernel of the Coupled solve); enable 0 stops
+ * recording.  _read returns the average duration of the
  * launches that ran (launches queued past convergence return at once and are left out),
  * their number, and the kernel's algorithmic HBM bytes per launch.  No counterpart in the
  * reference. */

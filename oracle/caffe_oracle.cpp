@@ -32,6 +32,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -486,12 +488,14 @@ struct PrimBC {
 
 // ---------------------------------------------------------------- sparse matrix
 // SparseMatrix.cpp:47-106: AIJ rows, INSERT_VALUES, negative columns dropped.
-struct Row { int n = 0; int col[9]; double val[9]; };
-struct SpMat {
-    std::vector<Row> rows;
-    void allocate(int n) { rows.assign(n, Row()); }
+template <int W>
+struct RowT { int n = 0; int col[W]; double val[W]; };
+template <int W>
+struct SpMatT {
+    std::vector<RowT<W>> rows;
+    void allocate(int n) { rows.assign(n, RowT<W>()); }
     void setRow(int r, int nc, const int* cols, const double* vals) {
-        Row& R = rows[r];
+        RowT<W>& R = rows[r];
         for (int q = 0; q < nc; ++q) {
             int c = cols[q];
             if (c < 0) continue;
@@ -505,6 +509,8 @@ struct SpMat {
         }
     }
 };
+using SpMat = SpMatT<9>;     // 7-point rows and 9-entry ghost rows
+using SpMat28 = SpMatT<28>;  // Coupled: 14-entry momentum rows, 28-entry continuity rows
 
 struct KspOpts { int kind; double rtol; int maxit; int nBlocks; int nThreads; };
 
@@ -525,11 +531,12 @@ double dotv(const std::vector<double>& a, const std::vector<double>& b, int nt) 
     return s;
 }
 
-void spmv(const SpMat& A, const std::vector<double>& x, std::vector<double>& y, int nt) {
+template <class M>
+void spmv(const M& A, const std::vector<double>& x, std::vector<double>& y, int nt) {
     int n = (int)A.rows.size();
     #pragma omp parallel for num_threads(nt) schedule(static)
     for (int r = 0; r < n; ++r) {
-        const Row& R = A.rows[r];
+        const auto& R = A.rows[r];
         double s = 0.;
         for (int q = 0; q < R.n; ++q) s += R.val[q] * x[R.col[q]];
         y[r] = s;
@@ -539,14 +546,15 @@ void spmv(const SpMat& A, const std::vector<double>& x, std::vector<double>& y, 
 // PCSOR(its=1, lits=1), SOR_LOCAL_SYMMETRIC_SWEEP, omega = 1, zero initial guess:
 // one forward and one backward Gauss-Seidel sweep inside each row block; couplings
 // to other blocks are ignored (processor-local SOR across ranks).
-void ssorApply(const SpMat& A, const std::vector<double>& r, std::vector<double>& z, int nBlocks, int nt) {
+template <class M>
+void ssorApply(const M& A, const std::vector<double>& r, std::vector<double>& z, int nBlocks, int nt) {
     int n = (int)A.rows.size();
     #pragma omp parallel for num_threads(nt) schedule(static)
     for (int bl = 0; bl < nBlocks; ++bl) {
         int lo = (int)((long)n * bl / nBlocks), hi = (int)((long)n * (bl + 1) / nBlocks);
         for (int i = lo; i < hi; ++i) z[i] = 0.;
         for (int i = lo; i < hi; ++i) {
-            const Row& R = A.rows[i];
+            const auto& R = A.rows[i];
             double s = r[i], d = 1.;
             for (int q = 0; q < R.n; ++q) {
                 int c = R.col[q];
@@ -556,7 +564,7 @@ void ssorApply(const SpMat& A, const std::vector<double>& r, std::vector<double>
             z[i] = s / d;
         }
         for (int i = hi - 1; i >= lo; --i) {
-            const Row& R = A.rows[i];
+            const auto& R = A.rows[i];
             double s = r[i], d = 1.;
             for (int q = 0; q < R.n; ++q) {
                 int c = R.col[q];
@@ -569,7 +577,8 @@ void ssorApply(const SpMat& A, const std::vector<double>& r, std::vector<double>
 }
 
 // Left-preconditioned BiCGStab with the preconditioned residual norm (PETSc KSPBCGS defaults).
-int bicgstabLeft(const SpMat& A, const std::vector<double>& b, std::vector<double>& x, const KspOpts& o, double* relres) {
+template <class M>
+int bicgstabLeft(const M& A, const std::vector<double>& b, std::vector<double>& x, const KspOpts& o, double* relres) {
     int n = (int)b.size(), nt = o.nThreads;
     std::vector<double> r(n), rp(n), p(n, 0.), v(n, 0.), s(n), t(n), tmp(n);
     std::fill(x.begin(), x.end(), 0.);
@@ -601,15 +610,20 @@ int bicgstabLeft(const SpMat& A, const std::vector<double>& b, std::vector<doubl
 }
 
 // Right-preconditioned BiCGStab to a true relative residual, restarted from the true residual.
-int bicgstabTight(const SpMat& A, const std::vector<double>& bIn, std::vector<double>& x, const KspOpts& o,
-                  bool project, double* relres) {
+// project: the rows and unknowns from projFrom on carry a Neumann null space (constants):
+// the right-hand side loses its (rounding) mean there and the solution is returned with
+// zero mean there.
+template <class M>
+int bicgstabTight(const M& A, const std::vector<double>& bIn, std::vector<double>& x, const KspOpts& o,
+                  bool project, double* relres, int projFrom = 0, bool gauge = true) {
     int n = (int)bIn.size(), nt = o.nThreads;
     std::vector<double> b = bIn;
-    if (project && n > 0) {  // Neumann null space: remove the (rounding) mean of b
+    const int np = n - projFrom;
+    if (project && np > 0) {
         double m = 0.;
-        for (int q = 0; q < n; ++q) m += b[q];
-        m /= n;
-        for (int q = 0; q < n; ++q) b[q] -= m;
+        for (int q = projFrom; q < n; ++q) m += b[q];
+        m /= np;
+        for (int q = projFrom; q < n; ++q) b[q] -= m;
     }
     std::fill(x.begin(), x.end(), 0.);
     double bn = std::sqrt(dotv(b, b, nt));
@@ -644,11 +658,11 @@ int bicgstabTight(const SpMat& A, const std::vector<double>& bIn, std::vector<do
             for (int q = 0; q < n; ++q) p[q] = r[q] + beta * (p[q] - omega * v[q]);
         }
     }
-    if (project && n > 0) {  // gauge of the Neumann problem: zero-mean solution
+    if (project && gauge && np > 0) {  // gauge of the Neumann problem: zero-mean solution
         double m = 0.;
-        for (int q = 0; q < n; ++q) m += x[q];
-        m /= n;
-        for (int q = 0; q < n; ++q) x[q] -= m;
+        for (int q = projFrom; q < n; ++q) m += x[q];
+        m /= np;
+        for (int q = projFrom; q < n; ++q) x[q] -= m;
     }
     spmv(A, x, r, nt);
     for (int q = 0; q < n; ++q) r[q] = b[q] - r[q];
@@ -673,6 +687,8 @@ struct Oracle {
     int flowType[6];
     SpMat A[2];
     std::vector<double> bv[3], xv[3];
+    SpMat28 Ac;                    // Coupled: the 4-variable system (Coupled.cpp:20)
+    std::vector<double> bc, xc;
     A3<int> ib;  // IbPiso cell types: 0 FLUID, 1 IB, 2 SOLID (IbPiso.h:44); patches are FLUID
     int itersMom = 0, itersPC = 0;
     double lastRelres = 0.;
@@ -716,12 +732,20 @@ struct Oracle {
         pCBC.init(pC);
         pcSingular = true;
         for (int i = 0; i < 6; ++i) { pCBC.changeType(i, pBC.types[i], 0.); if (pBC.types[i] == P_FIXED) pcSingular = false; }
+        rho.setAll(c->rho); mu.setAll(c->mu);
+        ib.resize(c->nI, c->nJ, c->nK);
+        if (c->algo == OC_COUPLED) {
+            // Coupled.cpp:5-34: createMatrices(4, 1, 1, 28); no initial conditions are read,
+            // so u and p start at zero
+            const int n4 = 4 * iMap.nActive;
+            Ac.allocate(n4);
+            bc.assign(n4, 0.); xc.assign(n4, 0.);
+            return;
+        }
         createMatrices(7);  // Simple.cpp:56
         // Simple.cpp:104-108, 71-73 (uniform initial conditions set cells only)
-        rho.setAll(c->rho); mu.setAll(c->mu);
         u.cells.assign(V3(c->uInit[0], c->uInit[1], c->uInit[2]));
         p.cells.assign(c->pInit);
-        ib.resize(c->nI, c->nJ, c->nK);
     }
 
     int nt() const { return cs.nThreads > 0 ? cs.nThreads : 1; }
@@ -1096,8 +1120,196 @@ struct Oracle {
         return r;
     }
 
+    // ---- Coupled (Coupled.cpp:38-338)
+    // Row numbering of IndexMap::operator()(i, j, k, varSetNo) (IndexMap.cpp:47-52): the
+    // cell's row plus varSetNo * nActive; u, v, w are variable sets 0-2 and p is set 3.
+    int row4(int i, int j, int k, int v) const { const int r = iMap(i, j, k); return r < 0 ? -1 : r + v * iMap.nActive; }
+
+    // Coupled.cpp:54-228: the momentum rows (14 entries), d = V / a_P, the d boundaries and
+    // faces, then the continuity rows (28 entries)
+    void assembleCoupled(double dt) {
+        const Mesh& M = mesh;
+        uStar.copyFrom(u);
+        const double om = cs.alphaU;
+        for (int k = 0; k < M.nK; ++k) for (int j = 0; j < M.nJ; ++j) for (int i = 0; i < M.nI; ++i) {
+            if (!iMap.isActive(i, j, k)) continue;
+            const double a0P = cs.steady ? 0. : rho(i, j, k) * M.vol(i, j, k) / dt;
+            const double mE = mflow.faceE(i, j, k), mW = mflow.faceW(i, j, k), mN = mflow.faceN(i, j, k);
+            const double mS = mflow.faceS(i, j, k), mT = mflow.faceT(i, j, k), mB = mflow.faceB(i, j, k);
+            const double muE = mu.faceE(i, j, k), muW = mu.faceW(i, j, k), muN = mu.faceN(i, j, k);
+            const double muS = mu.faceS(i, j, k), muT = mu.faceT(i, j, k), muB = mu.faceB(i, j, k);
+            double a[14];
+            a[0] = (std::max(mE, 0.) + muE * M.dE(i, j, k) - std::min(mW, 0.) + muW * M.dW(i, j, k)
+                    + std::max(mN, 0.) + muN * M.dN(i, j, k) - std::min(mS, 0.) + muS * M.dS(i, j, k)
+                    + std::max(mT, 0.) + muT * M.dT(i, j, k) - std::min(mB, 0.) + muB * M.dB(i, j, k) + a0P) / om;
+            a[1] = std::min(mE, 0.) - muE * M.dE(i, j, k);
+            a[2] = -std::max(mW, 0.) - muW * M.dW(i, j, k);
+            a[3] = std::min(mN, 0.) - muN * M.dN(i, j, k);
+            a[4] = -std::max(mS, 0.) - muS * M.dS(i, j, k);
+            a[5] = std::min(mT, 0.) - muT * M.dT(i, j, k);
+            a[6] = -std::max(mB, 0.) - muB * M.dB(i, j, k);
+            V3 bu = a0P * u0(i, j, k);
+            bu += (1. - om) * a[0] * uStar(i, j, k);
+            uBC.fold(i, j, k, a, bu);
+            const V3 S[6] = {M.SE(i, j, k), M.SW(i, j, k), M.SN(i, j, k), M.SS(i, j, k), M.ST(i, j, k), M.SB(i, j, k)};
+            const double g[6] = {M.gE(i, j, k), M.gW(i, j, k), M.gN(i, j, k), M.gS(i, j, k), M.gT(i, j, k), M.gB(i, j, k)};
+            for (int c = 0; c < 3; ++c) {
+                a[7] = g[0] * S[0](c) + g[1] * S[1](c) + g[2] * S[2](c) + g[3] * S[3](c) + g[4] * S[4](c) + g[5] * S[5](c);
+                for (int f = 0; f < 6; ++f) a[8 + f] = (1. - g[f]) * S[f](c);
+                const int row = row4(i, j, k, c);
+                const int cols[14] = {row, row4(i + 1, j, k, c), row4(i - 1, j, k, c), row4(i, j + 1, k, c),
+                                      row4(i, j - 1, k, c), row4(i, j, k + 1, c), row4(i, j, k - 1, c),
+                                      row4(i, j, k, 3), row4(i + 1, j, k, 3), row4(i - 1, j, k, 3), row4(i, j + 1, k, 3),
+                                      row4(i, j - 1, k, 3), row4(i, j, k + 1, 3), row4(i, j, k - 1, 3)};
+                pBC.fold(i, j, k, &a[7], bu(c));
+                Ac.setRow(row, 14, cols, a);
+                bc[row] = bu(c);
+            }
+            d(i, j, k) = M.vol(i, j, k) / a[0];
+        }
+        dBC.setBoundaries();
+        interpolate(d);
+        for (int k = 0; k < M.nK; ++k) for (int j = 0; j < M.nJ; ++j) for (int i = 0; i < M.nI; ++i) {
+            if (!iMap.isActive(i, j, k)) continue;
+            const int row = row4(i, j, k, 3);
+            double bp = 0., a[7];
+            const V3 S[6] = {M.SE(i, j, k), M.SW(i, j, k), M.SN(i, j, k), M.SS(i, j, k), M.ST(i, j, k), M.SB(i, j, k)};
+            const double g[6] = {M.gE(i, j, k), M.gW(i, j, k), M.gN(i, j, k), M.gS(i, j, k), M.gT(i, j, k), M.gB(i, j, k)};
+            const double rf[6] = {rho.faceE(i, j, k), rho.faceW(i, j, k), rho.faceN(i, j, k), rho.faceS(i, j, k),
+                                  rho.faceT(i, j, k), rho.faceB(i, j, k)};
+            for (int c = 0; c < 3; ++c) {
+                a[0] = rf[0] * g[0] * S[0](c) + rf[1] * g[1] * S[1](c) + rf[2] * g[2] * S[2](c) + rf[3] * g[3] * S[3](c)
+                       + rf[4] * g[4] * S[4](c) + rf[5] * g[5] * S[5](c);
+                for (int f = 0; f < 6; ++f) a[1 + f] = rf[f] * (1. - g[f]) * S[f](c);
+                const int cols[7] = {row4(i, j, k, c), row4(i + 1, j, k, c), row4(i - 1, j, k, c), row4(i, j + 1, k, c),
+                                     row4(i, j - 1, k, c), row4(i, j, k + 1, c), row4(i, j, k - 1, c)};
+                V3 bu(0., 0., 0.);
+                uBC.fold(i, j, k, a, bu);
+                bp += bu(c);
+                Ac.setRow(row, 7, cols, a);
+            }
+            const double df[6] = {d.faceE(i, j, k), d.faceW(i, j, k), d.faceN(i, j, k), d.faceS(i, j, k),
+                                  d.faceT(i, j, k), d.faceB(i, j, k)};
+            const double dd[6] = {M.dE(i, j, k), M.dW(i, j, k), M.dN(i, j, k), M.dS(i, j, k), M.dT(i, j, k), M.dB(i, j, k)};
+            a[0] = rf[0] * df[0] * dd[0] + rf[1] * df[1] * dd[1] + rf[2] * df[2] * dd[2] + rf[3] * df[3] * dd[3]
+                   + rf[4] * df[4] * dd[4] + rf[5] * df[5] * dd[5];
+            for (int f = 0; f < 6; ++f) a[1 + f] = -rf[f] * df[f] * dd[f];
+            const int cols[7] = {row, row4(i + 1, j, k, 3), row4(i - 1, j, k, 3), row4(i, j + 1, k, 3),
+                                 row4(i, j - 1, k, 3), row4(i, j, k + 1, 3), row4(i, j, k - 1, 3)};
+            const V3 gf[6] = {gradP.faceE(i, j, k), gradP.faceW(i, j, k), gradP.faceN(i, j, k), gradP.faceS(i, j, k),
+                              gradP.faceT(i, j, k), gradP.faceB(i, j, k)};
+            bp -= rf[0] * df[0] * dot(gf[0], S[0]) + rf[1] * df[1] * dot(gf[1], S[1]) + rf[2] * df[2] * dot(gf[2], S[2])
+                  + rf[3] * df[3] * dot(gf[3], S[3]) + rf[4] * df[4] * dot(gf[4], S[4]) + rf[5] * df[5] * dot(gf[5], S[5]);
+            pBC.fold(i, j, k, a, bp);
+            Ac.setRow(row, 7, cols, a);
+            bc[row] = bp;
+        }
+    }
+
+    // Gauge of a Coupled system without a fixed-pressure face (tight mode; the reference
+    // returns whatever its iteration reaches).  The continuity rows add up to zero, so the
+    // system is singular; its null vector is n = e_p + z with A z = -A e_p (e_p: p = 1,
+    // u = 0).  With planar faces A e_p = 0 (the face normals of a cell add up to zero) and n
+    // is the constant pressure; on cells with non-planar faces (normals from one triangle of
+    // each face, HexaFvmMesh.cpp:699-789) n has velocity components.  The solution is moved
+    // along n to zero-mean p.
+    void gaugeCoupled(const KspOpts& o) {
+        const int N = iMap.nActive, n4 = 4 * N;
+        std::vector<double> ep(n4, 0.), ae(n4), z(n4, 0.), r(n4);
+        for (int q = 3 * N; q < n4; ++q) ep[q] = 1.;
+        spmv(Ac, ep, ae, o.nThreads);
+        double g2 = 0.;
+        for (int q = 0; q < n4; ++q) { ae[q] = -ae[q]; g2 += ae[q] * ae[q]; }
+        double xbar = 0., bbar = 0.;
+        for (int q = 3 * N; q < n4; ++q) { xbar += xc[q]; bbar += bc[q]; }
+        xbar /= N; bbar /= N;
+        double b2 = 0.;
+        for (int q = 0; q < n4; ++q) { const double v = q >= 3 * N ? bc[q] - bbar : bc[q]; b2 += v * v; }
+        // a plain shift of p changes the residual by xbar * A e_p: below 1e-3 rtol it is kept
+        double zbar = 0.;
+        if (std::fabs(xbar) * std::sqrt(g2) > 1e-3 * o.rtol * std::sqrt(b2)) {
+            double rr = 0.;
+            bicgstabTight(Ac, ae, z, o, true, &rr, 3 * N, false);
+            for (int q = 3 * N; q < n4; ++q) zbar += z[q];
+            zbar /= N;
+        }
+        const double beta = xbar / (1. + zbar);
+        for (int q = 0; q < n4; ++q) xc[q] -= beta * (ep[q] + z[q]);
+    }
+
+    // Coupled.cpp:282-314: no density factor, and the W/S/B domain faces take the pressure
+    // difference with the sign of the E/N/T faces
+    void rhieChowCoupled() {
+        const Mesh& M = mesh;
+        for (int k = 0; k < M.nK; ++k) for (int j = 0; j < M.nJ; ++j) for (int i = 0; i < M.nI; ++i) {
+            if (iMap.isInactive(i, j, k)) continue;
+            if (i == 0)
+                mflow.faceW(i, j, k) = -dot(h.faceW(i, j, k), M.SW(i, j, k)) - d.faceW(i, j, k) * (p(i - 1, j, k) - p(i, j, k)) * M.dW(i, j, k);
+            if (j == 0)
+                mflow.faceS(i, j, k) = -dot(h.faceS(i, j, k), M.SS(i, j, k)) - d.faceS(i, j, k) * (p(i, j - 1, k) - p(i, j, k)) * M.dS(i, j, k);
+            if (k == 0)
+                mflow.faceB(i, j, k) = -dot(h.faceB(i, j, k), M.SB(i, j, k)) - d.faceB(i, j, k) * (p(i, j, k - 1) - p(i, j, k)) * M.dB(i, j, k);
+            mflow.faceE(i, j, k) = dot(h.faceE(i, j, k), M.SE(i, j, k)) - d.faceE(i, j, k) * (p(i + 1, j, k) - p(i, j, k)) * M.dE(i, j, k);
+            mflow.faceN(i, j, k) = dot(h.faceN(i, j, k), M.SN(i, j, k)) - d.faceN(i, j, k) * (p(i, j + 1, k) - p(i, j, k)) * M.dN(i, j, k);
+            mflow.faceT(i, j, k) = dot(h.faceT(i, j, k), M.ST(i, j, k)) - d.faceT(i, j, k) * (p(i, j, k + 1) - p(i, j, k)) * M.dT(i, j, k);
+        }
+    }
+
+    // Coupled.cpp:54-280.  The solve starts from x = 0 (KSPSolve's default guess).  The p
+    // faces are never interpolated by this solver, so the divergence-theorem gradient sees
+    // zero on interior faces and the patch on domain faces (Coupled.cpp:257-259).
+    void computeMomentumAndPressure(double dt) {
+        const Mesh& M = mesh;
+        assembleCoupled(dt);
+        std::fill(xc.begin(), xc.end(), 0.);
+        const int N = iMap.nActive;
+        KspOpts o = kopts();
+        if (o.kind == OC_KSP_TIGHT) {
+            itersPC = bicgstabTight(Ac, bc, xc, o, pcSingular, &lastRelres, 3 * N, false);
+            if (pcSingular) gaugeCoupled(o);
+        } else {
+            itersPC = bicgstabLeft(Ac, bc, xc, o, &lastRelres);
+        }
+        for (int k = 0; k < M.nK; ++k) for (int j = 0; j < M.nJ; ++j) for (int i = 0; i < M.nI; ++i) {
+            if (iMap.isInactive(i, j, k)) continue;
+            for (int c = 0; c < 3; ++c) u(i, j, k)(c) = xc[row4(i, j, k, c)];
+            p(i, j, k) = xc[row4(i, j, k, 3)];
+        }
+        pBC.setBoundaries();
+        gradScalar(p, gradP);
+        interpolate(gradP);
+        for (int k = 0; k < M.nK; ++k) for (int j = 0; j < M.nJ; ++j) for (int i = 0; i < M.nI; ++i) {
+            if (iMap.isActive(i, j, k)) h(i, j, k) = u(i, j, k) + d(i, j, k) * gradP(i, j, k);
+            else if (iMap.status(i, j, k) == GHOST) h(i, j, k) = u(i, j, k);
+        }
+        hBC.setBoundaries();
+        interpolate(h);
+        uBC.setBoundaries();
+        rhieChowCoupled();
+    }
+
+    // Coupled.cpp:316-338: every cell, |sum of outward mass flows| / V
+    double continuityCoupled() {
+        const Mesh& M = mesh;
+        double e = 0.;
+        for (int k = 0; k < M.nK; ++k) for (int j = 0; j < M.nJ; ++j) for (int i = 0; i < M.nI; ++i) {
+            const double v = std::fabs(mflow.faceE(i, j, k) - mflow.faceW(i, j, k) + mflow.faceN(i, j, k) - mflow.faceS(i, j, k)
+                                       + mflow.faceT(i, j, k) - mflow.faceB(i, j, k)) / M.vol(i, j, k);
+            if (v > e) e = v;
+        }
+        return e;
+    }
+
+    double solveCoupled(double dt) {
+        u0.copyFrom(u);
+        const int nInner = cs.steady ? 1 : cs.nInner;  // Coupled.cpp:25-28
+        for (int it = 0; it < nInner; ++it) computeMomentumAndPressure(dt);
+        return continuityCoupled();
+    }
+
     double solve(double dt) {
         if (cs.algo == OC_DIFFUSION) return solveDiffusion(dt);
+        if (cs.algo == OC_COUPLED) return solveCoupled(dt);
         u0.copyFrom(u);
         if (cs.algo == OC_IBPISO) computeIbField();
         int nInner = cs.steady ? 1 : cs.nInner;  // Simple.cpp:61-64
@@ -1155,6 +1367,16 @@ struct Oracle {
 };
 
 Oracle* O(void* h) { return static_cast<Oracle*>(h); }
+
+template <class M>
+void toCsr(const M& A, int* rowptr, int* cols, double* vals) {
+    long q = 0;
+    rowptr[0] = 0;
+    for (size_t r = 0; r < A.rows.size(); ++r) {
+        for (int e = 0; e < A.rows[r].n; ++e) { cols[q] = A.rows[r].col[e]; vals[q] = A.rows[r].val[e]; ++q; }
+        rowptr[r + 1] = (int)q;
+    }
+}
 
 }  // namespace
 
@@ -1280,20 +1502,16 @@ long oc_get_metric(void* h, const char* name, double* out) {
 int oc_nrows(void* h) { return O(h)->iMap.nActive; }
 long oc_matrix_nnz(void* h, int which) {
     long s = 0;
-    for (const Row& R : O(h)->A[which].rows) s += R.n;
+    if (which == 2) { for (const auto& R : O(h)->Ac.rows) s += R.n; return s; }
+    for (const auto& R : O(h)->A[which].rows) s += R.n;
     return s;
 }
 void oc_matrix_csr(void* h, int which, int* rowptr, int* cols, double* vals) {
-    const SpMat& A = O(h)->A[which];
-    long q = 0;
-    rowptr[0] = 0;
-    for (size_t r = 0; r < A.rows.size(); ++r) {
-        for (int e = 0; e < A.rows[r].n; ++e) { cols[q] = A.rows[r].col[e]; vals[q] = A.rows[r].val[e]; ++q; }
-        rowptr[r + 1] = (int)q;
-    }
+    if (which == 2) toCsr(O(h)->Ac, rowptr, cols, vals);
+    else toCsr(O(h)->A[which], rowptr, cols, vals);
 }
 void oc_rhs(void* h, int vec, double* out) {
-    const std::vector<double>& b = O(h)->bv[vec];
+    const std::vector<double>& b = vec == 3 ? O(h)->bc : O(h)->bv[vec];
     std::copy(b.begin(), b.end(), out);
 }
 void oc_imap(void* h, int* rowid, int* status) {

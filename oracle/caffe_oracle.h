@@ -25,7 +25,8 @@ extern "C" {
 enum { OC_INLET = 0, OC_OUTLET = 1, OC_WALL = 2, OC_EMPTY = 3, OC_ZERO_GRADIENT = 4,
        OC_FIXED = 5 /* PrimitiveBoundaryCondition FIXED, the Diffusion solver's boundaries */ };
 /* Solver family (Simple.h, Piso.h, IbPiso.h) */
-enum { OC_SIMPLE = 0, OC_PISO = 1, OC_IBPISO = 2, OC_DIFFUSION = 3 /* Diffusion.h */ };
+enum { OC_SIMPLE = 0, OC_PISO = 1, OC_IBPISO = 2, OC_DIFFUSION = 3 /* Diffusion.h */,
+       OC_COUPLED = 4 /* Coupled.h: the 4-variable (u, v, w, p) block system */ };
 /* Linear solver used in place of PETSc KSPBCGS + PCSOR (SparseMatrix.cpp:86-106) */
 enum { OC_KSP_PETSC_LIKE = 0, OC_KSP_TIGHT = 1 };
 
@@ -75,7 +76,8 @@ long   oc_get(void* h, const char* name, double* out);
 long   oc_set(void* h, const char* name, const double* in);
 long   oc_get_metric(void* h, const char* name, double* out);
 
-/* Assembled systems.  which: 0 = momentum (A_[0]), 1 = pressure correction (A_[1]) */
+/* Assembled systems.  which: 0 = momentum (A_[0]), 1 = pressure correction (A_[1]),
+ * 2 = the Coupled system (4 * nrows rows: u, v, w, p blocks; its right-hand side is vec 3) */
 int    oc_nrows(void* h);
 long   oc_matrix_nnz(void* h, int which);
 void   oc_matrix_csr(void* h, int which, int* rowptr, int* cols, double* vals);

@@ -190,8 +190,9 @@ class Oracle:
         return out.reshape(nc, self.P) if nc > 1 else out
 
     def matrix(self, which):
+        """CSR of A_[which]; which 2 is the Coupled system (4 rows per cell)."""
         L = lib()
-        n = L.oc_nrows(self.h)
+        n = L.oc_nrows(self.h) * (4 if which == 2 else 1)
         nnz = L.oc_matrix_nnz(self.h, which)
         rp = np.zeros(n + 1, dtype=np.int32)
         cols = np.zeros(max(nnz, 1), dtype=np.int32)
@@ -200,7 +201,8 @@ class Oracle:
         return rp, cols[:nnz], vals[:nnz]
 
     def rhs(self, vec):
-        n = lib().oc_nrows(self.h)
+        """b_[vec]; vec 3 is the Coupled right-hand side."""
+        n = lib().oc_nrows(self.h) * (4 if vec == 3 else 1)
         out = np.zeros(n)
         lib().oc_rhs(self.h, vec, _dp(out))
         return out
