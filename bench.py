@@ -58,6 +58,67 @@ def cpu_baseline(n, steps=2, warmup=1):
                       f"oracle BiCGStab+SSOR rtol 1e-5, {th} OpenMP threads, {dt:.1f} s timed"}
 
 
+def coupled_bench(a):
+    """The Coupled solver (Coupled.cpp:38-338) on the n^3 lid-driven cavity (Re 100, steady,
+    relaxationFactorMomentum 0.7): one "step" is Coupled::solve, i.e. assembly of the
+    4-variable system, its Jacobi BiCGStab solve to a true FP64 relative residual of 1e-8
+    from x = 0 (plus the null-vector gauge of the closed cavity), and the field update.
+    The roofline names KA of the joint BiCGStab (k_cp_a: 56 coefficient planes and the four
+    variables' operands per cell).  The CPU baseline is the oracle with the reference's
+    PETSc-like settings (BiCGStab + block SSOR, rtol 1e-5 on the preconditioned norm)."""
+    from caffe_old_amd import case as cs
+    from caffe_old_amd.solver import make_solver
+    n = a.coupled
+    case = cs.lid_cavity(n, 0.01, algo="coupled", max_iterations=a.steps + a.warmup + 1)
+    s = make_solver(case)
+    s.set_solve_opts("pcorr", rtol=1e-8, maxit=200000)
+    for _ in range(a.warmup):
+        s.solve()
+    s.kernel_probe("coupled_bicg_a", True)
+    s.sync()
+    t0 = time.perf_counter()
+    its = []
+    for _ in range(a.steps):
+        s.solve()
+        its.append(s.reports()[1]["iters"])
+    s.sync()
+    elapsed = time.perf_counter() - t0
+    kms, klaunches, kbytes = s.kernel_probe_read()
+    s.kernel_probe("coupled_bicg_a", False)
+    if a.calibrate_copy:
+        s.time_kernel("copy_fp64", reps=5)
+    ms_per_step = 1000.0 * elapsed / a.steps
+    achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get(f"coupled_bicg_a_{n}")
+    out = {
+        "metric": f"Coupled iterations/s, {n}^3 lid-driven cavity",
+        "value": round(1000.0 / ms_per_step, 4), "unit": "iterations/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic: lid-driven cavity from rest (no external data)",
+        "config": {"workload": f"3D lid-driven cavity {n}^3 uniform hex, Re 100 (mu 1e-2, lid 1), Coupled solver, steady, "
+                               f"relaxationFactorMomentum 0.7, 4-variable system solved to true relres 1e-8",
+                   "cells": case.n_cells, "bicgstab_iters_per_step": its},
+        "roofline": {"bound": "hbm", "kernel": "coupled_bicg_a", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "ms_per_launch": round(kms, 5), "bytes_per_launch": kbytes, "launches_timed": klaunches},
+    }
+    if not a.no_cpu:
+        from oracle.pyoracle import KSP_PETSC_LIKE, Oracle
+        th = _cpu_threads()
+        o = Oracle(case, ksp=KSP_PETSC_LIKE, rtol=1e-5, maxit=10000, n_blocks=th, n_threads=th)
+        o.solve()
+        t0 = time.perf_counter()
+        o.solve()
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": 1.0 / dt, "unit": "iterations/s", "cores": th, "kind": "port",
+                               "sample": f"iteration 2 of the {n}^3 Coupled cavity from rest (1 untimed), oracle "
+                                         f"BiCGStab+SSOR rtol 1e-5, {th} OpenMP threads, {dt:.1f} s timed"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,11 +135,16 @@ def main():
                     help="after the timed steps, 5 launches of a plain copy of 8 P doubles (counter calibration)")
     ap.add_argument("--kernel-only", action="store_true",
                     help="one warm-up step, then only the dominant kernel and the copy kernel (for counter runs)")
+    ap.add_argument("--coupled", type=int, default=0, metavar="N",
+                    help="measure the Coupled solver on the N^3 cavity instead (its own JSON line)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: --gpus {a.gpus} is not supported; this build runs the whole domain on one GPU "
               f"(multi-GPU block decomposition is design-only).", file=sys.stderr)
         sys.exit(2)
+    if a.coupled:
+        coupled_bench(a)
+        return
 
     from caffe_old_amd import _lib as L
     from caffe_old_amd import case as cs
