@@ -144,10 +144,12 @@ __device__ inline void writePartials(const double (&v)[K], double* buf) {
     __shared__ double lds[BS / 64][K];
     double w[K];
 #pragma unroll
-    for (int q = 0; q < K; ++q) {
-        w[q] = v[q];
-        for (int o = 32; o > 0; o >>= 1) w[q] += __shfl_xor(w[q], o, 64);
-    }
+    for (int q = 0; q < K; ++q) w[q] = v[q];
+    // butterfly rounds outside, sums inside: the K exchanges of a round are independent
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for (int q = 0; q < K; ++q) w[q] += __shfl_xor(w[q], o, 64);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0)
 #pragma unroll
@@ -161,23 +163,40 @@ __device__ inline void writePartials(const double (&v)[K], double* buf) {
 }
 
 // every thread of the block receives the K totals of nb rows: the first wave adds
-// rows lane, lane+64, ... in order and folds them by a butterfly; one barrier
+// rows lane, lane+64, ... in order and folds them by a butterfly; one barrier.  The rows
+// of a lane are loaded in groups (four at K <= 4, two above) before any is added, so a
+// grid of up to 256 blocks pays one or two memory latencies, not four; the addition order
+// is the row order either way.
 template <int BS, int K>
 __device__ inline void readPartials(const double* buf, int nb, double (&out)[K]) {
     static_assert(K <= RED_STRIDE, "more partial sums than a row holds");
+    constexpr int GR = K <= 4 ? 4 : 2;
     __shared__ double res[K];
     if (threadIdx.x < 64) {
         double a[K];
 #pragma unroll
         for (int q = 0; q < K; ++q) a[q] = 0.;
-        for (int b = threadIdx.x; b < nb; b += 64)
+        for (int b0 = threadIdx.x; b0 < nb; b0 += 64 * GR) {
+            double v[GR][K];
 #pragma unroll
-            for (int q = 0; q < K; ++q) a[q] += buf[b * RED_STRIDE + q];
+            for (int r = 0; r < GR; ++r) {
+                const int b = b0 + 64 * r;
 #pragma unroll
-        for (int q = 0; q < K; ++q) {
-            for (int o = 32; o > 0; o >>= 1) a[q] += __shfl_xor(a[q], o, 64);
-            if (threadIdx.x == 0) res[q] = a[q];
+                for (int q = 0; q < K; ++q) v[r][q] = b < nb ? buf[b * RED_STRIDE + q] : 0.;
+            }
+#pragma unroll
+            for (int r = 0; r < GR; ++r)
+#pragma unroll
+                for (int q = 0; q < K; ++q)
+                    if (b0 + 64 * r < nb) a[q] += v[r][q];
         }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+            for (int q = 0; q < K; ++q) a[q] += __shfl_xor(a[q], o, 64);
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int q = 0; q < K; ++q) res[q] = a[q];
     }
     __syncthreads();
 #pragma unroll

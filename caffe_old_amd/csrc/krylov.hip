@@ -139,8 +139,10 @@ __global__ void k_cg_init(Dims D, Sym4<T> A, const TB* b, double shift, double s
 template <class T>
 __global__ void k_cg_k1(Dims D, Sym4<T> A, const T* z, T* pb0, T* pb1, T* qv, const double* partIn, int nb,
                         double* partOut, double* st, LoopCtl lc) {
-    if (st[C_DONE] != 0.) return;
-    const int par = ((int)st[C_IT]) & 1;
+    // the scalars written by earlier kernels are loaded together, ahead of the partial sums
+    const double sDone = st[C_DONE], sIt = st[C_IT], sBb = st[C_BB], sRzPrev = st[C_RZPREV], sRatio = st[C_RATIO];
+    if (sDone != 0.) return;
+    const int par = ((int)sIt) & 1;
     const T* pold = par ? pb1 : pb0;
     T* pnew = par ? pb0 : pb1;
     double t[4];
@@ -149,15 +151,15 @@ __global__ void k_cg_k1(Dims D, Sym4<T> A, const T* z, T* pb0, T* pb1, T* qv, co
     const double rz = t[0] - zbar * t[3];
     // residual norm without its (unreachable) mean component
     const double rr = lc.nLive > 0. ? fmax(t[1] - t[3] * t[3] / lc.nLive, 0.) : t[1];
-    const double bb = st[C_BB] < 0. ? rr : st[C_BB];
-    const bool first = st[C_IT] == 0.;
-    const bool done = rr <= lc.tol2 * bb || st[C_IT] >= lc.maxit || rr == 0.;
+    const double bb = sBb < 0. ? rr : sBb;
+    const bool first = sIt == 0.;
+    const bool done = rr <= lc.tol2 * bb || sIt >= lc.maxit || rr == 0.;
     if (BLOCK0) {
         st[C_BB] = bb; st[C_RR] = rr; st[C_RZCUR] = rz; st[C_PBUF] = par ? 0. : 1.;
         if (done) st[C_DONE] = 1.;
     }
     if (done) return;
-    const double beta = first ? 0. : (rz / st[C_RZPREV]) * st[C_RATIO];
+    const double beta = first ? 0. : (rz / sRzPrev) * sRatio;
     double v[1] = {0.};
     // (singular systems have no ghost rows: halo cells meet zero coefficients, so the
     // shifted value there is never used)
@@ -175,13 +177,14 @@ __global__ void k_cg_k1(Dims D, Sym4<T> A, const T* z, T* pb0, T* pb1, T* qv, co
 template <class T>
 __global__ void k_cg_k2(Dims D, Sym4<T> A, T* x, T* r, T* z, const T* pb0, const T* pb1, const T* qv, const double* partIn,
                         int nb, double* partOut, double* st) {
-    if (st[C_DONE] != 0.) return;
-    const T* pv = st[C_PBUF] != 0. ? pb1 : pb0;  // the direction K1 of this iteration wrote
+    const double sDone = st[C_DONE], sPbuf = st[C_PBUF], sRz = st[C_RZCUR], sIt = st[C_IT];
+    if (sDone != 0.) return;
+    const T* pv = sPbuf != 0. ? pb1 : pb0;  // the direction K1 of this iteration wrote
     double t[1];
     readPartials<KBS, 1>(partIn, nb, t);
-    const double alpha = t[0] != 0. ? st[C_RZCUR] / t[0] : 0.;
+    const double alpha = t[0] != 0. ? sRz / t[0] : 0.;
     if (BLOCK0) {
-        st[C_RZPREV] = st[C_RZCUR]; st[C_IT] += 1.; st[C_RATIO] = 1.; st[C_PQ] = t[0];
+        st[C_RZPREV] = sRz; st[C_IT] = sIt + 1.; st[C_RATIO] = 1.; st[C_PQ] = t[0];
     }
     double v[4] = {0., 0., 0., 0.};
     FOR_CELLS(D, q, p) {
@@ -226,25 +229,43 @@ __global__ void k_bi_init(Dims D, Op A, const double* b, double shift, T* r, T* 
     }
 }
 
+// The state of the NV systems, loaded by every block in one batch at kernel start, ahead
+// of the partial sums of the previous kernel
+template <int NV>
+struct BiState {
+    double bb[NV], done[NV], it[NV], brk[NV], om[NV], rhoPrev[NV], al[NV], rho[NV];
+    __device__ __forceinline__ void load(const double* st) {
+        for (int s = 0; s < NV; ++s) {
+            const double* c = st + B_STRIDE * s;
+            bb[s] = c[B_BB]; done[s] = c[B_DONE]; it[s] = c[B_IT]; brk[s] = c[B_BRK];
+            om[s] = c[B_OMEGA]; rhoPrev[s] = c[B_RHOPREV]; al[s] = c[B_ALPHA]; rho[s] = c[B_RHOCUR];
+        }
+    }
+    __device__ __forceinline__ bool any() const {
+        bool a = false;
+        for (int s = 0; s < NV; ++s) a |= done[s] == 0.;
+        return a;
+    }
+};
+
 // KA head, every block: the convergence test on the partials (rh, r), (r, r) of KC,
 // beta and the omega of the previous step; block 0 records them
 template <int NV>
-__device__ __forceinline__ void biaHead(const double* partIn, int nb, double* st, const LoopCtl& lc, double* beta,
-                                        double* om, bool* live) {
+__device__ __forceinline__ void biaHead(const double* partIn, int nb, double* st, const LoopCtl& lc,
+                                        const BiState<NV>& S, double* beta, double* om, bool* live) {
     double t[2 * NV];
     readPartials<KBS, 2 * NV>(partIn, nb, t);
     for (int s = 0; s < NV; ++s) {
-        const double* c = st + B_STRIDE * s;
         const double rho = t[2 * s], rr = t[2 * s + 1];
-        const double bb = c[B_BB] < 0. ? rr : c[B_BB];
-        double done = c[B_DONE];
+        const double bb = S.bb[s] < 0. ? rr : S.bb[s];
+        double done = S.done[s];
         if (done == 0.) {
-            if (rr <= lc.tol2 * bb || c[B_IT] >= lc.maxit || rr == 0.) done = 1.;
-            else if (c[B_BRK] != 0. || rho == 0.) done = 2.;
+            if (rr <= lc.tol2 * bb || S.it[s] >= lc.maxit || rr == 0.) done = 1.;
+            else if (S.brk[s] != 0. || rho == 0.) done = 2.;
         }
         live[s] = done == 0.;
-        om[s] = c[B_OMEGA];
-        beta[s] = c[B_IT] == 0. ? 0. : (rho / c[B_RHOPREV]) * (c[B_ALPHA] / c[B_OMEGA]);
+        om[s] = S.om[s];
+        beta[s] = S.it[s] == 0. ? 0. : (rho / S.rhoPrev[s]) * (S.al[s] / S.om[s]);
         if (BLOCK0) {
             double* w = st + B_STRIDE * s;
             w[B_BB] = bb; w[B_RR] = rr; w[B_RHOCUR] = rho; w[B_DONE] = done;
@@ -253,17 +274,17 @@ __device__ __forceinline__ void biaHead(const double* partIn, int nb, double* st
 }
 // KB head: alpha = rho / (rh, v)
 template <int NV>
-__device__ __forceinline__ void bibHead(const double* partIn, int nb, double* st, double* al, bool* live) {
+__device__ __forceinline__ void bibHead(const double* partIn, int nb, double* st, const BiState<NV>& S, double* al,
+                                        bool* live) {
     double t[NV];
     readPartials<KBS, NV>(partIn, nb, t);
     for (int s = 0; s < NV; ++s) {
-        const double* c = st + B_STRIDE * s;
-        live[s] = c[B_DONE] == 0. && t[s] != 0.;
-        al[s] = live[s] ? c[B_RHOCUR] / t[s] : 0.;
+        live[s] = S.done[s] == 0. && t[s] != 0.;
+        al[s] = live[s] ? S.rho[s] / t[s] : 0.;
         if (BLOCK0) {
             double* w = st + B_STRIDE * s;
             w[B_ALPHA] = al[s];
-            if (c[B_DONE] == 0. && t[s] == 0.) w[B_BRK] = 1.;
+            if (S.done[s] == 0. && t[s] == 0.) w[B_BRK] = 1.;
         }
     }
 }
@@ -273,12 +294,12 @@ __device__ __forceinline__ void bibHead(const double* partIn, int nb, double* st
 template <int NV, class T, class Op>
 __global__ void k_bi_a(Dims D, Op A, const T* r, const T* rh, const T* pold, T* pnew, const T* vold, T* vnew,
                        const T* dinv, const double* partIn, int nb, double* partOut, double* st, LoopCtl lc) {
-    bool any = false;
-    for (int s = 0; s < NV; ++s) any |= st[B_STRIDE * s + B_DONE] == 0.;
-    if (!any) return;
+    BiState<NV> S;
+    S.load(st);
+    if (!S.any()) return;
     double beta[NV], om[NV];
     bool live[NV];
-    biaHead<NV>(partIn, nb, st, lc, beta, om, live);
+    biaHead<NV>(partIn, nb, st, lc, S, beta, om, live);
     double acc[NV];
     for (int s = 0; s < NV; ++s) acc[s] = 0.;
     const int P = D.P;
@@ -302,12 +323,12 @@ __global__ void k_bi_a(Dims D, Op A, const T* r, const T* rh, const T* pold, T* 
 template <int NV, class T, class Op>
 __global__ void k_bi_b(Dims D, Op A, const T* r, const T* v, T* sv, T* tv, const T* dinv, const double* partIn, int nb,
                        double* partOut, double* st) {
-    bool any = false;
-    for (int s = 0; s < NV; ++s) any |= st[B_STRIDE * s + B_DONE] == 0.;
-    if (!any) return;
+    BiState<NV> S;
+    S.load(st);
+    if (!S.any()) return;
     double al[NV];
     bool live[NV];
-    bibHead<NV>(partIn, nb, st, al, live);
+    bibHead<NV>(partIn, nb, st, S, al, live);
     double acc[2 * NV];
     for (int s = 0; s < 2 * NV; ++s) acc[s] = 0.;
     const int P = D.P;
@@ -336,12 +357,12 @@ __global__ void k_bi_a_band(Dims D, BandT<T> A, const T* __restrict__ r, const T
                             const T* __restrict__ pold, T* __restrict__ pnew, const T* __restrict__ vold,
                             T* __restrict__ vnew, const T* __restrict__ dinv, const double* partIn, int nb,
                             double* partOut, double* st, LoopCtl lc) {
-    bool any = false;
-    for (int s = 0; s < 3; ++s) any |= st[B_STRIDE * s + B_DONE] == 0.;
-    if (!any) return;
+    BiState<3> S;
+    S.load(st);
+    if (!S.any()) return;
     double beta[3], om[3];
     bool live[3];
-    biaHead<3>(partIn, nb, st, lc, beta, om, live);
+    biaHead<3>(partIn, nb, st, lc, S, beta, om, live);
     double acc[3] = {0., 0., 0.};
     const int P = D.P;
     const int o[7] = {0, 1, -1, D.sJ, -D.sJ, D.sK, -D.sK};
@@ -372,12 +393,12 @@ template <class T>
 __global__ void k_bi_b_band(Dims D, BandT<T> A, const T* __restrict__ r, const T* __restrict__ v, T* __restrict__ sv,
                             T* __restrict__ tv, const T* __restrict__ dinv, const double* partIn, int nb,
                             double* partOut, double* st) {
-    bool any = false;
-    for (int s = 0; s < 3; ++s) any |= st[B_STRIDE * s + B_DONE] == 0.;
-    if (!any) return;
+    BiState<3> S;
+    S.load(st);
+    if (!S.any()) return;
     double al[3];
     bool live[3];
-    bibHead<3>(partIn, nb, st, al, live);
+    bibHead<3>(partIn, nb, st, S, al, live);
     double acc[6] = {0., 0., 0., 0., 0., 0.};
     const int P = D.P;
     const int o[7] = {0, 1, -1, D.sJ, -D.sJ, D.sK, -D.sK};
@@ -411,22 +432,28 @@ __global__ void k_bi_b_band(Dims D, BandT<T> A, const T* __restrict__ r, const T
 template <int NV, class T, class Op>
 __global__ void k_bi_c(Dims D, Op A, T* x, T* r, const T* rh, const T* pv, const T* sv, const T* tv, const T* dinv,
                        const double* partIn, int nb, double* partOut, double* st) {
+    // B_BRK is 1 when KB found (rh, v) = 0 (it then wrote no s, t); this kernel marks a zero
+    // omega with 2, which other blocks may already see: they still update, as block 0 does
+    double sDone[NV], sBrk[NV], sAl[NV], sRho[NV], sIt[NV];
     bool any = false;
-    for (int s = 0; s < NV; ++s) any |= st[B_STRIDE * s + B_DONE] == 0.;
+    for (int s = 0; s < NV; ++s) {
+        const double* c = st + B_STRIDE * s;
+        sDone[s] = c[B_DONE]; sBrk[s] = c[B_BRK]; sAl[s] = c[B_ALPHA]; sRho[s] = c[B_RHOCUR]; sIt[s] = c[B_IT];
+        any |= sDone[s] == 0.;
+    }
     if (!any) return;
     double t[2 * NV];
     readPartials<KBS, 2 * NV>(partIn, nb, t);
     double al[NV], om[NV];
     bool live[NV];
     for (int s = 0; s < NV; ++s) {
-        const double* c = st + B_STRIDE * s;
-        live[s] = c[B_DONE] == 0. && c[B_BRK] == 0.;
-        al[s] = c[B_ALPHA];
+        live[s] = sDone[s] == 0. && sBrk[s] != 1.;
+        al[s] = sAl[s];
         om[s] = t[2 * s + 1] > 0. ? t[2 * s] / t[2 * s + 1] : 0.;
         if (BLOCK0 && live[s]) {
             double* w = st + B_STRIDE * s;
-            w[B_OMEGA] = om[s]; w[B_RHOPREV] = c[B_RHOCUR]; w[B_IT] = c[B_IT] + 1.;
-            if (om[s] == 0.) w[B_BRK] = 1.;
+            w[B_OMEGA] = om[s]; w[B_RHOPREV] = sRho[s]; w[B_IT] = sIt[s] + 1.;
+            if (om[s] == 0.) w[B_BRK] = 2.;
         }
     }
     double acc[2 * NV];
@@ -790,10 +817,12 @@ __global__ void k_cp_a(Dims D, CpOp A, const double* __restrict__ r, const doubl
                        const double* __restrict__ pold, double* __restrict__ pnew, const double* __restrict__ vold,
                        double* __restrict__ vnew, const double* __restrict__ dinv, const double* partIn, int nb,
                        double* partOut, double* st, LoopCtl lc) {
-    if (st[B_DONE] != 0.) return;
+    BiState<1> S;
+    S.load(st);
+    if (!S.any()) return;
     double beta, om;
     bool live;
-    biaHead<1>(partIn, nb, st, lc, &beta, &om, &live);
+    biaHead<1>(partIn, nb, st, lc, S, &beta, &om, &live);
     if (!live) return;
     double acc[1] = {0.};
     const int P = D.P;
@@ -816,10 +845,12 @@ __global__ void k_cp_a(Dims D, CpOp A, const double* __restrict__ r, const doubl
 __global__ void k_cp_b(Dims D, CpOp A, const double* __restrict__ r, const double* __restrict__ v,
                        double* __restrict__ sv, double* __restrict__ tv, const double* __restrict__ dinv,
                        const double* partIn, int nb, double* partOut, double* st) {
-    if (st[B_DONE] != 0.) return;
+    BiState<1> S;
+    S.load(st);
+    if (!S.any()) return;
     double al;
     bool live;
-    bibHead<1>(partIn, nb, st, &al, &live);
+    bibHead<1>(partIn, nb, st, S, &al, &live);
     if (!live) return;
     double acc[2] = {0., 0.};
     const int P = D.P;
@@ -843,16 +874,17 @@ __global__ void k_cp_b(Dims D, CpOp A, const double* __restrict__ r, const doubl
 // KC: omega = (t, s) / (t, t); x += alpha D^-1 p + omega D^-1 s; r = s - omega t; partials (rh, r), (r, r)
 __global__ void k_cp_c(Dims D, double* x, double* r, const double* rh, const double* pv, const double* sv,
                        const double* tv, const double* dinv, const double* partIn, int nb, double* partOut, double* st) {
-    if (st[B_DONE] != 0.) return;
+    const double sDone = st[B_DONE], sBrk = st[B_BRK], sAl = st[B_ALPHA], sRho = st[B_RHOCUR], sIt = st[B_IT];
+    if (sDone != 0.) return;
     double t[2];
     readPartials<KBS, 2>(partIn, nb, t);
     // B_BRK is 1 when KB found (rh, v) = 0 (it then wrote no s, t); this kernel marks a zero
     // omega with 2, which the other blocks may already see: they still update, as block 0 does
-    const bool live = st[B_BRK] != 1.;
-    const double al = st[B_ALPHA];
+    const bool live = sBrk != 1.;
+    const double al = sAl;
     const double om = t[1] > 0. ? t[0] / t[1] : 0.;
     if (BLOCK0 && live) {
-        st[B_OMEGA] = om; st[B_RHOPREV] = st[B_RHOCUR]; st[B_IT] = st[B_IT] + 1.;
+        st[B_OMEGA] = om; st[B_RHOPREV] = sRho; st[B_IT] = sIt + 1.;
         if (om == 0.) st[B_BRK] = 2.;
     }
     double acc[2] = {0., 0.};

@@ -319,14 +319,15 @@ __global__ void k_f_pre0(Dims D, SymF S, const float* r, float* w, const double*
 // would otherwise split at the barrier.  When the flag is read as set, the partials
 // already meet the criterion, so every wave reaches the same decision.
 __global__ void k_f_restrict(Dims D, const float* w, float* v, const double* partIn, int nb, double* st, FcgCtl lc) {
+    const double sBb = st[F_BB], sDone = st[F_DONE], sIt = st[F_IT];
     double t[2];
     readPartials<256, 2>(partIn, nb, t);
     const double rr = lc.nLive > 0. ? fmax(t[0] - t[1] * t[1] / lc.nLive, 0.) : t[0];
-    const double bb = st[F_BB] < 0. ? rr : st[F_BB];
-    const bool done = st[F_DONE] != 0. || rr <= lc.tol2 * bb || st[F_IT] >= lc.maxit || rr == 0.;
+    const double bb = sBb < 0. ? rr : sBb;
+    const bool done = sDone != 0. || rr <= lc.tol2 * bb || sIt >= lc.maxit || rr == 0.;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st[F_BB] = bb; st[F_RR] = rr; st[F_SR] = t[1];
-        if (st[F_IT] == 1.) st[F_FIRSTRR] = rr;
+        if (sIt == 1.) st[F_FIRSTRR] = rr;
         if (done) st[F_DONE] = 1.;
     }
     if (done) return;
@@ -335,8 +336,9 @@ __global__ void k_f_restrict(Dims D, const float* w, float* v, const double* par
 
 // y = w D^-1 r + P c
 __global__ void k_f_post1(Dims D, SymF S, const float* r0, const float* r1, const float* c, float* y, const double* st) {
-    if (st[F_DONE] != 0.) return;
-    const float* r = ((int)st[F_IT] & 1) ? r1 : r0;
+    const double sDone = st[F_DONE], sIt = st[F_IT];
+    if (sDone != 0.) return;
+    const float* r = ((int)sIt & 1) ? r1 : r0;
     const int cx = D.nI >> 1, cy = D.nJ >> 1, cz = D.nK >> 1;
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < D.N; q += gridDim.x * blockDim.x) {
         int i, j, k;
@@ -348,8 +350,9 @@ __global__ void k_f_post1(Dims D, SymF S, const float* r0, const float* r1, cons
 // z = y + w D^-1 (r - A y); partials (z, q_old), sum z, (z, r)
 __global__ void k_f_post2(Dims D, SymF S, const float* r0, const float* r1, const float* y, float* z, const float* q0,
                           const float* q1, double* part, const double* st) {
-    if (st[F_DONE] != 0.) return;
-    const int it = (int)st[F_IT];
+    const double sDone = st[F_DONE], sIt = st[F_IT];
+    if (sDone != 0.) return;
+    const int it = (int)sIt;
     const float* qold = (it & 1) ? q0 : q1;
     const float* r = (it & 1) ? r1 : r0;
     double v[3] = {0., 0., 0.};
@@ -371,17 +374,19 @@ __global__ void k_f_post2(Dims D, SymF S, const float* r0, const float* r1, cons
 // partials of k_f_post2; block 0 records it for k_f_k2pre.
 __global__ void k_f_k1(Dims D, SymF S, const float* z, float* p0, float* p1, float* q0, float* q1,
                        const double* partIn, int nb, double* partOut, double* st, FcgCtl lc) {
-    if (st[F_DONE] != 0.) return;
+    // the scalars written by earlier kernels are loaded together, ahead of the partial sums
+    const double sDone = st[F_DONE], sIt = st[F_IT], sSq = st[F_SQ], sPq = st[F_PQ], sSr = st[F_SR];
+    if (sDone != 0.) return;
     double t[3];
     readPartials<FB, 3>(partIn, nb, t);
-    const int it = (int)st[F_IT];
+    const int it = (int)sIt;
     const double zbar = lc.nLive > 0. ? t[1] / lc.nLive : 0.;
-    const double zq = t[0] - zbar * st[F_SQ];
-    const double beta = it == 0 || st[F_PQ] == 0. ? 0. : -zq / st[F_PQ];
+    const double zq = t[0] - zbar * sSq;
+    const double beta = it == 0 || sPq == 0. ? 0. : -zq / sPq;
     const float* pold = (it & 1) ? p0 : p1;
     float* pnew = (it & 1) ? p1 : p0;
     float* qnew = (it & 1) ? q1 : q0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) { st[F_PAR] = (double)(it & 1); st[F_PR] = t[2] - zbar * st[F_SR]; }
+    if (blockIdx.x == 0 && threadIdx.x == 0) { st[F_PAR] = (double)(it & 1); st[F_PR] = t[2] - zbar * sSr; }
     const float zb = (float)zbar, bt = (float)beta;
     auto pf = [&](int c) { return (z[c] - zb) + bt * pold[c]; };
     double v[2] = {0., 0.};
@@ -403,16 +408,17 @@ __global__ void k_f_k1(Dims D, SymF S, const float* z, float* p0, float* p1, flo
 __global__ void k_f_k2pre(Dims D, SymF S, float* e, const float* r0, const float* r1, float* w, const float* p0,
                           const float* p1, const float* q0, const float* q1, const double* partIn, int nb,
                           double* partOut, double* st) {
-    if (st[F_DONE] != 0.) return;
+    const double sDone = st[F_DONE], sPar = st[F_PAR], sPr = st[F_PR], sIt = st[F_IT];
+    if (sDone != 0.) return;
     double t[2];
     readPartials<FB, 2>(partIn, nb, t);
-    const int par = (int)st[F_PAR];          // this iteration's parity, set by k_f_k1
+    const int par = (int)sPar;               // this iteration's parity, set by k_f_k1
     const float* pv = par ? p1 : p0;
     const float* qv = par ? q1 : q0;
     const float* rc = par ? r1 : r0;
     float* rn = (float*)(par ? r0 : r1);
-    const double alpha = t[0] != 0. ? st[F_PR] / t[0] : 0.;
-    if (blockIdx.x == 0 && threadIdx.x == 0) { st[F_PQ] = t[0]; st[F_SQ] = t[1]; st[F_IT] += 1.; }
+    const double alpha = t[0] != 0. ? sPr / t[0] : 0.;
+    if (blockIdx.x == 0 && threadIdx.x == 0) { st[F_PQ] = t[0]; st[F_SQ] = t[1]; st[F_IT] = sIt + 1.; }
     const float al = (float)alpha;
     auto rnew = [&](int c) { return rc[c] - al * qv[c]; };
     auto z1 = [&](int c) { return NET_OMEGA * S.inv(c) * rnew(c); };

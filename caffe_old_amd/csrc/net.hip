@@ -55,6 +55,10 @@ __device__ __forceinline__ void ld4(const float* p, float (&v)[4]) {
     v[0] = f.x; v[1] = f.y; v[2] = f.z; v[3] = f.w;
 }
 __device__ __forceinline__ bool isDone(const double* done) { return done && *done != 0.; }
+// The forward kernels always get a flag (net_forward passes a slot that stays zero when the
+// caller has none), load it first and test it once their staging loads are in flight, so
+// the flag's latency overlaps theirs.
+__device__ __forceinline__ double flagOf(const double* done) { return *done; }
 
 constexpr int NB = 256;   // threads per block of the network kernels
 constexpr int LS = 20;    // LDS row of the weight-gradient kernel: 16 BF16 channels + 4 padding
@@ -278,7 +282,7 @@ __device__ __forceinline__ int tapOff(int t, int HX, int HY) {
 
 template <int TX, int TY, int TZ, int CIN, int MODE, bool POOLOUT>
 __global__ __launch_bounds__(NB) void k_convf(ConvFArgs a) {
-    if (isDone(a.done)) return;
+    const double flag = flagOf(a.done);
     constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2, HV = HX * HY * HZ;
     constexpr int TV = TX * TY * TZ, G = TV / 64;
     constexpr int C8 = CIN / 8;                               // 16-byte chunks per row
@@ -329,6 +333,7 @@ __global__ __launch_bounds__(NB) void k_convf(ConvFArgs a) {
         vw[u] = make_uint4(0u, 0u, 0u, 0u);
         if (it < NWI) vw[u] = *(const uint4*)(ws + (size_t)it * 8);   // packed [cb][t][co][ci]
     }
+    if (flag != 0.) return;
 #pragma unroll
     for (int u = 0; u < NIT; ++u) {
         const int it = tid + u * NB;
@@ -448,7 +453,7 @@ __device__ __forceinline__ bf16x4 asbf4(uint32_t x, uint32_t y) {
 
 template <int MODE, bool POOLOUT>
 __global__ __launch_bounds__(NB) void k_convf16(ConvFArgs a) {
-    if (isDone(a.done)) return;
+    const double flag = flagOf(a.done);
     constexpr int TX = 16, TY = 4, TZ = 4, CIN = 16, C8 = 2;
     constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2, HV = HX * HY * HZ;
     constexpr int TV = TX * TY * TZ, G = TV / 64;
@@ -486,6 +491,7 @@ __global__ __launch_bounds__(NB) void k_convf16(ConvFArgs a) {
             }
         }
     }
+    if (flag != 0.) return;
 #pragma unroll
     for (int u = 0; u < NIT; ++u) {
         const int it = tid + u * NB;
@@ -731,7 +737,7 @@ static int edgeBlocks(int nx, int ny, int nz) {
 // in: 1 -> 16 channels; W [27][16][1]
 __global__ __launch_bounds__(256) void k_in_fwd(const float* x, const float* W, uint16_t* out, int nx, int ny, int nz,
                                                 const double* done) {
-    if (isDone(done)) return;
+    const double flag = flagOf(done);
     __shared__ __align__(16) float sW[27 * 16];
     __shared__ float sx[EHV];
     int ox, oy, oz;
@@ -747,6 +753,7 @@ __global__ __launch_bounds__(256) void k_in_fwd(const float* x, const float* W, 
         const float val = x[g];
         tv[u] = (in && hv < EHV) ? val : 0.f;
     }
+    if (flag != 0.) return;
     for (int i = tid; i < 27 * 16; i += 256) sW[i] = W[i];
 #pragma unroll
     for (int u = 0; u < NIT; ++u)
@@ -790,7 +797,7 @@ __device__ __forceinline__ float dot2(uint32_t a, uint32_t b, float c) {
 }
 __global__ __launch_bounds__(256) void k_out_fwd(const uint16_t* a, const float* W, float* y, int nx, int ny, int nz,
                                                  const double* done) {
-    if (isDone(done)) return;
+    const double flag = flagOf(done);
     __shared__ __align__(16) uint32_t sWh[27 * 8], sWl[27 * 8];   // [t][channel pair]: hi and lo parts
     __shared__ __align__(16) uint4 sa[EHV * 2];   // 16 BF16 channels per halo voxel
     int ox, oy, oz;
@@ -807,6 +814,7 @@ __global__ __launch_bounds__(256) void k_out_fwd(const uint16_t* a, const float*
         const uint4 val = ((const uint4*)(a + (size_t)g * 16))[h];
         tv[u] = (in && it < NI) ? val : make_uint4(0u, 0u, 0u, 0u);
     }
+    if (flag != 0.) return;
     if (tid < 27 * 8) {
         const float w0 = W[2 * tid], w1 = W[2 * tid + 1];
         const uint32_t h0 = f2bf(w0), h1 = f2bf(w1);
@@ -1086,6 +1094,7 @@ static void convFLaunch(hipStream_t s, const ConvFArgs& a, int cin) {
 }
 
 void net_forward(Net& n, hipStream_t s, const float* vin, float* vout, const double* done) {
+    if (!done) done = n.scal + NET_ZERO_SLOT;   // a flag that stays zero
     hipLaunchKernelGGL(k_in_fwd, dim3(edgeBlocks(n.nx, n.ny, n.nz)), dim3(256), 0, s, vin, n.theta + n.off[0], n.act[0], n.nx, n.ny, n.nz, done);
     for (int l = 1; l < NET_OUT; ++l) {
         const LayerDef& d = kNet[l];

@@ -37,6 +37,7 @@ constexpr LayerDef kNet[NET_LAYERS] = {
     {16, 16, 0, MODE_UPADD, 9, 1},     // u0
     {16, 1, 0, MODE_PLAIN, 10, -1},    // out
 };
+constexpr int NET_ZERO_SLOT = 63;  // the forward pass's "not done" flag when the caller has none
 constexpr float NET_OMEGA = 0.8f;  // damped-Jacobi weight of the fine-grid smoothing sweeps
 
 // One training pair: the p' system (FP32 symmetric 4-array storage on the padded fine
@@ -89,7 +90,7 @@ struct Net {
 
     // reductions
     double* part = nullptr;          // 2 x MAX_RED_BLOCKS x RED_STRIDE
-    double* scal = nullptr;          // 64 device doubles
+    double* scal = nullptr;          // 64 device doubles; slot NET_ZERO_SLOT stays zero
     double* hscal = nullptr;         // pinned mirror
     float* losses = nullptr;         // device loss history
     long lossCap = 0;
