@@ -119,6 +119,89 @@ def coupled_bench(a):
     print(json.dumps(out), flush=True)
 
 
+def workload_bench(a):
+    """The two full-size configurations beside the headline one, each on one GPU:
+      ib-channel  channel 4 x 1 x 1 on 256 x 64 x 64 cells with an immersed sphere (D 0.2 at
+                  (1, .5, .5)), uniform inlet u = 1, Re_D 200, PISO 2 correctors, dt 0.5/64;
+      skewed      256^3 cavity whose interior nodes move by up to 0.15 h (seeded Philox),
+                  Re 1000, PISO 2 correctors, dt 0.5/256.
+    One step is Piso::solve / IbPiso::solve.  The p' solves of the timed steps use --mode
+    (auto: learned where the network applies, else FP32 IR; the ghost-row systems of the IB
+    channel go to FP64 BiCGStab in any mode).  After the timed steps the same p' system is
+    solved by each method to a true FP64 relres of 1e-8.  The roofline names the p' Krylov
+    stencil kernel of the timed steps; `device_bytes` is the context's HBM footprint."""
+    from caffe_old_amd import _lib as L
+    from caffe_old_amd import case as cs
+    from caffe_old_amd.learned import PressureNet, trained_weights_path
+    from caffe_old_amd.solver import make_solver
+    ib = a.workload == "ib-channel"
+    if ib:
+        case = cs.channel_ib_sphere(256, 64, 64, steps=a.steps + a.warmup)
+        desc = ("channel 4x1x1, 256x64x64 uniform hex, immersed sphere D 0.2 at (1,.5,.5), inlet u 1, Re_D 200, "
+                "IbPiso 2 correctors, dt 0.5/64")
+        s = make_solver(case)
+    else:
+        case = cs.skewed_cavity(256, steps=a.steps + a.warmup)
+        desc = ("256^3 lid-driven cavity, interior nodes moved by up to 0.15 h (Philox seed 0), Re 1000, PISO 2 "
+                "correctors, dt 0.5/256")
+        s = make_solver(case, uniform=False)
+    net = None
+    wpath = trained_weights_path(128)
+    if not ib and os.path.exists(wpath):
+        # the network trained on 128^3 cavities, applied on the 128^3 coarse grid of this mesh
+        net = PressureNet(seed=0).load(wpath)
+        s.attach_net(net)
+    mode_name = a.mode if a.mode != "auto" else ("ir" if ib or net is None else "learned")
+    modes = {"ir": L.SOLVE_IR_FP32, "plain": L.SOLVE_PLAIN_FP64, "learned": L.SOLVE_LEARNED_FCG}
+    inner = {"ir": 1e-2, "plain": 1e-2, "learned": 1e-4}
+    s.set_solve_opts("momentum", mode=L.SOLVE_IR_FP32 if not ib else L.SOLVE_PLAIN_FP64, rtol=1e-8, maxit=5000)
+    s.set_solve_opts("pcorr", mode=modes[mode_name], rtol=1e-8, maxit=200000, inner_rtol=inner[mode_name])
+    if ib:
+        kname = "pcorr_bicg_a_fp64"
+    else:
+        kname = {"learned": "fcg_k1", "ir": "pcorr_cg_k1_fp32", "plain": "pcorr_cg_k1_fp64"}[mode_name]
+    for w in range(a.warmup):
+        s.solve()
+    s.kernel_probe(kname, True)
+    s.sync()
+    t0 = time.perf_counter()
+    its = []
+    for _ in range(a.steps):
+        s.solve()
+        m, p = s.reports()
+        its.append({"p_iters": p["iters"], "fell_back": p["fell_back"], "mom_iters": m["iters"]})
+    s.sync()
+    elapsed = time.perf_counter() - t0
+    kms, klaunches, kbytes = s.kernel_probe_read()
+    s.kernel_probe(kname, False)
+    if a.calibrate_copy:
+        s.time_kernel("copy_fp64", reps=5)
+    psolve = {}
+    methods = [("plain_fp64", L.SOLVE_PLAIN_FP64, 1e-2), ("ir_fp32", L.SOLVE_IR_FP32, 1e-2)]
+    if net is not None:
+        methods.append(("learned_fcg", L.SOLVE_LEARNED_FCG, 1e-4))
+    for name, md, inr in methods:
+        s.pcorr_assemble()
+        r = s.pcorr_solve(mode=md, rtol=1e-8, maxit=200000, inner_rtol=inr)
+        psolve[name] = {"ms": round(r["ms"], 3), "iters": r["iters"], "refinements": r["refinements"],
+                        "relres": r["relres"], "fell_back": r["fell_back"]}
+    ms_per_step = 1000.0 * elapsed / a.steps
+    achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    out = {
+        "metric": f"{'IbPiso' if ib else 'PISO'} steps/s, {a.workload}", "value": round(1000.0 / ms_per_step, 4),
+        "unit": "steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"plain": "f64", "ir": "f64+f32", "learned": "f64+f32+bf16"}[mode_name],
+        "data": "synthetic (no external data)",
+        "config": {"workload": desc + f", p' solve to true relres 1e-8 ({mode_name})", "cells": case.n_cells,
+                   "device_bytes": s.device_bytes(), "steps_detail": its, "p_solve_ms": psolve},
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "ms_per_launch": round(kms, 5),
+                     "bytes_per_launch": kbytes, "launches_timed": klaunches},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +220,8 @@ def main():
                     help="one warm-up step, then only the dominant kernel and the copy kernel (for counter runs)")
     ap.add_argument("--coupled", type=int, default=0, metavar="N",
                     help="measure the Coupled solver on the N^3 cavity instead (its own JSON line)")
+    ap.add_argument("--workload", choices=["cavity", "ib-channel", "skewed"], default="cavity",
+                    help="ib-channel / skewed: the 256x64x64 IB channel or the skewed 256^3 cavity (own JSON line)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: --gpus {a.gpus} is not supported; this build runs the whole domain on one GPU "
@@ -144,6 +229,9 @@ def main():
         sys.exit(2)
     if a.coupled:
         coupled_bench(a)
+        return
+    if a.workload != "cavity":
+        workload_bench(a)
         return
 
     from caffe_old_amd import _lib as L

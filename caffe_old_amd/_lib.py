@@ -30,7 +30,7 @@ EXPORTS = [
     "caffe_metric_download", "caffe_matrix_download", "caffe_rhs_download", "caffe_ghost_rows", "caffe_time_kernel",
     "caffe_kernel_probe", "caffe_kernel_probe_read",
     "caffe_sync",
-    "caffe_padded_size",
+    "caffe_padded_size", "caffe_device_bytes",
     # learned pressure-correction preconditioner
     "caffe_net_create", "caffe_net_destroy", "caffe_net_last_error", "caffe_net_param_count", "caffe_net_get_params",
     "caffe_net_set_params", "caffe_net_forward", "caffe_net_backward", "caffe_net_precond", "caffe_net_loss_grad",
@@ -111,6 +111,7 @@ def load(path: str = LIB_PATH):
         "caffe_kernel_probe_read": ([vp, dp, lp, dp], C.c_int),
         "caffe_sync": ([vp], C.c_int),
         "caffe_padded_size": ([vp], C.c_long),
+        "caffe_device_bytes": ([vp], C.c_long),
         "caffe_net_create": ([C.c_int, C.c_ulonglong, C.POINTER(C.c_void_p)], C.c_int),
         "caffe_net_destroy": ([vp], C.c_int),
         "caffe_net_last_error": ([vp], C.c_char_p),

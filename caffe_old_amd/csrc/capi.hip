@@ -31,6 +31,7 @@ double* dalloc(Ctx& c, size_t n) {
     HIPCHECK(hipMalloc(&p, n * sizeof(double)));
     HIPCHECK(hipMemsetAsync(p, 0, n * sizeof(double), c.stream));
     c.allocs.push_back(p);
+    c.deviceBytes += n * sizeof(double);
     return (double*)p;
 }
 template <class T>
@@ -39,6 +40,7 @@ T* talloc(Ctx& c, size_t n) {
     HIPCHECK(hipMalloc(&p, n * sizeof(T)));
     HIPCHECK(hipMemsetAsync(p, 0, n * sizeof(T), c.stream));
     c.allocs.push_back(p);
+    c.deviceBytes += n * sizeof(T);
     return (T*)p;
 }
 
@@ -46,6 +48,7 @@ void freeAll(Ctx& c) {
     if (c.stream) (void)hipStreamSynchronize(c.stream);
     for (void* p : c.allocs) (void)hipFree(p);
     c.allocs.clear();
+    c.deviceBytes = 0;
     c.wb32 = nullptr; c.Am32 = nullptr; c.wm32 = nullptr;
     c.ov = Overlay();
     c.ready = false;
@@ -649,6 +652,8 @@ int caffe_kernel_probe(caffe_ctx* x, const char* which, int enable) {
         else if (w == "pcorr_cg_k1_fp32") kind = PROBE_CG_K1_FP32;
         else if (w == "pcorr_cg_k1_fp64") kind = PROBE_CG_K1_FP64;
         else if (w == "coupled_bicg_a") kind = PROBE_CP_A;
+        else if (w == "pcorr_bicg_a_fp32") kind = PROBE_PBI_A_FP32;
+        else if (w == "pcorr_bicg_a_fp64") kind = PROBE_PBI_A_FP64;
         else { c.err = "unknown probe kernel " + w; return CAFFE_ERR_ARG; }
         if (kind == PROBE_CP_A && !c.cA) { c.err = "not a Coupled case"; return CAFFE_ERR_STATE; }
         HIPCHECK(hipStreamSynchronize(c.stream));
@@ -689,8 +694,17 @@ int caffe_kernel_probe_read(caffe_ctx* x, double* avg_ms, long* launches, double
         // search direction's two inputs (z, p_old) and the four matrix arrays and write p
         // and q: 8 x 4 B in FP32, 8 x 8 B in FP64
         // The Coupled BiCGStab kernel KA reads the 56 coefficient planes, r, p_old, v_old, rh
-        // and D^-1 of the four variables and writes p and v: 84 doubles per cell
-        const double per = c.probe.which == PROBE_CP_A ? 672. : (c.probe.which == PROBE_CG_K1_FP64 ? 64. : 32.);
+        // and D^-1 of the four variables and writes p and v: 84 doubles per cell.  The p'
+        // BiCGStab KA reads the four matrix arrays, r, rh, p_old, v_old and D^-1 and writes
+        // p and v: 11 values per cell (ghost rows, about 0.1% of the cells, not counted).
+        double per = 32.;
+        switch (c.probe.which) {
+        case PROBE_CP_A: per = 672.; break;
+        case PROBE_CG_K1_FP64: per = 64.; break;
+        case PROBE_PBI_A_FP32: per = 44.; break;
+        case PROBE_PBI_A_FP64: per = 88.; break;
+        default: break;
+        }
         if (bytes) *bytes = per * (double)c.D.N;
         return CAFFE_OK;
     });
@@ -701,5 +715,7 @@ int caffe_sync(caffe_ctx* x) {
 }
 
 long caffe_padded_size(caffe_ctx* x) { return x ? x->c.D.P : -1; }
+
+long caffe_device_bytes(caffe_ctx* x) { return x ? (long)x->c.deviceBytes : -1; }
 
 }  // extern "C"

@@ -20,7 +20,8 @@ enum {
 };
 enum { FL_MDONE = 0, FL_PDONE = 1, FL_MITER = 2, FL_PITER = 3, N_FLAGS = 8 };
 
-enum { PROBE_NONE = 0, PROBE_FCG_K1 = 1, PROBE_CG_K1_FP32 = 2, PROBE_CG_K1_FP64 = 3, PROBE_CP_A = 4 };
+enum { PROBE_NONE = 0, PROBE_FCG_K1 = 1, PROBE_CG_K1_FP32 = 2, PROBE_CG_K1_FP64 = 3, PROBE_CP_A = 4,
+       PROBE_PBI_A_FP32 = 5, PROBE_PBI_A_FP64 = 6 };
 
 // Coupled system (Coupled.cpp:54-228): coefficient planes of P doubles per cell, each group
 // of seven in the neighbour order own, E, W, N, S, T, B
@@ -99,6 +100,7 @@ struct Ctx {
     double* metric = nullptr;
     double* dnodes = nullptr; // node array of general meshes (IB cell location)
     std::vector<void*> allocs;
+    size_t deviceBytes = 0;   // bytes held in allocs
 
     caffe_solve_opts momOpts{}, pcOpts{};
     caffe_solve_report lastMom{}, lastPc{};

@@ -617,7 +617,14 @@ static int runBicgstab(Ctx& c, Op A, const double* b, double shift, T* x, T* w, 
                     hipLaunchKernelGGL((k_bi_b<NV, T, Op>), dim3(g), dim3(KBS), 0, c.stream, D, A, r, vn, s, t, dinv, partA, g, partB, st);
                 }
             } else {
-                hipLaunchKernelGGL((k_bi_a<NV, T, Op>), dim3(g), dim3(KBS), 0, c.stream, D, A, r, rh, pc, pn, vc, vn, dinv, partC, g, partA, st, lc);
+                hipEvent_t e0, e1;
+                // the p' BiCGStab (one system: ghost-row or indefinite p' systems) can be probed
+                if (NV == 1 && c.probe.slot(std::is_same<T, float>::value ? PROBE_PBI_A_FP32 : PROBE_PBI_A_FP64, e0, e1))
+                    hipExtLaunchKernelGGL((k_bi_a<NV, T, Op>), dim3(g), dim3(KBS), 0, c.stream, e0, e1, 0, D, A, (const T*)r,
+                                          (const T*)rh, (const T*)pc, pn, (const T*)vc, vn, (const T*)dinv,
+                                          (const double*)partC, g, partA, st, lc);
+                else
+                    hipLaunchKernelGGL((k_bi_a<NV, T, Op>), dim3(g), dim3(KBS), 0, c.stream, D, A, r, rh, pc, pn, vc, vn, dinv, partC, g, partA, st, lc);
                 hipLaunchKernelGGL((k_bi_b<NV, T, Op>), dim3(g), dim3(KBS), 0, c.stream, D, A, r, vn, s, t, dinv, partA, g, partB, st);
             }
             hipLaunchKernelGGL((k_bi_c<NV, T, Op>), dim3(g), dim3(KBS), 0, c.stream, D, A, x, r, rh, pn, s, t, dinv, partB, g, partC, st);

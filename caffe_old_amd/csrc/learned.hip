@@ -50,6 +50,29 @@ static SymF symF(const float* A, const Dims& D) { return SymF{A, D.P, D.sJ, D.sK
              q##e_ = min((D).N, q - (int)threadIdx.x + q##c_), p = 0;                      \
          q < q##e_ && ((p = fineIdx((D), q)), true); q += blockDim.x)
 __device__ inline int fineIdx(const Dims& D, int q) { int i, j, k; return D.cell(q, i, j, k); }
+
+// Column tiles: a block takes (CTX x CTY) columns of (i, j) through a range of k planes,
+// one thread per column, k ascending, so that the k - 1 and k + 1 neighbours a cell reads
+// were, or will next be, loaded by the same block (on large grids the plane-chunk order
+// fetches them from HBM again).  Work item v (tile v / nseg, k segment v % nseg) goes to
+// blocks in the XCD-aware order, so an XCD holds whole tiles.
+constexpr int CTX = 64, CTY = 16;
+static_assert(CTX * CTY == FB, "one thread per column of a tile");
+struct ColMap { int ntx, nty, nseg, kper; };
+static ColMap colMap(const Dims& D, int g) {
+    ColMap m;
+    m.ntx = (D.nI + CTX - 1) / CTX;
+    m.nty = (D.nJ + CTY - 1) / CTY;
+    m.nseg = std::max(1, g / (m.ntx * m.nty));
+    m.kper = (D.nK + m.nseg - 1) / m.nseg;
+    return m;
+}
+#define FOR_COLS(D, M, p)                                                                           \
+    for (int v_ = xcdChunk(), vn_ = (M).ntx * (M).nty * (M).nseg; v_ < vn_; v_ += (int)gridDim.x)   \
+        for (int t_ = v_ / (M).nseg, i_ = (t_ % (M).ntx) * CTX + (int)(threadIdx.x % CTX),          \
+                 j_ = (t_ / (M).ntx) * CTY + (int)(threadIdx.x / CTX), k_ = (v_ % (M).nseg) * (M).kper, \
+                 ke_ = min((D).nK, k_ + (M).kper), p = 0;                                          \
+             i_ < (D).nI && j_ < (D).nJ && k_ < ke_ && ((p = (D).pidx(i_, j_, k_)), true); ++k_)
 __device__ __forceinline__ bool fdone(const double* done) { return done && *done != 0.; }
 static int fgrid(const Dims& D) { return blocksFor(D.N, FB, FGRID); }
 
@@ -372,7 +395,7 @@ __global__ void k_f_post2(Dims D, SymF S, const float* r0, const float* r1, cons
 // stencil points; q = A p; partials (p,q), sum q.  (p, r) needs no pass of its own:
 // alpha of the previous step makes (p_old, r) = 0, so (p, r) = (z - zbar, r), from the
 // partials of k_f_post2; block 0 records it for k_f_k2pre.
-__global__ void k_f_k1(Dims D, SymF S, const float* z, float* p0, float* p1, float* q0, float* q1,
+__global__ void k_f_k1(Dims D, SymF S, ColMap cm, const float* z, float* p0, float* p1, float* q0, float* q1,
                        const double* partIn, int nb, double* partOut, double* st, FcgCtl lc) {
     // the scalars written by earlier kernels are loaded together, ahead of the partial sums
     const double sDone = st[F_DONE], sIt = st[F_IT], sSq = st[F_SQ], sPq = st[F_PQ], sSr = st[F_SR];
@@ -388,15 +411,48 @@ __global__ void k_f_k1(Dims D, SymF S, const float* z, float* p0, float* p1, flo
     float* qnew = (it & 1) ? q1 : q0;
     if (blockIdx.x == 0 && threadIdx.x == 0) { st[F_PAR] = (double)(it & 1); st[F_PR] = t[2] - zbar * sSr; }
     const float zb = (float)zbar, bt = (float)beta;
-    auto pf = [&](int c) { return (z[c] - zb) + bt * pold[c]; };
     double v[2] = {0., 0.};
-    FOR_FINE(D, q, p) {
-        const float pn = pf(p);
-        const float qq = S.apply(pf, p);
-        pnew[p] = pn;
-        qnew[p] = qq;
-        v[0] += (double)pn * (double)qq;
-        v[1] += (double)qq;
+    // Each thread walks its column up k with a rolling window: the k - 1, k and k + 1 values
+    // of z and p_old stay in registers, and the loads of the next plane (its four side
+    // neighbours, the value above it and the seven matrix entries) are issued before the
+    // current cell's stores, which the compiler may not move past them.  (Two planes ahead
+    // measured no faster.)  The arithmetic is SymF::apply's,
+    // in its order.
+    const int P = S.P, sJ = S.sJ, sK = S.sK;
+    const float* A = S.A;
+    struct Plane { float zs[4], ps[4], zu, pu, a[7]; };   // sides W E S N, k + 1 (u), matrix
+    auto loadPlane = [&](int p, Plane& L) {
+        const int o[4] = {-1, 1, -sJ, sJ};
+#pragma unroll
+        for (int m = 0; m < 4; ++m) { L.zs[m] = z[p + o[m]]; L.ps[m] = pold[p + o[m]]; }
+        L.zu = z[p + sK]; L.pu = pold[p + sK];
+        L.a[0] = A[p]; L.a[1] = A[P + p]; L.a[2] = A[P + p + 1]; L.a[3] = A[2 * P + p];
+        L.a[4] = A[2 * P + p + sJ]; L.a[5] = A[3 * P + p]; L.a[6] = A[3 * P + p + sK];
+    };
+    auto f = [&](float zz, float pp) { return (zz - zb) + bt * pp; };
+    for (int v_ = xcdChunk(), vn_ = cm.ntx * cm.nty * cm.nseg; v_ < vn_; v_ += (int)gridDim.x) {
+        const int t_ = v_ / cm.nseg;
+        const int i = (t_ % cm.ntx) * CTX + (int)(threadIdx.x % CTX), j = (t_ / cm.ntx) * CTY + (int)(threadIdx.x / CTX);
+        int k = (v_ % cm.nseg) * cm.kper;
+        const int ke = min(D.nK, k + cm.kper);
+        if (!(i < D.nI && j < D.nJ) || k >= ke) continue;
+        int p = D.pidx(i, j, k);
+        float zd = z[p - sK], pd = pold[p - sK], zc = z[p], pc = pold[p];   // k - 1, k
+        Plane cur, nxt;
+        loadPlane(p, cur);
+        for (; k < ke; ++k, p += sK) {
+            if (k + 1 < ke) loadPlane(p + sK, nxt);
+            const float fc = f(zc, pc);
+            const float qq = cur.a[0] * fc + (cur.a[1] * f(cur.zs[0], cur.ps[0]) + cur.a[2] * f(cur.zs[1], cur.ps[1])
+                                              + cur.a[3] * f(cur.zs[2], cur.ps[2]) + cur.a[4] * f(cur.zs[3], cur.ps[3])
+                                              + cur.a[5] * f(zd, pd) + cur.a[6] * f(cur.zu, cur.pu));
+            pnew[p] = fc;
+            qnew[p] = qq;
+            v[0] += (double)fc * (double)qq;
+            v[1] += (double)qq;
+            zd = zc; pd = pc; zc = cur.zu; pc = cur.pu;
+            cur = nxt;
+        }
     }
     writePartials<FB, 2>(v, partOut);
 }
@@ -471,6 +527,7 @@ int learned_fcg(Ctx& c, double shift, double rtol, int maxit, double delta, int*
     double rel = bb > 0. ? rr / bb : 0.;
     int its = 0;
     FcgCtl lc{delta * delta, maxit, c.singularP ? (double)D.N : 0.};
+    const ColMap cm = colMap(D, g);
     int lastRound = 0;
     while (rel > rtol && its < maxit && *refs < 40) {
         // the inner solve gains `delta`, or only what the last round still needs
@@ -499,10 +556,10 @@ int learned_fcg(Ctx& c, double shift, double rtol, int maxit, double delta, int*
                                    z, (const float*)q0, (const float*)q1, partA, (const double*)st);
                 hipEvent_t e0, e1;
                 if (c.probe.slot(PROBE_FCG_K1, e0, e1))   // dispatch-bracketing events: the kernel's own time
-                    hipExtLaunchKernelGGL(k_f_k1, dim3(g), dim3(FB), 0, s, e0, e1, 0, D, S, (const float*)z,
+                    hipExtLaunchKernelGGL(k_f_k1, dim3(g), dim3(FB), 0, s, e0, e1, 0, D, S, cm, (const float*)z,
                                           p0, p1, q0, q1, (const double*)partA, g, partB, st, lc);
                 else
-                    hipLaunchKernelGGL(k_f_k1, dim3(g), dim3(FB), 0, s, D, S, z, p0, p1, q0, q1, partA, g, partB, st, lc);
+                    hipLaunchKernelGGL(k_f_k1, dim3(g), dim3(FB), 0, s, D, S, cm, z, p0, p1, q0, q1, partA, g, partB, st, lc);
                 hipLaunchKernelGGL(k_f_k2pre, dim3(g), dim3(FB), 0, s, D, S, e, (const float*)r, (const float*)r1, w,
                                    (const float*)p0, (const float*)p1, (const float*)q0, (const float*)q1,
                                    (const double*)partB, g, partC, st);

@@ -176,6 +176,10 @@ class Solver:
                     "caffe_kernel_probe_read")
         return ms.value, n.value, by.value
 
+    def device_bytes(self) -> int:
+        """Device memory held by the context (an attached network holds its own)."""
+        return int(self.lib.caffe_device_bytes(self._h))
+
     def sync(self):
         self._check(self.lib.caffe_sync(self._h), "caffe_sync")
 

@@ -163,8 +163,9 @@ int caffe_time_kernel(caffe_ctx* ctx, const char* which, int reps, double* ms, d
 /* Live timing of one kernel inside the solves that follow (HIP event pairs around each of
  * its launches on the context's stream, up to 4096): which = "fcg_k1" (stencil kernel of the
  * learned flexible CG), "pcorr_cg_k1_fp32" / "pcorr_cg_k1_fp64" (stencil kernel of the
- * pressure CG), "coupled_bicg_a" (first BiCGStab kernel of the Coupled solve); enable 0 stops
- * recording.  _read returns the average duration of the
+ * pressure CG), "coupled_bicg_a" (first BiCGStab kernel of the Coupled solve),
+ * "pcorr_bicg_a_fp32" / "pcorr_bicg_a_fp64" (first kernel of the p' BiCGStab used for
+ * ghost-row and indefinite systems); enable 0 stops recording.  _read returns the average duration of the
  * launches that ran (launches queued past convergence return at once and are left out),
  * their number, and the kernel's algorithmic HBM bytes per launch.  No counterpart in the
  * reference. */
@@ -172,6 +173,9 @@ int caffe_kernel_probe(caffe_ctx* ctx, const char* which, int enable);
 int caffe_kernel_probe_read(caffe_ctx* ctx, double* avg_ms, long* launches, double* bytes);
 int caffe_sync(caffe_ctx* ctx);
 long caffe_padded_size(caffe_ctx* ctx);
+/* Device memory held by the context (fields, metrics, systems, Krylov work), in bytes;
+ * a learned network attached to it holds its own.  No counterpart in the reference. */
+long caffe_device_bytes(caffe_ctx* ctx);
 
 /* ---------------------------------------------------------------------------------
  * Learned multilevel preconditioner of the pressure-correction solve.  No counterpart in
