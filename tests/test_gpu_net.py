@@ -55,7 +55,9 @@ def test_params_roundtrip(net):
     assert np.array_equal(net.get_params(), th)
 
 
-@pytest.mark.parametrize("shape", [(16, 16, 16), (8, 16, 24)])
+# [K][J][I]: cubic, small anisotropic, and the coarse grids of the 128x32x32 and
+# 256x64x64 channels (I >= 32 takes the 16-channel row kernels at level 0 and 1)
+@pytest.mark.parametrize("shape", [(16, 16, 16), (8, 16, 24), (16, 16, 64), (32, 32, 128)])
 def test_forward_matches_oracle(net, shape):
     th = _theta(3)
     net.set_params(th)
