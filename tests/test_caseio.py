@@ -312,3 +312,26 @@ def test_diffusion_deck():
     # refVector after refValue (FlowBoundaryConditions.cpp:27-29)
     with pytest.raises(io.InfoError, match=r"Boundaries\.east\.refVector"):
         io.case_from_info(p, ics, io.generate_mesh(v, (9, 5, 2)), "simple")
+
+
+def test_coupled_deck_reads_only_its_keys():
+    # The Coupled solver reads the flow boundaries, numberOfInnerIterations when
+    # unsteady, relaxationFactorMomentum, rho and mu (Coupled.cpp:18-33); it reads no
+    # relaxationFactorPCorr and no initial conditions
+    d = _case("Cavity2D")
+    p = io.read_info(os.path.join(d, "case", "case.info"))
+    ics = io.read_info(os.path.join(d, "case", "initialConditions.info"))
+    name, v, res = io.read_mesh_info(os.path.join(d, "mesh", "structuredMesh.info"))
+    nodes = io.generate_mesh(v, (9, 9, 2))
+    c = io.case_from_info(p, ics, nodes, "coupled")
+    assert c.algo == "coupled" and c.time_accurate
+    assert c.n_inner == p.get("Solver.numberOfInnerIterations", int)
+    assert (c.alpha_u, c.rho, c.mu) == (0.5, 1.0, 0.001)
+    assert c.u_init == (0.0, 0.0, 0.0) and c.p_init == 0.0
+    assert [b.type for b in c.boundaries] == ["wall"] * 4 + ["empty"] * 2
+    # a deck without relaxationFactorPCorr still loads for the Coupled solver, not for SIMPLE
+    text = open(os.path.join(d, "case", "case.info")).read().replace("relaxationFactorPCorr 0.5", "")
+    p2 = io.parse_info(text)
+    io.case_from_info(p2, ics, nodes, "coupled")
+    with pytest.raises(io.InfoError, match="relaxationFactorPCorr"):
+        io.case_from_info(p2, ics, nodes, "simple")
