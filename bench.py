@@ -126,10 +126,10 @@ def workload_bench(a):
       skewed      256^3 cavity whose interior nodes move by up to 0.15 h (seeded Philox),
                   Re 1000, PISO 2 correctors, dt 0.5/256.
     One step is Piso::solve / IbPiso::solve.  The p' solves of the timed steps use --mode
-    (auto: learned where the network applies, else FP32 IR; the ghost-row systems of the IB
-    channel go to FP64 BiCGStab in any mode).  After the timed steps the same p' system is
-    solved by each method to a true FP64 relres of 1e-8.  The roofline names the p' Krylov
-    stencil kernel of the timed steps; `device_bytes` is the context's HBM footprint."""
+    (auto: learned where the network applies, else the multigrid BiCGStab on the IB channel's
+    ghost-row systems and FP32 IR elsewhere).  After the timed steps the same p' system is
+    solved by each method to a true FP64 relres of 1e-8.  The roofline names the dominant p'
+    kernel of the timed steps; `device_bytes` is the context's HBM footprint."""
     from caffe_old_amd import _lib as L
     from caffe_old_amd import case as cs
     from caffe_old_amd.learned import PressureNet, trained_weights_path
@@ -151,15 +151,15 @@ def workload_bench(a):
         # the network trained on 128^3 cavities, applied on the 128^3 coarse grid of this mesh
         net = PressureNet(seed=0).load(wpath)
         s.attach_net(net)
-    mode_name = a.mode if a.mode != "auto" else ("ir" if ib or net is None else "learned")
-    modes = {"ir": L.SOLVE_IR_FP32, "plain": L.SOLVE_PLAIN_FP64, "learned": L.SOLVE_LEARNED_FCG}
-    inner = {"ir": 1e-2, "plain": 1e-2, "learned": 1e-4}
+    mode_name = a.mode if a.mode != "auto" else ("mg" if ib else ("ir" if net is None else "learned"))
+    modes = {"ir": L.SOLVE_IR_FP32, "plain": L.SOLVE_PLAIN_FP64, "learned": L.SOLVE_LEARNED_FCG, "mg": L.SOLVE_MG}
+    inner = {"ir": 1e-2, "plain": 1e-2, "learned": 1e-4, "mg": 1e-2}
     s.set_solve_opts("momentum", mode=L.SOLVE_IR_FP32 if not ib else L.SOLVE_PLAIN_FP64, rtol=1e-8, maxit=5000)
     s.set_solve_opts("pcorr", mode=modes[mode_name], rtol=1e-8, maxit=200000, inner_rtol=inner[mode_name])
     if ib:
-        kname = "pcorr_bicg_a_fp64"
+        kname = "pcorr_bicg_a_fp64" if mode_name == "plain" else "mg_post0"
     else:
-        kname = {"learned": "fcg_k1", "ir": "pcorr_cg_k1_fp32", "plain": "pcorr_cg_k1_fp64"}[mode_name]
+        kname = {"learned": "fcg_k1", "ir": "pcorr_cg_k1_fp32", "plain": "pcorr_cg_k1_fp64", "mg": "mg_post0"}[mode_name]
     for w in range(a.warmup):
         s.solve()
     s.kernel_probe(kname, True)
@@ -177,7 +177,7 @@ def workload_bench(a):
     if a.calibrate_copy:
         s.time_kernel("copy_fp64", reps=5)
     psolve = {}
-    methods = [("plain_fp64", L.SOLVE_PLAIN_FP64, 1e-2), ("ir_fp32", L.SOLVE_IR_FP32, 1e-2)]
+    methods = [("plain_fp64", L.SOLVE_PLAIN_FP64, 1e-2), ("ir_fp32", L.SOLVE_IR_FP32, 1e-2), ("mg", L.SOLVE_MG, 1e-2)]
     if net is not None:
         methods.append(("learned_fcg", L.SOLVE_LEARNED_FCG, 1e-4))
     for name, md, inr in methods:
@@ -191,7 +191,7 @@ def workload_bench(a):
         "metric": f"{'IbPiso' if ib else 'PISO'} steps/s, {a.workload}", "value": round(1000.0 / ms_per_step, 4),
         "unit": "steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_per_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": {"plain": "f64", "ir": "f64+f32", "learned": "f64+f32+bf16"}[mode_name],
+        "dtype": {"plain": "f64", "ir": "f64+f32", "learned": "f64+f32+bf16", "mg": "f64+f32"}[mode_name],
         "data": "synthetic (no external data)",
         "config": {"workload": desc + f", p' solve to true relres 1e-8 ({mode_name})", "cells": case.n_cells,
                    "device_bytes": s.device_bytes(), "steps_detail": its, "p_solve_ms": psolve},
@@ -208,8 +208,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--mode", choices=["auto", "ir", "plain", "learned"], default="auto",
-                    help="p' solve: learned FCG (auto when trained weights exist for the grid), FP32 IR, plain FP64")
+    ap.add_argument("--mode", choices=["auto", "ir", "plain", "learned", "mg"], default="auto",
+                    help="p' solve: learned FCG (auto when trained weights exist for the grid), FP32 IR, plain FP64, "
+                         "multigrid BiCGStab")
     ap.add_argument("--mom-mode", choices=["ir", "plain"], default="ir",
                     help="momentum solves: FP32 BiCGStab inside FP64 refinement, or plain FP64")
     ap.add_argument("--no-cpu", action="store_true")
@@ -242,8 +243,8 @@ def main():
     wpath = trained_weights_path(a.n)
     if a.mode == "auto":
         a.mode = "learned" if os.path.exists(wpath) else "ir"
-    mode = {"ir": L.SOLVE_IR_FP32, "plain": L.SOLVE_PLAIN_FP64, "learned": L.SOLVE_LEARNED_FCG}[a.mode]
-    inner = {"ir": 1e-2, "plain": 1e-2, "learned": 1e-4}
+    mode = {"ir": L.SOLVE_IR_FP32, "plain": L.SOLVE_PLAIN_FP64, "learned": L.SOLVE_LEARNED_FCG, "mg": L.SOLVE_MG}[a.mode]
+    inner = {"ir": 1e-2, "plain": 1e-2, "learned": 1e-4, "mg": 1e-2}
     case = cs.cavity_piso_128(a.n, steps=a.steps + a.warmup)
     s = make_solver(case)
     mom_mode = {"ir": L.SOLVE_IR_FP32, "plain": L.SOLVE_PLAIN_FP64}[a.mom_mode]
@@ -255,7 +256,7 @@ def main():
         net = PressureNet(seed=0).load(wpath)
         s.attach_net(net)
     # the dominant kernel of the timed steps: the pressure-correction stencil kernel
-    kname = {"learned": "fcg_k1", "ir": "pcorr_cg_k1_fp32", "plain": "pcorr_cg_k1_fp64"}[a.mode]
+    kname = {"learned": "fcg_k1", "ir": "pcorr_cg_k1_fp32", "plain": "pcorr_cg_k1_fp64", "mg": "mg_post0"}[a.mode]
 
     def log(*x):
         if not a.quiet:
@@ -290,10 +291,11 @@ def main():
     rep = s.step_report()
     log(f"timed {a.steps} steps: {ms_per_step:.2f} ms/step, last p' iters {p['iters']}, courant {rep['courant']:.3f}")
 
-    # pressure-correction solve: plain FP64 vs FP32-inner / FP64-refinement, same system
+    # pressure-correction solve on the same system: plain FP64, FP32-inner / FP64-refinement,
+    # multigrid BiCGStab, learned FCG
     psolve = {}
     for name, md, inr in (("plain_fp64", L.SOLVE_PLAIN_FP64, 1e-2), ("ir_fp32", L.SOLVE_IR_FP32, 1e-2),
-                          ("learned_fcg", L.SOLVE_LEARNED_FCG, 1e-4)):
+                          ("mg", L.SOLVE_MG, 1e-2), ("learned_fcg", L.SOLVE_LEARNED_FCG, 1e-4)):
         s.pcorr_assemble()
         r = s.pcorr_solve(mode=md, rtol=1e-8, maxit=100000, inner_rtol=inr)
         psolve[name] = {"ms": round(r["ms"], 3), "iters": r["iters"], "refinements": r["refinements"],
@@ -331,7 +333,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"plain": "f64", "ir": "f64+f32", "learned": "f64+f32+bf16"}[a.mode],
+        "dtype": {"plain": "f64", "ir": "f64+f32", "learned": "f64+f32+bf16", "mg": "f64+f32"}[a.mode],
         "data": "synthetic: lid-driven cavity from rest (no external data)",
         "config": {"workload": f"3D lid-driven cavity {a.n}^3 uniform hex, Re 1000 (mu 1e-3, lid 1), PISO 2 correctors, "
                                f"dt {case.time_step:g} (CFL 0.5), p' solve to true relres 1e-8 ({a.mode})",

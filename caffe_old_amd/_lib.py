@@ -20,6 +20,7 @@ CAFFE_ERR_NOT_CONVERGED = -4
 SOLVE_PLAIN_FP64 = 0
 SOLVE_IR_FP32 = 1
 SOLVE_LEARNED_FCG = 2
+SOLVE_MG = 3
 
 # every symbol include/caffe_hip.h declares
 EXPORTS = [

@@ -51,6 +51,7 @@ void freeAll(Ctx& c) {
     c.deviceBytes = 0;
     c.wb32 = nullptr; c.Am32 = nullptr; c.wm32 = nullptr;
     c.ov = Overlay();
+    c.mg = MgHier();
     c.ready = false;
 }
 
@@ -452,6 +453,7 @@ int caffe_case_setup(caffe_ctx* x, const caffe_case* cs, const double* nodes) {
 int caffe_set_solve_opts(caffe_ctx* x, int which, const caffe_solve_opts* o) {
     return guard(x, [&](Ctx& c) {
         if (!o || which < 0 || which > 1 || !(o->rtol > 0.) || o->maxit < 1) { c.err = "invalid solve options"; return CAFFE_ERR_ARG; }
+        if (o->mode < CAFFE_SOLVE_PLAIN_FP64 || o->mode > CAFFE_SOLVE_MG) { c.err = "unknown solve mode"; return CAFFE_ERR_ARG; }
         if (which == 0 && o->mode != CAFFE_SOLVE_PLAIN_FP64 && o->mode != CAFFE_SOLVE_IR_FP32) {
             c.err = "momentum solves are FP64 or FP32 inside FP64 refinement";
             return CAFFE_ERR_ARG;
@@ -477,7 +479,11 @@ int caffe_momentum_solve(caffe_ctx* x, caffe_solve_report* rep) {
 int caffe_rhiechow(caffe_ctx* x) { return guard(x, [&](Ctx& c) { READY(c); launch_rhie_chow(c); return CAFFE_OK; }); }
 int caffe_pcorr_assemble(caffe_ctx* x) { return guard(x, [&](Ctx& c) { READY(c); launch_pc_assemble(c); return CAFFE_OK; }); }
 int caffe_pcorr_solve(caffe_ctx* x, const caffe_solve_opts* o, caffe_solve_report* rep) {
-    return guard(x, [&](Ctx& c) { READY(c); return pcorrSolve(c, o ? *o : c.pcOpts, rep); });
+    return guard(x, [&](Ctx& c) {
+        READY(c);
+        if (o && (o->mode < CAFFE_SOLVE_PLAIN_FP64 || o->mode > CAFFE_SOLVE_MG)) { c.err = "unknown solve mode"; return CAFFE_ERR_ARG; }
+        return pcorrSolve(c, o ? *o : c.pcOpts, rep);
+    });
 }
 int caffe_correct(caffe_ctx* x) { return guard(x, [&](Ctx& c) { READY(c); return correctStep(c); }); }
 int caffe_ib_field(caffe_ctx* x) {
@@ -654,6 +660,7 @@ int caffe_kernel_probe(caffe_ctx* x, const char* which, int enable) {
         else if (w == "coupled_bicg_a") kind = PROBE_CP_A;
         else if (w == "pcorr_bicg_a_fp32") kind = PROBE_PBI_A_FP32;
         else if (w == "pcorr_bicg_a_fp64") kind = PROBE_PBI_A_FP64;
+        else if (w == "mg_post0") kind = PROBE_MG_POST0;
         else { c.err = "unknown probe kernel " + w; return CAFFE_ERR_ARG; }
         if (kind == PROBE_CP_A && !c.cA) { c.err = "not a Coupled case"; return CAFFE_ERR_STATE; }
         HIPCHECK(hipStreamSynchronize(c.stream));
@@ -697,12 +704,16 @@ int caffe_kernel_probe_read(caffe_ctx* x, double* avg_ms, long* launches, double
         // and D^-1 of the four variables and writes p and v: 84 doubles per cell.  The p'
         // BiCGStab KA reads the four matrix arrays, r, rh, p_old, v_old and D^-1 and writes
         // p and v: 11 values per cell (ghost rows, about 0.1% of the cells, not counted).
+        // The multigrid post-smoothing sweep on the fine level reads the four FP32 matrix
+        // arrays, the FP64 right-hand side and the FP32 iterate and writes the new iterate:
+        // 32 B per cell.
         double per = 32.;
         switch (c.probe.which) {
         case PROBE_CP_A: per = 672.; break;
         case PROBE_CG_K1_FP64: per = 64.; break;
         case PROBE_PBI_A_FP32: per = 44.; break;
         case PROBE_PBI_A_FP64: per = 88.; break;
+        case PROBE_MG_POST0: per = 32.; break;
         default: break;
         }
         if (bytes) *bytes = per * (double)c.D.N;

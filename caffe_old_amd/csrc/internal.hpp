@@ -21,7 +21,7 @@ enum {
 enum { FL_MDONE = 0, FL_PDONE = 1, FL_MITER = 2, FL_PITER = 3, N_FLAGS = 8 };
 
 enum { PROBE_NONE = 0, PROBE_FCG_K1 = 1, PROBE_CG_K1_FP32 = 2, PROBE_CG_K1_FP64 = 3, PROBE_CP_A = 4,
-       PROBE_PBI_A_FP32 = 5, PROBE_PBI_A_FP64 = 6 };
+       PROBE_PBI_A_FP32 = 5, PROBE_PBI_A_FP64 = 6, PROBE_MG_POST0 = 7 };
 
 // Coupled system (Coupled.cpp:54-228): coefficient planes of P doubles per cell, each group
 // of seven in the neighbour order own, E, W, N, S, T, B
@@ -42,6 +42,30 @@ struct Overlay {          // ghost-cell rows (IbPiso.cpp:148-185), 9 entries eac
     double* valP = nullptr;   // 9 * n pressure-row values
     float* valP32 = nullptr;
     int* rowOf = nullptr;     // P: overlay row of a padded cell or -1
+};
+
+// Multigrid hierarchy of the p' system (mg.hip): level 0 is the context's own grid and
+// storage; each coarser level holds its operator (aP and the three low-face coefficients,
+// FP32), right-hand side and two iterates.  f is the aggregation factor per direction
+// from the finer level.
+constexpr int MG_MAX_LEVELS = 16;
+struct MgLevel {
+    Dims D{};
+    int f[3] = {1, 1, 1};
+    float* A = nullptr;
+    float* b = nullptr;
+    float* z = nullptr;
+    float* t = nullptr;
+    float* w = nullptr;   // damped Jacobi weights 0.8 / aP
+};
+struct MgHier {
+    int nl = 0;                      // levels, 0 until planned for the grid
+    MgLevel L[MG_MAX_LEVELS];
+    double* cinv = nullptr;          // dense inverse of the coarsest operator
+    float* z0[3] = {nullptr, nullptr, nullptr};   // level 0: the two preconditioned vectors, a ping-pong
+    float* w0 = nullptr;             // level 0 Jacobi weights
+    float* g1 = nullptr;             // first-sweep values of the ghost cells, one per ghost row
+    int g1n = 0;
 };
 
 struct Ctx {
@@ -131,6 +155,8 @@ struct Ctx {
     double* cx = nullptr;
     double* cw = nullptr;
 
+    MgHier mg;
+
     // learned preconditioner (optional)
     void* net = nullptr;
 };
@@ -173,6 +199,11 @@ int pcorr_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep);
 // relative residual rtol; p is returned with zero mean when no face fixes it
 int coupled_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep);
 void pcorr_make_fp32(Ctx& c);
+// ---- multigrid (mg.hip): BiCGStab on the p' system with the FP32 V-cycle as right
+// preconditioner, FP64 recurrences restarted from the true residual; continues from c.xp.
+// Returns the iteration count, or -1 when the system does not qualify (64 cells or fewer,
+// or an indefinite system)
+int mg_bicgstab(Ctx& c, double shift, double rtol, int maxit, int* refs, double* relOut);
 void add_vec(Ctx& c, double* x, const double* d, int n);
 double time_cg_k1(Ctx& c, bool fp32, int reps);
 

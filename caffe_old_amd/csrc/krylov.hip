@@ -26,23 +26,15 @@
 #include <hip/hip_ext.h>
 
 #include "internal.hpp"
+#include "krylov.hpp"
+#include "sym4.hpp"
 
 #include <cmath>
 #include <type_traits>
 
 namespace caffe {
 
-constexpr int KBS = 1024;
-constexpr int KGRID = 256;   // one block of 16 waves per CU: the prologue that adds the partial rows
-                             // is paid once per CU, and the rows are few
-
 // ------------------------------------------------------------------ operators
-struct OvView {
-    const int* rowOf;      // nullptr when there are no ghost rows
-    const int* cols;
-    const void* vals;      // double or float, 9 per row
-};
-
 template <class T>
 struct BandT {             // momentum matrix; T = float only without ghost rows
     const T* A; int P, sJ, sK; OvView ov; const int8_t* st;
@@ -64,40 +56,6 @@ struct BandT {             // momentum matrix; T = float only without ghost rows
     __device__ double diag(int p) const { return (double)A[p]; }
 };
 using Band7 = BandT<double>;
-
-template <class T>
-struct Sym4 {              // pressure-correction matrix
-    const T* A; int P, sJ, sK; OvView ov; const int8_t* st;
-    template <class F>
-    __device__ double apply(F x, int p) const {
-        if (ov.rowOf) {
-            if (st[p] == ST_INACTIVE) return x(p);
-            int r = ov.rowOf[p];
-            if (r >= 0) {
-                const T* v = (const T*)ov.vals + 9 * r;
-                double s = (double)v[0] * x(p);
-                for (int m = 1; m < 9; ++m) { int c = ov.cols[9 * r + m]; if (c >= 0) s += (double)v[m] * x(c); }
-                return s;
-            }
-        }
-        return (double)A[p] * x(p) + (double)A[P + p] * x(p - 1) + (double)A[P + p + 1] * x(p + 1)
-             + (double)A[2 * P + p] * x(p - sJ) + (double)A[2 * P + p + sJ] * x(p + sJ)
-             + (double)A[3 * P + p] * x(p - sK) + (double)A[3 * P + p + sK] * x(p + sK);
-    }
-    __device__ double diag(int p) const { return (double)A[p]; }
-};
-
-// interior cells in one contiguous chunk per block (XCD-aware chunk order): q = interior
-// id, p = padded index
-__device__ inline int cellIdx(const Dims& D, int q) { int i, j, k; return D.cell(q, i, j, k); }
-#define FOR_CELLS(D, q, p)                                                                 \
-    for (int q##c_ = ((D).N + (int)gridDim.x - 1) / (int)gridDim.x, q = xcdChunk() * q##c_ + (int)threadIdx.x, \
-             q##e_ = min((D).N, q - (int)threadIdx.x + q##c_), p = 0;                      \
-         q < q##e_ && ((p = cellIdx((D), q)), true); q += blockDim.x)
-
-#define BLOCK0 (blockIdx.x == 0 && threadIdx.x == 0)
-
-struct LoopCtl { double tol2; int maxit; double nLive; };
 
 // ================================================================== CG
 // state slots (doubles)
@@ -201,8 +159,6 @@ __global__ void k_cg_k2(Dims D, Sym4<T> A, T* x, T* r, T* z, const T* pb0, const
 }
 
 // ================================================================== BiCGStab, NV systems
-enum { B_RHOCUR = 0, B_RHOPREV, B_ALPHA, B_OMEGA, B_BB, B_RR, B_IT, B_DONE, B_BRK, B_STRIDE = 16 };
-
 template <int NV, class T, class Op>
 __global__ void k_bi_init(Dims D, Op A, const double* b, double shift, T* r, T* rh, T* p0, T* v0, T* x, T* dinv,
                           double* part, double* st) {
@@ -225,66 +181,6 @@ __global__ void k_bi_init(Dims D, Op A, const double* b, double shift, T* r, T* 
         for (int s = 0; s < NV; ++s) {
             double* c = st + B_STRIDE * s;
             c[B_IT] = 0.; c[B_DONE] = 0.; c[B_BB] = -1.; c[B_RHOPREV] = 1.; c[B_ALPHA] = 1.; c[B_OMEGA] = 1.; c[B_BRK] = 0.;
-        }
-    }
-}
-
-// The state of the NV systems, loaded by every block in one batch at kernel start, ahead
-// of the partial sums of the previous kernel
-template <int NV>
-struct BiState {
-    double bb[NV], done[NV], it[NV], brk[NV], om[NV], rhoPrev[NV], al[NV], rho[NV];
-    __device__ __forceinline__ void load(const double* st) {
-        for (int s = 0; s < NV; ++s) {
-            const double* c = st + B_STRIDE * s;
-            bb[s] = c[B_BB]; done[s] = c[B_DONE]; it[s] = c[B_IT]; brk[s] = c[B_BRK];
-            om[s] = c[B_OMEGA]; rhoPrev[s] = c[B_RHOPREV]; al[s] = c[B_ALPHA]; rho[s] = c[B_RHOCUR];
-        }
-    }
-    __device__ __forceinline__ bool any() const {
-        bool a = false;
-        for (int s = 0; s < NV; ++s) a |= done[s] == 0.;
-        return a;
-    }
-};
-
-// KA head, every block: the convergence test on the partials (rh, r), (r, r) of KC,
-// beta and the omega of the previous step; block 0 records them
-template <int NV>
-__device__ __forceinline__ void biaHead(const double* partIn, int nb, double* st, const LoopCtl& lc,
-                                        const BiState<NV>& S, double* beta, double* om, bool* live) {
-    double t[2 * NV];
-    readPartials<KBS, 2 * NV>(partIn, nb, t);
-    for (int s = 0; s < NV; ++s) {
-        const double rho = t[2 * s], rr = t[2 * s + 1];
-        const double bb = S.bb[s] < 0. ? rr : S.bb[s];
-        double done = S.done[s];
-        if (done == 0.) {
-            if (rr <= lc.tol2 * bb || S.it[s] >= lc.maxit || rr == 0.) done = 1.;
-            else if (S.brk[s] != 0. || rho == 0.) done = 2.;
-        }
-        live[s] = done == 0.;
-        om[s] = S.om[s];
-        beta[s] = S.it[s] == 0. ? 0. : (rho / S.rhoPrev[s]) * (S.al[s] / S.om[s]);
-        if (BLOCK0) {
-            double* w = st + B_STRIDE * s;
-            w[B_BB] = bb; w[B_RR] = rr; w[B_RHOCUR] = rho; w[B_DONE] = done;
-        }
-    }
-}
-// KB head: alpha = rho / (rh, v)
-template <int NV>
-__device__ __forceinline__ void bibHead(const double* partIn, int nb, double* st, const BiState<NV>& S, double* al,
-                                        bool* live) {
-    double t[NV];
-    readPartials<KBS, NV>(partIn, nb, t);
-    for (int s = 0; s < NV; ++s) {
-        live[s] = S.done[s] == 0. && t[s] != 0.;
-        al[s] = live[s] ? S.rho[s] / t[s] : 0.;
-        if (BLOCK0) {
-            double* w = st + B_STRIDE * s;
-            w[B_ALPHA] = al[s];
-            if (S.done[s] == 0. && t[s] == 0.) w[B_BRK] = 1.;
         }
     }
 }
@@ -1264,14 +1160,20 @@ int pcorr_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep) {
         }
         if (!learnedDone) {
             const double delta = o.inner_rtol > 0. ? o.inner_rtol : 1e-2;
-            if (sym) {
+            if (sym && o.mode != CAFFE_SOLVE_MG) {
                 iters += cgSolve<float>(c, shift, o.rtol, o.maxit, delta, &refs, &rel);
             } else {
-                // ghost-row systems: FP32 BiCGStab does not reach 1e-8 on them (measured on
-                // the 256x64x64 channel: every inner tolerance from 1e-2 to 1e-5 fell back),
-                // so they go to the FP64 solve directly and the report says so
-                fell = 1;
-                iters += bicgPressure<double>(c, shift, o.rtol, o.maxit, 0., &refs, &rel);
+                // the multigrid mode, and ghost-row systems in the FP32 modes: FP32 Jacobi
+                // BiCGStab does not reach 1e-8 on those (on the 256x64x64 channel every inner
+                // tolerance from 1e-2 to 1e-5 fell back), the FP64 recurrence with FP32
+                // V-cycles does.  Systems the cycle does not take go to the FP64 solve.
+                const int it = mg_bicgstab(c, shift, o.rtol, o.maxit, &refs, &rel);
+                if (it >= 0) {
+                    iters += it;
+                } else {
+                    fell = 1;
+                    iters += bicgPressure<double>(c, shift, o.rtol, o.maxit, 0., &refs, &rel);
+                }
             }
         }
         if (!(rel <= o.rtol)) {  // the plain FP64 solve stays as fallback and judge, from x = 0 (also for NaN)

@@ -1,0 +1,764 @@
+// mg.hip -- geometric multigrid preconditioner for the pressure-correction systems, and
+// the BiCGStab that uses it.
+//
+// The reference hands each p' system (assembled in Simple.cpp:231-269; with the ghost-cell
+// rows of IbPiso.cpp:148-185 it is nonsymmetric) to a preconditioned PETSc Krylov solve.
+// With a pointwise preconditioner the iteration count grows with the square of the
+// longest cell path between fixed-pressure faces: a 256-cell channel needs thousands.
+// Here BiCGStab keeps its recurrences in FP64 and is preconditioned on the right by one
+// V-cycle in FP32:
+//   - level 0 is the system itself, ghost rows included; damped Jacobi (weight 0.8),
+//     two sweeps before and two after the coarse correction, each sweep ending with the
+//     ghost rows solved exactly for their cells;
+//   - coarser levels aggregate 2 x 2 x 2 cells (1 along a direction of 2 cells or fewer).
+//     A coarse face coefficient is the sum of the fine face coefficients across it divided
+//     by the aggregation factor along its normal (4x the area at 2x the distance), so the
+//     coarse operator is the finite-volume Laplacian of the coarse cells; the fixed-
+//     pressure part of a diagonal carries over the same way.  Ghost cells join the coarse
+//     graph through their faces to fluid cells; inactive cells drop out, and a coarse cell
+//     made of inactive cells only is an identity row;
+//   - restriction sums the fine residuals of an aggregate, prolongation is cell-centred
+//     trilinear (3/4 and 1/4 per direction);
+//   - the coarsest level (64 cells or fewer) is solved with its dense inverse, formed once
+//     per solve by one workgroup.
+// Fixed sweep counts and the exact coarse solve make the cycle a fixed linear map, which
+// BiCGStab requires of its preconditioner.
+#include <hip/hip_ext.h>
+
+#include "internal.hpp"
+#include "krylov.hpp"
+#include "sym4.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace caffe {
+
+constexpr int MBS = 256;          // block of the per-level kernels (one thread per cell)
+constexpr double MG_W = 0.8;      // Jacobi damping
+constexpr int MG_COARSEST = 64;   // cells of the level solved densely
+
+__host__ __device__ inline Sym4<float> levOp(const MgLevel& L) {
+    return Sym4<float>{L.A, L.D.P, L.D.sJ, L.D.sK, OvView{nullptr, nullptr, nullptr}, nullptr};
+}
+
+// Damped Jacobi weights w = 0.8 / aP of a level's storage, formed once per solve (zero on
+// the halo, whose diagonal is zero).  The sweeps then run in FP32 without divisions.
+__global__ void k_mg_weights(const float* A, int P, float* w) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < P) {
+        const float a = A[q];
+        w[q] = a != 0.f ? (float)(MG_W / (double)a) : 0.f;
+    }
+}
+
+// the plain stencil of the p' storage at p on the values x at p, W, E, S, N, B, T
+__device__ __forceinline__ float stencil7(const float* A, int P, int sJ, int sK, int p, const float (&x)[7]) {
+    return A[p] * x[0] + A[P + p] * x[1] + A[P + p + 1] * x[2] + A[2 * P + p] * x[3] + A[2 * P + p + sJ] * x[4]
+         + A[3 * P + p] * x[5] + A[3 * P + p + sK] * x[6];
+}
+__device__ __forceinline__ void nbrs(int p, int sJ, int sK, int (&nb)[7]) {
+    nb[0] = p; nb[1] = p - 1; nb[2] = p + 1; nb[3] = p - sJ; nb[4] = p + sJ; nb[5] = p - sK; nb[6] = p + sK;
+}
+
+// two damped Jacobi sweeps from zero, at cell p: z1 = w b, z = z1 + w (b - A z1)
+template <class B>
+__device__ __forceinline__ float preSmooth(const Sym4<float>& A, const float* wt, B b, int p) {
+    int nb[7];
+    nbrs(p, A.sJ, A.sK, nb);
+    float z1[7];
+#pragma unroll
+    for (int m = 0; m < 7; ++m) z1[m] = wt[nb[m]] * (float)b(nb[m]);
+    return z1[0] + wt[p] * ((float)b(p) - stencil7(A.A, A.P, A.sJ, A.sK, p, z1));
+}
+
+// Ghost rows on the fine level.  A ghost row ties its cell to the interpolated value at the
+// image point (IbPiso.cpp:148-185); damped Jacobi smooths such rows badly (on the
+// 64 x 16 x 16 channel the cycle then needs 40 BiCGStab iterations, 7 when they are solved
+// exactly).  So every fine-level sweep ends with each ghost row solved exactly for its own
+// cell, from the values this sweep gave the other cells of the row (their input values
+// where those are ghost cells too).  The sweep kernels run the plain stencil on every
+// cell, and one thread per ghost row then writes the ghost cells (k_mg_ghost_*): it reads
+// fluid cells only where it writes, so the result does not depend on thread order.
+template <class T>
+__device__ __forceinline__ int ghostRow(const Sym4<T>& A, int c) { return A.ov.rowOf ? A.ov.rowOf[c] : -1; }
+template <class T, class B, class Z>
+__device__ __forceinline__ double ghostSolve(const Sym4<T>& A, int r, B b, int p, Z z) {
+    const T* v = (const T*)A.ov.vals + 9 * r;
+    int cols[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) cols[m] = A.ov.cols[9 * r + 1 + m];
+    double s = b(p);
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+        if (cols[m] >= 0) s -= (double)v[1 + m] * z(cols[m]);
+    return s / (double)v[0];
+}
+// preSmooth on the fine level.  A fluid cell's ghost neighbours take their first-sweep
+// value, the ghost row solved from w b; a ghost cell gets that value here (also into g1 at
+// its row, for k_mg_ghost_pre) and its second sweep from k_mg_ghost_pre.  The seven
+// neighbours' values, weights and ghost-row numbers (G: the system has ghost rows) are
+// loaded together; only a cell next to a ghost cell takes the ghost branch.  An inactive
+// cell's stored row is the identity with zero faces, so the plain stencil is its row.
+template <bool G, class B>
+__device__ __forceinline__ float preSmooth0(const Sym4<float>& A, const float* __restrict__ wt, B b,
+                                            float* __restrict__ g1, int p) {
+    int nb[7];
+    nbrs(p, A.sJ, A.sK, nb);
+    float z1[7];
+    int gr[7];
+#pragma unroll
+    for (int m = 0; m < 7; ++m) {
+        z1[m] = wt[nb[m]] * (float)b(nb[m]);
+        gr[m] = G ? A.ov.rowOf[nb[m]] : -1;
+    }
+    if (G) {
+        bool anyG = false;
+#pragma unroll
+        for (int m = 0; m < 7; ++m) anyG |= gr[m] >= 0;
+        if (anyG) {
+            auto zJ = [&](int q) { return (double)(wt[q] * (float)b(q)); };
+            for (int m = 0; m < 7; ++m)
+                if (gr[m] >= 0) z1[m] = (float)ghostSolve(A, gr[m], b, nb[m], zJ);
+            if (gr[0] >= 0) {
+                g1[gr[0]] = z1[0];
+                return z1[0];
+            }
+        }
+    }
+    return z1[0] + wt[p] * ((float)b(p) - stencil7(A.A, A.P, A.sJ, A.sK, p, z1));
+}
+
+// ------------------------------------------------------------------ hierarchy set-up
+// Fine coefficients seen by the aggregation: face(dir, i, j, k) is the stored low-face
+// coefficient of cell (i, j, k) along dir; surplus(i, j, k) the part of -aP that no
+// stored face accounts for (fixed-pressure boundary faces), zero where the row is not a
+// plain fluid row.
+struct FineP {            // level 0: the FP64 p' storage and the cell statuses
+    const double* A; const int8_t* st; Dims D;
+    __device__ double face(int dir, int i, int j, int k) const { return A[(1 + dir) * D.P + D.pidx(i, j, k)]; }
+    __device__ double surplus(int i, int j, int k) const {
+        const int p = D.pidx(i, j, k), P = D.P;
+        if (st[p] != ST_ACTIVE) return 0.;
+        const double s = -(A[p] + A[P + p] + A[P + p + 1] + A[2 * P + p] + A[2 * P + p + D.sJ] + A[3 * P + p]
+                           + A[3 * P + p + D.sK]);
+        return s > 1e-12 * -A[p] ? s : 0.;
+    }
+};
+struct FineL {            // a coarse level as the fine side of the next aggregation
+    const float* A; Dims D;
+    __device__ double face(int dir, int i, int j, int k) const { return A[(1 + dir) * D.P + D.pidx(i, j, k)]; }
+    __device__ double surplus(int i, int j, int k) const {
+        const int p = D.pidx(i, j, k), P = D.P;
+        const double a = A[p];
+        if (a > 0.) return 0.;   // identity row of an inactive aggregate
+        const double s = -(a + (double)A[P + p] + A[P + p + 1] + A[2 * P + p] + A[2 * P + p + D.sJ] + A[3 * P + p]
+                           + A[3 * P + p + D.sK]);
+        return s > 1e-6 * -a ? s : 0.;
+    }
+};
+
+template <class Fine>
+__global__ void k_mg_coarsen(Fine F, MgLevel c) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= c.D.N) return;
+    const Dims& Df = F.D;
+    int I, J, K;
+    const int pc = c.D.cell(e, I, J, K);
+    const int i0 = c.f[0] * I, i1 = min(i0 + c.f[0], Df.nI);
+    const int j0 = c.f[1] * J, j1 = min(j0 + c.f[1], Df.nJ);
+    const int k0 = c.f[2] * K, k1 = min(k0 + c.f[2], Df.nK);
+    double lo[3] = {0., 0., 0.}, hi[3] = {0., 0., 0.}, sur = 0.;
+    for (int k = k0; k < k1; ++k)
+        for (int j = j0; j < j1; ++j) {
+            if (I > 0) lo[0] += F.face(0, i0, j, k);
+            if (I + 1 < c.D.nI) hi[0] += F.face(0, i0 + c.f[0], j, k);
+        }
+    for (int k = k0; k < k1; ++k)
+        for (int i = i0; i < i1; ++i) {
+            if (J > 0) lo[1] += F.face(1, i, j0, k);
+            if (J + 1 < c.D.nJ) hi[1] += F.face(1, i, j0 + c.f[1], k);
+        }
+    for (int j = j0; j < j1; ++j)
+        for (int i = i0; i < i1; ++i) {
+            if (K > 0) lo[2] += F.face(2, i, j, k0);
+            if (K + 1 < c.D.nK) hi[2] += F.face(2, i, j, k0 + c.f[2]);
+        }
+    for (int k = k0; k < k1; ++k)
+        for (int j = j0; j < j1; ++j)
+            for (int i = i0; i < i1; ++i) sur += F.surplus(i, j, k);
+    double aP = 0.;
+    for (int d = 0; d < 3; ++d) {
+        lo[d] /= c.f[d];
+        hi[d] /= c.f[d];
+        aP -= lo[d] + hi[d];
+    }
+    aP -= sur / (double)max(c.f[0], max(c.f[1], c.f[2]));
+    if (aP == 0.) aP = 1.;   // no face and no fixed value: an aggregate of inactive cells
+    const int P = c.D.P;
+    c.A[pc] = (float)aP;
+    c.A[P + pc] = (float)lo[0];
+    c.A[2 * P + pc] = (float)lo[1];
+    c.A[3 * P + pc] = (float)lo[2];
+}
+
+// Dense inverse of the coarsest operator (n <= 64 cells) by in-place Gauss-Jordan without
+// pivoting (the operator is symmetric negative definite, or made so by grounding cell 0
+// when the system has no fixed-pressure face); one workgroup, inverse stored row-major.
+__global__ void k_mg_dense(MgLevel L, int n, int ground, double* inv) {
+    __shared__ double M[MG_COARSEST][MG_COARSEST + 1];
+    __shared__ double col[MG_COARSEST];
+    const Dims& D = L.D;
+    const int P = D.P;
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) M[e / n][e % n] = 0.;
+    __syncthreads();
+    if ((int)threadIdx.x < n) {
+        const int q = threadIdx.x;
+        int i, j, k;
+        const int p = D.cell(q, i, j, k);
+        M[q][q] = L.A[p];
+        if (i > 0) M[q][q - 1] = L.A[P + p];
+        if (i + 1 < D.nI) M[q][q + 1] = L.A[P + p + 1];
+        if (j > 0) M[q][q - D.nI] = L.A[2 * P + p];
+        if (j + 1 < D.nJ) M[q][q + D.nI] = L.A[2 * P + p + D.sJ];
+        if (k > 0) M[q][q - D.nI * D.nJ] = L.A[3 * P + p];
+        if (k + 1 < D.nK) M[q][q + D.nI * D.nJ] = L.A[3 * P + p + D.sK];
+    }
+    __syncthreads();
+    if (ground && threadIdx.x == 0) {
+        const double d = M[0][0];
+        for (int m = 0; m < n; ++m) { M[0][m] = 0.; M[m][0] = 0.; }
+        M[0][0] = d != 0. ? d : -1.;
+    }
+    __syncthreads();
+    for (int k = 0; k < n; ++k) {
+        if (threadIdx.x == 0) {
+            const double piv = 1. / M[k][k];
+            M[k][k] = 1.;
+            for (int m = 0; m < n; ++m) M[k][m] *= piv;
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < n) col[threadIdx.x] = M[threadIdx.x][k];
+        __syncthreads();
+        for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+            const int r = e / n, m = e % n;
+            if (r == k) continue;
+            M[r][m] = (m == k ? 0. : M[r][m]) - col[r] * M[k][m];
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) inv[e] = M[e / n][e % n];
+}
+
+// ------------------------------------------------------------------ cycle kernels
+__device__ __forceinline__ bool mgDone(const double* st) { return st[B_DONE] != 0.; }
+
+// coarse level: z = two Jacobi sweeps from zero on b
+__global__ void k_mg_pre(MgLevel L, const double* st) {
+    if (mgDone(st)) return;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= L.D.N) return;
+    int i, j, k;
+    const int p = L.D.cell(e, i, j, k);
+    L.z[p] = preSmooth(levOp(L), L.w, [&](int c) { return L.b[c]; }, p);
+}
+
+// zout = zin + w (b - A zin), coarse level
+__global__ void k_mg_post(MgLevel L, const float* zin, float* zout, const double* st) {
+    if (mgDone(st)) return;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= L.D.N) return;
+    int i, j, k;
+    const int p = L.D.cell(e, i, j, k);
+    int nb[7];
+    nbrs(p, L.D.sJ, L.D.sK, nb);
+    float x[7];
+#pragma unroll
+    for (int m = 0; m < 7; ++m) x[m] = zin[nb[m]];
+    zout[p] = x[0] + L.w[p] * (L.b[p] - stencil7(L.A, L.D.P, L.D.sJ, L.D.sK, p, x));
+}
+
+// c.b = the sum over each aggregate of the fine residual b - A z (Op: the fine level's
+// operator, b its right-hand side, z its iterate).  Eight consecutive lanes take the (up to)
+// eight fine cells of one aggregate, lane s at offset (s & 1, s >> 1 & 1, s >> 2); the
+// butterfly over those lanes adds in a fixed order.
+template <class Op, class B>
+__device__ __forceinline__ void restrictAgg(const Dims& Df, const Op& A, B b, const float* z, const MgLevel& c) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = t >> 3, sub = t & 7;
+    int I = 0, J = 0, K = 0, pc = 0;
+    double s = 0.;
+    if (e < c.D.N) {
+        pc = c.D.cell(e, I, J, K);
+        const int di = sub & 1, dj = (sub >> 1) & 1, dk = sub >> 2;
+        const int i = c.f[0] * I + di, j = c.f[1] * J + dj, k = c.f[2] * K + dk;
+        if (di < c.f[0] && dj < c.f[1] && dk < c.f[2] && i < Df.nI && j < Df.nJ && k < Df.nK) {
+            const int p = Df.pidx(i, j, k);
+            auto zf = [&](int q) { return (double)z[q]; };
+            s = b(p) - A.apply(zf, p);
+        }
+    }
+    s += __shfl_xor(s, 1, 64);
+    s += __shfl_xor(s, 2, 64);
+    s += __shfl_xor(s, 4, 64);
+    if (e < c.D.N && sub == 0) c.b[pc] = (float)s;
+}
+__global__ void k_mg_restrict0(Dims Df, Sym4<float> A, const double* b, const float* z, MgLevel c, const double* st) {
+    if (mgDone(st)) return;
+    restrictAgg(Df, A, [&](int p) { return b[p]; }, z, c);
+}
+__global__ void k_mg_restrict(MgLevel f, MgLevel c, const double* st) {
+    if (mgDone(st)) return;
+    restrictAgg(f.D, levOp(f), [&](int p) { return (double)f.b[p]; }, f.z, c);
+}
+
+// coarsest level: z = inverse * b (one workgroup)
+__global__ void k_mg_coarse(MgLevel L, const double* inv, int n, const double* st) {
+    if (mgDone(st)) return;
+    __shared__ double bs[MG_COARSEST];
+    int i, j, k;
+    if ((int)threadIdx.x < n) bs[threadIdx.x] = L.b[L.D.cell(threadIdx.x, i, j, k)];
+    __syncthreads();
+    if ((int)threadIdx.x < n) {
+        double s = 0.;
+        for (int m = 0; m < n; ++m) s += inv[threadIdx.x * n + m] * bs[m];
+        L.z[L.D.cell(threadIdx.x, i, j, k)] = (float)s;
+    }
+}
+
+// cell-centred linear interpolation along one direction: fine index f, aggregation factor
+// fa, nc coarse cells -> coarse cells c0, c1 with weights w0, w1
+__device__ __forceinline__ void lin(int f, int fa, int nc, int& c0, int& c1, float& w0, float& w1) {
+    if (fa == 1) { c0 = c1 = f; w0 = 1.f; w1 = 0.f; return; }
+    c0 = f >> 1;
+    c1 = (f & 1) ? c0 + 1 : c0 - 1;
+    c1 = c1 < 0 ? 0 : (c1 >= nc ? nc - 1 : c1);
+    w0 = 0.75f; w1 = 0.25f;
+}
+
+// zf += P zc on the fine level (cells of status INACTIVE keep their value)
+__global__ void k_mg_prolong(Dims Df, MgLevel c, float* zf, const int8_t* stf, const double* st) {
+    if (mgDone(st)) return;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= Df.N) return;
+    int i, j, k;
+    const int p = Df.cell(e, i, j, k);
+    if (stf && stf[p] == ST_INACTIVE) return;
+    int cx[2], cy[2], cz[2];
+    float wx[2], wy[2], wz[2];
+    lin(i, c.f[0], c.D.nI, cx[0], cx[1], wx[0], wx[1]);
+    lin(j, c.f[1], c.D.nJ, cy[0], cy[1], wy[0], wy[1]);
+    lin(k, c.f[2], c.D.nK, cz[0], cz[1], wz[0], wz[1]);
+    float s = 0.f;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int d = 0; d < 2; ++d) s += wz[a] * wy[b] * wx[d] * c.z[c.D.pidx(cx[d], cy[b], cz[a])];
+    zf[p] += s;
+}
+
+// level 0: zout = zin + w (b - A zin) on every cell with the plain stencil (the ghost
+// cells are rewritten by k_mg_ghost_post)
+__global__ void k_mg_post0(Dims D, const float* __restrict__ A, const float* __restrict__ wt, const double* __restrict__ b,
+                           const float* __restrict__ zin, float* __restrict__ zout, const double* st) {
+    if (mgDone(st)) return;
+    FOR_CELLS(D, q, p) {
+        int nb[7];
+        nbrs(p, D.sJ, D.sK, nb);
+        float x[7];
+#pragma unroll
+        for (int m = 0; m < 7; ++m) x[m] = zin[nb[m]];
+        zout[p] = x[0] + wt[p] * ((float)b[p] - stencil7(A, D.P, D.sJ, D.sK, p, x));
+    }
+}
+
+// the ghost rows after the pre-smoothing: z holds the second sweep on fluid cells, g1 the
+// first sweep of the ghost cells.  The row's eight cells, their ghost-row numbers and
+// values are loaded together.
+__global__ void k_mg_ghost_pre(int n, const int* cell, Sym4<float> A, const double* b, const float* g1, float* z,
+                               const double* st) {
+    if (mgDone(st)) return;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int p = cell[r];
+    const float* v = (const float*)A.ov.vals + 9 * r;
+    int cols[8], gr[8];
+    float zc[8], gc[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) cols[m] = A.ov.cols[9 * r + 1 + m];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const int c = cols[m] >= 0 ? cols[m] : p;
+        gr[m] = A.ov.rowOf[c];
+        zc[m] = z[c];
+    }
+#pragma unroll
+    for (int m = 0; m < 8; ++m) gc[m] = g1[gr[m] >= 0 ? gr[m] : r];
+    double s = b[p];
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+        if (cols[m] >= 0) s -= (double)v[1 + m] * (double)(gr[m] >= 0 ? gc[m] : zc[m]);
+    z[p] = (float)(s / (double)v[0]);
+}
+// the ghost rows after a post-smoothing sweep zin -> zout: fluid cells of the row at their
+// swept value, ghost cells at their input value
+__global__ void k_mg_ghost_post(int n, const int* cell, Sym4<float> A, const double* b, const float* zin, float* zout,
+                                const double* st) {
+    if (mgDone(st)) return;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int p = cell[r];
+    const float* v = (const float*)A.ov.vals + 9 * r;
+    int cols[8];
+    float zi[8], zo[8];
+    int gr[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) cols[m] = A.ov.cols[9 * r + 1 + m];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const int c = cols[m] >= 0 ? cols[m] : p;
+        gr[m] = A.ov.rowOf[c];
+        zi[m] = zin[c];
+        zo[m] = zout[c];
+    }
+    double s = b[p];
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+        if (cols[m] >= 0) s -= (double)v[1 + m] * (double)(gr[m] >= 0 ? zi[m] : zo[m]);
+    zout[p] = (float)(s / (double)v[0]);
+}
+
+// ------------------------------------------------------------------ BiCGStab kernels
+// r = b on live cells, rh = r, p = v = x = 0; partials (rh, r), (r, r)
+__global__ void k_mg_init(Dims D, const int8_t* stc, const double* b, double* r, double* rh, double* p0, double* p1,
+                          double* v, double* x, double* part, double* st) {
+    double acc[2] = {0., 0.};
+    FOR_CELLS(D, q, p) {
+        const double bv = stc[p] != ST_INACTIVE ? b[p] : 0.;
+        r[p] = bv; rh[p] = bv; p0[p] = 0.; p1[p] = 0.; v[p] = 0.; x[p] = 0.;
+        acc[0] += bv * bv;
+        acc[1] += bv * bv;
+    }
+    writePartials<KBS, 2>(acc, part);
+    if (BLOCK0) {
+        st[B_IT] = 0.; st[B_DONE] = 0.; st[B_BB] = -1.; st[B_RHOPREV] = 1.; st[B_ALPHA] = 1.; st[B_OMEGA] = 1.;
+        st[B_BRK] = 0.;
+    }
+}
+
+// [convergence on (rh, r), (r, r); beta] p_new = r + beta (p - omega v); the cycle's
+// level-0 pre-smoothing of p_new into z (G: the system has ghost rows)
+template <bool G>
+__global__ void k_mg_pre_p(Dims D, Sym4<float> A, const float* __restrict__ wt, float* __restrict__ g1,
+                           const double* __restrict__ r, const double* __restrict__ pold, double* __restrict__ pnew,
+                           const double* __restrict__ v, float* __restrict__ z, const double* partIn, int nb, double* st,
+                           LoopCtl lc) {
+    BiState<1> S;
+    S.load(st);
+    if (!S.any()) return;
+    double beta[1], om[1];
+    bool live[1];
+    biaHead<1>(partIn, nb, st, lc, S, beta, om, live);
+    if (!live[0]) return;
+    const double bt = beta[0], w = om[0];
+    auto pf = [&](int c) { return r[c] + bt * (pold[c] - w * v[c]); };
+    FOR_CELLS(D, q, p) {
+        const double pp = pf(p);
+        const float zz = preSmooth0<G>(A, wt, pf, g1, p);
+        pnew[p] = pp;
+        z[p] = zz;
+    }
+}
+
+// [alpha = rho / (rh, v)] s = r - alpha v; the level-0 pre-smoothing of s into z
+template <bool G>
+__global__ void k_mg_pre_s(Dims D, Sym4<float> A, const float* __restrict__ wt, float* __restrict__ g1,
+                           const double* __restrict__ r, const double* __restrict__ v, double* __restrict__ s,
+                           float* __restrict__ z, const double* partIn, int nb, double* st) {
+    BiState<1> S;
+    S.load(st);
+    if (!S.any()) return;
+    double al[1];
+    bool live[1];
+    bibHead<1>(partIn, nb, st, S, al, live);
+    if (!live[0]) return;
+    const double a = al[0];
+    auto sf = [&](int c) { return r[c] - a * v[c]; };
+    FOR_CELLS(D, q, p) {
+        const double ss = sf(p);
+        const float zz = preSmooth0<G>(A, wt, sf, g1, p);
+        s[p] = ss;
+        z[p] = zz;
+    }
+}
+
+// v = A ph; partial (rh, v)
+__global__ void k_mg_v(Dims D, Sym4<double> A, const float* ph, const double* rh, double* v, double* partOut,
+                       const double* st) {
+    if (mgDone(st)) return;
+    double acc[1] = {0.};
+    auto xf = [&](int c) { return (double)ph[c]; };
+    FOR_CELLS(D, q, p) {
+        const double vv = A.apply(xf, p);
+        v[p] = vv;
+        acc[0] += rh[p] * vv;
+    }
+    writePartials<KBS, 1>(acc, partOut);
+}
+
+// t = A sh; partials (t, s), (t, t)
+__global__ void k_mg_t(Dims D, Sym4<double> A, const float* sh, const double* s, double* t, double* partOut,
+                       const double* st) {
+    if (mgDone(st) || st[B_BRK] == 1.) return;
+    double acc[2] = {0., 0.};
+    auto xf = [&](int c) { return (double)sh[c]; };
+    FOR_CELLS(D, q, p) {
+        const double tt = A.apply(xf, p);
+        t[p] = tt;
+        acc[0] += tt * s[p];
+        acc[1] += tt * tt;
+    }
+    writePartials<KBS, 2>(acc, partOut);
+}
+
+// [omega = (t, s) / (t, t)] x += alpha ph + omega sh; r = s - omega t; partials (rh, r), (r, r)
+__global__ void k_mg_x(Dims D, double* x, double* r, const double* rh, const float* ph, const float* sh, const double* s,
+                       const double* t, const double* partIn, int nb, double* partOut, double* st) {
+    const double sDone = st[B_DONE], sBrk = st[B_BRK], sAl = st[B_ALPHA], sRho = st[B_RHOCUR], sIt = st[B_IT];
+    if (sDone != 0.) return;
+    double tt[2];
+    readPartials<KBS, 2>(partIn, nb, tt);
+    const bool live = sBrk != 1.;   // KB found (rh, v) = 0: no s, t this step
+    const double om = tt[1] > 0. ? tt[0] / tt[1] : 0.;
+    if (BLOCK0 && live) {
+        st[B_OMEGA] = om; st[B_RHOPREV] = sRho; st[B_IT] = sIt + 1.;
+        if (om == 0.) st[B_BRK] = 2.;
+    }
+    double acc[2] = {0., 0.};
+    FOR_CELLS(D, q, p) {
+        double rn;
+        if (live) {
+            x[p] += sAl * (double)ph[p] + om * (double)sh[p];
+            rn = s[p] - om * t[p];
+            r[p] = rn;
+        } else {
+            rn = r[p];
+        }
+        acc[0] += rh[p] * rn;
+        acc[1] += rn * rn;
+    }
+    writePartials<KBS, 2>(acc, partOut);
+}
+
+// ------------------------------------------------------------------ host side
+static int kgrid(const Dims& D) { return blocksFor(D.N, KBS, KGRID); }
+static int lgrid(const Dims& D) { return blocksFor(D.N, MBS); }
+static int rgrid(const Dims& Dc) { return blocksFor(8L * Dc.N, MBS); }   // 8 lanes per coarse cell
+
+static Dims mkDims(int nI, int nJ, int nK) {
+    Dims D;
+    D.nI = nI; D.nJ = nJ; D.nK = nK;
+    D.sJ = nI + 2; D.sK = (nI + 2) * (nJ + 2);
+    D.P = D.sK * (nK + 2);
+    D.N = nI * nJ * nK;
+    return D;
+}
+
+// Levels for the context's grid, allocated once per grid: each level halves every
+// dimension of more than 2 cells (rounding up) until 64 cells or fewer remain.  Returns
+// false when the grid itself is that small.
+static bool mgPlan(Ctx& c) {
+    MgHier& H = c.mg;
+    if (H.nl > 0) return H.nl > 1;
+    const Dims& D = c.D;
+    H.L[0].D = D;
+    H.nl = 1;
+    Dims cur = D;
+    while (cur.N > MG_COARSEST && H.nl < MG_MAX_LEVELS) {
+        int f[3], n[3] = {cur.nI, cur.nJ, cur.nK};
+        for (int d = 0; d < 3; ++d) f[d] = n[d] > 2 ? 2 : 1;
+        if (f[0] == 1 && f[1] == 1 && f[2] == 1) break;
+        MgLevel& L = H.L[H.nl];
+        L.D = mkDims((n[0] + f[0] - 1) / f[0], (n[1] + f[1] - 1) / f[1], (n[2] + f[2] - 1) / f[2]);
+        for (int d = 0; d < 3; ++d) L.f[d] = f[d];
+        const size_t Pl = (size_t)L.D.P;
+        L.A = falloc(c, 4 * Pl);
+        L.b = falloc(c, Pl);
+        L.z = falloc(c, Pl);
+        L.t = falloc(c, Pl);
+        L.w = falloc(c, Pl);
+        cur = L.D;
+        ++H.nl;
+    }
+    if (H.nl == 1 || cur.N > MG_COARSEST) { H.nl = 1; return false; }
+    for (int q = 0; q < 3; ++q) H.z0[q] = falloc(c, (size_t)D.P);
+    H.w0 = falloc(c, (size_t)D.P);
+    double* inv = nullptr;
+    HIPCHECK(hipMalloc(&inv, sizeof(double) * MG_COARSEST * MG_COARSEST));
+    c.allocs.push_back(inv);
+    c.deviceBytes += sizeof(double) * MG_COARSEST * MG_COARSEST;
+    H.cinv = inv;
+    return true;
+}
+
+// coarse operators of the current p' system (its FP32 copy made), the Jacobi weights of
+// every level and the dense inverse of the coarsest
+static void mgBuild(Ctx& c) {
+    MgHier& H = c.mg;
+    if (c.hasIB && c.ov.n > H.g1n) {
+        H.g1 = falloc(c, (size_t)c.ov.n);
+        H.g1n = c.ov.n;
+    }
+    hipLaunchKernelGGL(k_mg_weights, dim3(blocksFor(c.D.P, MBS)), dim3(MBS), 0, c.stream, (const float*)c.Ap32, c.D.P, H.w0);
+    hipLaunchKernelGGL(k_mg_coarsen<FineP>, dim3(lgrid(H.L[1].D)), dim3(MBS), 0, c.stream, FineP{c.Ap, c.st, c.D}, H.L[1]);
+    for (int l = 2; l < H.nl; ++l)
+        hipLaunchKernelGGL(k_mg_coarsen<FineL>, dim3(lgrid(H.L[l].D)), dim3(MBS), 0, c.stream,
+                           FineL{H.L[l - 1].A, H.L[l - 1].D}, H.L[l]);
+    for (int l = 1; l < H.nl; ++l)
+        hipLaunchKernelGGL(k_mg_weights, dim3(blocksFor(H.L[l].D.P, MBS)), dim3(MBS), 0, c.stream, (const float*)H.L[l].A,
+                           H.L[l].D.P, H.L[l].w);
+    const MgLevel& Lc = H.L[H.nl - 1];
+    hipLaunchKernelGGL(k_mg_dense, dim3(1), dim3(MBS), 0, c.stream, Lc, Lc.D.N, c.singularP ? 1 : 0, H.cinv);
+}
+
+// the cycle after the level-0 pre-smoothing: z (level 0) holds that smoothing of b
+static void mgCycleRest(Ctx& c, const Sym4<float>& A0, const double* b, float* z, const double* st) {
+    MgHier& H = c.mg;
+    const Dims& D = c.D;
+    hipStream_t s = c.stream;
+    const int g0 = kgrid(D);
+    const bool ghosts = A0.ov.rowOf && c.ov.n > 0;
+    const int gg = blocksFor(c.ov.n, MBS);
+    if (ghosts)
+        hipLaunchKernelGGL(k_mg_ghost_pre, dim3(gg), dim3(MBS), 0, s, c.ov.n, (const int*)c.ov.cell, A0, b,
+                           (const float*)H.g1, z, st);
+    hipLaunchKernelGGL(k_mg_restrict0, dim3(rgrid(H.L[1].D)), dim3(MBS), 0, s, D, A0, b, (const float*)z, H.L[1], st);
+    const int nl = H.nl;
+    for (int l = 1; l < nl - 1; ++l) {
+        hipLaunchKernelGGL(k_mg_pre, dim3(lgrid(H.L[l].D)), dim3(MBS), 0, s, H.L[l], st);
+        hipLaunchKernelGGL(k_mg_restrict, dim3(rgrid(H.L[l + 1].D)), dim3(MBS), 0, s, H.L[l], H.L[l + 1], st);
+    }
+    const MgLevel& Lc = H.L[nl - 1];
+    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(MG_COARSEST), 0, s, Lc, (const double*)H.cinv, Lc.D.N, st);
+    for (int l = nl - 2; l >= 1; --l) {
+        const MgLevel& L = H.L[l];
+        hipLaunchKernelGGL(k_mg_prolong, dim3(lgrid(L.D)), dim3(MBS), 0, s, L.D, H.L[l + 1], L.z, (const int8_t*)nullptr, st);
+        hipLaunchKernelGGL(k_mg_post, dim3(lgrid(L.D)), dim3(MBS), 0, s, L, (const float*)L.z, L.t, st);
+        hipLaunchKernelGGL(k_mg_post, dim3(lgrid(L.D)), dim3(MBS), 0, s, L, (const float*)L.t, L.z, st);
+    }
+    hipLaunchKernelGGL(k_mg_prolong, dim3(lgrid(D)), dim3(MBS), 0, s, D, H.L[1], z, (const int8_t*)c.st, st);
+    for (int sweep = 0; sweep < 2; ++sweep) {
+        const float* zin = sweep == 0 ? z : H.z0[2];
+        float* zout = sweep == 0 ? H.z0[2] : z;
+        hipEvent_t e0, e1;
+        if (c.probe.slot(PROBE_MG_POST0, e0, e1))   // dispatch-bracketing events: the kernel's own time
+            hipExtLaunchKernelGGL(k_mg_post0, dim3(g0), dim3(KBS), 0, s, e0, e1, 0, D, A0.A, (const float*)H.w0, b, zin, zout,
+                                  st);
+        else
+            hipLaunchKernelGGL(k_mg_post0, dim3(g0), dim3(KBS), 0, s, D, A0.A, (const float*)H.w0, b, zin, zout, st);
+        if (ghosts)
+            hipLaunchKernelGGL(k_mg_ghost_post, dim3(gg), dim3(MBS), 0, s, c.ov.n, (const int*)c.ov.cell, A0, b, zin, zout, st);
+    }
+}
+
+// BiCGStab with the cycle on A e = b from e = 0, to ||r|| <= rtol ||b|| on its recurrence.
+// Returns the iteration count; done holds 1 (converged), 2 (breakdown) or 0 (cap).
+static int mgRun(Ctx& c, const double* b, double* e, double rtol, int maxit, int* doneOut) {
+    MgHier& H = c.mg;
+    const Dims& D = c.D;
+    const int P = D.P;
+    hipStream_t s = c.stream;
+    const OvView ov64{c.hasIB ? c.ov.rowOf : nullptr, c.ov.cols, c.ov.valP};
+    const OvView ov32{c.hasIB ? c.ov.rowOf : nullptr, c.ov.cols, c.ov.valP32};
+    const Sym4<double> A64{c.Ap, P, D.sJ, D.sK, ov64, c.st};
+    const Sym4<float> A32{c.Ap32, P, D.sJ, D.sK, ov32, c.st};
+    // FP64 vectors in the momentum work block (double views only, halos stay zero)
+    double* W = c.wm;
+    double *r = W, *rh = W + P, *p0 = W + 2 * (size_t)P, *p1 = W + 3 * (size_t)P, *v = W + 4 * (size_t)P;
+    double *sv = W + 5 * (size_t)P, *tv = W + 6 * (size_t)P;
+    float *ph = H.z0[0], *sh = H.z0[1];
+    double* st = c.scal + SC_BI;
+    double* partC = c.red.part;
+    double* partA = c.red.part + MAX_RED_BLOCKS * RED_STRIDE;
+    double* partB = c.red.part + 2 * MAX_RED_BLOCKS * RED_STRIDE;
+    const int g = kgrid(D);
+    const bool ghosts = A32.ov.rowOf && c.ov.n > 0;
+    LoopCtl lc{rtol * rtol, maxit, 0.};
+    hipLaunchKernelGGL(k_mg_init, dim3(g), dim3(KBS), 0, s, D, (const int8_t*)c.st, b, r, rh, p0, p1, v, e, partC, st);
+    double *pc = p0, *pn = p1;
+    int it = 0, chunk = 4;
+    while (true) {
+        for (int q = 0; q < chunk; ++q) {
+            if (ghosts)
+                hipLaunchKernelGGL(k_mg_pre_p<true>, dim3(g), dim3(KBS), 0, s, D, A32, (const float*)H.w0, H.g1, (const double*)r,
+                                   (const double*)pc, pn, (const double*)v, ph, (const double*)partC, g, st, lc);
+            else
+                hipLaunchKernelGGL(k_mg_pre_p<false>, dim3(g), dim3(KBS), 0, s, D, A32, (const float*)H.w0, H.g1, (const double*)r,
+                                   (const double*)pc, pn, (const double*)v, ph, (const double*)partC, g, st, lc);
+            mgCycleRest(c, A32, pn, ph, st);
+            hipLaunchKernelGGL(k_mg_v, dim3(g), dim3(KBS), 0, s, D, A64, (const float*)ph, (const double*)rh, v, partA,
+                               (const double*)st);
+            if (ghosts)
+                hipLaunchKernelGGL(k_mg_pre_s<true>, dim3(g), dim3(KBS), 0, s, D, A32, (const float*)H.w0, H.g1, (const double*)r,
+                                   (const double*)v, sv, sh, (const double*)partA, g, st);
+            else
+                hipLaunchKernelGGL(k_mg_pre_s<false>, dim3(g), dim3(KBS), 0, s, D, A32, (const float*)H.w0, H.g1, (const double*)r,
+                                   (const double*)v, sv, sh, (const double*)partA, g, st);
+            mgCycleRest(c, A32, sv, sh, st);
+            hipLaunchKernelGGL(k_mg_t, dim3(g), dim3(KBS), 0, s, D, A64, (const float*)sh, (const double*)sv, tv, partB,
+                               (const double*)st);
+            hipLaunchKernelGGL(k_mg_x, dim3(g), dim3(KBS), 0, s, D, e, r, (const double*)rh, (const float*)ph,
+                               (const float*)sh, (const double*)sv, (const double*)tv, (const double*)partB, g, partC, st);
+            std::swap(pc, pn);
+        }
+        HIPCHECK(hipMemcpyAsync(c.hscal + SC_BI, st, B_STRIDE * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        it = (int)c.hscal[SC_BI + B_IT];
+        if (c.hscal[SC_BI + B_DONE] != 0. || it >= maxit) break;
+        // B_RR is the residual tested before the last queued iteration (it - 1 done): queue
+        // what the observed rate of decrease still needs (an iteration queued past
+        // convergence costs 55 launches that return at once)
+        const double rr0 = c.hscal[SC_BI + B_RR], bb0 = c.hscal[SC_BI + B_BB];
+        chunk = 4;
+        if (it > 1 && rr0 > 0. && bb0 > 0. && rr0 < bb0) {
+            const double rate = std::log(rr0 / bb0) / (it - 1);
+            const double need = std::log(lc.tol2 * bb0 / rr0) / rate - 1.;
+            chunk = std::max(1, std::min(16, (int)std::ceil(need)));
+        }
+    }
+    *doneOut = (int)c.hscal[SC_BI + B_DONE];
+    return it;
+}
+
+int mg_bicgstab(Ctx& c, double shift, double rtol, int maxit, int* refs, double* relOut) {
+    *refs = 0;
+    if (c.indefiniteP) return -1;   // the coarse Laplacians do not model a reaction term
+    if (!mgPlan(c)) return -1;
+    pcorr_make_fp32(c);
+    mgBuild(c);
+    const int P = c.D.P;
+    double* e = c.wm + 7 * (size_t)P;
+    const double* r64 = c.wp + 6 * (size_t)P;
+    double rr, bb;
+    pcorr_true_resid(c, shift, &rr, &bb);
+    double rel = bb > 0. ? rr / bb : 0.;
+    int its = 0, done = 0;
+    // each round solves A e = r_true from e = 0 and adds e: the recurrence residual drifts
+    // from the true one, and a breakdown restarts from the true residual
+    while (rel > rtol && its < maxit && *refs < 30) {
+        const double tol = std::min(0.5, 0.5 * rtol / rel);
+        its += mgRun(c, r64, e, tol, maxit - its, &done);
+        add_vec(c, c.xp, e, P);
+        const double prev = rel;
+        pcorr_true_resid(c, shift, &rr, &bb);
+        rel = rr / bb;
+        ++*refs;
+        if (!(rel < prev)) break;
+    }
+    *relOut = rel;
+    return its;
+}
+
+}  // namespace caffe

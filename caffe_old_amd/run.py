@@ -2,7 +2,7 @@
 """Run a reference case directory unchanged on the GPU solver.
 
     python -m caffe_old_amd.run hexamesh CASE_DIR
-    python -m caffe_old_amd.run {simple,piso,ibpiso} CASE_DIR [--pcorr plain|ir|learned] [--momentum plain|ir]
+    python -m caffe_old_amd.run {simple,piso,ibpiso} CASE_DIR [--pcorr plain|ir|learned|mg] [--momentum plain|ir]
     python -m caffe_old_amd.run diffusion CASE_DIR
     python -m caffe_old_amd.run coupled CASE_DIR
 
@@ -68,7 +68,7 @@ def run(algo: str, case_dir: str, pcorr: str = "plain", momentum: str = "plain",
 
     name, case, rc = caseio.load_case(case_dir, algo)
     s = make_solver(case)
-    modes = {"plain": L.SOLVE_PLAIN_FP64, "ir": L.SOLVE_IR_FP32, "learned": L.SOLVE_LEARNED_FCG}
+    modes = {"plain": L.SOLVE_PLAIN_FP64, "ir": L.SOLVE_IR_FP32, "learned": L.SOLVE_LEARNED_FCG, "mg": L.SOLVE_MG}
     s.set_solve_opts("momentum", mode=modes[momentum], rtol=rtol, maxit=5000)
     s.set_solve_opts("pcorr", mode=modes[pcorr], rtol=rtol, maxit=200000, inner_rtol=1e-4 if pcorr == "learned" else 1e-2)
     if pcorr == "learned":
@@ -107,7 +107,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("module", choices=["hexamesh", "simple", "piso", "ibpiso", "diffusion", "coupled"])
     ap.add_argument("case_dir", nargs="?", default=".")
-    ap.add_argument("--pcorr", choices=["plain", "ir", "learned"], default="plain")
+    ap.add_argument("--pcorr", choices=["plain", "ir", "learned", "mg"], default="plain")
     ap.add_argument("--momentum", choices=["plain", "ir"], default="plain")
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--quiet", action="store_true")

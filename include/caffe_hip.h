@@ -69,9 +69,13 @@ enum { CAFFE_SIMPLE = 0, CAFFE_PISO = 1, CAFFE_IBPISO = 2,
 enum {
     CAFFE_SOLVE_PLAIN_FP64 = 0,   /* preconditioned Krylov in FP64 (the judge and fallback) */
     CAFFE_SOLVE_IR_FP32 = 1,      /* FP32 inner Krylov inside FP64 iterative refinement */
-    CAFFE_SOLVE_LEARNED_FCG = 2   /* flexible CG with the learned two-grid cycle (FP32 inside
+    CAFFE_SOLVE_LEARNED_FCG = 2,  /* flexible CG with the learned two-grid cycle (FP32 inside
                                      FP64 refinement); accepted only at rtol, else the plain
                                      solve runs (fell_back) */
+    CAFFE_SOLVE_MG = 3            /* p' only: BiCGStab in FP64, right-preconditioned by one FP32
+                                     geometric multigrid V-cycle (no counterpart in the
+                                     reference, whose PETSc solve it replaces); the FP32 modes
+                                     use it for ghost-row systems too */
 };
 
 /* The keys of case/case.info and case/initialConditions.info the hot path reads
@@ -164,8 +168,10 @@ int caffe_time_kernel(caffe_ctx* ctx, const char* which, int reps, double* ms, d
  * its launches on the context's stream, up to 4096): which = "fcg_k1" (stencil kernel of the
  * learned flexible CG), "pcorr_cg_k1_fp32" / "pcorr_cg_k1_fp64" (stencil kernel of the
  * pressure CG), "coupled_bicg_a" (first BiCGStab kernel of the Coupled solve),
- * "pcorr_bicg_a_fp32" / "pcorr_bicg_a_fp64" (first kernel of the p' BiCGStab used for
- * ghost-row and indefinite systems); enable 0 stops recording.  _read returns the average duration of the
+ * "pcorr_bicg_a_fp32" / "pcorr_bicg_a_fp64" (first kernel of the Jacobi p' BiCGStab:
+ * ghost-row systems in the plain mode, indefinite systems), "mg_post0" (fine-level
+ * post-smoothing sweep of the multigrid cycle: CAFFE_SOLVE_MG, and ghost-row systems in
+ * the FP32 modes); enable 0 stops recording.  _read returns the average duration of the
  * launches that ran (launches queued past convergence return at once and are left out),
  * their number, and the kernel's algorithmic HBM bytes per launch.  No counterpart in the
  * reference. */

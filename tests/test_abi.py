@@ -68,3 +68,10 @@ def test_philox_known_answer():
     assert [hex(int(v)) for v in z] == ["0x6627e8d5", "0xe169c58d", "0xbc57ac4c", "0x9b00dbd8"]
     f = cs.philox4x32(np.full((1, 4), 0xFFFFFFFF, dtype=np.uint32), (0xFFFFFFFF, 0xFFFFFFFF))[0]
     assert [hex(int(v)) for v in f] == ["0x408f276d", "0x41c83b0e", "0xa20bc7c6", "0x6d5451fd"]
+
+
+def test_solve_mode_constants_match_header():
+    txt = open(os.path.join(ROOT, "include", "caffe_hip.h")).read()
+    modes = dict((k, int(v)) for k, v in re.findall(r"CAFFE_SOLVE_(\w+)\s*=\s*(\d+)", txt))
+    assert modes == {"PLAIN_FP64": _lib.SOLVE_PLAIN_FP64, "IR_FP32": _lib.SOLVE_IR_FP32,
+                     "LEARNED_FCG": _lib.SOLVE_LEARNED_FCG, "MG": _lib.SOLVE_MG}
