@@ -78,31 +78,18 @@ __device__ __forceinline__ float preSmooth(const Sym4<float>& A, const float* wt
 // exactly).  So every fine-level sweep ends with each ghost row solved exactly for its own
 // cell, from the values this sweep gave the other cells of the row (their input values
 // where those are ghost cells too).  The sweep kernels run the plain stencil on every
-// cell, and one thread per ghost row then writes the ghost cells (k_mg_ghost_*): it reads
-// fluid cells only where it writes, so the result does not depend on thread order.
-template <class T>
-__device__ __forceinline__ int ghostRow(const Sym4<T>& A, int c) { return A.ov.rowOf ? A.ov.rowOf[c] : -1; }
-template <class T, class B, class Z>
-__device__ __forceinline__ double ghostSolve(const Sym4<T>& A, int r, B b, int p, Z z) {
-    const T* v = (const T*)A.ov.vals + 9 * r;
-    int cols[8];
-#pragma unroll
-    for (int m = 0; m < 8; ++m) cols[m] = A.ov.cols[9 * r + 1 + m];
-    double s = b(p);
-#pragma unroll
-    for (int m = 0; m < 8; ++m)
-        if (cols[m] >= 0) s -= (double)v[1 + m] * z(cols[m]);
-    return s / (double)v[0];
-}
+// cell, and one thread per ghost row then writes the ghost cells (k_mg_first_*,
+// k_mg_ghost_*): it reads other cells than those it writes, so the result does not depend
+// on thread order.
 // preSmooth on the fine level.  A fluid cell's ghost neighbours take their first-sweep
-// value, the ghost row solved from w b; a ghost cell gets that value here (also into g1 at
-// its row, for k_mg_ghost_pre) and its second sweep from k_mg_ghost_pre.  The seven
-// neighbours' values, weights and ghost-row numbers (G: the system has ghost rows) are
-// loaded together; only a cell next to a ghost cell takes the ghost branch.  An inactive
-// cell's stored row is the identity with zero faces, so the plain stencil is its row.
+// value g1 (the ghost row solved from w b, formed by k_mg_first_*); a ghost cell gets its
+// g1 here and its second sweep from k_mg_ghost_pre.  The seven neighbours' values, weights
+// and ghost-row numbers (G: the system has ghost rows) are loaded together; a cell next to
+// a ghost cell then loads those neighbours' g1.  An inactive cell's stored row is the
+// identity with zero faces, so the plain stencil is its row.
 template <bool G, class B>
 __device__ __forceinline__ float preSmooth0(const Sym4<float>& A, const float* __restrict__ wt, B b,
-                                            float* __restrict__ g1, int p) {
+                                            const float* __restrict__ g1, int p) {
     int nb[7];
     nbrs(p, A.sJ, A.sK, nb);
     float z1[7];
@@ -117,16 +104,31 @@ __device__ __forceinline__ float preSmooth0(const Sym4<float>& A, const float* _
 #pragma unroll
         for (int m = 0; m < 7; ++m) anyG |= gr[m] >= 0;
         if (anyG) {
-            auto zJ = [&](int q) { return (double)(wt[q] * (float)b(q)); };
+#pragma unroll
             for (int m = 0; m < 7; ++m)
-                if (gr[m] >= 0) z1[m] = (float)ghostSolve(A, gr[m], b, nb[m], zJ);
-            if (gr[0] >= 0) {
-                g1[gr[0]] = z1[0];
-                return z1[0];
-            }
+                if (gr[m] >= 0) z1[m] = g1[gr[m]];
+            if (gr[0] >= 0) return z1[0];
         }
     }
     return z1[0] + wt[p] * ((float)b(p) - stencil7(A.A, A.P, A.sJ, A.sK, p, z1));
+}
+
+// g1 = the ghost rows solved from w b (the first sweep from zero), one thread per row
+template <class B>
+__device__ __forceinline__ void firstGhost(const Sym4<float>& A, const float* wt, B b, const int* cell, float* g1, int r) {
+    const int p = cell[r];
+    const float* v = (const float*)A.ov.vals + 9 * r;
+    int cols[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) cols[m] = A.ov.cols[9 * r + 1 + m];
+    double s = b(p);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const int c = cols[m] >= 0 ? cols[m] : p;
+        const double zJ = (double)(wt[c] * (float)b(c));
+        s -= cols[m] >= 0 ? (double)v[1 + m] * zJ : 0.;
+    }
+    g1[r] = (float)(s / (double)v[0]);
 }
 
 // ------------------------------------------------------------------ hierarchy set-up
@@ -278,13 +280,11 @@ __global__ void k_mg_post(MgLevel L, const float* zin, float* zout, const double
     zout[p] = x[0] + L.w[p] * (L.b[p] - stencil7(L.A, L.D.P, L.D.sJ, L.D.sK, p, x));
 }
 
-// c.b = the sum over each aggregate of the fine residual b - A z (Op: the fine level's
-// operator, b its right-hand side, z its iterate).  Eight consecutive lanes take the (up to)
-// eight fine cells of one aggregate, lane s at offset (s & 1, s >> 1 & 1, s >> 2); the
-// butterfly over those lanes adds in a fixed order.
-template <class Op, class B>
-__device__ __forceinline__ void restrictAgg(const Dims& Df, const Op& A, B b, const float* z, const MgLevel& c) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+// c.b = the sum over each aggregate of the fine residuals res(p).  Eight consecutive lanes
+// take the (up to) eight fine cells of one aggregate, lane s at offset (s & 1, s >> 1 & 1,
+// s >> 2); the butterfly over those lanes adds in a fixed order.
+template <class R>
+__device__ __forceinline__ void restrictAgg(const Dims& Df, R res, const MgLevel& c, int t) {
     const int e = t >> 3, sub = t & 7;
     int I = 0, J = 0, K = 0, pc = 0;
     double s = 0.;
@@ -292,38 +292,45 @@ __device__ __forceinline__ void restrictAgg(const Dims& Df, const Op& A, B b, co
         pc = c.D.cell(e, I, J, K);
         const int di = sub & 1, dj = (sub >> 1) & 1, dk = sub >> 2;
         const int i = c.f[0] * I + di, j = c.f[1] * J + dj, k = c.f[2] * K + dk;
-        if (di < c.f[0] && dj < c.f[1] && dk < c.f[2] && i < Df.nI && j < Df.nJ && k < Df.nK) {
-            const int p = Df.pidx(i, j, k);
-            auto zf = [&](int q) { return (double)z[q]; };
-            s = b(p) - A.apply(zf, p);
-        }
+        if (di < c.f[0] && dj < c.f[1] && dk < c.f[2] && i < Df.nI && j < Df.nJ && k < Df.nK) s = res(Df.pidx(i, j, k));
     }
     s += __shfl_xor(s, 1, 64);
     s += __shfl_xor(s, 2, 64);
     s += __shfl_xor(s, 4, 64);
     if (e < c.D.N && sub == 0) c.b[pc] = (float)s;
 }
-__global__ void k_mg_restrict0(Dims Df, Sym4<float> A, const double* b, const float* z, MgLevel c, const double* st) {
+// residual b - A z of a level's plain stencil at p
+__device__ __forceinline__ float levResid(const float* A, const float* b, const float* z, const Dims& D, int p) {
+    int nb[7];
+    nbrs(p, D.sJ, D.sK, nb);
+    float x[7];
+#pragma unroll
+    for (int m = 0; m < 7; ++m) x[m] = z[nb[m]];
+    return b[p] - stencil7(A, D.P, D.sJ, D.sK, p, x);
+}
+// Level 0: the plain stencil's residual on fluid and inactive cells (an inactive row is the
+// identity, zero at b = z = 0); ghost rows restrict nothing.  Their residuals are values
+// of the interpolation constraint, not flux balances, and after the sweep's exact solve
+// they hold only the coupling among ghost cells: restricted, they spoil the coarse right-
+// hand side (on the 128 x 32 x 32 channel with a sphere of radius 6.4 cells the cycle then
+// needs 12 BiCGStab iterations instead of 9).  G: the system has ghost rows.
+template <bool G>
+__global__ void k_mg_restrict0(Dims Df, const float* __restrict__ A, const int* __restrict__ rowOf,
+                               const double* __restrict__ b, const float* __restrict__ z, MgLevel c, const double* st) {
     if (mgDone(st)) return;
-    restrictAgg(Df, A, [&](int p) { return b[p]; }, z, c);
+    restrictAgg(Df, [&](int p) -> double {
+        if (G && rowOf[p] >= 0) return 0.;
+        int nb[7];
+        nbrs(p, Df.sJ, Df.sK, nb);
+        float x[7];
+#pragma unroll
+        for (int m = 0; m < 7; ++m) x[m] = z[nb[m]];
+        return b[p] - (double)stencil7(A, Df.P, Df.sJ, Df.sK, p, x);
+    }, c, blockIdx.x * blockDim.x + threadIdx.x);
 }
 __global__ void k_mg_restrict(MgLevel f, MgLevel c, const double* st) {
     if (mgDone(st)) return;
-    restrictAgg(f.D, levOp(f), [&](int p) { return (double)f.b[p]; }, f.z, c);
-}
-
-// coarsest level: z = inverse * b (one workgroup)
-__global__ void k_mg_coarse(MgLevel L, const double* inv, int n, const double* st) {
-    if (mgDone(st)) return;
-    __shared__ double bs[MG_COARSEST];
-    int i, j, k;
-    if ((int)threadIdx.x < n) bs[threadIdx.x] = L.b[L.D.cell(threadIdx.x, i, j, k)];
-    __syncthreads();
-    if ((int)threadIdx.x < n) {
-        double s = 0.;
-        for (int m = 0; m < n; ++m) s += inv[threadIdx.x * n + m] * bs[m];
-        L.z[L.D.cell(threadIdx.x, i, j, k)] = (float)s;
-    }
+    restrictAgg(f.D, [&](int p) { return (double)levResid(f.A, f.b, f.z, f.D, p); }, c, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // cell-centred linear interpolation along one direction: fine index f, aggregation factor
@@ -357,6 +364,83 @@ __global__ void k_mg_prolong(Dims Df, MgLevel c, float* zf, const int8_t* stf, c
 #pragma unroll
             for (int d = 0; d < 2; ++d) s += wz[a] * wy[b] * wx[d] * c.z[c.D.pidx(cx[d], cy[b], cz[a])];
     zf[p] += s;
+}
+
+// The bottom of the cycle in one workgroup: from level L[0] (its b restricted) down to the
+// coarsest level L[n - 1] and back up to L[0].z, the per-level steps of the kernels above
+// separated by workgroup barriers (global memory written before a barrier is visible to
+// the whole workgroup after it).  It replaces five launches per level on levels small
+// enough for one workgroup (at most MG_TAIL_CELLS cells).
+constexpr int MG_TAIL_LEVELS = 8;
+constexpr int MG_TAIL_CELLS = 4096;
+struct MgTail { int n; MgLevel L[MG_TAIL_LEVELS]; };
+__global__ void __launch_bounds__(1024) k_mg_tail(MgTail T, const double* inv, const double* st) {
+    if (mgDone(st)) return;
+    const int tid = threadIdx.x, bd = blockDim.x;
+    for (int l = 0; l < T.n - 1; ++l) {
+        const MgLevel& L = T.L[l];
+        for (int e = tid; e < L.D.N; e += bd) {
+            int i, j, k;
+            const int p = L.D.cell(e, i, j, k);
+            L.z[p] = preSmooth(levOp(L), L.w, [&](int c) { return L.b[c]; }, p);
+        }
+        __syncthreads();
+        const MgLevel& C = T.L[l + 1];
+        for (int base = 0; base < 8 * C.D.N; base += bd)   // every lane of a wave takes part in the butterfly
+            restrictAgg(L.D, [&](int p) { return (double)levResid(L.A, L.b, L.z, L.D, p); }, C, base + tid);
+        __syncthreads();
+    }
+    {   // coarsest level: z = inverse * b
+        __shared__ double bs[MG_COARSEST];
+        const MgLevel& L = T.L[T.n - 1];
+        const int n = L.D.N;
+        int i, j, k;
+        if (tid < n) bs[tid] = L.b[L.D.cell(tid, i, j, k)];
+        __syncthreads();
+        if (tid < n) {
+            double acc = 0.;
+            for (int m = 0; m < n; ++m) acc += inv[tid * n + m] * bs[m];
+            L.z[L.D.cell(tid, i, j, k)] = (float)acc;
+        }
+        __syncthreads();
+    }
+    for (int l = T.n - 2; l >= 0; --l) {
+        const MgLevel& L = T.L[l];
+        const MgLevel& C = T.L[l + 1];
+        for (int e = tid; e < L.D.N; e += bd) {
+            int i, j, k;
+            const int p = L.D.cell(e, i, j, k);
+            int cx[2], cy[2], cz[2];
+            float wx[2], wy[2], wz[2];
+            lin(i, C.f[0], C.D.nI, cx[0], cx[1], wx[0], wx[1]);
+            lin(j, C.f[1], C.D.nJ, cy[0], cy[1], wy[0], wy[1]);
+            lin(k, C.f[2], C.D.nK, cz[0], cz[1], wz[0], wz[1]);
+            float s = 0.f;
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int d = 0; d < 2; ++d) s += wz[a] * wy[b] * wx[d] * C.z[C.D.pidx(cx[d], cy[b], cz[a])];
+            L.z[p] += s;
+        }
+        __syncthreads();
+        for (int sweep = 0; sweep < 2; ++sweep) {
+            const float* zin = sweep == 0 ? L.z : L.t;
+            float* zout = sweep == 0 ? L.t : L.z;
+            for (int e = tid; e < L.D.N; e += bd) {
+                int i, j, k;
+                const int p = L.D.cell(e, i, j, k);
+                int nb[7];
+                nbrs(p, L.D.sJ, L.D.sK, nb);
+                float x[7];
+#pragma unroll
+                for (int m = 0; m < 7; ++m) x[m] = zin[nb[m]];
+                zout[p] = x[0] + L.w[p] * (L.b[p] - stencil7(L.A, L.D.P, L.D.sJ, L.D.sK, p, x));
+            }
+            __syncthreads();
+        }
+    }
 }
 
 // level 0: zout = zin + w (b - A zin) on every cell with the plain stencil (the ghost
@@ -448,10 +532,39 @@ __global__ void k_mg_init(Dims D, const int8_t* stc, const double* b, double* r,
     }
 }
 
+// The ghost rows' first sweep ahead of the two level-0 heads: each computes the head's
+// beta (alpha) from the same partial sums and state, without recording anything.
+__global__ void k_mg_first_p(int n, const int* cell, Sym4<float> A, const float* wt, float* g1, const double* r,
+                             const double* pold, const double* v, const double* partIn, int nb, const double* st,
+                             LoopCtl lc) {
+    BiState<1> S;
+    S.load(st);
+    if (!S.any()) return;
+    double t[2];
+    readPartials<MBS, 2>(partIn, nb, t);
+    const double rho = t[0], rr = t[1], bb = S.bb[0] < 0. ? rr : S.bb[0];
+    if (rr <= lc.tol2 * bb || S.it[0] >= lc.maxit || rr == 0. || S.brk[0] != 0. || rho == 0.) return;
+    const double bt = S.it[0] == 0. ? 0. : (rho / S.rhoPrev[0]) * (S.al[0] / S.om[0]), w = S.om[0];
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) firstGhost(A, wt, [&](int c) { return r[c] + bt * (pold[c] - w * v[c]); }, cell, g1, q);
+}
+__global__ void k_mg_first_s(int n, const int* cell, Sym4<float> A, const float* wt, float* g1, const double* r,
+                             const double* v, const double* partIn, int nb, const double* st) {
+    BiState<1> S;
+    S.load(st);
+    if (!S.any()) return;
+    double t[1];
+    readPartials<MBS, 1>(partIn, nb, t);
+    if (t[0] == 0.) return;
+    const double a = S.rho[0] / t[0];
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) firstGhost(A, wt, [&](int c) { return r[c] - a * v[c]; }, cell, g1, q);
+}
+
 // [convergence on (rh, r), (r, r); beta] p_new = r + beta (p - omega v); the cycle's
 // level-0 pre-smoothing of p_new into z (G: the system has ghost rows)
 template <bool G>
-__global__ void k_mg_pre_p(Dims D, Sym4<float> A, const float* __restrict__ wt, float* __restrict__ g1,
+__global__ void k_mg_pre_p(Dims D, Sym4<float> A, const float* __restrict__ wt, const float* __restrict__ g1,
                            const double* __restrict__ r, const double* __restrict__ pold, double* __restrict__ pnew,
                            const double* __restrict__ v, float* __restrict__ z, const double* partIn, int nb, double* st,
                            LoopCtl lc) {
@@ -474,7 +587,7 @@ __global__ void k_mg_pre_p(Dims D, Sym4<float> A, const float* __restrict__ wt, 
 
 // [alpha = rho / (rh, v)] s = r - alpha v; the level-0 pre-smoothing of s into z
 template <bool G>
-__global__ void k_mg_pre_s(Dims D, Sym4<float> A, const float* __restrict__ wt, float* __restrict__ g1,
+__global__ void k_mg_pre_s(Dims D, Sym4<float> A, const float* __restrict__ wt, const float* __restrict__ g1,
                            const double* __restrict__ r, const double* __restrict__ v, double* __restrict__ s,
                            float* __restrict__ z, const double* partIn, int nb, double* st) {
     BiState<1> S;
@@ -634,15 +747,25 @@ static void mgCycleRest(Ctx& c, const Sym4<float>& A0, const double* b, float* z
     if (ghosts)
         hipLaunchKernelGGL(k_mg_ghost_pre, dim3(gg), dim3(MBS), 0, s, c.ov.n, (const int*)c.ov.cell, A0, b,
                            (const float*)H.g1, z, st);
-    hipLaunchKernelGGL(k_mg_restrict0, dim3(rgrid(H.L[1].D)), dim3(MBS), 0, s, D, A0, b, (const float*)z, H.L[1], st);
+    if (ghosts)
+        hipLaunchKernelGGL(k_mg_restrict0<true>, dim3(rgrid(H.L[1].D)), dim3(MBS), 0, s, D, (const float*)A0.A,
+                           (const int*)A0.ov.rowOf, b, (const float*)z, H.L[1], st);
+    else
+        hipLaunchKernelGGL(k_mg_restrict0<false>, dim3(rgrid(H.L[1].D)), dim3(MBS), 0, s, D, (const float*)A0.A,
+                           (const int*)nullptr, b, (const float*)z, H.L[1], st);
     const int nl = H.nl;
-    for (int l = 1; l < nl - 1; ++l) {
+    // levels lt .. nl - 1 run in one workgroup
+    int lt = nl - 1;
+    while (lt > 1 && H.L[lt - 1].D.N <= MG_TAIL_CELLS && nl - (lt - 1) <= MG_TAIL_LEVELS) --lt;
+    for (int l = 1; l < lt; ++l) {
         hipLaunchKernelGGL(k_mg_pre, dim3(lgrid(H.L[l].D)), dim3(MBS), 0, s, H.L[l], st);
         hipLaunchKernelGGL(k_mg_restrict, dim3(rgrid(H.L[l + 1].D)), dim3(MBS), 0, s, H.L[l], H.L[l + 1], st);
     }
-    const MgLevel& Lc = H.L[nl - 1];
-    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(MG_COARSEST), 0, s, Lc, (const double*)H.cinv, Lc.D.N, st);
-    for (int l = nl - 2; l >= 1; --l) {
+    MgTail T;
+    T.n = nl - lt;
+    for (int l = lt; l < nl; ++l) T.L[l - lt] = H.L[l];
+    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(1024), 0, s, T, (const double*)H.cinv, st);
+    for (int l = lt - 1; l >= 1; --l) {
         const MgLevel& L = H.L[l];
         hipLaunchKernelGGL(k_mg_prolong, dim3(lgrid(L.D)), dim3(MBS), 0, s, L.D, H.L[l + 1], L.z, (const int8_t*)nullptr, st);
         hipLaunchKernelGGL(k_mg_post, dim3(lgrid(L.D)), dim3(MBS), 0, s, L, (const float*)L.z, L.t, st);
@@ -685,24 +808,33 @@ static int mgRun(Ctx& c, const double* b, double* e, double rtol, int maxit, int
     double* partB = c.red.part + 2 * MAX_RED_BLOCKS * RED_STRIDE;
     const int g = kgrid(D);
     const bool ghosts = A32.ov.rowOf && c.ov.n > 0;
+    const int gg = blocksFor(c.ov.n, MBS);
     LoopCtl lc{rtol * rtol, maxit, 0.};
     hipLaunchKernelGGL(k_mg_init, dim3(g), dim3(KBS), 0, s, D, (const int8_t*)c.st, b, r, rh, p0, p1, v, e, partC, st);
     double *pc = p0, *pn = p1;
     int it = 0, chunk = 4;
     while (true) {
         for (int q = 0; q < chunk; ++q) {
-            if (ghosts)
-                hipLaunchKernelGGL(k_mg_pre_p<true>, dim3(g), dim3(KBS), 0, s, D, A32, (const float*)H.w0, H.g1, (const double*)r,
-                                   (const double*)pc, pn, (const double*)v, ph, (const double*)partC, g, st, lc);
+            if (ghosts) {
+                hipLaunchKernelGGL(k_mg_first_p, dim3(gg), dim3(MBS), 0, s, c.ov.n, (const int*)c.ov.cell, A32,
+                                   (const float*)H.w0, H.g1, (const double*)r, (const double*)pc, (const double*)v,
+                                   (const double*)partC, g, (const double*)st, lc);
+                hipLaunchKernelGGL(k_mg_pre_p<true>, dim3(g), dim3(KBS), 0, s, D, A32, (const float*)H.w0, (const float*)H.g1,
+                                   (const double*)r, (const double*)pc, pn, (const double*)v, ph, (const double*)partC, g, st, lc);
+            }
             else
                 hipLaunchKernelGGL(k_mg_pre_p<false>, dim3(g), dim3(KBS), 0, s, D, A32, (const float*)H.w0, H.g1, (const double*)r,
                                    (const double*)pc, pn, (const double*)v, ph, (const double*)partC, g, st, lc);
             mgCycleRest(c, A32, pn, ph, st);
             hipLaunchKernelGGL(k_mg_v, dim3(g), dim3(KBS), 0, s, D, A64, (const float*)ph, (const double*)rh, v, partA,
                                (const double*)st);
-            if (ghosts)
-                hipLaunchKernelGGL(k_mg_pre_s<true>, dim3(g), dim3(KBS), 0, s, D, A32, (const float*)H.w0, H.g1, (const double*)r,
-                                   (const double*)v, sv, sh, (const double*)partA, g, st);
+            if (ghosts) {
+                hipLaunchKernelGGL(k_mg_first_s, dim3(gg), dim3(MBS), 0, s, c.ov.n, (const int*)c.ov.cell, A32,
+                                   (const float*)H.w0, H.g1, (const double*)r, (const double*)v, (const double*)partA, g,
+                                   (const double*)st);
+                hipLaunchKernelGGL(k_mg_pre_s<true>, dim3(g), dim3(KBS), 0, s, D, A32, (const float*)H.w0, (const float*)H.g1,
+                                   (const double*)r, (const double*)v, sv, sh, (const double*)partA, g, st);
+            }
             else
                 hipLaunchKernelGGL(k_mg_pre_s<false>, dim3(g), dim3(KBS), 0, s, D, A32, (const float*)H.w0, H.g1, (const double*)r,
                                    (const double*)v, sv, sh, (const double*)partA, g, st);
