@@ -66,6 +66,7 @@ struct MgHier {
     float* w0 = nullptr;             // level 0 Jacobi weights
     float* g1 = nullptr;             // first-sweep values of the ghost cells, one per ghost row
     int g1n = 0;
+    int lastIts = 0;                 // iterations of the last solve's first round
 };
 
 struct Ctx {

@@ -788,7 +788,7 @@ static void mgCycleRest(Ctx& c, const Sym4<float>& A0, const double* b, float* z
 
 // BiCGStab with the cycle on A e = b from e = 0, to ||r|| <= rtol ||b|| on its recurrence.
 // Returns the iteration count; done holds 1 (converged), 2 (breakdown) or 0 (cap).
-static int mgRun(Ctx& c, const double* b, double* e, double rtol, int maxit, int* doneOut) {
+static int mgRun(Ctx& c, const double* b, double* e, double rtol, int maxit, int firstChunk, int* doneOut) {
     MgHier& H = c.mg;
     const Dims& D = c.D;
     const int P = D.P;
@@ -812,7 +812,7 @@ static int mgRun(Ctx& c, const double* b, double* e, double rtol, int maxit, int
     LoopCtl lc{rtol * rtol, maxit, 0.};
     hipLaunchKernelGGL(k_mg_init, dim3(g), dim3(KBS), 0, s, D, (const int8_t*)c.st, b, r, rh, p0, p1, v, e, partC, st);
     double *pc = p0, *pn = p1;
-    int it = 0, chunk = 4;
+    int it = 0, chunk = firstChunk;
     while (true) {
         for (int q = 0; q < chunk; ++q) {
             if (ghosts) {
@@ -881,7 +881,12 @@ int mg_bicgstab(Ctx& c, double shift, double rtol, int maxit, int* refs, double*
     // from the true one, and a breakdown restarts from the true residual
     while (rel > rtol && its < maxit && *refs < 30) {
         const double tol = std::min(0.5, 0.5 * rtol / rel);
-        its += mgRun(c, r64, e, tol, maxit - its, &done);
+        // the first round queues one iteration fewer than the last solve's first round needed
+        // (one more read-back costs less than an iteration that returns at once: ~55 launches)
+        const int first = *refs == 0 ? (c.mg.lastIts > 1 ? c.mg.lastIts - 1 : 4) : 2;
+        const int it = mgRun(c, r64, e, tol, maxit - its, first, &done);
+        if (*refs == 0) c.mg.lastIts = it;
+        its += it;
         add_vec(c, c.xp, e, P);
         const double prev = rel;
         pcorr_true_resid(c, shift, &rr, &bb);
