@@ -126,10 +126,11 @@ def workload_bench(a):
       skewed      256^3 cavity whose interior nodes move by up to 0.15 h (seeded Philox),
                   Re 1000, PISO 2 correctors, dt 0.5/256.
     One step is Piso::solve / IbPiso::solve.  The p' solves of the timed steps use --mode
-    (auto: the multigrid BiCGStab, the fastest of the methods on both systems).  After the
-    timed steps the same p' system is solved by each method to a true FP64 relres of 1e-8.
-    The roofline names the dominant p' kernel of the timed steps; `device_bytes` is the
-    context's HBM footprint."""
+    (auto: the learned FCG on the skewed cavity, as that configuration names it; the
+    multigrid BiCGStab on the IB channel's ghost-row systems, which the learned cycle does
+    not take).  After the timed steps the same p' system is solved by each method to a true
+    FP64 relres of 1e-8.  The roofline names the dominant p' kernel of the timed steps;
+    `device_bytes` is the context's HBM footprint."""
     from caffe_old_amd import _lib as L
     from caffe_old_amd import case as cs
     from caffe_old_amd.learned import PressureNet, trained_weights_path
@@ -151,7 +152,7 @@ def workload_bench(a):
         # the network trained on 128^3 cavities, applied on the 128^3 coarse grid of this mesh
         net = PressureNet(seed=0).load(wpath)
         s.attach_net(net)
-    mode_name = a.mode if a.mode != "auto" else "mg"
+    mode_name = a.mode if a.mode != "auto" else ("mg" if ib or net is None else "learned")
     modes = {"ir": L.SOLVE_IR_FP32, "plain": L.SOLVE_PLAIN_FP64, "learned": L.SOLVE_LEARNED_FCG, "mg": L.SOLVE_MG}
     inner = {"ir": 1e-2, "plain": 1e-2, "learned": 1e-4, "mg": 1e-2}
     s.set_solve_opts("momentum", mode=L.SOLVE_IR_FP32 if not ib else L.SOLVE_PLAIN_FP64, rtol=1e-8, maxit=5000)
