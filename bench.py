@@ -155,7 +155,7 @@ def workload_bench(a):
     mode_name = a.mode if a.mode != "auto" else ("mg" if ib or net is None else "learned")
     modes = {"ir": L.SOLVE_IR_FP32, "plain": L.SOLVE_PLAIN_FP64, "learned": L.SOLVE_LEARNED_FCG, "mg": L.SOLVE_MG}
     inner = {"ir": 1e-2, "plain": 1e-2, "learned": 1e-4, "mg": 1e-2}
-    s.set_solve_opts("momentum", mode=L.SOLVE_IR_FP32 if not ib else L.SOLVE_PLAIN_FP64, rtol=1e-8, maxit=5000)
+    s.set_solve_opts("momentum", mode=L.SOLVE_IR_FP32, rtol=1e-8, maxit=5000)
     s.set_solve_opts("pcorr", mode=modes[mode_name], rtol=1e-8, maxit=200000, inner_rtol=inner[mode_name])
     if ib:
         kname = "pcorr_bicg_a_fp64" if mode_name == "plain" else "mg_post0"

@@ -244,11 +244,28 @@ __global__ void k_bi_b(Dims D, Op A, const T* r, const T* v, T* sv, T* tv, const
     writePartials<KBS, 2 * NV>(acc, partOut);
 }
 
-// KA and KB for the three momentum components on the band without ghost rows: the
-// seven coefficients and D^-1 at the seven stencil points are loaded once per cell for
-// the three components (the generic kernels reload them per component, as their stores
-// may alias them).  Same arithmetic as Band7::apply.
-template <class T>
+// KA and KB for the three momentum components on the band: the seven coefficients and
+// D^-1 at the seven stencil points are loaded once per cell for the three components (the
+// generic kernels reload them per component, as their stores may alias them).  Same
+// arithmetic as Band7::apply.  G: the system has ghost rows; a ghost cell (about 0.1% of
+// the cells) evaluates its 9-entry row (FP64 values, as BandT::apply reads them) on the
+// operand formed at the row's cells.
+template <class T, bool G>
+__device__ __forceinline__ bool bandGhost(const BandT<T>& A, int p, int& r) {
+    r = G ? A.ov.rowOf[p] : -1;
+    return G && r >= 0;
+}
+template <class T, class F>
+__device__ __forceinline__ double ghostApply(const BandT<T>& A, int r, int p, F x) {
+    const double* v = (const double*)A.ov.vals + 9 * r;
+    double s = v[0] * x(p);
+    for (int m = 1; m < 9; ++m) {
+        const int c = A.ov.cols[9 * r + m];
+        if (c >= 0) s += v[m] * x(c);
+    }
+    return s;
+}
+template <class T, bool G>
 __global__ void k_bi_a_band(Dims D, BandT<T> A, const T* __restrict__ r, const T* __restrict__ rh,
                             const T* __restrict__ pold, T* __restrict__ pnew, const T* __restrict__ vold,
                             T* __restrict__ vnew, const T* __restrict__ dinv, const double* partIn, int nb,
@@ -266,6 +283,8 @@ __global__ void k_bi_a_band(Dims D, BandT<T> A, const T* __restrict__ r, const T
         double a[7], di[7];
 #pragma unroll
         for (int m = 0; m < 7; ++m) { a[m] = (double)A.A[m * P + p]; di[m] = (double)dinv[p + o[m]]; }
+        int gr;
+        const bool ghost = bandGhost<T, G>(A, p, gr);
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
             if (!live[s]) continue;
@@ -277,15 +296,21 @@ __global__ void k_bi_a_band(Dims D, BandT<T> A, const T* __restrict__ r, const T
                 if (m == 0) pn = pf;
                 ph[m] = pf * di[m];
             }
-            const double vn = a[0] * ph[0] + a[1] * ph[1] + a[2] * ph[2] + a[3] * ph[3] + a[4] * ph[4] + a[5] * ph[5]
-                            + a[6] * ph[6];
+            double vn = a[0] * ph[0] + a[1] * ph[1] + a[2] * ph[2] + a[3] * ph[3] + a[4] * ph[4] + a[5] * ph[5]
+                      + a[6] * ph[6];
+            if (ghost) {
+                const int so = s * P;
+                vn = ghostApply(A, gr, p, [&](int c) {
+                    return ((double)r[so + c] + beta[s] * ((double)pold[so + c] - om[s] * (double)vold[so + c])) * (double)dinv[c];
+                });
+            }
             pnew[b] = (T)pn; vnew[b] = (T)vn;
             acc[s] += (double)rh[b] * vn;
         }
     }
     writePartials<KBS, 3>(acc, partOut);
 }
-template <class T>
+template <class T, bool G>
 __global__ void k_bi_b_band(Dims D, BandT<T> A, const T* __restrict__ r, const T* __restrict__ v, T* __restrict__ sv,
                             T* __restrict__ tv, const T* __restrict__ dinv, const double* partIn, int nb,
                             double* partOut, double* st) {
@@ -302,6 +327,8 @@ __global__ void k_bi_b_band(Dims D, BandT<T> A, const T* __restrict__ r, const T
         double a[7], di[7];
 #pragma unroll
         for (int m = 0; m < 7; ++m) { a[m] = (double)A.A[m * P + p]; di[m] = (double)dinv[p + o[m]]; }
+        int gr;
+        const bool ghost = bandGhost<T, G>(A, p, gr);
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
             if (!live[s]) continue;
@@ -313,8 +340,14 @@ __global__ void k_bi_b_band(Dims D, BandT<T> A, const T* __restrict__ r, const T
                 if (m == 0) ss = sf;
                 sh[m] = sf * di[m];
             }
-            const double tt = a[0] * sh[0] + a[1] * sh[1] + a[2] * sh[2] + a[3] * sh[3] + a[4] * sh[4] + a[5] * sh[5]
-                            + a[6] * sh[6];
+            double tt = a[0] * sh[0] + a[1] * sh[1] + a[2] * sh[2] + a[3] * sh[3] + a[4] * sh[4] + a[5] * sh[5]
+                      + a[6] * sh[6];
+            if (ghost) {
+                const int so = s * P;
+                tt = ghostApply(A, gr, p, [&](int c) {
+                    return ((double)r[so + c] - al[s] * (double)v[so + c]) * (double)dinv[c];
+                });
+            }
             sv[b] = (T)ss; tv[b] = (T)tt;
             acc[2 * s] += tt * ss;
             acc[2 * s + 1] += tt * tt;
@@ -504,13 +537,15 @@ static int runBicgstab(Ctx& c, Op A, const double* b, double shift, T* x, T* w, 
         for (int q = 0; q < chunk; ++q) {
             if constexpr (NV == 3 && std::is_same<Op, BandT<T>>::value) {
                 if (!A.ov.rowOf) {
-                    hipLaunchKernelGGL((k_bi_a_band<T>), dim3(g), dim3(KBS), 0, c.stream, D, A, (const T*)r, (const T*)rh,
+                    hipLaunchKernelGGL((k_bi_a_band<T, false>), dim3(g), dim3(KBS), 0, c.stream, D, A, (const T*)r, (const T*)rh,
                                        (const T*)pc, pn, (const T*)vc, vn, (const T*)dinv, (const double*)partC, g, partA, st, lc);
-                    hipLaunchKernelGGL((k_bi_b_band<T>), dim3(g), dim3(KBS), 0, c.stream, D, A, (const T*)r, (const T*)vn, s, t,
+                    hipLaunchKernelGGL((k_bi_b_band<T, false>), dim3(g), dim3(KBS), 0, c.stream, D, A, (const T*)r, (const T*)vn, s, t,
                                        (const T*)dinv, (const double*)partA, g, partB, st);
                 } else {
-                    hipLaunchKernelGGL((k_bi_a<NV, T, Op>), dim3(g), dim3(KBS), 0, c.stream, D, A, r, rh, pc, pn, vc, vn, dinv, partC, g, partA, st, lc);
-                    hipLaunchKernelGGL((k_bi_b<NV, T, Op>), dim3(g), dim3(KBS), 0, c.stream, D, A, r, vn, s, t, dinv, partA, g, partB, st);
+                    hipLaunchKernelGGL((k_bi_a_band<T, true>), dim3(g), dim3(KBS), 0, c.stream, D, A, (const T*)r, (const T*)rh,
+                                       (const T*)pc, pn, (const T*)vc, vn, (const T*)dinv, (const double*)partC, g, partA, st, lc);
+                    hipLaunchKernelGGL((k_bi_b_band<T, true>), dim3(g), dim3(KBS), 0, c.stream, D, A, (const T*)r, (const T*)vn, s, t,
+                                       (const T*)dinv, (const double*)partA, g, partB, st);
                 }
             } else {
                 hipEvent_t e0, e1;
@@ -549,7 +584,7 @@ __global__ void k_add_f32_3(Dims D, double* x, const float* e) {
     }
 }
 
-// FP32 BiCGStab rounds inside FP64 refinement (systems without ghost rows).  x starts at
+// FP32 BiCGStab rounds inside FP64 refinement (ghost rows in FP64 values).  x starts at
 // zero; each round solves A dx = r64 for the three components in FP32 from dx = 0 to a
 // relative tolerance of at most 1e-5 (or only what is still needed), adds dx in FP64
 // and recomputes the true FP64 residual, until the worst component meets rtol.
@@ -562,7 +597,8 @@ static int momentumIR(Ctx& c, const caffe_solve_opts& o, int* rounds, double* re
     }
     hipLaunchKernelGGL(k_cast_f32, dim3(blocksFor(7 * P, KBS)), dim3(KBS), 0, c.stream, c.Am, c.Am32, 7 * P);
     Band7 A = bandM(c);
-    BandT<float> A32{c.Am32, P, D.sJ, D.sK, OvView{nullptr, nullptr, nullptr}, c.st};
+    // ghost rows keep their FP64 values (BandT reads the overlay in FP64 whatever T is)
+    BandT<float> A32{c.Am32, P, D.sJ, D.sK, ovM(c), c.st};
     float* w = c.wm32;
     float* dx = c.wm32 + 30 * (size_t)P;
     double* rt = c.wm + 33 * (size_t)P;
@@ -600,7 +636,7 @@ int momentum_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep) {
     HIPCHECK(hipEventRecord(c.ev0, c.stream));
     const int P = c.D.P;
     Band7 A = bandM(c);
-    if (o.mode == CAFFE_SOLVE_IR_FP32 && !c.hasIB) {
+    if (o.mode == CAFFE_SOLVE_IR_FP32) {
         int rounds = 0, fell = 0;
         double rel = 0.;
         int iters = momentumIR(c, o, &rounds, &rel);
