@@ -51,6 +51,7 @@ void freeAll(Ctx& c) {
     c.deviceBytes = 0;
     c.wb32 = nullptr; c.Am32 = nullptr; c.wm32 = nullptr;
     c.ov = Overlay();
+    if (c.mg.iter2) (void)hipGraphExecDestroy(c.mg.iter2);
     c.mg = MgHier();
     c.ready = false;
 }

@@ -67,6 +67,8 @@ struct MgHier {
     float* g1 = nullptr;             // first-sweep values of the ghost cells, one per ghost row
     int g1n = 0;
     int lastIts = 0;                 // iterations of the last solve's first round
+    hipGraphExec_t iter2 = nullptr;  // two BiCGStab iterations, captured once per context
+    bool graphOff = false;           // capture failed: iterations are launched one by one
 };
 
 struct Ctx {

@@ -22,7 +22,10 @@ __device__ inline int cellIdx(const Dims& D, int q) { int i, j, k; return D.cell
 struct LoopCtl { double tol2; int maxit; double nLive; };
 
 // BiCGStab state slots (doubles), B_STRIDE per system
-enum { B_RHOCUR = 0, B_RHOPREV, B_ALPHA, B_OMEGA, B_BB, B_RR, B_IT, B_DONE, B_BRK, B_STRIDE = 16 };
+enum { B_RHOCUR = 0, B_RHOPREV, B_ALPHA, B_OMEGA, B_BB, B_RR, B_IT, B_DONE, B_BRK,
+       B_TOL2, B_MAXIT,   // loop limits kept on the device (the multigrid BiCGStab, whose
+                          // iterations replay as a captured graph)
+       B_STRIDE = 16 };
 
 // The state of the NV systems, loaded by every block in one batch at kernel start, ahead
 // of the partial sums of the previous kernel

@@ -517,7 +517,7 @@ __global__ void k_mg_ghost_post(int n, const int* cell, Sym4<float> A, const dou
 // ------------------------------------------------------------------ BiCGStab kernels
 // r = b on live cells, rh = r, p = v = x = 0; partials (rh, r), (r, r)
 __global__ void k_mg_init(Dims D, const int8_t* stc, const double* b, double* r, double* rh, double* p0, double* p1,
-                          double* v, double* x, double* part, double* st) {
+                          double* v, double* x, double* part, double* st, double tol2, int maxit) {
     double acc[2] = {0., 0.};
     FOR_CELLS(D, q, p) {
         const double bv = stc[p] != ST_INACTIVE ? b[p] : 0.;
@@ -528,18 +528,19 @@ __global__ void k_mg_init(Dims D, const int8_t* stc, const double* b, double* r,
     writePartials<KBS, 2>(acc, part);
     if (BLOCK0) {
         st[B_IT] = 0.; st[B_DONE] = 0.; st[B_BB] = -1.; st[B_RHOPREV] = 1.; st[B_ALPHA] = 1.; st[B_OMEGA] = 1.;
-        st[B_BRK] = 0.;
+        st[B_BRK] = 0.; st[B_TOL2] = tol2; st[B_MAXIT] = (double)maxit;
     }
 }
+__device__ __forceinline__ LoopCtl mgLoop(const double* st) { return LoopCtl{st[B_TOL2], (int)st[B_MAXIT], 0.}; }
 
 // The ghost rows' first sweep ahead of the two level-0 heads: each computes the head's
 // beta (alpha) from the same partial sums and state, without recording anything.
 __global__ void k_mg_first_p(int n, const int* cell, Sym4<float> A, const float* wt, float* g1, const double* r,
-                             const double* pold, const double* v, const double* partIn, int nb, const double* st,
-                             LoopCtl lc) {
+                             const double* pold, const double* v, const double* partIn, int nb, const double* st) {
     BiState<1> S;
     S.load(st);
     if (!S.any()) return;
+    const LoopCtl lc = mgLoop(st);
     double t[2];
     readPartials<MBS, 2>(partIn, nb, t);
     const double rho = t[0], rr = t[1], bb = S.bb[0] < 0. ? rr : S.bb[0];
@@ -566,11 +567,11 @@ __global__ void k_mg_first_s(int n, const int* cell, Sym4<float> A, const float*
 template <bool G>
 __global__ void k_mg_pre_p(Dims D, Sym4<float> A, const float* __restrict__ wt, const float* __restrict__ g1,
                            const double* __restrict__ r, const double* __restrict__ pold, double* __restrict__ pnew,
-                           const double* __restrict__ v, float* __restrict__ z, const double* partIn, int nb, double* st,
-                           LoopCtl lc) {
+                           const double* __restrict__ v, float* __restrict__ z, const double* partIn, int nb, double* st) {
     BiState<1> S;
     S.load(st);
     if (!S.any()) return;
+    const LoopCtl lc = mgLoop(st);
     double beta[1], om[1];
     bool live[1];
     biaHead<1>(partIn, nb, st, lc, S, beta, om, live);
@@ -723,6 +724,11 @@ static void mgBuild(Ctx& c) {
     if (c.hasIB && c.ov.n > H.g1n) {
         H.g1 = falloc(c, (size_t)c.ov.n);
         H.g1n = c.ov.n;
+        if (H.iter2) {   // the captured iterations hold the old pointer
+            HIPCHECK(hipStreamSynchronize(c.stream));
+            HIPCHECK(hipGraphExecDestroy(H.iter2));
+            H.iter2 = nullptr;
+        }
     }
     hipLaunchKernelGGL(k_mg_weights, dim3(blocksFor(c.D.P, MBS)), dim3(MBS), 0, c.stream, (const float*)c.Ap32, c.D.P, H.w0);
     hipLaunchKernelGGL(k_mg_coarsen<FineP>, dim3(lgrid(H.L[1].D)), dim3(MBS), 0, c.stream, FineP{c.Ap, c.st, c.D}, H.L[1]);
@@ -786,9 +792,20 @@ static void mgCycleRest(Ctx& c, const Sym4<float>& A0, const double* b, float* z
     }
 }
 
-// BiCGStab with the cycle on A e = b from e = 0, to ||r|| <= rtol ||b|| on its recurrence.
-// Returns the iteration count; done holds 1 (converged), 2 (breakdown) or 0 (cap).
-static int mgRun(Ctx& c, const double* b, double* e, double rtol, int maxit, int firstChunk, int* doneOut) {
+// The buffers of the multigrid BiCGStab: FP64 vectors in the momentum work block (double
+// views only, halos stay zero), FP32 preconditioned vectors on the fine grid
+struct MgVecs {
+    double *r, *rh, *p0, *p1, *v, *sv, *tv;
+    float *ph, *sh;
+};
+static MgVecs mgVecs(Ctx& c) {
+    const size_t P = (size_t)c.D.P;
+    double* W = c.wm;
+    return MgVecs{W, W + P, W + 2 * P, W + 3 * P, W + 4 * P, W + 5 * P, W + 6 * P, c.mg.z0[0], c.mg.z0[1]};
+}
+
+// one BiCGStab iteration: direction p from pc into pn, both V-cycles, the update of e
+static void mgIteration(Ctx& c, const MgVecs& V, const double* pc, double* pn, double* e) {
     MgHier& H = c.mg;
     const Dims& D = c.D;
     const int P = D.P;
@@ -797,11 +814,6 @@ static int mgRun(Ctx& c, const double* b, double* e, double rtol, int maxit, int
     const OvView ov32{c.hasIB ? c.ov.rowOf : nullptr, c.ov.cols, c.ov.valP32};
     const Sym4<double> A64{c.Ap, P, D.sJ, D.sK, ov64, c.st};
     const Sym4<float> A32{c.Ap32, P, D.sJ, D.sK, ov32, c.st};
-    // FP64 vectors in the momentum work block (double views only, halos stay zero)
-    double* W = c.wm;
-    double *r = W, *rh = W + P, *p0 = W + 2 * (size_t)P, *p1 = W + 3 * (size_t)P, *v = W + 4 * (size_t)P;
-    double *sv = W + 5 * (size_t)P, *tv = W + 6 * (size_t)P;
-    float *ph = H.z0[0], *sh = H.z0[1];
     double* st = c.scal + SC_BI;
     double* partC = c.red.part;
     double* partA = c.red.part + MAX_RED_BLOCKS * RED_STRIDE;
@@ -809,41 +821,84 @@ static int mgRun(Ctx& c, const double* b, double* e, double rtol, int maxit, int
     const int g = kgrid(D);
     const bool ghosts = A32.ov.rowOf && c.ov.n > 0;
     const int gg = blocksFor(c.ov.n, MBS);
-    LoopCtl lc{rtol * rtol, maxit, 0.};
-    hipLaunchKernelGGL(k_mg_init, dim3(g), dim3(KBS), 0, s, D, (const int8_t*)c.st, b, r, rh, p0, p1, v, e, partC, st);
-    double *pc = p0, *pn = p1;
+    if (ghosts) {
+        hipLaunchKernelGGL(k_mg_first_p, dim3(gg), dim3(MBS), 0, s, c.ov.n, (const int*)c.ov.cell, A32, (const float*)H.w0,
+                           H.g1, (const double*)V.r, pc, (const double*)V.v, (const double*)partC, g, (const double*)st);
+        hipLaunchKernelGGL(k_mg_pre_p<true>, dim3(g), dim3(KBS), 0, s, D, A32, (const float*)H.w0, (const float*)H.g1,
+                           (const double*)V.r, pc, pn, (const double*)V.v, V.ph, (const double*)partC, g, st);
+    } else {
+        hipLaunchKernelGGL(k_mg_pre_p<false>, dim3(g), dim3(KBS), 0, s, D, A32, (const float*)H.w0, (const float*)H.g1,
+                           (const double*)V.r, pc, pn, (const double*)V.v, V.ph, (const double*)partC, g, st);
+    }
+    mgCycleRest(c, A32, pn, V.ph, st);
+    hipLaunchKernelGGL(k_mg_v, dim3(g), dim3(KBS), 0, s, D, A64, (const float*)V.ph, (const double*)V.rh, V.v, partA,
+                       (const double*)st);
+    if (ghosts) {
+        hipLaunchKernelGGL(k_mg_first_s, dim3(gg), dim3(MBS), 0, s, c.ov.n, (const int*)c.ov.cell, A32, (const float*)H.w0,
+                           H.g1, (const double*)V.r, (const double*)V.v, (const double*)partA, g, (const double*)st);
+        hipLaunchKernelGGL(k_mg_pre_s<true>, dim3(g), dim3(KBS), 0, s, D, A32, (const float*)H.w0, (const float*)H.g1,
+                           (const double*)V.r, (const double*)V.v, V.sv, V.sh, (const double*)partA, g, st);
+    } else {
+        hipLaunchKernelGGL(k_mg_pre_s<false>, dim3(g), dim3(KBS), 0, s, D, A32, (const float*)H.w0, (const float*)H.g1,
+                           (const double*)V.r, (const double*)V.v, V.sv, V.sh, (const double*)partA, g, st);
+    }
+    mgCycleRest(c, A32, V.sv, V.sh, st);
+    hipLaunchKernelGGL(k_mg_t, dim3(g), dim3(KBS), 0, s, D, A64, (const float*)V.sh, (const double*)V.sv, V.tv, partB,
+                       (const double*)st);
+    hipLaunchKernelGGL(k_mg_x, dim3(g), dim3(KBS), 0, s, D, e, V.r, (const double*)V.rh, (const float*)V.ph,
+                       (const float*)V.sh, (const double*)V.sv, (const double*)V.tv, (const double*)partB, g, partC, st);
+}
+
+// Two iterations (the direction buffers back at their starting parity) captured once per
+// context as a HIP graph: every pointer and launch shape in them is fixed per context, and
+// the loop limits live in the device state.  The graph holds kernel launches only.  A
+// capture that fails leaves the iterations launched one by one.
+static bool mgGraph(Ctx& c, const MgVecs& V, double* e) {
+    MgHier& H = c.mg;
+    if (H.iter2) return true;
+    if (H.graphOff) return false;
+    hipStream_t s = c.stream;
+    hipGraph_t graph = nullptr;
+    bool ok = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (ok) {
+        mgIteration(c, V, V.p0, V.p1, e);
+        mgIteration(c, V, V.p1, V.p0, e);
+        ok = hipStreamEndCapture(s, &graph) == hipSuccess && graph;
+    }
+    if (ok) ok = hipGraphInstantiate(&H.iter2, graph, nullptr, nullptr, 0) == hipSuccess;
+    if (graph) (void)hipGraphDestroy(graph);
+    if (!ok) {
+        (void)hipGetLastError();
+        H.iter2 = nullptr;
+        H.graphOff = true;
+    }
+    return ok;
+}
+
+// BiCGStab with the cycle on A e = b from e = 0, to ||r|| <= rtol ||b|| on its recurrence.
+// Returns the iteration count; done holds 1 (converged), 2 (breakdown) or 0 (cap).  The
+// iterations replay the captured graph, two at a time, except while a kernel probe records
+// (its events cannot be part of a graph).
+static int mgRun(Ctx& c, const double* b, double* e, double rtol, int maxit, int firstChunk, int* doneOut) {
+    const Dims& D = c.D;
+    hipStream_t s = c.stream;
+    const MgVecs V = mgVecs(c);
+    double* st = c.scal + SC_BI;
+    const int g = kgrid(D);
+    const double tol2 = rtol * rtol;
+    hipLaunchKernelGGL(k_mg_init, dim3(g), dim3(KBS), 0, s, D, (const int8_t*)c.st, b, V.r, V.rh, V.p0, V.p1, V.v, e,
+                       c.red.part, st, tol2, maxit);
+    const bool graph = c.probe.which == PROBE_NONE && mgGraph(c, V, e);
+    double *pc = V.p0, *pn = V.p1;
     int it = 0, chunk = firstChunk;
     while (true) {
-        for (int q = 0; q < chunk; ++q) {
-            if (ghosts) {
-                hipLaunchKernelGGL(k_mg_first_p, dim3(gg), dim3(MBS), 0, s, c.ov.n, (const int*)c.ov.cell, A32,
-                                   (const float*)H.w0, H.g1, (const double*)r, (const double*)pc, (const double*)v,
-                                   (const double*)partC, g, (const double*)st, lc);
-                hipLaunchKernelGGL(k_mg_pre_p<true>, dim3(g), dim3(KBS), 0, s, D, A32, (const float*)H.w0, (const float*)H.g1,
-                                   (const double*)r, (const double*)pc, pn, (const double*)v, ph, (const double*)partC, g, st, lc);
+        if (graph) {
+            for (int q = 0; q < (chunk + 1) / 2; ++q) HIPCHECK(hipGraphLaunch(c.mg.iter2, s));
+        } else {
+            for (int q = 0; q < chunk; ++q) {
+                mgIteration(c, V, pc, pn, e);
+                std::swap(pc, pn);
             }
-            else
-                hipLaunchKernelGGL(k_mg_pre_p<false>, dim3(g), dim3(KBS), 0, s, D, A32, (const float*)H.w0, H.g1, (const double*)r,
-                                   (const double*)pc, pn, (const double*)v, ph, (const double*)partC, g, st, lc);
-            mgCycleRest(c, A32, pn, ph, st);
-            hipLaunchKernelGGL(k_mg_v, dim3(g), dim3(KBS), 0, s, D, A64, (const float*)ph, (const double*)rh, v, partA,
-                               (const double*)st);
-            if (ghosts) {
-                hipLaunchKernelGGL(k_mg_first_s, dim3(gg), dim3(MBS), 0, s, c.ov.n, (const int*)c.ov.cell, A32,
-                                   (const float*)H.w0, H.g1, (const double*)r, (const double*)v, (const double*)partA, g,
-                                   (const double*)st);
-                hipLaunchKernelGGL(k_mg_pre_s<true>, dim3(g), dim3(KBS), 0, s, D, A32, (const float*)H.w0, (const float*)H.g1,
-                                   (const double*)r, (const double*)v, sv, sh, (const double*)partA, g, st);
-            }
-            else
-                hipLaunchKernelGGL(k_mg_pre_s<false>, dim3(g), dim3(KBS), 0, s, D, A32, (const float*)H.w0, H.g1, (const double*)r,
-                                   (const double*)v, sv, sh, (const double*)partA, g, st);
-            mgCycleRest(c, A32, sv, sh, st);
-            hipLaunchKernelGGL(k_mg_t, dim3(g), dim3(KBS), 0, s, D, A64, (const float*)sh, (const double*)sv, tv, partB,
-                               (const double*)st);
-            hipLaunchKernelGGL(k_mg_x, dim3(g), dim3(KBS), 0, s, D, e, r, (const double*)rh, (const float*)ph,
-                               (const float*)sh, (const double*)sv, (const double*)tv, (const double*)partB, g, partC, st);
-            std::swap(pc, pn);
         }
         HIPCHECK(hipMemcpyAsync(c.hscal + SC_BI, st, B_STRIDE * sizeof(double), hipMemcpyDeviceToHost, s));
         HIPCHECK(hipStreamSynchronize(s));
@@ -856,7 +911,7 @@ static int mgRun(Ctx& c, const double* b, double* e, double rtol, int maxit, int
         chunk = 4;
         if (it > 1 && rr0 > 0. && bb0 > 0. && rr0 < bb0) {
             const double rate = std::log(rr0 / bb0) / (it - 1);
-            const double need = std::log(lc.tol2 * bb0 / rr0) / rate - 1.;
+            const double need = std::log(tol2 * bb0 / rr0) / rate - 1.;
             chunk = std::max(1, std::min(16, (int)std::ceil(need)));
         }
     }

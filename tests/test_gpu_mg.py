@@ -101,3 +101,37 @@ def test_mg_iterations_vs_jacobi():
     assert res["plain"]["converged"] == 1 and res["mg"]["converged"] == 1, res
     assert res["mg"]["fell_back"] == 0
     assert 5 * res["mg"]["iters"] <= res["plain"]["iters"], res
+
+
+@pytest.mark.parametrize("name", ["ib_channel", "cavity"])
+def test_mg_graph_replays_match_eager(name):
+    """The multigrid BiCGStab replays its iterations as a captured HIP graph; with a kernel
+    probe recording, the same iterations are launched one by one.  The same p' system is
+    solved several times, alternating the two, with the eager solves and synchronisations
+    in between: every graph replay gives the eager solution bit for bit."""
+    from caffe_old_amd.solver import make_solver
+    if name == "ib_channel":
+        case = cs.channel_ib_sphere(64, 16, 16, steps=2)
+        case.sphere = (0.2, (1.0, 0.5, 0.5))
+    else:
+        case = cs.lid_cavity(24, 1e-3, 1.0, "piso", alpha_u=1.0, alpha_p=1.0, time_accurate=True, dt=0.02, n_pcorr=2,
+                             max_iterations=2)
+    print(f"case: {name}, graph replays against eager launches", flush=True)
+    g = make_solver(case)
+    g.set_solve_opts("momentum", rtol=1e-10, maxit=5000)
+    g.set_solve_opts("pcorr", mode=L.SOLVE_PLAIN_FP64, rtol=1e-10, maxit=50000)
+    g.solve()
+    ref = None
+    for k in range(6):
+        eager = k % 2 == 1
+        g.kernel_probe("mg_post0", eager)
+        g.pcorr_assemble()
+        r = g.pcorr_solve(mode=L.SOLVE_MG, rtol=1e-10, maxit=50000)
+        g.sync()
+        x = g.get("pCorr").copy()
+        assert r["converged"] == 1 and r["fell_back"] == 0, r
+        if ref is None:
+            ref = (x, r["iters"])
+        assert r["iters"] == ref[1] and np.array_equal(x, ref[0]), (k, r)
+    g.kernel_probe("mg_post0", False)
+    g.close()
