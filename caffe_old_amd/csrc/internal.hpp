@@ -204,7 +204,7 @@ int coupled_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep);
 void pcorr_make_fp32(Ctx& c);
 // ---- multigrid (mg.hip): BiCGStab on the p' system with the FP32 V-cycle as right
 // preconditioner, FP64 recurrences restarted from the true residual; continues from c.xp.
-// Returns the iteration count, or -1 when the system does not qualify (64 cells or fewer,
+// Returns the iteration count, or -1 when the system does not qualify (8 cells or fewer,
 // or an indefinite system)
 int mg_bicgstab(Ctx& c, double shift, double rtol, int maxit, int* refs, double* relOut);
 void add_vec(Ctx& c, double* x, const double* d, int n);

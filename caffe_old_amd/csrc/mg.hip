@@ -19,7 +19,7 @@
 //     made of inactive cells only is an identity row;
 //   - restriction sums the fine residuals of an aggregate, prolongation is cell-centred
 //     trilinear (3/4 and 1/4 per direction);
-//   - the coarsest level (64 cells or fewer) is solved with its dense inverse, formed once
+//   - the coarsest level (8 cells or fewer) is solved with its dense inverse, formed once
 //     per solve by one workgroup.
 // Fixed sweep counts and the exact coarse solve make the cycle a fixed linear map, which
 // BiCGStab requires of its preconditioner.
@@ -36,7 +36,8 @@ namespace caffe {
 
 constexpr int MBS = 256;          // block of the per-level kernels (one thread per cell)
 constexpr double MG_W = 0.8;      // Jacobi damping
-constexpr int MG_COARSEST = 64;   // cells of the level solved densely
+constexpr int MG_COARSEST = 8;    // cells of the level solved densely (a 64-cell level took
+                                  // 262 us to invert in one workgroup; 8 cells take a few)
 
 __host__ __device__ inline Sym4<float> levOp(const MgLevel& L) {
     return Sym4<float>{L.A, L.D.P, L.D.sJ, L.D.sK, OvView{nullptr, nullptr, nullptr}, nullptr};
@@ -204,7 +205,7 @@ __global__ void k_mg_coarsen(Fine F, MgLevel c) {
     c.A[3 * P + pc] = (float)lo[2];
 }
 
-// Dense inverse of the coarsest operator (n <= 64 cells) by in-place Gauss-Jordan without
+// Dense inverse of the coarsest operator (n <= MG_COARSEST cells) by in-place Gauss-Jordan without
 // pivoting (the operator is symmetric negative definite, or made so by grounding cell 0
 // when the system has no fixed-pressure face); one workgroup, inverse stored row-major.
 __global__ void k_mg_dense(MgLevel L, int n, int ground, double* inv) {
@@ -681,8 +682,8 @@ static Dims mkDims(int nI, int nJ, int nK) {
 }
 
 // Levels for the context's grid, allocated once per grid: each level halves every
-// dimension of more than 2 cells (rounding up) until 64 cells or fewer remain.  Returns
-// false when the grid itself is that small.
+// dimension of more than 2 cells (rounding up) until MG_COARSEST cells or fewer remain.
+// Returns false when the grid itself is that small.
 static bool mgPlan(Ctx& c) {
     MgHier& H = c.mg;
     if (H.nl > 0) return H.nl > 1;
