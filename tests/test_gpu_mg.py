@@ -8,6 +8,7 @@ Each case also checks that the multigrid path ran (no fallback to Jacobi BiCGSta
 Cases:
   * the immersed-sphere channel: nonsymmetric ghost rows, a fixed-pressure outlet;
   * the closed cavity: a singular pure-Neumann system, so the coarsest level is grounded;
+  * the closed cavity with an immersed sphere: ghost rows, and coarse levels singular;
   * a cavity of odd sizes 13 x 7 x 5: uneven aggregates, and directions of 2 cells or
     fewer are not coarsened;
   * the skewed mesh: general metrics.
@@ -61,6 +62,21 @@ def test_mg_closed_cavity_singular():
     case = cs.lid_cavity(10, 0.01, 1.0, "simple", alpha_u=0.7, alpha_p=0.3, max_iterations=6)
     print("case: closed cavity 10^3 (singular p'), multigrid p' solves", flush=True)
     g, o, errs = _run(case, uniform=True)
+    assert max(errs.values()) <= 1e-6, errs
+
+
+def test_mg_ib_closed_cavity():
+    """Ghost rows in a closed box: no face fixes the pressure, so the coarse levels are
+    singular and grounded, while the ghost rows that reach into the solid anchor the fine
+    system."""
+    case = cs.lid_cavity(16, 1e-2, 1.0, "piso", alpha_u=1.0, alpha_p=1.0, time_accurate=True, dt=0.02, n_pcorr=2,
+                         max_iterations=3)
+    case.algo = "ibpiso"
+    case.sphere = (0.2, (0.5, 0.45, 0.5))
+    print("case: closed cavity 16^3 with an immersed sphere, multigrid p' solves", flush=True)
+    g, o, errs = _run(case, uniform=True)
+    p = g.reports()[1]
+    assert p["iters"] <= 30, p
     assert max(errs.values()) <= 1e-6, errs
 
 
