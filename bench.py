@@ -58,6 +58,47 @@ def cpu_baseline(n, steps=2, warmup=1):
                       f"oracle BiCGStab+SSOR rtol 1e-5, {th} OpenMP threads, {dt:.1f} s timed"}
 
 
+def field_errors(n, mode, inner, net, mom_mode, steps=2):
+    """Part of the CPU leg, the oracle as the checker: the error of every field against the
+    FP64 oracle on this configuration.  A fresh GPU run with the timed run's solve settings
+    (true relres 1e-8) and the oracle with tight solves (true relres 1e-12) each take `steps`
+    PISO steps from rest; relative L2 over the interior cells, p after removing the mean (the
+    pure-Neumann pressure is defined up to a constant)."""
+    import numpy as np
+    from caffe_old_amd import case as cs
+    from caffe_old_amd.solver import make_solver
+    from oracle.pyoracle import Oracle, interior
+    th = _cpu_threads()
+    c = cs.cavity_piso_128(n, steps=steps)
+    g = make_solver(c)
+    g.set_solve_opts("momentum", mode=mom_mode, rtol=1e-8, maxit=5000)
+    g.set_solve_opts("pcorr", mode=mode, rtol=1e-8, maxit=100000, inner_rtol=inner)
+    if net is not None:
+        g.attach_net(net)
+    # at most 8 SSOR blocks: with 16 the oracle's BiCGStab broke down on the 128^3 p' system
+    o = Oracle(c, rtol=1e-12, maxit=20000, n_blocks=min(th, 8), n_threads=th)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.solve()
+        o.solve()
+    sh = g.shape
+    ug, uo = interior(g.get("u"), sh), interior(o.get("u"), sh)
+    pg, po = interior(g.get("p"), sh), interior(o.get("p"), sh)
+
+    def rel(x, y):
+        ny = float(np.linalg.norm(y))
+        if not np.isfinite(ny):
+            return float("nan")
+        return float(np.linalg.norm(x - y)) / ny if ny > 0 else float(np.linalg.norm(x))
+
+    out = {comp: rel(ug[i], uo[i]) for i, comp in enumerate("uvw")}
+    out["p"] = rel(pg - pg.mean(), po - po.mean())
+    g.close()
+    out.update({"steps": steps, "gpu_rtol": 1e-8, "oracle_rtol": 1e-12, "norm": "relative L2 over interior cells",
+                "seconds": round(time.perf_counter() - t0, 1)})
+    return out
+
+
 def coupled_bench(a):
     """The Coupled solver (Coupled.cpp:38-338) on the n^3 lid-driven cavity (Re 100, steady,
     relaxationFactorMomentum 0.7): one "step" is Coupled::solve, i.e. assembly of the
@@ -366,6 +407,8 @@ def main():
     if not a.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(a.n)
+        log("field errors against the oracle ...")
+        out["field_error_vs_oracle"] = field_errors(a.n, mode, inner[a.mode], net, mom_mode)
     print(json.dumps(out), flush=True)
 
 
