@@ -400,24 +400,17 @@ __global__ void k_f_k1(Dims D, SymF S, ColMap cm, const float* z, float* p0, flo
     // the scalars written by earlier kernels are loaded together, ahead of the partial sums
     const double sDone = st[F_DONE], sIt = st[F_IT], sSq = st[F_SQ], sPq = st[F_PQ], sSr = st[F_SR];
     if (sDone != 0.) return;
-    double t[3];
-    readPartials<FB, 3>(partIn, nb, t);
     const int it = (int)sIt;
-    const double zbar = lc.nLive > 0. ? t[1] / lc.nLive : 0.;
-    const double zq = t[0] - zbar * sSq;
-    const double beta = it == 0 || sPq == 0. ? 0. : -zq / sPq;
     const float* pold = (it & 1) ? p0 : p1;
     float* pnew = (it & 1) ? p1 : p0;
     float* qnew = (it & 1) ? q1 : q0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) { st[F_PAR] = (double)(it & 1); st[F_PR] = t[2] - zbar * sSr; }
-    const float zb = (float)zbar, bt = (float)beta;
-    double v[2] = {0., 0.};
     // Each thread walks its column up k with a rolling window: the k - 1, k and k + 1 values
     // of z and p_old stay in registers, and the loads of the next plane (its four side
     // neighbours, the value above it and the seven matrix entries) are issued before the
     // current cell's stores, which the compiler may not move past them.  (Two planes ahead
-    // measured no faster.)  The arithmetic is SymF::apply's,
-    // in its order.
+    // measured no faster.)  The first column's first loads need no scalar, so they are
+    // issued before the partial sums are read, and the two latencies overlap (at 128^3
+    // every block takes one column tile).  The arithmetic is SymF::apply's, in its order.
     const int P = S.P, sJ = S.sJ, sK = S.sK;
     const float* A = S.A;
     struct Plane { float zs[4], ps[4], zu, pu, a[7]; };   // sides W E S N, k + 1 (u), matrix
@@ -429,17 +422,43 @@ __global__ void k_f_k1(Dims D, SymF S, ColMap cm, const float* z, float* p0, flo
         L.a[0] = A[p]; L.a[1] = A[P + p]; L.a[2] = A[P + p + 1]; L.a[3] = A[2 * P + p];
         L.a[4] = A[2 * P + p + sJ]; L.a[5] = A[3 * P + p]; L.a[6] = A[3 * P + p + sK];
     };
-    auto f = [&](float zz, float pp) { return (zz - zb) + bt * pp; };
-    for (int v_ = xcdChunk(), vn_ = cm.ntx * cm.nty * cm.nseg; v_ < vn_; v_ += (int)gridDim.x) {
+    // work item v: this thread's column (i, j) and its k range [k, ke); false if empty
+    auto item = [&](int v_, int& p, int& k, int& ke) {
         const int t_ = v_ / cm.nseg;
         const int i = (t_ % cm.ntx) * CTX + (int)(threadIdx.x % CTX), j = (t_ / cm.ntx) * CTY + (int)(threadIdx.x / CTX);
-        int k = (v_ % cm.nseg) * cm.kper;
-        const int ke = min(D.nK, k + cm.kper);
-        if (!(i < D.nI && j < D.nJ) || k >= ke) continue;
-        int p = D.pidx(i, j, k);
-        float zd = z[p - sK], pd = pold[p - sK], zc = z[p], pc = pold[p];   // k - 1, k
-        Plane cur, nxt;
+        k = (v_ % cm.nseg) * cm.kper;
+        ke = min(D.nK, k + cm.kper);
+        if (!(i < D.nI && j < D.nJ) || k >= ke) return false;
+        p = D.pidx(i, j, k);
+        return true;
+    };
+    const int v0 = xcdChunk(), vn = cm.ntx * cm.nty * cm.nseg;
+    int p = 0, k = 0, ke = 0;
+    bool live = v0 < vn && item(v0, p, k, ke);
+    float zd = 0.f, pd = 0.f, zc = 0.f, pc = 0.f;   // k - 1, k
+    Plane cur, nxt;
+    if (live) {
+        zd = z[p - sK]; pd = pold[p - sK]; zc = z[p]; pc = pold[p];
         loadPlane(p, cur);
+    }
+    double t[3];
+    readPartials<FB, 3>(partIn, nb, t);
+    const double zbar = lc.nLive > 0. ? t[1] / lc.nLive : 0.;
+    const double zq = t[0] - zbar * sSq;
+    const double beta = it == 0 || sPq == 0. ? 0. : -zq / sPq;
+    if (blockIdx.x == 0 && threadIdx.x == 0) { st[F_PAR] = (double)(it & 1); st[F_PR] = t[2] - zbar * sSr; }
+    const float zb = (float)zbar, bt = (float)beta;
+    double v[2] = {0., 0.};
+    auto f = [&](float zz, float pp) { return (zz - zb) + bt * pp; };
+    for (int v_ = v0; v_ < vn; v_ += (int)gridDim.x) {
+        if (v_ != v0) {
+            live = item(v_, p, k, ke);
+            if (live) {
+                zd = z[p - sK]; pd = pold[p - sK]; zc = z[p]; pc = pold[p];
+                loadPlane(p, cur);
+            }
+        }
+        if (!live) continue;
         for (; k < ke; ++k, p += sK) {
             if (k + 1 < ke) loadPlane(p + sK, nxt);
             const float fc = f(zc, pc);
