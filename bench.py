@@ -102,20 +102,25 @@ def field_errors(n, mode, inner, net, mom_mode, steps=2):
 def coupled_bench(a):
     """The Coupled solver (Coupled.cpp:38-338) on the n^3 lid-driven cavity (Re 100, steady,
     relaxationFactorMomentum 0.7): one "step" is Coupled::solve, i.e. assembly of the
-    4-variable system, its Jacobi BiCGStab solve to a true FP64 relative residual of 1e-8
-    from x = 0 (plus the null-vector gauge of the closed cavity), and the field update.
-    The roofline names KA of the joint BiCGStab (k_cp_a: 56 coefficient planes and the four
-    variables' operands per cell).  The CPU baseline is the oracle with the reference's
+    4-variable system, its BiCGStab solve to a true FP64 relative residual of 1e-8 from
+    x = 0 (plus the null-vector gauge of the closed cavity), and the field update.  The
+    BiCGStab is Jacobi-preconditioned, or with --mode mg block-triangular with a multigrid
+    V-cycle on the pressure block.  The roofline names KA of the Jacobi BiCGStab (k_cp_a: 56
+    coefficient planes and the four variables' operands per cell), or with --mode mg the
+    level-0 post-smoothing sweep of the cycle.  The CPU baseline is the oracle with the reference's
     PETSc-like settings (BiCGStab + block SSOR, rtol 1e-5 on the preconditioned norm)."""
     from caffe_old_amd import case as cs
     from caffe_old_amd.solver import make_solver
+    from caffe_old_amd import _lib as L
     n = a.coupled
+    mg = a.mode == "mg"
+    probe = "mg_post0" if mg else "coupled_bicg_a"
     case = cs.lid_cavity(n, 0.01, algo="coupled", max_iterations=a.steps + a.warmup + 1)
     s = make_solver(case)
-    s.set_solve_opts("pcorr", rtol=1e-8, maxit=200000)
+    s.set_solve_opts("pcorr", mode=L.SOLVE_MG if mg else L.SOLVE_PLAIN_FP64, rtol=1e-8, maxit=200000)
     for _ in range(a.warmup):
         s.solve()
-    s.kernel_probe("coupled_bicg_a", True)
+    s.kernel_probe(probe, True)
     s.sync()
     t0 = time.perf_counter()
     its = []
@@ -125,7 +130,7 @@ def coupled_bench(a):
     s.sync()
     elapsed = time.perf_counter() - t0
     kms, klaunches, kbytes = s.kernel_probe_read()
-    s.kernel_probe("coupled_bicg_a", False)
+    s.kernel_probe(probe, False)
     if a.calibrate_copy:
         s.time_kernel("copy_fp64", reps=5)
     ms_per_step = 1000.0 * elapsed / a.steps
@@ -133,7 +138,7 @@ def coupled_bench(a):
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get(f"coupled_bicg_a_{n}")
+        traffic = json.load(open(tfile)).get(f"{probe}_coupled_{n}" if mg else f"coupled_bicg_a_{n}")
     out = {
         "metric": f"Coupled iterations/s, {n}^3 lid-driven cavity",
         "value": round(1000.0 / ms_per_step, 4), "unit": "iterations/s", "n_gpus": 1, "steps": a.steps,
@@ -141,8 +146,9 @@ def coupled_bench(a):
         "vs_baseline": None, "dtype": "f64", "data": "synthetic: lid-driven cavity from rest (no external data)",
         "config": {"workload": f"3D lid-driven cavity {n}^3 uniform hex, Re 100 (mu 1e-2, lid 1), Coupled solver, steady, "
                                f"relaxationFactorMomentum 0.7, 4-variable system solved to true relres 1e-8",
-                   "cells": case.n_cells, "bicgstab_iters_per_step": its},
-        "roofline": {"bound": "hbm", "kernel": "coupled_bicg_a", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                   "cells": case.n_cells, "bicgstab_iters_per_step": its,
+                   "preconditioner": "block triangle, multigrid V-cycle on the pressure block" if mg else "Jacobi"},
+        "roofline": {"bound": "hbm", "kernel": probe, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "ms_per_launch": round(kms, 5), "bytes_per_launch": kbytes, "launches_timed": klaunches},
     }

@@ -700,33 +700,8 @@ int momentum_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep) {
 // The 4-variable system of Coupled.cpp:54-234 (replacing A_[0].solve on it).  One BiCGStab
 // over the joint vector (u, v, w, p blocks of P): its dot products add up all four blocks.
 // Jacobi preconditioning (right): the diagonal is a_P / omega in the momentum rows and the
-// d-Laplacian diagonal in the continuity rows.
-struct CpOp {
-    const double* A; int P, sJ, sK;
-    // y = the four rows of cell p applied to f(v, c): variable v at padded cell c
-    template <class F>
-    __device__ __forceinline__ void apply(F f, int p, double* y) const {
-        const int o[7] = {0, 1, -1, sJ, -sJ, sK, -sK};
-        double xp[7];
-#pragma unroll
-        for (int m = 0; m < 7; ++m) xp[m] = f(3, p + o[m]);
-        double cont = 0.;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            double mom = 0.;
-#pragma unroll
-            for (int m = 0; m < 7; ++m) {
-                const double xv = f(c, p + o[m]);
-                mom += A[(CP_M + m) * P + p] * xv + A[(CP_G + 7 * c + m) * P + p] * xp[m];
-                cont += A[(CP_D + 7 * c + m) * P + p] * xv;
-            }
-            y[c] = mom;
-        }
-#pragma unroll
-        for (int m = 0; m < 7; ++m) cont += A[(CP_L + m) * P + p] * xp[m];
-        y[3] = cont;
-    }
-};
+// d-Laplacian diagonal in the continuity rows.  In the CAFFE_SOLVE_MG mode the block-
+// triangular multigrid preconditioner (cp_mg_run) replaces it.
 
 // r = b (minus shift on the continuity rows), rh = r, p = v = x = 0, D^-1; partials (rh, r), (r, r)
 __global__ void k_cp_init(Dims D, CpOp A, const double* b, double shift, double* r, double* rh, double* p0, double* v0,
@@ -957,6 +932,10 @@ struct CpSolver {
     int g;
     double* rt;
     double* dx;
+    bool mg;
+    int run(const double* b, double shift, double* x, double rtol, int maxit, int* done) {
+        return mg ? cp_mg_run(c, A, b, shift, x, rtol, maxit, done) : runCoupled(c, A, b, shift, x, rtol, maxit, 8, done);
+    }
     // r = b - shift - A x: returns ||r|| (without the component along the continuity rows'
     // zero sum when singular), ||b - shift|| and the sum of the p block of x
     void resid(const double* b, double shift, const double* x, double* rr, double* bb, double* xsum) {
@@ -981,13 +960,13 @@ struct CpSolver {
     double solve(const double* b, double shift, double* x, double rtol, int maxit, int* iters, int* restarts,
                  double* xsum) {
         int done = 0;
-        *iters += runCoupled(c, A, b, shift, x, rtol, maxit - *iters, 8, &done);
+        *iters += run(b, shift, x, rtol, maxit - *iters, &done);
         double rr, bb;
         resid(b, shift, x, &rr, &bb, xsum);
         double rel = bb > 0. ? rr / bb : 0.;
         for (int k = 0; rel > rtol && k < 20 && *iters < maxit; ++k) {
             const double sub = std::min(0.5, 0.5 * rtol / rel);
-            *iters += runCoupled(c, A, rt, 0., dx, sub, maxit - *iters, 8, &done);
+            *iters += run(rt, 0., dx, sub, maxit - *iters, &done);
             hipLaunchKernelGGL(k_cp_add, dim3(g), dim3(KBS), 0, c.stream, c.D, x, (const double*)dx);
             const double prev = rel;
             resid(b, shift, x, &rr, &bb, xsum);
@@ -1004,7 +983,9 @@ int coupled_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep) {
     HIPCHECK(hipEventRecord(c.ev0, c.stream));
     const Dims& D = c.D;
     const size_t V = 4 * (size_t)D.P;
-    CpSolver S{c, CpOp{c.cA, D.P, D.sJ, D.sK}, gridFor(D), c.cw + 36 * (size_t)D.P, c.cw + 40 * (size_t)D.P};
+    const bool wantMg = o.mode == CAFFE_SOLVE_MG;
+    const bool mg = wantMg && cp_mg_setup(c);
+    CpSolver S{c, CpOp{c.cA, D.P, D.sJ, D.sK}, gridFor(D), c.cw + 36 * (size_t)D.P, c.cw + 40 * (size_t)D.P, mg};
     double* gv = c.cw + 44 * (size_t)D.P;   // -A e_p
     double* z = gv + V;                     // A z = -A e_p
     // Without a fixed-pressure face the continuity rows add up to zero: the system is
@@ -1050,7 +1031,7 @@ int coupled_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep) {
     HIPCHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
     if (rep) {
         rep->iters = iters; rep->refinements = restarts; rep->relres = rel; rep->converged = rel <= o.rtol;
-        rep->fell_back = 0; rep->ms = ms;
+        rep->fell_back = wantMg && !mg; rep->ms = ms;
     }
     return rel <= o.rtol ? CAFFE_OK : CAFFE_ERR_NOT_CONVERGED;
 }

@@ -87,4 +87,42 @@ __device__ __forceinline__ void bibHead(const double* partIn, int nb, double* st
     }
 }
 
+// The operator of the Coupled system (Coupled.cpp:54-234): planes CP_M (momentum), CP_G
+// (pressure gradient, per component), CP_D (continuity, per velocity component) and CP_L
+// (continuity, pressure) over the offsets own, E, W, N, S, T, B.
+struct CpOp {
+    const double* A; int P, sJ, sK;
+    // y = the four rows of cell p applied to f(v, c): variable v at padded cell c
+    template <class F>
+    __device__ __forceinline__ void apply(F f, int p, double* y) const {
+        const int o[7] = {0, 1, -1, sJ, -sJ, sK, -sK};
+        double xp[7];
+#pragma unroll
+        for (int m = 0; m < 7; ++m) xp[m] = f(3, p + o[m]);
+        double cont = 0.;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double mom = 0.;
+#pragma unroll
+            for (int m = 0; m < 7; ++m) {
+                const double xv = f(c, p + o[m]);
+                mom += A[(CP_M + m) * P + p] * xv + A[(CP_G + 7 * c + m) * P + p] * xp[m];
+                cont += A[(CP_D + 7 * c + m) * P + p] * xv;
+            }
+            y[c] = mom;
+        }
+#pragma unroll
+        for (int m = 0; m < 7; ++m) cont += A[(CP_L + m) * P + p] * xp[m];
+        y[3] = cont;
+    }
+};
+
+// The Coupled system with the block-triangular multigrid preconditioner (mg.hip).
+// cp_mg_setup: the multigrid hierarchy of the assembled system's continuity-pressure block
+// (false when the grid is too small for a coarse level).  cp_mg_run: BiCGStab on
+// A x = b - shift (shift on the continuity rows) from x = 0 to the recurrence tolerance
+// rtol; returns the iterations, *done the state's done flag.
+bool cp_mg_setup(Ctx& c);
+int cp_mg_run(Ctx& c, const CpOp& A, const double* b, double shift, double* x, double rtol, int maxit, int* doneOut);
+
 }  // namespace caffe

@@ -72,10 +72,12 @@ enum {
     CAFFE_SOLVE_LEARNED_FCG = 2,  /* flexible CG with the learned two-grid cycle (FP32 inside
                                      FP64 refinement); accepted only at rtol, else the plain
                                      solve runs (fell_back) */
-    CAFFE_SOLVE_MG = 3            /* p' only: BiCGStab in FP64, right-preconditioned by one FP32
-                                     geometric multigrid V-cycle (no counterpart in the
+    CAFFE_SOLVE_MG = 3            /* p' and Coupled: BiCGStab in FP64, right-preconditioned by one
+                                     FP32 geometric multigrid V-cycle (no counterpart in the
                                      reference, whose PETSc solve it replaces); the FP32 modes
-                                     use it for ghost-row systems too */
+                                     use it for ghost-row systems too.  Coupled: the block
+                                     lower triangle [diag M, 0; D, L], the cycle on L; set
+                                     with which = 1 (the pressure / Coupled options) */
 };
 
 /* The keys of case/case.info and case/initialConditions.info the hot path reads
