@@ -933,8 +933,15 @@ struct CpSolver {
     double* rt;
     double* dx;
     bool mg;
-    int run(const double* b, double shift, double* x, double rtol, int maxit, int* done) {
-        return mg ? cp_mg_run(c, A, b, shift, x, rtol, maxit, done) : runCoupled(c, A, b, shift, x, rtol, maxit, 8, done);
+    // the multigrid runs queue first one iteration fewer than the last solve's first run took
+    // (the first solve 4, a restart 2): a queued iteration past convergence costs more than
+    // one more read-back
+    int run(const double* b, double shift, double* x, double rtol, int maxit, int* done, bool restart) {
+        if (!mg) return runCoupled(c, A, b, shift, x, rtol, maxit, 8, done);
+        const int first = restart ? 2 : (c.mg.lastIts > 1 ? c.mg.lastIts - 1 : 4);
+        const int it = cp_mg_run(c, A, b, shift, x, rtol, maxit, first, done);
+        if (!restart) c.mg.lastIts = it;
+        return it;
     }
     // r = b - shift - A x: returns ||r|| (without the component along the continuity rows'
     // zero sum when singular), ||b - shift|| and the sum of the p block of x
@@ -960,13 +967,13 @@ struct CpSolver {
     double solve(const double* b, double shift, double* x, double rtol, int maxit, int* iters, int* restarts,
                  double* xsum) {
         int done = 0;
-        *iters += run(b, shift, x, rtol, maxit - *iters, &done);
+        *iters += run(b, shift, x, rtol, maxit - *iters, &done, false);
         double rr, bb;
         resid(b, shift, x, &rr, &bb, xsum);
         double rel = bb > 0. ? rr / bb : 0.;
         for (int k = 0; rel > rtol && k < 20 && *iters < maxit; ++k) {
             const double sub = std::min(0.5, 0.5 * rtol / rel);
-            *iters += run(rt, 0., dx, sub, maxit - *iters, &done);
+            *iters += run(rt, 0., dx, sub, maxit - *iters, &done, true);
             hipLaunchKernelGGL(k_cp_add, dim3(g), dim3(KBS), 0, c.stream, c.D, x, (const double*)dx);
             const double prev = rel;
             resid(b, shift, x, &rr, &bb, xsum);

@@ -121,8 +121,11 @@ struct CpOp {
 // cp_mg_setup: the multigrid hierarchy of the assembled system's continuity-pressure block
 // (false when the grid is too small for a coarse level).  cp_mg_run: BiCGStab on
 // A x = b - shift (shift on the continuity rows) from x = 0 to the recurrence tolerance
-// rtol; returns the iterations, *done the state's done flag.
+// rtol; returns the iterations, *done the state's done flag.  The iterations are queued in
+// chunks between read-backs of the state: firstChunk, then 2, 4, ... 16 (2 * firstChunk, ...
+// when firstChunk is 4 or less).
 bool cp_mg_setup(Ctx& c);
-int cp_mg_run(Ctx& c, const CpOp& A, const double* b, double shift, double* x, double rtol, int maxit, int* doneOut);
+int cp_mg_run(Ctx& c, const CpOp& A, const double* b, double shift, double* x, double rtol, int maxit, int firstChunk,
+              int* doneOut);
 
 }  // namespace caffe

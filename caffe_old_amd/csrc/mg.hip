@@ -1161,7 +1161,8 @@ bool cp_mg_setup(Ctx& c) {
     return true;
 }
 
-int cp_mg_run(Ctx& c, const CpOp& A, const double* b, double shift, double* x, double rtol, int maxit, int* doneOut) {
+int cp_mg_run(Ctx& c, const CpOp& A, const double* b, double shift, double* x, double rtol, int maxit, int firstChunk,
+              int* doneOut) {
     const Dims& D = c.D;
     const size_t V = 4 * (size_t)D.P;
     double* r = c.cw; double* rh = r + V; double* p0 = rh + V; double* p1 = p0 + V; double* v = p1 + V;
@@ -1179,7 +1180,7 @@ int cp_mg_run(Ctx& c, const CpOp& A, const double* b, double shift, double* x, d
     LoopCtl lc{rtol * rtol, maxit, 0.};
     hipLaunchKernelGGL(k_cpm_init, dim3(g), dim3(KBS), 0, q, D, A, b, shift, r, rh, p0, p1, v, x, dinv, partC, st);
     double *pc = p0, *pn = p1;
-    int it = 0, chunk = 4;
+    int it = 0, chunk = std::max(firstChunk, 1), next = firstChunk > 4 ? 2 : 2 * chunk;
     while (true) {
         for (int k = 0; k < chunk; ++k) {
             hipLaunchKernelGGL(k_cpm_head_p, dim3(g), dim3(KBS), 0, q, D, A, (const double*)r, (const double*)pc, pn,
@@ -1203,7 +1204,8 @@ int cp_mg_run(Ctx& c, const CpOp& A, const double* b, double shift, double* x, d
         HIPCHECK(hipStreamSynchronize(q));
         it = (int)c.hscal[SC_BI + B_IT];
         if (c.hscal[SC_BI + B_DONE] != 0. || it >= maxit) break;
-        if (chunk < 16) chunk *= 2;
+        chunk = next;
+        next = std::min(2 * next, 16);
     }
     *doneOut = (int)c.hscal[SC_BI + B_DONE];
     return it;
