@@ -4,7 +4,7 @@
     python -m caffe_old_amd.run hexamesh CASE_DIR
     python -m caffe_old_amd.run {simple,piso,ibpiso} CASE_DIR [--pcorr plain|ir|learned|mg] [--momentum plain|ir]
     python -m caffe_old_amd.run diffusion CASE_DIR
-    python -m caffe_old_amd.run coupled CASE_DIR
+    python -m caffe_old_amd.run coupled CASE_DIR [--pcorr plain|mg]
 
 ``hexamesh`` does what Modules/hexaMesh.cpp:8-28 does: read mesh/structuredMesh.info,
 generate the nodes, check the faces are planar, and write mesh/structuredMesh.dat.
