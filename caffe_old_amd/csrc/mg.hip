@@ -1175,7 +1175,9 @@ int cp_mg_run(Ctx& c, const CpOp& A, const double* b, double shift, double* x, d
     float* zp = c.mg.z0[0];
     float* zs = c.mg.z0[1];
     const float* w0 = c.mg.w0;
-    const int g = kgrid(D);
+    // up to 4 blocks of 16 waves per CU: the operator kernels hold 59-64 VGPRs, so a SIMD
+    // takes 8 of their waves, and one block per CU left it at 4
+    const int g = blocksFor(D.N, KBS, 4 * KGRID);
     hipStream_t q = c.stream;
     LoopCtl lc{rtol * rtol, maxit, 0.};
     hipLaunchKernelGGL(k_cpm_init, dim3(g), dim3(KBS), 0, q, D, A, b, shift, r, rh, p0, p1, v, x, dinv, partC, st);
