@@ -104,16 +104,16 @@ def coupled_bench(a):
     relaxationFactorMomentum 0.7): one "step" is Coupled::solve, i.e. assembly of the
     4-variable system, its BiCGStab solve to a true FP64 relative residual of 1e-8 from
     x = 0 (plus the null-vector gauge of the closed cavity), and the field update.  The
-    BiCGStab is Jacobi-preconditioned, or with --mode mg block-triangular with a multigrid
-    V-cycle on the pressure block.  The roofline names KA of the Jacobi BiCGStab (k_cp_a: 56
-    coefficient planes and the four variables' operands per cell), or with --mode mg the
-    level-0 post-smoothing sweep of the cycle.  The CPU baseline is the oracle with the reference's
+    BiCGStab is right-preconditioned by the block lower triangle with a multigrid V-cycle on
+    the pressure block (--mode auto or mg), or by Jacobi (--mode plain).  The roofline names
+    the cycle's level-0 post-smoothing sweep, or KA of the Jacobi BiCGStab (k_cp_a: 56
+    coefficient planes and the four variables' operands per cell).  The CPU baseline is the oracle with the reference's
     PETSc-like settings (BiCGStab + block SSOR, rtol 1e-5 on the preconditioned norm)."""
     from caffe_old_amd import case as cs
     from caffe_old_amd.solver import make_solver
     from caffe_old_amd import _lib as L
     n = a.coupled
-    mg = a.mode == "mg"
+    mg = a.mode in ("mg", "auto")
     probe = "mg_post0" if mg else "coupled_bicg_a"
     case = cs.lid_cavity(n, 0.01, algo="coupled", max_iterations=a.steps + a.warmup + 1)
     s = make_solver(case)
