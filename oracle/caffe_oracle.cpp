@@ -612,53 +612,86 @@ int bicgstabLeft(const M& A, const std::vector<double>& b, std::vector<double>& 
 // Right-preconditioned BiCGStab to a true relative residual, restarted from the true residual.
 // project: the rows and unknowns from projFrom on carry a Neumann null space (constants):
 // the right-hand side loses its (rounding) mean there and the solution is returned with
-// zero mean there.
+// zero mean there.  The residual's mean over those rows comes only from the rounding of
+// the zero row sums; no Krylov step can reduce it, so it is removed from every residual
+// and left out of the residual norm (otherwise a tight tolerance stalls on it); the
+// preconditioned directions are projected to zero mean there too, so the iterate gathers
+// no constant whose rounding in A x would floor the true residual.
+// Safeguards near the attainable accuracy: a cycle restarts on breakdown (rho or (r^, v)
+// vanishing, a non-finite step length), when its recurrence residual grows 1e3-fold over
+// its best, or after 100 iterations without a new best; the best iterate (by the true
+// residual at the restarts) is kept, and the solve ends, returning it, once a restart no
+// longer improves on it (a restart from the same iterate would replay the same cycle).
 template <class M>
 int bicgstabTight(const M& A, const std::vector<double>& bIn, std::vector<double>& x, const KspOpts& o,
                   bool project, double* relres, int projFrom = 0, bool gauge = true) {
     int n = (int)bIn.size(), nt = o.nThreads;
     std::vector<double> b = bIn;
     const int np = n - projFrom;
-    if (project && np > 0) {
+    const bool proj = project && np > 0;
+    auto demean = [&](std::vector<double>& w) {
+        if (!proj) return;
         double m = 0.;
-        for (int q = projFrom; q < n; ++q) m += b[q];
+        for (int q = projFrom; q < n; ++q) m += w[q];
         m /= np;
-        for (int q = projFrom; q < n; ++q) b[q] -= m;
-    }
+        for (int q = projFrom; q < n; ++q) w[q] -= m;
+    };
+    demean(b);
     std::fill(x.begin(), x.end(), 0.);
     double bn = std::sqrt(dotv(b, b, nt));
     if (bn == 0.) { if (relres) *relres = 0.; return 0; }
-    std::vector<double> r(n), rh(n), p(n), v(n), s(n), t(n), ph(n), sh(n);
+    std::vector<double> r(n), rh(n), p(n), v(n), s(n), t(n), ph(n), sh(n), xbest = x;
     int total = 0;
-    double rr = 1.;
-    for (int restart = 0; restart < 50 && total < o.maxit; ++restart) {
+    double rrBest = 2.;
+    const double tiny = 1e-30;
+    for (int restart = 0; restart < 200 && total < o.maxit; ++restart) {
         spmv(A, x, r, nt);
         for (int q = 0; q < n; ++q) r[q] = b[q] - r[q];
-        rr = std::sqrt(dotv(r, r, nt)) / bn;
+        demean(r);
+        const double rr = std::sqrt(dotv(r, r, nt)) / bn;
+        if (!(rr < rrBest)) break;  // no gain since the last restart: the best iterate stands
+        rrBest = rr;
+        xbest = x;
         if (rr <= o.rtol) break;
         rh = r; p = r;
+        const double rhn = std::sqrt(dotv(rh, rh, nt));
         double rho = dotv(rh, r, nt);
+        double rnMin = rr;
+        int itMin = total;
         for (; total < o.maxit; ++total) {
             ssorApply(A, p, ph, o.nBlocks, nt);
+            demean(ph);  // the preconditioned direction projected off the null space
             spmv(A, ph, v, nt);
+            demean(v);
             double d = dotv(rh, v, nt);
-            if (d == 0. || rho == 0.) break;
             double alpha = rho / d;
+            if (!(std::fabs(d) > tiny * rhn * std::sqrt(dotv(v, v, nt))) || !std::isfinite(alpha)) break;
             for (int q = 0; q < n; ++q) s[q] = r[q] - alpha * v[q];
             ssorApply(A, s, sh, o.nBlocks, nt);
+            demean(sh);
             spmv(A, sh, t, nt);
+            demean(t);
             double tt = dotv(t, t, nt);
             double omega = tt > 0. ? dotv(t, s, nt) / tt : 0.;
+            if (!std::isfinite(omega)) break;
             for (int q = 0; q < n; ++q) { x[q] += alpha * ph[q] + omega * sh[q]; r[q] = s[q] - omega * t[q]; }
             double rn = std::sqrt(dotv(r, r, nt)) / bn;
             if (rn <= 0.5 * o.rtol || omega == 0.) { ++total; break; }
+            if (rn < rnMin) { rnMin = rn; itMin = total; }
+            if (!(rn < 1e3 * rnMin) || total - itMin > 100) { ++total; break; }
             double rhoN = dotv(rh, r, nt);
+            if (!(std::fabs(rhoN) > tiny * rhn * rn * bn)) { ++total; break; }
             double beta = (rhoN / rho) * (alpha / omega);
             rho = rhoN;
             for (int q = 0; q < n; ++q) p[q] = r[q] + beta * (p[q] - omega * v[q]);
         }
     }
-    if (project && gauge && np > 0) {  // gauge of the Neumann problem: zero-mean solution
+    // the cycle that ran last may have ended worse than the best restart point
+    spmv(A, x, r, nt);
+    for (int q = 0; q < n; ++q) r[q] = b[q] - r[q];
+    demean(r);
+    if (!(std::sqrt(dotv(r, r, nt)) / bn < rrBest)) x = xbest;
+    if (proj && gauge) {  // gauge of the Neumann problem: zero-mean solution
         double m = 0.;
         for (int q = projFrom; q < n; ++q) m += x[q];
         m /= np;
@@ -666,8 +699,8 @@ int bicgstabTight(const M& A, const std::vector<double>& bIn, std::vector<double
     }
     spmv(A, x, r, nt);
     for (int q = 0; q < n; ++q) r[q] = b[q] - r[q];
-    rr = std::sqrt(dotv(r, r, nt)) / bn;
-    if (relres) *relres = rr;
+    demean(r);
+    if (relres) *relres = std::sqrt(dotv(r, r, nt)) / bn;
     return total;
 }
 

@@ -114,3 +114,94 @@ def copy_state(src, dst):
     """oracle -> device (or device -> oracle): every field the hot path reads"""
     for n in STATE:
         dst.set(n, src.get(n))
+
+
+def threads():
+    """host threads for the oracle: the CPU share this process may use, at most 16"""
+    import os
+    try:
+        n = len(os.sched_getaffinity(0))
+    except Exception:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def imbalance(b_padded, vol_padded, active):
+    """max over active cells of |b| / V: the cell mass imbalance the p' assembly puts in
+    its right-hand side (Simple.cpp:248-252), scaled as computeContinuityError scales it"""
+    return float(np.max(np.abs(b_padded[active]) / vol_padded[active]))
+
+
+def stepped_solve(g, o, dt, pc_opts, vol, active, n_pcorr):
+    """Solver::solve(dt) of SIMPLE / PISO (Simple.cpp:78-91, Piso.cpp:12-29) run as its
+    sub-steps on the device g and the oracle o side by side.  Returns the pre-correction
+    imbalance of the first p' assembly on each side (the outer-residual history the
+    physics determines: it does not depend on the Krylov path of the p' solve)."""
+    g.save_u0()
+    o.save_u0()
+    g.momentum_assemble(dt)
+    g.momentum_solve()
+    g.rhie_chow()
+    o.compute_momentum(dt)
+    hg = ho = None
+    for q in range(n_pcorr):
+        g.pcorr_assemble()
+        o.assemble_pcorr()
+        if q == 0:
+            hg = imbalance(g.rhs(1), vol, active)
+            ho = imbalance(oracle_rhs_padded(o, 0, g.P), vol, active)
+        g.pcorr_solve(**pc_opts)
+        g.correct()
+        o.compute_pcorr()
+        o.correct()
+    return hg, ho
+
+
+def assembly_errors(g2, o, dt):
+    """Momentum and p' assembly of the device context g2 and the oracle from the oracle's
+    current state (copied into g2).  Leaves the oracle's state as it was.  Returns the
+    max-norm relative differences of the band, the three momentum right-hand sides, d,
+    the p' coefficients and the p' right-hand side."""
+    snap = {n: o.get(n) for n in STATE}
+    copy_state(o, g2)
+    g2.momentum_assemble(dt)
+    o.assemble_momentum(dt)
+    band_o, status = oracle_momentum_band(o)
+    act = status == 0
+    out = {"band": rel_inf(g2.matrix(0), band_o, act)}
+    bg = g2.rhs(0)
+    for c in range(3):
+        out[f"rhs{c}"] = rel_inf(bg[c], oracle_rhs_padded(o, c, g2.P), act)
+    out["d"] = rel_inf(g2.get("d"), o.get("d"), act)
+    for n, v in snap.items():
+        o.set(n, v)
+    copy_state(o, g2)
+    g2.pcorr_assemble()
+    o.assemble_pcorr()
+    sym_o, _ = oracle_pcorr_sym(o)
+    sym_g = g2.matrix(1)
+    out["aP"] = rel_inf(sym_g[0], sym_o[0], act)
+    out["aFaces"] = max(rel_inf(sym_g[a], sym_o[a]) for a in (1, 2, 3))
+    out["pc_rhs"] = rel_inf(g2.rhs(1), oracle_rhs_padded(o, 0, g2.P), act)
+    for n, v in snap.items():
+        o.set(n, v)
+    return out
+
+
+def field_errors(g, o):
+    """relative L2 of u, v, w over the interior cells, and of p after removing the mean
+    (the pure-Neumann pressure is defined up to a constant)"""
+    sh = g.shape
+    ug, uo = interior(g.get("u"), sh), interior(o.get("u"), sh)
+    pg, po = interior(g.get("p"), sh), interior(o.get("p"), sh)
+    errs = {c: rel_l2(ug[i], uo[i]) for i, c in enumerate("uvw") if np.linalg.norm(uo[i]) > 0}
+    errs["p"] = rel_l2(pg - pg.mean(), po - po.mean())
+    return errs
+
+
+def active_mask(o):
+    """padded boolean mask of the ACTIVE cells of the oracle's IndexMap"""
+    rows, rid, st, pad = padded_of_rows(o)
+    m = np.zeros(o.P, dtype=bool)
+    m[pad[st == 0]] = True
+    return m
