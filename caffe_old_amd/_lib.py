@@ -37,6 +37,12 @@ EXPORTS = [
     "caffe_net_set_params", "caffe_net_forward", "caffe_net_backward", "caffe_net_precond", "caffe_net_loss_grad",
     "caffe_net_collect", "caffe_net_clear", "caffe_net_dataset_size", "caffe_net_train", "caffe_net_eval",
     "caffe_ctx_attach_net",
+    # SparseMatrix / SparseVector compatibility path
+    "caffe_matrix_create", "caffe_matrix_destroy", "caffe_matrix_last_error", "caffe_matrix_set_rows",
+    "caffe_matrix_set_value", "caffe_matrix_zero_entries", "caffe_matrix_assemble", "caffe_matrix_solve",
+    "caffe_matrix_last_report", "caffe_matrix_multiply", "caffe_matrix_multiply_add", "caffe_matrix_scale",
+    "caffe_vector_create", "caffe_vector_destroy", "caffe_vector_last_error", "caffe_vector_set_values",
+    "caffe_vector_get_values", "caffe_vector_zero_entries", "caffe_vector_scale", "caffe_vector_norm",
 ]
 
 
@@ -129,6 +135,27 @@ def load(path: str = LIB_PATH):
         "caffe_net_train": ([vp, C.c_int, C.c_int, C.c_float, C.c_double, C.c_ulonglong, fp], C.c_int),
         "caffe_net_eval": ([vp, fp, C.c_long], C.c_int),
         "caffe_ctx_attach_net": ([vp, vp], C.c_int),
+        "caffe_matrix_create": ([C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
+                                 C.POINTER(C.c_void_p)], C.c_int),
+        "caffe_matrix_destroy": ([vp], C.c_int),
+        "caffe_matrix_last_error": ([vp], C.c_char_p),
+        "caffe_matrix_set_rows": ([vp, C.c_int, ip, ip, ip, dp], C.c_int),
+        "caffe_matrix_set_value": ([vp, C.c_int, C.c_int, C.c_double], C.c_int),
+        "caffe_matrix_zero_entries": ([vp], C.c_int),
+        "caffe_matrix_assemble": ([vp], C.c_int),
+        "caffe_matrix_solve": ([vp, vp, vp, ip], C.c_int),
+        "caffe_matrix_last_report": ([vp, C.POINTER(SolveReport)], C.c_int),
+        "caffe_matrix_multiply": ([vp, vp, vp], C.c_int),
+        "caffe_matrix_multiply_add": ([vp, vp, vp, vp], C.c_int),
+        "caffe_matrix_scale": ([C.c_double, vp], C.c_int),
+        "caffe_vector_create": ([C.c_int, C.c_int, C.POINTER(C.c_void_p)], C.c_int),
+        "caffe_vector_destroy": ([vp], C.c_int),
+        "caffe_vector_last_error": ([vp], C.c_char_p),
+        "caffe_vector_set_values": ([vp, C.c_int, ip, dp], C.c_int),
+        "caffe_vector_get_values": ([vp, C.c_int, ip, dp], C.c_int),
+        "caffe_vector_zero_entries": ([vp], C.c_int),
+        "caffe_vector_scale": ([C.c_double, vp], C.c_int),
+        "caffe_vector_norm": ([vp, C.c_int, dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -236,6 +236,60 @@ int caffe_net_eval(caffe_net* net, float* losses, long n);
  * must outlive the attachment */
 int caffe_ctx_attach_net(caffe_ctx* ctx, caffe_net* net);
 
+/* ---------------------------------------------------------------------------------
+ * The SparseMatrix / SparseVector compatibility path: for callers that assemble their
+ * systems row by row, as the reference's solvers do (Simple.cpp:162-175, 265-275,
+ * IbPiso.cpp:148-185, Diffusion.cpp:97-130, Coupled.cpp:137-234), and solve them through
+ * SparseMatrix::solve.  A matrix keeps its rows on the host while they are set; assembly
+ * copies them to HBM (ELL layout), where the products and the solve run.  Every call is
+ * synchronous.  Matrices and vectors are independent of any caffe_ctx.
+ * ------------------------------------------------------------------------------- */
+typedef struct caffe_matrix caffe_matrix;
+typedef struct caffe_vector caffe_vector;
+enum { CAFFE_NORM_L1 = 0, CAFFE_NORM_L2 = 1, CAFFE_NORM_INF = 2 };
+
+/* SparseMatrix(rToler, absToler, maxIters) + allocate(m, n, nnz) (src/Math/SparseMatrix.h:38-46,
+ * SparseMatrix.cpp:26-63): an m x n matrix with nnz entries reserved per row (a row may grow
+ * past it, as PETSc AIJ lets it).  rtol <= 0, abstol < 0 and maxit <= 0 take the PETSc
+ * defaults 1e-5, 1e-50 and 10000.  destroy = deallocate (SparseMatrix.cpp:65-69). */
+int caffe_matrix_create(int device, int m, int n, int nnz, double rtol, double abstol, int maxit, caffe_matrix** out);
+int caffe_matrix_destroy(caffe_matrix* A);
+const char* caffe_matrix_last_error(caffe_matrix* A);
+/* SparseMatrix::setRow / setValue (SparseMatrix.cpp:71-79: MatSetValues, INSERT_VALUES) for
+ * nrows rows at once: row rows[q] gets columns cols[rowptr[q] .. rowptr[q+1]) with vals; an
+ * entry set again is overwritten, other entries of the row stay; negative column and row
+ * indices are ignored (IndexMap::operator() returns -1 off the domain, IndexMap.cpp:77). */
+int caffe_matrix_set_rows(caffe_matrix* A, int nrows, const int* rows, const int* rowptr, const int* cols,
+                          const double* vals);
+int caffe_matrix_set_value(caffe_matrix* A, int i, int j, double v);
+/* SparseMatrix::zeroEntries (MatZeroEntries: values zero, structure kept) and assemble */
+int caffe_matrix_zero_entries(caffe_matrix* A);
+int caffe_matrix_assemble(caffe_matrix* A);
+/* SparseMatrix::solve(b, x) (SparseMatrix.cpp:86-106): x from zero; BiCGStab in FP64,
+ * right-preconditioned by Jacobi, to ||b - A x||_2 <= rtol ||b||_2 (or <= abstol), restarted
+ * from the true residual on breakdown; *iters = the iteration count.  Not converging is not
+ * an error (the reference never checks KSP's converged reason): caffe_matrix_last_report
+ * gives the true relative residual and whether it met rtol. */
+int caffe_matrix_solve(caffe_matrix* A, const caffe_vector* b, caffe_vector* x, int* iters);
+int caffe_matrix_last_report(caffe_matrix* A, caffe_solve_report* rep);
+/* multiply(A, x, y): y = A x; multiplyAdd(A, x1, x2, y): y = A x1 + x2 (y may be x2);
+ * scale(alpha, A) (SparseMatrix.h:76-79, SparseMatrix.cpp:135-148).  The matrix-matrix
+ * multiply(A, B, C) is not provided: no solver of the reference calls it. */
+int caffe_matrix_multiply(caffe_matrix* A, const caffe_vector* x, caffe_vector* y);
+int caffe_matrix_multiply_add(caffe_matrix* A, const caffe_vector* x1, const caffe_vector* x2, caffe_vector* y);
+int caffe_matrix_scale(double alpha, caffe_matrix* A);
+/* SparseVector (src/Math/SparseVector.h:8-43): allocate(m), setValue (VecSetValues,
+ * INSERT_VALUES; negative indices ignored), operator() (VecGetValues), zeroEntries,
+ * l1Norm / l2Norm / infNorm, and scale(alpha, v) (Diffusion.cpp:127) */
+int caffe_vector_create(int device, int m, caffe_vector** out);
+int caffe_vector_destroy(caffe_vector* v);
+const char* caffe_vector_last_error(caffe_vector* v);
+int caffe_vector_set_values(caffe_vector* v, int n, const int* idx, const double* vals);
+int caffe_vector_get_values(caffe_vector* v, int n, const int* idx, double* out);
+int caffe_vector_zero_entries(caffe_vector* v);
+int caffe_vector_scale(double alpha, caffe_vector* v);
+int caffe_vector_norm(caffe_vector* v, int type, double* out);
+
 #ifdef __cplusplus
 }
 #endif
