@@ -40,6 +40,7 @@ EXPORTS = [
     # SparseMatrix / SparseVector compatibility path
     "caffe_matrix_create", "caffe_matrix_destroy", "caffe_matrix_last_error", "caffe_matrix_set_rows",
     "caffe_matrix_set_value", "caffe_matrix_zero_entries", "caffe_matrix_assemble", "caffe_matrix_solve",
+    "caffe_matrix_solve_noalloc",
     "caffe_matrix_last_report", "caffe_matrix_multiply", "caffe_matrix_multiply_add", "caffe_matrix_scale",
     "caffe_vector_create", "caffe_vector_destroy", "caffe_vector_last_error", "caffe_vector_set_values",
     "caffe_vector_get_values", "caffe_vector_zero_entries", "caffe_vector_scale", "caffe_vector_norm",
@@ -144,6 +145,7 @@ def load(path: str = LIB_PATH):
         "caffe_matrix_zero_entries": ([vp], C.c_int),
         "caffe_matrix_assemble": ([vp], C.c_int),
         "caffe_matrix_solve": ([vp, vp, vp, ip], C.c_int),
+        "caffe_matrix_solve_noalloc": ([vp, vp, vp, ip], C.c_int),
         "caffe_matrix_last_report": ([vp, C.POINTER(SolveReport)], C.c_int),
         "caffe_matrix_multiply": ([vp, vp, vp], C.c_int),
         "caffe_matrix_multiply_add": ([vp, vp, vp, vp], C.c_int),

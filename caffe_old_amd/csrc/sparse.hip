@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -242,7 +243,16 @@ __global__ void k_sp_norm_final(const double* part, int nb, double* out) {
 
 int gridRows(long m) { return blocksFor(m, SBS, SGRID); }
 
+constexpr long NOALLOC_MAX_ROWS = CAFFE_NOALLOC_MAX_ROWS;
+constexpr size_t NOALLOC_DOUBLES = 9 * (size_t)NOALLOC_MAX_ROWS + (size_t)N_PART_ROWS * MAX_RED_BLOCKS * RED_STRIDE + N_SSLOTS;
+
 }  // namespace
+
+// The scratch of caffe_matrix_solve_noalloc: the BiCGStab work (vectors, partial sums and
+// scalar slots) of systems up to CAFFE_NOALLOC_MAX_ROWS rows, 19.3 MB, held by every GPU the
+// library runs on until the process ends.  Host code never uses this name as an address: it
+// uses the address hipGetSymbolAddress returns for the device (g_scratch below).
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
 
 // ---------------------------------------------------------------- host objects
 struct caffe_vector {
@@ -353,6 +363,20 @@ int setRow(caffe_matrix& A, int row, int nc, const int* cols, const double* vals
     return CAFFE_OK;
 }
 
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratchMu;
+double* g_scratch[MAX_DEVICES] = {nullptr};
+
+// the device address of scratch, once per device, with that device current
+void initScratch(int device) {
+    if (device < 0 || device >= MAX_DEVICES) throw std::string("device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratchMu);
+    if (g_scratch[device]) return;
+    void* p = nullptr;
+    HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch[device] = (double*)p;
+}
+
 // BiCGStab on the assembled A from x = 0 into x, work in w (solveWorkDoubles(m)).  Every
 // value the kernels read is written first in this call (k_sp_init and the fill below), so
 // the work buffer may hold anything on entry.
@@ -448,6 +472,7 @@ int caffe_matrix_create(int device, int m, int n, int nnz, double rtol, double a
         HIPCHECK(hipEventCreate(&a.ev1));
         a.rows.assign(m, {});
         for (auto& r : a.rows) r.reserve(nnz);
+        initScratch(device);
         return CAFFE_OK;
     });
     if (rc != CAFFE_OK) {
@@ -515,6 +540,32 @@ int caffe_matrix_solve(caffe_matrix* A, const caffe_vector* b, caffe_vector* x, 
         vecToDevice(*const_cast<caffe_vector*>(b));
         if (!a.work) HIPCHECK(hipMalloc((void**)&a.work, sizeof(double) * solveWorkDoubles(a.m)));
         const int its = runSolve(a, b->dev, x->dev, a.work, nullptr);
+        x->newer = 2;
+        if (iters) *iters = its;
+        return CAFFE_OK;
+    });
+}
+
+int caffe_matrix_solve_noalloc(caffe_matrix* A, const caffe_vector* b, caffe_vector* x, int* iters) {
+    // no hipMalloc, hipFree or allocating routine: the work is scratch (its address taken at
+    // caffe_matrix_create), the matrix must be assembled, and b reaches HBM by a copy into its
+    // existing buffer
+    return sguard(A, [&](caffe_matrix& a) {
+        int rc = checkSolveArgs(a, b, x);
+        if (rc != CAFFE_OK) return rc;
+        if (a.m > NOALLOC_MAX_ROWS) {
+            a.err = "caffe_matrix_solve_noalloc: " + std::to_string(a.m) + " rows, more than CAFFE_NOALLOC_MAX_ROWS (" +
+                    std::to_string(NOALLOC_MAX_ROWS) + ")";
+            return CAFFE_ERR_ARG;
+        }
+        if (a.dirty) {
+            a.err = "caffe_matrix_solve_noalloc: rows changed since the last caffe_matrix_assemble (assembly may allocate)";
+            return CAFFE_ERR_STATE;
+        }
+        double* w = g_scratch[a.device];
+        if (!w) { a.err = "caffe_matrix_solve_noalloc: scratch address not initialised for this device"; return CAFFE_ERR_STATE; }
+        vecToDevice(*const_cast<caffe_vector*>(b));
+        const int its = runSolve(a, b->dev, x->dev, w, nullptr);
         x->newer = 2;
         if (iters) *iters = its;
         return CAFFE_OK;

@@ -271,6 +271,17 @@ int caffe_matrix_assemble(caffe_matrix* A);
  * an error (the reference never checks KSP's converged reason): caffe_matrix_last_report
  * gives the true relative residual and whether it met rtol. */
 int caffe_matrix_solve(caffe_matrix* A, const caffe_vector* b, caffe_vector* x, int* iters);
+/* No counterpart in the reference.  caffe_matrix_solve for callers that cannot allocate
+ * device memory (code under HIP graph capture set-up, tight serving loops, harnesses that
+ * forbid hipMalloc): the same solve with the same results, whose BiCGStab work lives in a
+ * fixed per-GPU library array of CAFFE_NOALLOC_MAX_ROWS rows (19.3 MB, held by each GPU the
+ * library runs on until the process ends) instead of a per-matrix allocation.  It calls no
+ * hipMalloc or hipFree: A must have been assembled (caffe_matrix_assemble) since its rows last
+ * changed, or CAFFE_ERR_STATE; more rows than the maximum give CAFFE_ERR_ARG.  Every call on
+ * one GPU shares that array, so calls that overlap on one GPU (from several host threads)
+ * race; each call writes every work value before it reads it. */
+#define CAFFE_NOALLOC_MAX_ROWS 262144
+int caffe_matrix_solve_noalloc(caffe_matrix* A, const caffe_vector* b, caffe_vector* x, int* iters);
 int caffe_matrix_last_report(caffe_matrix* A, caffe_solve_report* rep);
 /* multiply(A, x, y): y = A x; multiplyAdd(A, x1, x2, y): y = A x1 + x2 (y may be x2);
  * scale(alpha, A) (SparseMatrix.h:76-79, SparseMatrix.cpp:135-148).  The matrix-matrix
