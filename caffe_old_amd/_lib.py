@@ -21,6 +21,7 @@ SOLVE_PLAIN_FP64 = 0
 SOLVE_IR_FP32 = 1
 SOLVE_LEARNED_FCG = 2
 SOLVE_MG = 3
+SOLVE_SGS_FP64 = 4
 
 # every symbol include/caffe_hip.h declares
 EXPORTS = [

@@ -454,7 +454,7 @@ int caffe_case_setup(caffe_ctx* x, const caffe_case* cs, const double* nodes) {
 int caffe_set_solve_opts(caffe_ctx* x, int which, const caffe_solve_opts* o) {
     return guard(x, [&](Ctx& c) {
         if (!o || which < 0 || which > 1 || !(o->rtol > 0.) || o->maxit < 1) { c.err = "invalid solve options"; return CAFFE_ERR_ARG; }
-        if (o->mode < CAFFE_SOLVE_PLAIN_FP64 || o->mode > CAFFE_SOLVE_MG) { c.err = "unknown solve mode"; return CAFFE_ERR_ARG; }
+        if (o->mode < CAFFE_SOLVE_PLAIN_FP64 || o->mode > CAFFE_SOLVE_SGS_FP64) { c.err = "unknown solve mode"; return CAFFE_ERR_ARG; }
         if (which == 0 && o->mode != CAFFE_SOLVE_PLAIN_FP64 && o->mode != CAFFE_SOLVE_IR_FP32) {
             c.err = "momentum solves are FP64 or FP32 inside FP64 refinement";
             return CAFFE_ERR_ARG;
@@ -482,7 +482,7 @@ int caffe_pcorr_assemble(caffe_ctx* x) { return guard(x, [&](Ctx& c) { READY(c);
 int caffe_pcorr_solve(caffe_ctx* x, const caffe_solve_opts* o, caffe_solve_report* rep) {
     return guard(x, [&](Ctx& c) {
         READY(c);
-        if (o && (o->mode < CAFFE_SOLVE_PLAIN_FP64 || o->mode > CAFFE_SOLVE_MG)) { c.err = "unknown solve mode"; return CAFFE_ERR_ARG; }
+        if (o && (o->mode < CAFFE_SOLVE_PLAIN_FP64 || o->mode > CAFFE_SOLVE_SGS_FP64)) { c.err = "unknown solve mode"; return CAFFE_ERR_ARG; }
         return pcorrSolve(c, o ? *o : c.pcOpts, rep);
     });
 }

@@ -1121,6 +1121,121 @@ static int cgSolve(Ctx& c, double shift, double rtol, int maxit, double delta, i
     return it;
 }
 
+// ---------------------------------------------------------------- red-black SGS-PCG
+// CG preconditioned by one symmetric Gauss-Seidel sweep (SSOR with omega = 1) in red-black
+// order, the parallel analogue of PETSc's PCSOR with its = lits = 1 and the symmetric sweep
+// (SparseMatrix.cpp:97-98).  Cell (i, j, k) is red when i + j + k is even.  z = M^-1 r is the
+// forward sweep (red: z = r / aP; black: z = (r - sum a z_red) / aP) followed by the backward
+// sweep, whose black half repeats the forward one (black cells couple only to red cells) and
+// is skipped, and whose red half is z = (r - sum a z_black) / aP.  M is symmetric, so the
+// iteration is CG.  An iteration: k_cg_k1 (direction, A p), k_sgs_upd (x, r and the forward
+// red half), k_sgs_black, k_sgs_red (backward red half, partial sums for the next k_cg_k1).
+__device__ __forceinline__ int cellColour(const Dims& D, int q) {
+    int i, j, k;
+    D.cell(q, i, j, k);
+    return (i + j + k) & 1;
+}
+// the off-diagonal part of row p applied to z
+__device__ __forceinline__ double offDiag(const Sym4<double>& A, const double* z, int p) {
+    return A.A[A.P + p] * z[p - 1] + A.A[A.P + p + 1] * z[p + 1] + A.A[2 * A.P + p] * z[p - A.sJ] +
+           A.A[2 * A.P + p + A.sJ] * z[p + A.sJ] + A.A[3 * A.P + p] * z[p - A.sK] + A.A[3 * A.P + p + A.sK] * z[p + A.sK];
+}
+// [alpha = rz / (p, q)] x += alpha p; r -= alpha q; red cells: z = r / aP
+__global__ void k_sgs_upd(Dims D, Sym4<double> A, double* x, double* r, double* z, const double* pb0, const double* pb1,
+                          const double* qv, const double* partIn, int nb, double* st) {
+    const double sDone = st[C_DONE], sPbuf = st[C_PBUF], sRz = st[C_RZCUR], sIt = st[C_IT];
+    if (sDone != 0.) return;
+    const double* pv = sPbuf != 0. ? pb1 : pb0;
+    double t[1];
+    readPartials<KBS, 1>(partIn, nb, t);
+    const double alpha = t[0] != 0. ? sRz / t[0] : 0.;
+    if (BLOCK0) { st[C_RZPREV] = sRz; st[C_IT] = sIt + 1.; st[C_RATIO] = 1.; st[C_PQ] = t[0]; }
+    FOR_CELLS(D, q, p) {
+        x[p] += alpha * pv[p];
+        const double rn = r[p] - alpha * qv[p];
+        r[p] = rn;
+        if (cellColour(D, q) == 0) z[p] = rn / A.diag(p);
+    }
+}
+// black cells: z = (r - sum_nb a z_red) / aP
+__global__ void k_sgs_black(Dims D, Sym4<double> A, const double* r, double* z, const double* st) {
+    if (st[C_DONE] != 0.) return;
+    FOR_CELLS(D, q, p) {
+        if (cellColour(D, q) == 1) z[p] = (r[p] - offDiag(A, z, p)) / A.diag(p);
+    }
+}
+// red cells: z = (r - sum_nb a z_black) / aP; partials (r, z), (r, r), sum z, sum r over
+// all cells (as k_cg_k2 writes them)
+__global__ void k_sgs_red(Dims D, Sym4<double> A, const double* r, double* z, double* partOut, const double* st) {
+    if (st[C_DONE] != 0.) return;
+    double v[4] = {0., 0., 0., 0.};
+    FOR_CELLS(D, q, p) {
+        const double rv = r[p];
+        double zv;
+        if (cellColour(D, q) == 0) { zv = (rv - offDiag(A, z, p)) / A.diag(p); z[p] = zv; }
+        else zv = z[p];
+        v[0] += rv * zv; v[1] += rv * rv; v[2] += zv; v[3] += rv;
+    }
+    writePartials<KBS, 4>(v, partOut);
+}
+
+// SGS-PCG in FP64 on the symmetric p' system from the current c.xp, to the true FP64 relative
+// residual rtol (the recurrence aims at what is still needed; restarts from the true
+// residual keep the direction, as cgSolve does)
+static int sgsCgSolve(Ctx& c, double shift, double rtol, int maxit, int* updates, double* relOut) {
+    const Dims& D = c.D;
+    const int P = D.P;
+    const int g = gridFor(D);
+    Sym4<double> A = symP<double>(c);
+    double* w = c.wp;
+    double* r = w; double* z = r + P; double* p0 = z + P; double* p1 = p0 + P; double* qv = p1 + P;
+    double* r64 = c.wp + 6 * P;
+    double* st = c.scal + SC_CG;
+    double* partB = c.red.part;
+    double* partA = c.red.part + MAX_RED_BLOCKS * RED_STRIDE;
+    double rr, bb;
+    trueResid<1>(c, A, c.bp, shift, c.xp, r64, &rr, &bb, c.singularP);
+    double rel = bb > 0. ? rr / bb : 0.;
+    *updates = 0;
+    if (rel <= rtol || bb == 0.) { *relOut = rel; return 0; }
+    // k_cg_init: r = b - A x (from r64), z = r / aP on every cell (the forward red half for the
+    // red cells), x kept; then the black half and the backward red half complete z = M^-1 r
+    auto start = [&](int zeroP) {
+        hipLaunchKernelGGL((k_cg_init<double, double>), dim3(g), dim3(KBS), 0, c.stream, D, A, r64, 0., 1.0, r, z, p0, p1,
+                           (double*)c.xp, zeroP, 0, partB, st, 1.0);
+        hipLaunchKernelGGL(k_sgs_black, dim3(g), dim3(KBS), 0, c.stream, D, A, (const double*)r, z, (const double*)st);
+        hipLaunchKernelGGL(k_sgs_red, dim3(g), dim3(KBS), 0, c.stream, D, A, (const double*)r, z, partB, (const double*)st);
+    };
+    start(1);
+    double tolIn = std::min(0.5, rtol / rel);
+    LoopCtl lc{tolIn * tolIn, maxit, c.singularP ? (double)D.N : 0.};
+    int it = 0, chunk = 8;
+    while (true) {
+        for (int q = 0; q < chunk; ++q) {
+            hipLaunchKernelGGL((k_cg_k1<double>), dim3(g), dim3(KBS), 0, c.stream, D, A, (const double*)z, p0, p1, qv,
+                               (const double*)partB, g, partA, st, lc);
+            hipLaunchKernelGGL(k_sgs_upd, dim3(g), dim3(KBS), 0, c.stream, D, A, (double*)c.xp, r, z, (const double*)p0,
+                               (const double*)p1, (const double*)qv, (const double*)partA, g, st);
+            hipLaunchKernelGGL(k_sgs_black, dim3(g), dim3(KBS), 0, c.stream, D, A, (const double*)r, z, (const double*)st);
+            hipLaunchKernelGGL(k_sgs_red, dim3(g), dim3(KBS), 0, c.stream, D, A, (const double*)r, z, partB, (const double*)st);
+        }
+        readBack(c, st, SC_CG, 8);
+        it = (int)c.hscal[SC_CG + C_IT];
+        const bool recDone = c.hscal[SC_CG + C_DONE] != 0.;
+        if (chunk < 32) chunk *= 2;
+        if (!recDone && it < maxit) continue;
+        trueResid<1>(c, A, c.bp, shift, c.xp, r64, &rr, &bb, c.singularP);
+        rel = rr / bb;
+        ++*updates;
+        if (rel <= rtol || it >= maxit || *updates > 200) break;
+        start(0);
+        tolIn = std::min(0.5, rtol / rel);
+        lc.tol2 = tolIn * tolIn;
+    }
+    *relOut = rel;
+    return it;
+}
+
 // BiCGStab on the nonsymmetric (ghost-row) p' system, restarted from the true FP64
 // residual; T = float runs each restart in FP32 to relative tolerance `inner`.
 template <class T>
@@ -1172,6 +1287,11 @@ int pcorr_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep) {
     if (o.mode == CAFFE_SOLVE_PLAIN_FP64) {
         if (sym) iters += cgSolve<double>(c, shift, o.rtol, o.maxit, 0., &refs, &rel);
         else iters += bicgPressure<double>(c, shift, o.rtol, o.maxit, 0., &refs, &rel);
+    } else if (o.mode == CAFFE_SOLVE_SGS_FP64) {
+        // ghost-row systems (not a 7-point red-black coupling) and indefinite ones take the
+        // plain solve
+        if (sym && !c.hasIB) iters += sgsCgSolve(c, shift, o.rtol, o.maxit, &refs, &rel);
+        else { fell = 1; iters += bicgPressure<double>(c, shift, o.rtol, o.maxit, 0., &refs, &rel); }
     } else {
         pcorr_make_fp32(c);
         bool learnedDone = false;

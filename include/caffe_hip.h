@@ -72,6 +72,11 @@ enum {
     CAFFE_SOLVE_LEARNED_FCG = 2,  /* flexible CG with the learned two-grid cycle (FP32 inside
                                      FP64 refinement); accepted only at rtol, else the plain
                                      solve runs (fell_back) */
+    CAFFE_SOLVE_SGS_FP64 = 4,     /* p' system: CG in FP64 preconditioned by one red-black symmetric
+                                     Gauss-Seidel sweep (SSOR, omega = 1): the parallel analogue of
+                                     the reference's PCSOR(its 1, lits 1) (SparseMatrix.cpp:97-98);
+                                     ghost-row and indefinite systems take the plain solve
+                                     (fell_back = 1) */
     CAFFE_SOLVE_MG = 3            /* p' and Coupled: BiCGStab in FP64, right-preconditioned by one
                                      FP32 geometric multigrid V-cycle (no counterpart in the
                                      reference, whose PETSc solve it replaces); the FP32 modes

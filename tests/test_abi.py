@@ -74,7 +74,7 @@ def test_solve_mode_constants_match_header():
     txt = open(os.path.join(ROOT, "include", "caffe_hip.h")).read()
     modes = dict((k, int(v)) for k, v in re.findall(r"CAFFE_SOLVE_(\w+)\s*=\s*(\d+)", txt))
     assert modes == {"PLAIN_FP64": _lib.SOLVE_PLAIN_FP64, "IR_FP32": _lib.SOLVE_IR_FP32,
-                     "LEARNED_FCG": _lib.SOLVE_LEARNED_FCG, "MG": _lib.SOLVE_MG}
+                     "LEARNED_FCG": _lib.SOLVE_LEARNED_FCG, "MG": _lib.SOLVE_MG, "SGS_FP64": _lib.SOLVE_SGS_FP64}
 
 
 def test_cpp_wrapper_header_compiles(tmp_path):
