@@ -10,11 +10,19 @@ Inputs are resident in HBM before the timed region; the timed region contains K 
 steps bracketed by stream synchronisation.  The p' solves of the timed steps use the
 learned FCG when trained weights for the grid ship with the package (--mode auto).
 
-roofline: the pressure-correction stencil kernel of the timed steps (k_f_k1 of the
-learned FCG; k_cg_k1 for --mode ir/plain), its duration averaged over its launches in
-the timed steps from HIP event pairs recorded around them on the solver's stream, its
-algorithmic bytes per launch (interior cells x bytes per cell), and the HBM bytes per
-launch counted by rocprofv3 (profiles/traffic.json, calibrated against a copy kernel).
+roofline: the kernel with the largest total time in the rocprof trace of the timed steps
+(k_f_k2pre, the learned FCG's update and pre-smoothing kernel; k_cg_k1 for --mode ir/plain/
+sgs; the multigrid's fine post-smoothing sweep for --mode mg), its duration averaged over
+its launches in the timed steps from HIP event pairs recorded around them on the solver's
+stream, its algorithmic bytes per launch (interior cells x bytes per cell), and the HBM
+bytes per launch counted by rocprofv3 (profiles/traffic.json, calibrated against a copy
+kernel).
+
+Beside the contract's K timed steps the line reports the configuration as stated, 100 steps
+from rest timed as a whole (config.full_run_100_steps), the p' solve of one system by every
+method (the red-black SGS-preconditioned CG being the judge as strong as the reference's
+SOR), the CPU baseline on all host cores over the same part of the run, and the error of
+every field against the FP64 oracle after each of four steps.
 
 Prints one JSON line.  --gpus N > 1 is not supported: the whole domain lives on one GPU.
 """
@@ -31,17 +39,33 @@ HBM_PEAK_GBS = 8000.0
 
 
 def _cpu_threads():
+    """every host core this process may run on (the box's CPU share), or OMP_NUM_THREADS when
+    that is set lower"""
     try:
         n = len(os.sched_getaffinity(0))
     except Exception:
         n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
 
 
-def cpu_baseline(n, steps=2, warmup=1):
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(n, steps=2, warmup=5):
     """The oracle (C++ FP64 restatement of the reference path, BiCGStab + block-local SSOR
     at rtol 1e-5 like the reference's PETSc settings, OpenMP threads standing in for MPI
-    ranks) on the same 128^3 case: `warmup` untimed PISO steps, then `steps` timed ones."""
+    ranks, one SSOR block per thread) on the same 128^3 case and the same part of the run as
+    the GPU's timed steps: `warmup` untimed PISO steps from rest, then `steps` timed ones."""
     from caffe_old_amd import case as cs
     from oracle.pyoracle import KSP_PETSC_LIKE, Oracle
     th = _cpu_threads()
@@ -53,12 +77,12 @@ def cpu_baseline(n, steps=2, warmup=1):
     for _ in range(steps):
         o.solve()
     dt = time.perf_counter() - t0
-    return {"value": steps / dt, "unit": "steps/s", "cores": th, "kind": "port",
+    return {"value": steps / dt, "unit": "steps/s", "cores": th, "kind": "port", "cpu_model": _cpu_model(),
             "sample": f"steps {warmup + 1}-{warmup + steps} of the {n}^3 cavity PISO run from rest ({warmup} untimed), "
-                      f"oracle BiCGStab+SSOR rtol 1e-5, {th} OpenMP threads, {dt:.1f} s timed"}
+                      f"oracle BiCGStab+SSOR rtol 1e-5, {th} OpenMP threads on {_cpu_model()}, {dt:.1f} s timed"}
 
 
-def field_errors(n, mode, inner, net, mom_mode, steps=2):
+def field_errors(n, mode, inner, net, mom_mode, steps=4):
     """Part of the CPU leg, the oracle as the checker: the error of every field against the
     FP64 oracle on this configuration.  A fresh GPU run with the timed run's solve settings
     (true relres 1e-8) and the oracle with tight solves (true relres 1e-12) each take `steps`
@@ -75,15 +99,8 @@ def field_errors(n, mode, inner, net, mom_mode, steps=2):
     g.set_solve_opts("pcorr", mode=mode, rtol=1e-8, maxit=100000, inner_rtol=inner)
     if net is not None:
         g.attach_net(net)
-    # at most 8 SSOR blocks: with 16 the oracle's BiCGStab broke down on the 128^3 p' system
-    o = Oracle(c, rtol=1e-12, maxit=20000, n_blocks=min(th, 8), n_threads=th)
+    o = Oracle(c, rtol=1e-12, maxit=20000, n_blocks=th, n_threads=th)
     t0 = time.perf_counter()
-    for _ in range(steps):
-        g.solve()
-        o.solve()
-    sh = g.shape
-    ug, uo = interior(g.get("u"), sh), interior(o.get("u"), sh)
-    pg, po = interior(g.get("p"), sh), interior(o.get("p"), sh)
 
     def rel(x, y):
         ny = float(np.linalg.norm(y))
@@ -91,12 +108,52 @@ def field_errors(n, mode, inner, net, mom_mode, steps=2):
             return float("nan")
         return float(np.linalg.norm(x - y)) / ny if ny > 0 else float(np.linalg.norm(x))
 
-    out = {comp: rel(ug[i], uo[i]) for i, comp in enumerate("uvw")}
-    out["p"] = rel(pg - pg.mean(), po - po.mean())
+    sh = g.shape
+    per_step = []
+    for _ in range(steps):
+        g.solve()
+        o.solve()
+        ug, uo = interior(g.get("u"), sh), interior(o.get("u"), sh)
+        pg, po = interior(g.get("p"), sh), interior(o.get("p"), sh)
+        e = {comp: rel(ug[i], uo[i]) for i, comp in enumerate("uvw")}
+        e["p"] = rel(pg - pg.mean(), po - po.mean())
+        per_step.append(e)
+    out = dict(per_step[-1])
     g.close()
-    out.update({"steps": steps, "gpu_rtol": 1e-8, "oracle_rtol": 1e-12, "norm": "relative L2 over interior cells",
+    out.update({"steps": steps, "max_over_steps": max(max(e.values()) for e in per_step),
+                "per_step_max": [max(e.values()) for e in per_step], "gpu_rtol": 1e-8, "oracle_rtol": 1e-12,
+                "norm": "relative L2 over interior cells, p with the mean removed",
                 "seconds": round(time.perf_counter() - t0, 1)})
     return out
+
+
+def full_run(n, mode, inner, net, mom_mode):
+    """The configuration as stated: 100 PISO steps of the n^3 cavity from rest, in a fresh
+    context with the timed run's solve settings, timed as a whole (synchronised before and
+    after).  Reported beside the headline's K timed steps."""
+    from caffe_old_amd import case as cs
+    from caffe_old_amd.solver import make_solver
+    c = cs.cavity_piso_128(n, steps=100)
+    g = make_solver(c)
+    g.set_solve_opts("momentum", mode=mom_mode, rtol=1e-8, maxit=5000)
+    g.set_solve_opts("pcorr", mode=mode, rtol=1e-8, maxit=100000, inner_rtol=inner)
+    if net is not None:
+        g.attach_net(net)
+    g.sync()
+    t0 = time.perf_counter()
+    fell = not_conv = 0
+    for _ in range(c.n_solves):
+        g.solve()
+        m, p = g.reports()
+        fell += p["fell_back"]
+        not_conv += 0 if (p["converged"] and m["converged"]) else 1
+    g.sync()
+    dt = time.perf_counter() - t0
+    cont = g.continuityError
+    g.close()
+    return {"steps": c.n_solves, "seconds": round(dt, 3), "steps_per_s": round(c.n_solves / dt, 3),
+            "p_fell_back_steps": fell, "not_converged_steps": not_conv, "final_continuity": cont,
+            "note": "from rest, including the first solves that allocate work and capture graphs"}
 
 
 def coupled_bench(a):
@@ -123,14 +180,20 @@ def coupled_bench(a):
     s.kernel_probe(probe, True)
     s.sync()
     t0 = time.perf_counter()
-    its = []
+    its, steps_detail = [], []
     for _ in range(a.steps):
         s.solve()
-        its.append(s.reports()[1]["iters"])
+        p = s.reports()[1]
+        its.append(p["iters"])
+        steps_detail.append({"iters": p["iters"], "converged": p["converged"], "relres": p["relres"],
+                             "fell_back": p["fell_back"]})
     s.sync()
     elapsed = time.perf_counter() - t0
     kms, klaunches, kbytes = s.kernel_probe_read()
     s.kernel_probe(probe, False)
+    bad = sum(1 for d in steps_detail if not d["converged"] or d["fell_back"])
+    if bad:
+        print(f"bench.py --coupled: {bad} of {a.steps} timed steps did not converge or fell back", file=sys.stderr)
     if a.calibrate_copy:
         s.time_kernel("copy_fp64", reps=5)
     ms_per_step = 1000.0 * elapsed / a.steps
@@ -146,7 +209,8 @@ def coupled_bench(a):
         "vs_baseline": None, "dtype": "f64", "data": "synthetic: lid-driven cavity from rest (no external data)",
         "config": {"workload": f"3D lid-driven cavity {n}^3 uniform hex, Re 100 (mu 1e-2, lid 1), Coupled solver, steady, "
                                f"relaxationFactorMomentum 0.7, 4-variable system solved to true relres 1e-8",
-                   "cells": case.n_cells, "bicgstab_iters_per_step": its,
+                   "cells": case.n_cells, "bicgstab_iters_per_step": its, "steps_detail": steps_detail,
+                   "all_steps_converged_without_fallback": bad == 0,
                    "preconditioner": "block triangle, multigrid V-cycle on the pressure block" if mg else "Jacobi"},
         "roofline": {"bound": "hbm", "kernel": probe, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -265,12 +329,13 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--mode", choices=["auto", "ir", "plain", "learned", "mg"], default="auto",
+    ap.add_argument("--mode", choices=["auto", "ir", "plain", "learned", "mg", "sgs"], default="auto",
                     help="p' solve: learned FCG (auto when trained weights exist for the grid), FP32 IR, plain FP64, "
                          "multigrid BiCGStab")
     ap.add_argument("--mom-mode", choices=["ir", "plain"], default="ir",
                     help="momentum solves: FP32 BiCGStab inside FP64 refinement, or plain FP64")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-full-run", action="store_true", help="skip the 100-step run from rest")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, 5 launches of a plain copy of 8 P doubles (counter calibration)")
@@ -300,8 +365,9 @@ def main():
     wpath = trained_weights_path(a.n)
     if a.mode == "auto":
         a.mode = "learned" if os.path.exists(wpath) else "ir"
-    mode = {"ir": L.SOLVE_IR_FP32, "plain": L.SOLVE_PLAIN_FP64, "learned": L.SOLVE_LEARNED_FCG, "mg": L.SOLVE_MG}[a.mode]
-    inner = {"ir": 1e-2, "plain": 1e-2, "learned": 1e-4, "mg": 1e-2}
+    mode = {"ir": L.SOLVE_IR_FP32, "plain": L.SOLVE_PLAIN_FP64, "learned": L.SOLVE_LEARNED_FCG, "mg": L.SOLVE_MG,
+            "sgs": L.SOLVE_SGS_FP64}[a.mode]
+    inner = {"ir": 1e-2, "plain": 1e-2, "learned": 1e-4, "mg": 1e-2, "sgs": 1e-2}
     case = cs.cavity_piso_128(a.n, steps=a.steps + a.warmup)
     s = make_solver(case)
     mom_mode = {"ir": L.SOLVE_IR_FP32, "plain": L.SOLVE_PLAIN_FP64}[a.mom_mode]
@@ -312,8 +378,11 @@ def main():
     if os.path.exists(wpath):
         net = PressureNet(seed=0).load(wpath)
         s.attach_net(net)
-    # the dominant kernel of the timed steps: the pressure-correction stencil kernel
-    kname = {"learned": "fcg_k1", "ir": "pcorr_cg_k1_fp32", "plain": "pcorr_cg_k1_fp64", "mg": "mg_post0"}[a.mode]
+    # the dominant kernel of the timed steps (the largest total time in the rocprof kernel
+    # trace of the same command): the learned FCG's update and pre-smoothing kernel, the CG
+    # stencil kernel, or the multigrid's fine-level post-smoothing sweep
+    kname = {"learned": "fcg_k2pre", "ir": "pcorr_cg_k1_fp32", "plain": "pcorr_cg_k1_fp64", "mg": "mg_post0",
+             "sgs": "pcorr_cg_k1_fp64"}[a.mode]
 
     def log(*x):
         if not a.quiet:
@@ -339,10 +408,13 @@ def main():
     s.sync()
     t0 = time.perf_counter()
     iters_p = 0
+    fell_steps = not_conv_steps = 0
     for k in range(a.steps):
         s.solve()
         m, p = s.reports()
         iters_p += p["iters"]
+        fell_steps += p["fell_back"]
+        not_conv_steps += 0 if (p["converged"] and m["converged"]) else 1
     s.sync()
     elapsed = time.perf_counter() - t0
     if not probe_in_steps:
@@ -357,18 +429,27 @@ def main():
     rep = s.step_report()
     log(f"timed {a.steps} steps: {ms_per_step:.2f} ms/step, last p' iters {p['iters']}, courant {rep['courant']:.3f}")
 
-    # pressure-correction solve on the same system: plain FP64, FP32-inner / FP64-refinement,
-    # multigrid BiCGStab, learned FCG; each method's second solve is reported (the first
-    # one of a method in a context also allocates its work and captures its graph)
+    # pressure-correction solve on the same system: plain FP64 (Jacobi CG, and CG with the
+    # red-black SGS sweep, the analogue of the reference's PCSOR), FP32-inner / FP64-refinement,
+    # multigrid BiCGStab, learned FCG; each method's second solve is reported (the first one of
+    # a method in a context also allocates its work and captures its graph)
     psolve = {}
-    for name, md, inr in (("plain_fp64", L.SOLVE_PLAIN_FP64, 1e-2), ("ir_fp32", L.SOLVE_IR_FP32, 1e-2),
-                          ("mg", L.SOLVE_MG, 1e-2), ("learned_fcg", L.SOLVE_LEARNED_FCG, 1e-4)):
+    for name, md, inr in (("plain_fp64", L.SOLVE_PLAIN_FP64, 1e-2), ("sgs_fp64", L.SOLVE_SGS_FP64, 1e-2),
+                          ("ir_fp32", L.SOLVE_IR_FP32, 1e-2), ("mg", L.SOLVE_MG, 1e-2),
+                          ("learned_fcg", L.SOLVE_LEARNED_FCG, 1e-4)):
         s.pcorr_assemble()
         s.pcorr_solve(mode=md, rtol=1e-8, maxit=100000, inner_rtol=inr)
         r = s.pcorr_solve(mode=md, rtol=1e-8, maxit=100000, inner_rtol=inr)
         psolve[name] = {"ms": round(r["ms"], 3), "iters": r["iters"], "refinements": r["refinements"],
                         "relres": r["relres"], "fell_back": r["fell_back"]}
     log("p' solves:", psolve)
+
+    def ratio(a_, b_):
+        return round(psolve[a_]["ms"] / psolve[b_]["ms"], 2) if psolve[b_]["ms"] > 0 else None
+
+    speedup = {"learned_vs_sgs_fp64": ratio("sgs_fp64", "learned_fcg"),
+               "learned_vs_plain_fp64": ratio("plain_fp64", "learned_fcg"),
+               "mg_vs_sgs_fp64": ratio("sgs_fp64", "mg"), "learned_vs_mg": ratio("mg", "learned_fcg")}
     # momentum solve (3 components): plain FP64 vs FP32-inner / FP64-refinement, same system
     msolve = {}
     s.save_u0()
@@ -401,18 +482,24 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"plain": "f64", "ir": "f64+f32", "learned": "f64+f32+bf16", "mg": "f64+f32"}[a.mode],
+        "dtype": {"plain": "f64", "ir": "f64+f32", "learned": "f64+f32+bf16", "mg": "f64+f32", "sgs": "f64"}[a.mode],
         "data": "synthetic: lid-driven cavity from rest (no external data)",
         "config": {"workload": f"3D lid-driven cavity {a.n}^3 uniform hex, Re 1000 (mu 1e-3, lid 1), PISO 2 correctors, "
                                f"dt {case.time_step:g} (CFL 0.5), p' solve to true relres 1e-8 ({a.mode})",
-                   "momentum": a.mom_mode, "cells": case.n_cells, "p_solve_ms": psolve, "momentum_solve_ms": msolve},
+                   "momentum": a.mom_mode, "cells": case.n_cells, "p_solve_ms": psolve,
+                   "p_solve_speedup": speedup, "momentum_solve_ms": msolve,
+                   "timed_steps": f"steps {a.warmup + 1}-{a.warmup + a.steps} of a run from rest",
+                   "timed_p_fell_back_steps": fell_steps, "timed_not_converged_steps": not_conv_steps},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "ms_per_launch": round(kms, 5), "bytes_per_launch": kbytes, "launches_timed": klaunches},
     }
+    if not a.no_full_run:
+        log("the configured run: 100 steps from rest ...")
+        out["config"]["full_run_100_steps"] = full_run(a.n, mode, inner[a.mode], net, mom_mode)
     if not a.no_cpu:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(a.n)
+        out["cpu_baseline"] = cpu_baseline(a.n, steps=2, warmup=a.warmup)
         log("field errors against the oracle ...")
         out["field_error_vs_oracle"] = field_errors(a.n, mode, inner[a.mode], net, mom_mode)
     print(json.dumps(out), flush=True)

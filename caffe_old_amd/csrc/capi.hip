@@ -662,6 +662,7 @@ int caffe_kernel_probe(caffe_ctx* x, const char* which, int enable) {
         else if (w == "pcorr_bicg_a_fp32") kind = PROBE_PBI_A_FP32;
         else if (w == "pcorr_bicg_a_fp64") kind = PROBE_PBI_A_FP64;
         else if (w == "mg_post0") kind = PROBE_MG_POST0;
+        else if (w == "fcg_k2pre") kind = PROBE_FCG_K2PRE;
         else { c.err = "unknown probe kernel " + w; return CAFFE_ERR_ARG; }
         if (kind == PROBE_CP_A && !c.cA) { c.err = "not a Coupled case"; return CAFFE_ERR_STATE; }
         HIPCHECK(hipStreamSynchronize(c.stream));
@@ -707,7 +708,9 @@ int caffe_kernel_probe_read(caffe_ctx* x, double* avg_ms, long* launches, double
         // p and v: 11 values per cell (ghost rows, about 0.1% of the cells, not counted).
         // The multigrid post-smoothing sweep on the fine level reads the four FP32 matrix
         // arrays, the FP64 right-hand side and the FP32 iterate and writes the new iterate:
-        // 32 B per cell.
+        // 32 B per cell.  The learned FCG's update and pre-smoothing kernel (k_f_k2pre) reads
+        // e, p, r, q and the four FP32 matrix arrays and writes e, the next r and the
+        // pre-smoothed w: 44 B per cell (r and q at the stencil points come from cache).
         double per = 32.;
         switch (c.probe.which) {
         case PROBE_CP_A: per = 672.; break;
@@ -715,6 +718,7 @@ int caffe_kernel_probe_read(caffe_ctx* x, double* avg_ms, long* launches, double
         case PROBE_PBI_A_FP32: per = 44.; break;
         case PROBE_PBI_A_FP64: per = 88.; break;
         case PROBE_MG_POST0: per = 32.; break;
+        case PROBE_FCG_K2PRE: per = 44.; break;
         default: break;
         }
         if (bytes) *bytes = per * (double)c.D.N;

@@ -21,7 +21,7 @@ enum {
 enum { FL_MDONE = 0, FL_PDONE = 1, FL_MITER = 2, FL_PITER = 3, N_FLAGS = 8 };
 
 enum { PROBE_NONE = 0, PROBE_FCG_K1 = 1, PROBE_CG_K1_FP32 = 2, PROBE_CG_K1_FP64 = 3, PROBE_CP_A = 4,
-       PROBE_PBI_A_FP32 = 5, PROBE_PBI_A_FP64 = 6, PROBE_MG_POST0 = 7 };
+       PROBE_PBI_A_FP32 = 5, PROBE_PBI_A_FP64 = 6, PROBE_MG_POST0 = 7, PROBE_FCG_K2PRE = 8 };
 
 // Coupled system (Coupled.cpp:54-228): coefficient planes of P doubles per cell, each group
 // of seven in the neighbour order own, E, W, N, S, T, B
@@ -67,6 +67,7 @@ struct MgHier {
     float* g1 = nullptr;             // first-sweep values of the ghost cells, one per ghost row
     int g1n = 0;
     int lastIts = 0;                 // iterations of the last solve's first round
+    int lastItsGauge = 0;            // the same for the Coupled null-vector solve
     hipGraphExec_t iter2 = nullptr;  // two BiCGStab iterations, captured once per context
     bool graphOff = false;           // capture failed: iterations are launched one by one
 };

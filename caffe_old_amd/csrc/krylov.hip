@@ -933,14 +933,15 @@ struct CpSolver {
     double* rt;
     double* dx;
     bool mg;
-    // the multigrid runs queue first one iteration fewer than the last solve's first run took
-    // (the first solve 4, a restart 2): a queued iteration past convergence costs more than
-    // one more read-back
-    int run(const double* b, double shift, double* x, double rtol, int maxit, int* done, bool restart) {
+    // the multigrid runs queue first one iteration fewer than the last solve of the same kind
+    // took in its first run (the first solve 4, a restart 2): a queued iteration past
+    // convergence costs more than one more read-back.  lastIts is the main solve's count or
+    // the null-vector solve's, so neither sizes the other's chunks.
+    int run(const double* b, double shift, double* x, double rtol, int maxit, int* done, bool restart, int& lastIts) {
         if (!mg) return runCoupled(c, A, b, shift, x, rtol, maxit, 8, done);
-        const int first = restart ? 2 : (c.mg.lastIts > 1 ? c.mg.lastIts - 1 : 4);
+        const int first = restart ? 2 : (lastIts > 1 ? lastIts - 1 : 4);
         const int it = cp_mg_run(c, A, b, shift, x, rtol, maxit, first, done);
-        if (!restart) c.mg.lastIts = it;
+        if (!restart) lastIts = it;
         return it;
     }
     // r = b - shift - A x: returns ||r|| (without the component along the continuity rows'
@@ -965,15 +966,15 @@ struct CpSolver {
     // A x = b - shift from x = 0 to a true relative residual rtol: BiCGStab, restarted from the
     // true residual while that gains; returns the relative residual
     double solve(const double* b, double shift, double* x, double rtol, int maxit, int* iters, int* restarts,
-                 double* xsum) {
+                 double* xsum, int& lastIts) {
         int done = 0;
-        *iters += run(b, shift, x, rtol, maxit - *iters, &done, false);
+        *iters += run(b, shift, x, rtol, maxit - *iters, &done, false, lastIts);
         double rr, bb;
         resid(b, shift, x, &rr, &bb, xsum);
         double rel = bb > 0. ? rr / bb : 0.;
         for (int k = 0; rel > rtol && k < 20 && *iters < maxit; ++k) {
             const double sub = std::min(0.5, 0.5 * rtol / rel);
-            *iters += run(rt, 0., dx, sub, maxit - *iters, &done, true);
+            *iters += run(rt, 0., dx, sub, maxit - *iters, &done, true, lastIts);
             hipLaunchKernelGGL(k_cp_add, dim3(g), dim3(KBS), 0, c.stream, c.D, x, (const double*)dx);
             const double prev = rel;
             resid(b, shift, x, &rr, &bb, xsum);
@@ -1005,7 +1006,7 @@ int coupled_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep) {
     const double shift = c.singularP ? S.meanP(c.cb) : 0.;
     int iters = 0, restarts = 0;
     double xs = 0.;
-    double rel = S.solve(c.cb, shift, c.cx, o.rtol, o.maxit, &iters, &restarts, &xs);
+    double rel = S.solve(c.cb, shift, c.cx, o.rtol, o.maxit, &iters, &restarts, &xs, c.mg.lastIts);
     if (c.singularP) {
         const double xbar = xs / (double)D.N;
         const int g = S.g;
@@ -1020,7 +1021,7 @@ int coupled_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep) {
         if (std::fabs(xbar) * gnorm > 1e-3 * o.rtol * bb) {
             double zs = 0.;
             const double zsh = S.meanP(gv);
-            const double zrel = S.solve(gv, zsh, z, o.rtol, o.maxit, &iters, &restarts, &zs);
+            const double zrel = S.solve(gv, zsh, z, o.rtol, o.maxit, &iters, &restarts, &zs, c.mg.lastItsGauge);
             zbar = zs / (double)D.N;
             plain = !(zrel <= o.rtol) || !std::isfinite(zbar);   // no null vector: shift p only
         }

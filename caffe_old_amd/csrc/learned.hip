@@ -579,9 +579,14 @@ int learned_fcg(Ctx& c, double shift, double rtol, int maxit, double delta, int*
                                           p0, p1, q0, q1, (const double*)partA, g, partB, st, lc);
                 else
                     hipLaunchKernelGGL(k_f_k1, dim3(g), dim3(FB), 0, s, D, S, cm, z, p0, p1, q0, q1, partA, g, partB, st, lc);
-                hipLaunchKernelGGL(k_f_k2pre, dim3(g), dim3(FB), 0, s, D, S, e, (const float*)r, (const float*)r1, w,
-                                   (const float*)p0, (const float*)p1, (const float*)q0, (const float*)q1,
-                                   (const double*)partB, g, partC, st);
+                if (c.probe.slot(PROBE_FCG_K2PRE, e0, e1))
+                    hipExtLaunchKernelGGL(k_f_k2pre, dim3(g), dim3(FB), 0, s, e0, e1, 0, D, S, e, (const float*)r,
+                                          (const float*)r1, w, (const float*)p0, (const float*)p1, (const float*)q0,
+                                          (const float*)q1, (const double*)partB, g, partC, st);
+                else
+                    hipLaunchKernelGGL(k_f_k2pre, dim3(g), dim3(FB), 0, s, D, S, e, (const float*)r, (const float*)r1, w,
+                                       (const float*)p0, (const float*)p1, (const float*)q0, (const float*)q1,
+                                       (const double*)partB, g, partC, st);
             }
             HIPCHECK(hipMemcpyAsync(c.hscal + SC_CG, st, 8 * sizeof(double), hipMemcpyDeviceToHost, s));
             HIPCHECK(hipStreamSynchronize(s));

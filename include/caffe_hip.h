@@ -178,7 +178,8 @@ int caffe_time_kernel(caffe_ctx* ctx, const char* which, int reps, double* ms, d
  * "pcorr_bicg_a_fp32" / "pcorr_bicg_a_fp64" (first kernel of the Jacobi p' BiCGStab:
  * ghost-row systems in the plain mode, indefinite systems), "mg_post0" (fine-level
  * post-smoothing sweep of the multigrid cycle: CAFFE_SOLVE_MG, and ghost-row systems in
- * the FP32 modes); enable 0 stops recording.  _read returns the average duration of the
+ * the FP32 modes), "fcg_k2pre" (the update and pre-smoothing kernel of the learned flexible CG);
+ * enable 0 stops recording.  _read returns the average duration of the
  * launches that ran (launches queued past convergence return at once and are left out),
  * their number, and the kernel's algorithmic HBM bytes per launch.  No counterpart in the
  * reference. */
