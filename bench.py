@@ -254,8 +254,10 @@ def workload_bench(a):
         s = make_solver(case)
     else:
         case = cs.skewed_cavity(256, steps=a.steps + a.warmup)
+        case.non_orth = a.non_orth
         desc = ("256^3 lid-driven cavity, interior nodes moved by up to 0.15 h (Philox seed 0), Re 1000, PISO 2 "
-                "correctors, dt 0.5/256")
+                "correctors, dt 0.5/256, non-orthogonal term " +
+                ("corrected (grad u faces filled)" if a.non_orth else "as the reference (zero)"))
         s = make_solver(case, uniform=False)
     net = None
     wpath = trained_weights_path(128)
@@ -315,6 +317,7 @@ def workload_bench(a):
         "dtype": {"plain": "f64", "ir": "f64+f32", "learned": "f64+f32+bf16", "mg": "f64+f32"}[mode_name],
         "data": "synthetic (no external data)",
         "config": {"workload": desc + f", p' solve to true relres 1e-8 ({mode_name})", "cells": case.n_cells,
+                   "non_orthogonal_correction": "n/a (uniform mesh)" if ib else ("corrected" if a.non_orth else "reference"),
                    "device_bytes": s.device_bytes(), "steps_detail": its, "p_solve_ms": psolve},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "ms_per_launch": round(kms, 5),
@@ -336,6 +339,8 @@ def main():
                     help="momentum solves: FP32 BiCGStab inside FP64 refinement, or plain FP64")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-full-run", action="store_true", help="skip the 100-step run from rest")
+    ap.add_argument("--non-orth", action="store_true",
+                    help="--workload skewed: the corrected mode of the non-orthogonal momentum term")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, 5 launches of a plain copy of 8 P doubles (counter calibration)")

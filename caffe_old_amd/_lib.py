@@ -57,7 +57,7 @@ class CaffeCase(C.Structure):
         ("bcType", C.c_int * 6), ("bcVec", C.c_double * 18), ("bcVal", C.c_double * 6),
         ("uInit", C.c_double * 3), ("pInit", C.c_double),
         ("sphereR", C.c_double), ("sphereC", C.c_double * 3),
-        ("uniform", C.c_int),
+        ("uniform", C.c_int), ("nonOrth", C.c_int),
     ]
 
 

@@ -69,6 +69,10 @@ class Case:
     sphere: Optional[Tuple[float, Tuple[float, float, float]]] = None
     time_step: float = 1.0
     max_iterations: int = 2
+    # corrected mode: the non-orthogonal term mu_f (grad u)_f . c_f of Simple.cpp:153-158 with
+    # the faces of grad u filled (interior: volume-weighted interpolation, domain faces: the
+    # boundary cell's gradient); False keeps the reference, where those faces stay zero
+    non_orth: bool = False
 
     def __post_init__(self):
         self.nodes = np.ascontiguousarray(self.nodes, dtype=np.float64)

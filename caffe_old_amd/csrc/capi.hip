@@ -181,6 +181,10 @@ int setup(Ctx& c, const caffe_case* cs, const double* nodes) {
         for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) g.fc[a][b] = nx();
         for (int a = 0; a < 3; ++a) g.dF[a] = nx();
         for (int a = 0; a < 3; ++a) g.gF[a] = nx();
+        if (cs->nonOrth) {   // the corrected mode's c metric: 9 more arrays
+            double* mc = dalloc(c, 9 * D.P);
+            for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) g.cF[a][b] = mc + (3 * a + b) * (long)D.P;
+        }
         compute_metrics(c, c.dnodes);
     }
 
@@ -191,6 +195,8 @@ int setup(Ctx& c, const caffe_case* cs, const double* nodes) {
         c.mF[q] = dalloc(c, D.P); c.mF2[q] = dalloc(c, D.P);
     }
     for (int q = 0; q < 9; ++q) c.gu[q] = nullptr;
+    // the corrected mode keeps grad u (Simple.cpp:355) for the next momentum assembly
+    if (cs->nonOrth && !c.uniform) for (int q = 0; q < 9; ++q) c.gu[q] = dalloc(c, D.P);
     c.p = dalloc(c, D.P); c.pc = dalloc(c, D.P); c.d = dalloc(c, D.P); c.mflow = dalloc(c, D.P);
     c.st = talloc<int8_t>(c, D.P);
     {

@@ -62,16 +62,42 @@ void launch_halo(Ctx& c, double* const* f, int nf, const int* types) {
 // Simple::computeMomentum assembly loop (Simple.cpp:117-179) with the BC fold of
 // PrimitiveBoundaryCondition::setImplicitBoundaryCoefficients (PrimitiveBoundaryCondition.tpp:144-255).
 // The non-orthogonal term mu_f (grad u)_f . c_f is identically zero in the reference:
-// gradUField_ faces are never assigned (Simple.cpp:153-158, 355), so it is not evaluated.
+// gradUField_ faces are never assigned (Simple.cpp:153-158, 355), so in reference mode it
+// is not evaluated.  In the corrected mode (caffe_case.nonOrth, general meshes) the face
+// value of grad u is the volume-weighted interpolation of the two cell gradients on an
+// interior face and the cell's own gradient on a domain face, and c is the face metric
+// stored as seen from the low-side cell (the high-side cell's is its negation).
 struct MomArgs {
     const int8_t* st;
     const double *mI, *mJ, *mK;
     const double *u[3], *u0[3], *gp[3];
+    const double* gu[9];      // grad u cells, row-major (d u_r / d x_c at 3 r + c); corrected mode
     double* A; double* b; double* d;
     double rho, mu, om, dt;
     int steady;
+    int nonOrth;
     int uT[6];
 };
+
+// (grad u)_f . c over one face in the corrected mode: the tensor times the vector by rows
+// (Tensor3D.cpp:223-228), the tensor interpolated at an interior face (FvScheme.tpp:12-40)
+__device__ inline void nonOrthFace(const Geo& g, const MomArgs& a, long p, long s, int dir, bool hi, bool bnd,
+                                   double* out) {
+    double T[9];
+    if (bnd) {
+        for (int q = 0; q < 9; ++q) T[q] = a.gu[q][p];
+    } else if (hi) {
+        const double w = G<false>::gw(g, dir, p + s);
+        for (int q = 0; q < 9; ++q) T[q] = w * a.gu[q][p] + (1. - w) * a.gu[q][p + s];
+    } else {
+        const double w = G<false>::gw(g, dir, p);
+        for (int q = 0; q < 9; ++q) T[q] = w * a.gu[q][p - s] + (1. - w) * a.gu[q][p];
+    }
+    const long f = hi ? p + s : p;
+    V3 c = mk(g.cF[dir][0][f], g.cF[dir][1][f], g.cF[dir][2][f]);
+    if (!hi) c = -c;
+    for (int r = 0; r < 3; ++r) out[r] = T[3 * r] * c.x + T[3 * r + 1] * c.y + T[3 * r + 2] * c.z;
+}
 
 template <bool U>
 __global__ void k_mom_assemble(Geo g, MomArgs a) {
@@ -111,6 +137,19 @@ __global__ void k_mom_assemble(Geo g, MomArgs a) {
     bx += a.gp[0][p] * mv; by += a.gp[1][p] * mv; bz += a.gp[2][p] * mv;
     const double r = (1. - a.om) * A[0];
     bx += a.u[0][p] * r; by += a.u[1][p] * r; bz += a.u[2][p] * r;
+    if (!U && a.nonOrth) {   // Simple.cpp:153-158, faces E, W, N, S, T, B
+        const long str[3] = {1, sJ, sK};
+        const int ijk[3] = {i, j, k}, nn[3] = {D.nI, D.nJ, D.nK};
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            const int dir = f >> 1;
+            const bool hi = (f & 1) == 0;
+            const bool bnd = hi ? ijk[dir] == nn[dir] - 1 : ijk[dir] == 0;
+            double t[3];
+            nonOrthFace(g, a, p, str[dir], dir, hi, bnd, t);
+            bx += mu * t[0]; by += mu * t[1]; bz += mu * t[2];
+        }
+    }
     // boundary fold, in the reference's face order E, W, N, S, T, B
     const long nb[7] = {0, p + 1, p - 1, p + sJ, p - sJ, p + sK, p - sK};
     const bool atB[7] = {false, i == D.nI - 1, i == 0, j == D.nJ - 1, j == 0, k == D.nK - 1, k == 0};
@@ -139,8 +178,10 @@ void launch_mom_assemble(Ctx& c, double dt) {
     MomArgs a;
     a.st = c.st; a.mI = c.mF[0]; a.mJ = c.mF[1]; a.mK = c.mF[2];
     for (int q = 0; q < 3; ++q) { a.u[q] = c.u[q]; a.u0[q] = c.u0[q]; a.gp[q] = c.gp[q]; }
+    for (int q = 0; q < 9; ++q) a.gu[q] = c.gu[q];
     a.A = c.Am; a.b = c.bm; a.d = c.d;
     a.rho = c.cs.rho; a.mu = c.cs.mu; a.om = c.cs.alphaU; a.dt = dt; a.steady = c.cs.steady;
+    a.nonOrth = (c.cs.nonOrth && !c.uniform && c.gu[0]) ? 1 : 0;
     for (int q = 0; q < 6; ++q) a.uT[q] = c.uT[q];
     int nb = blocksFor(c.D.N, BS);
     if (c.uniform) hipLaunchKernelGGL(k_mom_assemble<true>, dim3(nb), dim3(BS), 0, c.stream, c.geo, a);

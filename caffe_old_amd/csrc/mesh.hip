@@ -53,6 +53,7 @@ struct MeshOut {
     double* fc[3][3];
     double* d[3];
     double* g[3];
+    double* c[3][3];      // c = S - r d seen from the low-side cell (null: not computed)
 };
 
 __device__ inline V3 node(const Dims& D, const double* n, int i, int j, int k) {
@@ -107,26 +108,35 @@ __global__ void k_face_metrics(Dims D, MeshOut m, int dir) {
     V3 S = mk(m.S[dir][0][f], m.S[dir][1][f], m.S[dir][2][f]);
     V3 fcv = mk(m.fc[dir][0][f], m.fc[dir][1][f], m.fc[dir][2][f]);
     double dval, gval;
+    // c = S - r (S.S) / (S.r) (HexaFvmMesh.cpp:879-884; Vector3D's operator/ multiplies by
+    // the reciprocal), stored as seen from the low-side cell: the high-side cell's c is its
+    // negation bit for bit, and on a west/south/bottom domain face (seen only from the cell
+    // above it) the negation of that cell's c is stored
+    V3 cv;
     if (idx == 0) {  // west/south/bottom domain face, seen from cell f with normal -S
         V3 Sw = -S;
         V3 xcv = mk(m.xc[0][f], m.xc[1][f], m.xc[2][f]);
         V3 rf = fcv - xcv;
         dval = dot(Sw, Sw) / dot(Sw, rf);
         gval = 0.;
+        cv = -(Sw - rf * dot(Sw, Sw) / dot(Sw, rf));
     } else if (idx == nmax) {  // east/north/top domain face, seen from cell f - s
         V3 xcv = mk(m.xc[0][f - s], m.xc[1][f - s], m.xc[2][f - s]);
         V3 rf = fcv - xcv;
         dval = dot(S, S) / dot(S, rf);
         gval = 0.;
+        cv = S - rf * dot(S, S) / dot(S, rf);
     } else {
         V3 lo = mk(m.xc[0][f - s], m.xc[1][f - s], m.xc[2][f - s]);
         V3 hi = mk(m.xc[0][f], m.xc[1][f], m.xc[2][f]);
         V3 rc = hi - lo;
         dval = dot(S, S) / dot(S, rc);
         gval = m.vol[f] / (m.vol[f] + m.vol[f - s]);
+        cv = S - rc * dot(S, S) / dot(S, rc);
     }
     m.d[dir][f] = dval;
     m.g[dir][f] = gval;
+    if (m.c[dir][0]) { m.c[dir][0][f] = cv.x; m.c[dir][1][f] = cv.y; m.c[dir][2][f] = cv.z; }
 }
 
 void compute_metrics(Ctx& c, const double* dNodes) {
@@ -137,7 +147,11 @@ void compute_metrics(Ctx& c, const double* dNodes) {
         m.xc[a] = const_cast<double*>(c.geo.xc[a]);
         m.d[a] = const_cast<double*>(c.geo.dF[a]);
         m.g[a] = const_cast<double*>(c.geo.gF[a]);
-        for (int b = 0; b < 3; ++b) { m.S[a][b] = const_cast<double*>(c.geo.S[a][b]); m.fc[a][b] = const_cast<double*>(c.geo.fc[a][b]); }
+        for (int b = 0; b < 3; ++b) {
+            m.S[a][b] = const_cast<double*>(c.geo.S[a][b]);
+            m.fc[a][b] = const_cast<double*>(c.geo.fc[a][b]);
+            m.c[a][b] = const_cast<double*>(c.geo.cF[a][b]);
+        }
     }
     const Dims& D = c.D;
     hipLaunchKernelGGL(k_cells, dim3(blocksFor(D.N, 256)), dim3(256), 0, c.stream, D, m);

@@ -53,6 +53,7 @@ class Solver:
         if uniform is None:
             uniform = _is_uniform_box(case.nodes)
         cs.uniform = 1 if uniform else 0
+        cs.nonOrth = 1 if getattr(case, "non_orth", False) else 0
         self.uniform = bool(uniform)
         nodes = np.ascontiguousarray(case.nodes, dtype=np.float64)
         self._check(self.lib.caffe_case_setup(self._h, C.byref(cs), _dp(nodes)), "caffe_case_setup")

@@ -102,6 +102,13 @@ typedef struct {
     double pInit;
     double sphereR, sphereC[3];
     int uniform;             /* 1: cells form a uniform box; metrics in closed form */
+    int nonOrth;             /* 1: corrected mode: the momentum right-hand side gets the non-orthogonal
+                                term mu_f (grad u)_f . c_f of Simple.cpp:153-158 with the faces of
+                                grad u filled (interior: volume-weighted interpolation of the cell
+                                gradients, domain faces: the boundary cell's gradient).  0: reference
+                                mode, where those faces are never assigned and the term vanishes.
+                                No effect on uniform boxes (c = 0).  No counterpart key in the
+                                reference's case.info. */
 } caffe_case;
 
 typedef struct {

@@ -77,6 +77,8 @@ struct T3 {
 };
 inline T3 operator+(T3 l, const T3& r) { return l += r; }
 inline T3 operator/(T3 l, double s) { for (int i = 0; i < 9; ++i) l.a[i] /= s; return l; }
+// scalar times tensor, each entry (Tensor3D.cpp operator*=)
+inline T3 operator*(double s, T3 t) { for (int i = 0; i < 9; ++i) t.a[i] *= s; return t; }
 // tensor(u, v) = u v^T (Vector3D.cpp:176-181)
 inline T3 tensor(const V3& u, const V3& v) {
     T3 t;
@@ -947,6 +949,24 @@ struct Oracle {
         interpolate(p); interpolate(u);
         gradScalar(p, gradP);
         gradVector(u, gradU);
+        if (cs.nonOrth) gradUFaces();
+    }
+
+    // Corrected mode only (the reference never assigns gradUField_'s faces, Simple.cpp:355):
+    // interior faces of grad u by the volume-weighted interpolation of FvScheme.tpp:12-40,
+    // domain faces from the boundary cell's own gradient.
+    void gradUFaces() {
+        interpolate(gradU);
+        const Mesh& M = mesh;
+        for (int k = 0; k < M.nK; ++k) for (int j = 0; j < M.nJ; ++j) for (int i = 0; i < M.nI; ++i) {
+            const T3 g = gradU(i, j, k);
+            if (i == 0) gradU.faceW(i, j, k) = g;
+            if (i == M.nI - 1) gradU.faceE(i, j, k) = g;
+            if (j == 0) gradU.faceS(i, j, k) = g;
+            if (j == M.nJ - 1) gradU.faceN(i, j, k) = g;
+            if (k == 0) gradU.faceB(i, j, k) = g;
+            if (k == M.nK - 1) gradU.faceT(i, j, k) = g;
+        }
     }
 
     // Simple.cpp:410-449
@@ -1495,6 +1515,11 @@ long oc_set(void* h, const char* name, const double* in) {
     if (n == "mI" || n == "mJ" || n == "mK") {
         o.importFaces(o.mflow, n == "mI" ? 0 : (n == "mJ" ? 1 : 2), in, [](double& v, double x) { v = x; });
         return (long)P;
+    }
+    if (n == "gradU") {  // cells; in the corrected mode the faces follow as correct() forms them
+        for (int c = 0; c < 9; ++c) o.importCells(o.gradU, in + c * P, [c](T3& t, double x) { t.a[c] = x; });
+        if (o.cs.nonOrth) o.gradUFaces();
+        return (long)(9 * P);
     }
     return -1;
 }

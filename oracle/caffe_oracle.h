@@ -48,6 +48,11 @@ typedef struct {
     int maxit;
     int nBlocks;                 /* row blocks of the block-local SSOR ("MPI ranks") */
     int nThreads;                /* OpenMP threads for assembly / Krylov (1 = serial, deterministic) */
+    int nonOrth;                 /* 1: corrected mode, the non-orthogonal term mu_f (grad u)_f . c_f of
+                                    Simple.cpp:153-158 with the faces of grad u filled (interior:
+                                    volume-weighted interpolation; domain faces: the cell's own
+                                    gradient); 0: reference mode, the faces stay zero (the term
+                                    vanishes, as in the reference) */
 } oc_case;
 
 void*  oc_create(const oc_case* c, const double* nodes /* (nI+1)(nJ+1)(nK+1) x 3, i fastest */);

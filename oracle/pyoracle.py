@@ -29,7 +29,7 @@ class OcCase(C.Structure):
         ("uInit", C.c_double * 3), ("pInit", C.c_double),
         ("sphereR", C.c_double), ("sphereC", C.c_double * 3),
         ("kspKind", C.c_int), ("rtol", C.c_double), ("maxit", C.c_int),
-        ("nBlocks", C.c_int), ("nThreads", C.c_int),
+        ("nBlocks", C.c_int), ("nThreads", C.c_int), ("nonOrth", C.c_int),
     ]
 
 
@@ -144,6 +144,7 @@ class Oracle:
             for q in range(3):
                 c.sphereC[q] = case.sphere[1][q]
         c.kspKind, c.rtol, c.maxit, c.nBlocks, c.nThreads = ksp, rtol, maxit, n_blocks, n_threads
+        c.nonOrth = 1 if getattr(case, "non_orth", False) else 0
         self._c = c
         nodes = np.ascontiguousarray(case.nodes, dtype=np.float64)
         self.h = lib().oc_create(C.byref(c), _dp(nodes))
