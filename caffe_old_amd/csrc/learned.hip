@@ -224,7 +224,7 @@ __global__ void k_spmv_f(Dims D, SymF S, const float* x, float* y) {
 }
 
 // ---------------------------------------------------------------- host: cycle and loss
-enum { FV_R = 0, FV_E, FV_W, FV_Y, FV_Z, FV_P0, FV_P1, FV_Q0, FV_Q1, FV_H, FV_AE, FV_XS, FV_BS, FV_R1, N_FV };
+enum { FV_R = 0, FV_E, FV_W, FV_Y, FV_Z, FV_P0, FV_P1, FV_Q0, FV_Q1, FV_H, FV_AE, FV_XS, FV_BS, FV_R1, FV_DI, N_FV };
 
 void tg_ensure_fine(Net& n, const Dims& D) {
     if (n.fineP == D.P && n.fw) return;
@@ -308,11 +308,14 @@ static double sampleNorm(Net& n, hipStream_t s, const Sample& smp) {
 }
 
 // ---------------------------------------------------------------- flexible CG
-enum { F_RR = 0, F_BB, F_IT, F_DONE, F_PQ, F_SQ, F_PAR, F_FIRSTRR, F_SR, F_PR };
-struct FcgCtl { double tol2; int maxit; double nLive; };
+// state slots; the round's loop limits F_TOL2 and F_MAXIT live on the device, so the
+// iteration's launches are the same in every round
+enum { F_RR = 0, F_BB, F_IT, F_DONE, F_PQ, F_SQ, F_PAR, F_FIRSTRR, F_SR, F_PR, F_TOL2, F_MAXIT };
+struct FcgCtl { double nLive; };   // live cells of a singular system (0: nonsingular)
 
-// r = s r64 (live cells), e = 0; partials (r,r), sum r
-__global__ void k_f_init(Dims D, const double* r64, double s, float* r, float* e, double* part, double* st) {
+// r = s r64 (live cells), e = 0; partials (r,r), sum r; the round's loop limits
+__global__ void k_f_init(Dims D, const double* r64, double s, float* r, float* e, double* part, double* st, double tol2,
+                         int maxit) {
     double v[2] = {0., 0.};
     FOR_FINE(D, q, p) {
         const float rr = (float)(s * r64[p]);
@@ -322,7 +325,9 @@ __global__ void k_f_init(Dims D, const double* r64, double s, float* r, float* e
         v[1] += (double)rr;
     }
     writePartials<FB, 2>(v, part);
-    if (blockIdx.x == 0 && threadIdx.x == 0) { st[F_IT] = 0.; st[F_DONE] = 0.; st[F_BB] = -1.; }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st[F_IT] = 0.; st[F_DONE] = 0.; st[F_BB] = -1.; st[F_TOL2] = tol2; st[F_MAXIT] = (double)maxit;
+    }
 }
 
 // The residual alternates between two buffers by iteration parity (r[F_IT & 1] is the
@@ -330,10 +335,10 @@ __global__ void k_f_init(Dims D, const double* r64, double s, float* r, float* e
 // still read the current one.
 
 // w = D^-1 (r - A w D^-1 r) for the first iteration of a round (r = r0)
-__global__ void k_f_pre0(Dims D, SymF S, const float* r, float* w, const double* st) {
+__global__ void k_f_pre0(Dims D, SymF S, const float* dinv, const float* r, float* w, const double* st) {
     if (st[F_DONE] != 0.) return;
-    auto z1 = [&](int c) { return NET_OMEGA * S.inv(c) * r[c]; };
-    FOR_FINE(D, q, p) { w[p] = S.inv(p) * (r[p] - S.apply(z1, p)); }
+    auto z1 = [&](int c) { return NET_OMEGA * dinv[c] * r[c]; };
+    FOR_FINE(D, q, p) { w[p] = dinv[p] * (r[p] - S.apply(z1, p)); }
 }
 
 // Convergence check on the partials of (r,r), sum r (from k_f_init or k_f_k2pre), then
@@ -342,12 +347,12 @@ __global__ void k_f_pre0(Dims D, SymF S, const float* r, float* w, const double*
 // would otherwise split at the barrier.  When the flag is read as set, the partials
 // already meet the criterion, so every wave reaches the same decision.
 __global__ void k_f_restrict(Dims D, const float* w, float* v, const double* partIn, int nb, double* st, FcgCtl lc) {
-    const double sBb = st[F_BB], sDone = st[F_DONE], sIt = st[F_IT];
+    const double sBb = st[F_BB], sDone = st[F_DONE], sIt = st[F_IT], tol2 = st[F_TOL2], maxit = st[F_MAXIT];
     double t[2];
     readPartials<256, 2>(partIn, nb, t);
     const double rr = lc.nLive > 0. ? fmax(t[0] - t[1] * t[1] / lc.nLive, 0.) : t[0];
     const double bb = sBb < 0. ? rr : sBb;
-    const bool done = sDone != 0. || rr <= lc.tol2 * bb || sIt >= lc.maxit || rr == 0.;
+    const bool done = sDone != 0. || rr <= tol2 * bb || sIt >= maxit || rr == 0.;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st[F_BB] = bb; st[F_RR] = rr; st[F_SR] = t[1];
         if (sIt == 1.) st[F_FIRSTRR] = rr;
@@ -358,7 +363,8 @@ __global__ void k_f_restrict(Dims D, const float* w, float* v, const double* par
 }
 
 // y = w D^-1 r + P c
-__global__ void k_f_post1(Dims D, SymF S, const float* r0, const float* r1, const float* c, float* y, const double* st) {
+__global__ void k_f_post1(Dims D, const float* dinv, const float* r0, const float* r1, const float* c, float* y,
+                          const double* st) {
     const double sDone = st[F_DONE], sIt = st[F_IT];
     if (sDone != 0.) return;
     const float* r = ((int)sIt & 1) ? r1 : r0;
@@ -366,13 +372,13 @@ __global__ void k_f_post1(Dims D, SymF S, const float* r0, const float* r1, cons
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < D.N; q += gridDim.x * blockDim.x) {
         int i, j, k;
         const int p = D.cell(q, i, j, k);
-        y[p] = NET_OMEGA * S.inv(p) * r[p] + prolongAt(c, cx, cy, cz, i, j, k);
+        y[p] = NET_OMEGA * dinv[p] * r[p] + prolongAt(c, cx, cy, cz, i, j, k);
     }
 }
 
 // z = y + w D^-1 (r - A y); partials (z, q_old), sum z, (z, r)
-__global__ void k_f_post2(Dims D, SymF S, const float* r0, const float* r1, const float* y, float* z, const float* q0,
-                          const float* q1, double* part, const double* st) {
+__global__ void k_f_post2(Dims D, SymF S, const float* dinv, const float* r0, const float* r1, const float* y, float* z,
+                          const float* q0, const float* q1, double* part, const double* st) {
     const double sDone = st[F_DONE], sIt = st[F_IT];
     if (sDone != 0.) return;
     const int it = (int)sIt;
@@ -382,7 +388,7 @@ __global__ void k_f_post2(Dims D, SymF S, const float* r0, const float* r1, cons
     auto yf = [&](int c) { return y[c]; };
     FOR_FINE(D, q, p) {
         const float rp = r[p];
-        const float zz = y[p] + NET_OMEGA * S.inv(p) * (rp - S.apply(yf, p));
+        const float zz = y[p] + NET_OMEGA * dinv[p] * (rp - S.apply(yf, p));
         z[p] = zz;
         v[0] += (double)zz * (double)qold[p];
         v[1] += (double)zz;
@@ -480,38 +486,151 @@ __global__ void k_f_k1(Dims D, SymF S, ColMap cm, const float* z, float* p0, flo
 // partials (r',r'), sum r'; and the next iteration's pre-smoothing from r' formed at the
 // stencil points: w = D^-1 (r' - A w D^-1 r').  The convergence test on these partials
 // is made by the next k_f_restrict, so a w computed after the last iteration is unused.
-__global__ void k_f_k2pre(Dims D, SymF S, float* e, const float* r0, const float* r1, float* w, const float* p0,
-                          const float* p1, const float* q0, const float* q1, const double* partIn, int nb,
-                          double* partOut, double* st) {
+// Column tiles as in k_f_k1: each thread walks its column up k with z1 = w D^-1 r' of the
+// k - 1, k and k + 1 cells in registers; the loads of the next plane (r, q and D^-1 at the
+// four side neighbours and above, the seven matrix entries, e and p) are issued before the
+// current cell's stores.  D^-1 comes from the array k_f_dinv fills once per solve (the
+// same values SymF::inv gives); the first plane's loads go ahead of the partial sums.
+// The arithmetic is SymF::apply's, in its order.
+__global__ void k_f_k2pre(Dims D, SymF S, ColMap cm, const float* __restrict__ dinv, float* e, const float* r0,
+                          const float* r1, float* w, const float* p0, const float* p1, const float* q0, const float* q1,
+                          const double* partIn, int nb, double* partOut, double* st) {
     const double sDone = st[F_DONE], sPar = st[F_PAR], sPr = st[F_PR], sIt = st[F_IT];
     if (sDone != 0.) return;
+    const int par = (int)sPar;               // this iteration's parity, set by k_f_k1
+    const float* __restrict__ pv = par ? p1 : p0;
+    const float* __restrict__ qv = par ? q1 : q0;
+    const float* __restrict__ rc = par ? r1 : r0;
+    float* __restrict__ rn = (float*)(par ? r0 : r1);
+    const int P = S.P, sJ = S.sJ, sK = S.sK;
+    const float* __restrict__ A = S.A;
+    struct Plane { float rs[4], qs[4], ds[4], ru, qu, du, a[7], ee, pp; };   // sides W E S N, k + 1 (u)
+    auto loadPlane = [&](int p, Plane& L) {
+        const int o[4] = {-1, 1, -sJ, sJ};
+#pragma unroll
+        for (int m = 0; m < 4; ++m) { L.rs[m] = rc[p + o[m]]; L.qs[m] = qv[p + o[m]]; L.ds[m] = dinv[p + o[m]]; }
+        L.ru = rc[p + sK]; L.qu = qv[p + sK]; L.du = dinv[p + sK];
+        L.a[0] = A[p]; L.a[1] = A[P + p]; L.a[2] = A[P + p + 1]; L.a[3] = A[2 * P + p];
+        L.a[4] = A[2 * P + p + sJ]; L.a[5] = A[3 * P + p]; L.a[6] = A[3 * P + p + sK];
+        L.ee = e[p]; L.pp = pv[p];
+    };
+    auto item = [&](int v_, int& p, int& k, int& ke) {
+        const int t_ = v_ / cm.nseg;
+        const int i = (t_ % cm.ntx) * CTX + (int)(threadIdx.x % CTX), j = (t_ / cm.ntx) * CTY + (int)(threadIdx.x / CTX);
+        k = (v_ % cm.nseg) * cm.kper;
+        ke = min(D.nK, k + cm.kper);
+        if (!(i < D.nI && j < D.nJ) || k >= ke) return false;
+        p = D.pidx(i, j, k);
+        return true;
+    };
+    const int v0 = xcdChunk(), vn = cm.ntx * cm.nty * cm.nseg;
+    int p = 0, k = 0, ke = 0;
+    bool live = v0 < vn && item(v0, p, k, ke);
+    // raw values below and at the cell (r, q, D^-1), loaded before alpha is known
+    float rd = 0.f, qd = 0.f, dd = 0.f, rcc = 0.f, qcc = 0.f, dcc = 0.f;
+    Plane cur, nxt;
+    if (live) {
+        rd = rc[p - sK]; qd = qv[p - sK]; dd = dinv[p - sK];
+        rcc = rc[p]; qcc = qv[p]; dcc = dinv[p];
+        loadPlane(p, cur);
+    }
     double t[2];
     readPartials<FB, 2>(partIn, nb, t);
-    const int par = (int)sPar;               // this iteration's parity, set by k_f_k1
-    const float* pv = par ? p1 : p0;
-    const float* qv = par ? q1 : q0;
-    const float* rc = par ? r1 : r0;
-    float* rn = (float*)(par ? r0 : r1);
     const double alpha = t[0] != 0. ? sPr / t[0] : 0.;
     if (blockIdx.x == 0 && threadIdx.x == 0) { st[F_PQ] = t[0]; st[F_SQ] = t[1]; st[F_IT] = sIt + 1.; }
     const float al = (float)alpha;
-    auto rnew = [&](int c) { return rc[c] - al * qv[c]; };
-    auto z1 = [&](int c) { return NET_OMEGA * S.inv(c) * rnew(c); };
+    auto rnew = [&](float rr_, float qq_) { return rr_ - al * qq_; };
+    auto z1 = [&](float d_, float rr_, float qq_) { return NET_OMEGA * d_ * rnew(rr_, qq_); };
     double v[2] = {0., 0.};
-    FOR_FINE(D, q, p) {
-        e[p] += al * pv[p];
-        const float rp = rnew(p);
-        rn[p] = rp;
-        w[p] = S.inv(p) * (rp - S.apply(z1, p));
-        v[0] += (double)rp * (double)rp;
-        v[1] += (double)rp;
+    for (int v_ = v0; v_ < vn; v_ += (int)gridDim.x) {
+        if (v_ != v0) {
+            live = item(v_, p, k, ke);
+            if (live) {
+                rd = rc[p - sK]; qd = qv[p - sK]; dd = dinv[p - sK];
+                rcc = rc[p]; qcc = qv[p]; dcc = dinv[p];
+                loadPlane(p, cur);
+            }
+        }
+        if (!live) continue;
+        float zd = z1(dd, rd, qd);
+        float zc = z1(dcc, rcc, qcc);
+        for (; k < ke; ++k, p += sK) {
+            if (k + 1 < ke) loadPlane(p + sK, nxt);
+            const float rp = rnew(rcc, qcc);
+            const float zu = z1(cur.du, cur.ru, cur.qu);
+            const float Az = cur.a[0] * zc + (cur.a[1] * z1(cur.ds[0], cur.rs[0], cur.qs[0])
+                                              + cur.a[2] * z1(cur.ds[1], cur.rs[1], cur.qs[1])
+                                              + cur.a[3] * z1(cur.ds[2], cur.rs[2], cur.qs[2])
+                                              + cur.a[4] * z1(cur.ds[3], cur.rs[3], cur.qs[3]) + cur.a[5] * zd
+                                              + cur.a[6] * zu);
+            e[p] = cur.ee + al * cur.pp;
+            rn[p] = rp;
+            w[p] = dcc * (rp - Az);
+            v[0] += (double)rp * (double)rp;
+            v[1] += (double)rp;
+            zd = zc; zc = zu;
+            rcc = cur.ru; qcc = cur.qu; dcc = cur.du;
+            cur = nxt;
+        }
     }
     writePartials<FB, 2>(v, partOut);
+}
+
+// D^-1 of the FP32 p' matrix over the whole padded array, as SymF::inv computes it, once per
+// solve (the halo holds the identity rows' 1)
+__global__ void k_f_dinv(SymF S, float* dinv) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < S.P) dinv[p] = S.inv(p);
 }
 
 __global__ void k_f_axpy(Dims D, double* x, const float* e, double s) {
     FOR_FINE(D, q, p) { x[p] += s * (double)e[p]; }
 }
+
+// The launches of one FCG iteration: restriction (with the convergence test), the network,
+// the post-smoothing, the direction and the update with the next pre-smoothing.  The same
+// launches serve every iteration of every round: the parities and the round's loop limits
+// live in the device state.  (Replaying them from a captured HIP graph measured no faster
+// at 128^3, 3.79 against 3.76 ms per solve: the gaps between the dependent kernels are not
+// launch overhead.)
+struct FcgIter {
+    Ctx& c;
+    Net& n;
+    SymF S;
+    ColMap cm;
+    int g;
+    float *r, *r1, *e, *w, *y, *z, *p0, *p1, *q0, *q1, *dinv;
+    double *st, *partA, *partB, *partC;
+    FcgCtl lc;
+    // probe: record the kernel probe's events around its launches
+    void launch(hipStream_t s, bool probe) const {
+        const Dims& D = c.D;
+        const int nc = (D.nI / 2) * (D.nJ / 2) * (D.nK / 2);
+        const double* done = st + F_DONE;
+        hipLaunchKernelGGL(k_f_restrict, dim3((nc + 255) / 256), dim3(256), 0, s, D, (const float*)w, n.vin,
+                           (const double*)partC, g, st, lc);
+        net_forward(n, s, n.vin, n.vout, done);
+        hipLaunchKernelGGL(k_f_post1, dim3(g), dim3(FB), 0, s, D, (const float*)dinv, (const float*)r, (const float*)r1,
+                           (const float*)n.vout, y, (const double*)st);
+        hipLaunchKernelGGL(k_f_post2, dim3(g), dim3(FB), 0, s, D, S, (const float*)dinv, (const float*)r, (const float*)r1,
+                           (const float*)y, z, (const float*)q0, (const float*)q1, partA, (const double*)st);
+        hipEvent_t e0, e1;
+        if (probe && c.probe.slot(PROBE_FCG_K1, e0, e1))   // dispatch-bracketing events: the kernel's own time
+            hipExtLaunchKernelGGL(k_f_k1, dim3(g), dim3(FB), 0, s, e0, e1, 0, D, S, cm, (const float*)z, p0, p1, q0, q1,
+                                  (const double*)partA, g, partB, st, lc);
+        else
+            hipLaunchKernelGGL(k_f_k1, dim3(g), dim3(FB), 0, s, D, S, cm, (const float*)z, p0, p1, q0, q1,
+                               (const double*)partA, g, partB, st, lc);
+        if (probe && c.probe.slot(PROBE_FCG_K2PRE, e0, e1))
+            hipExtLaunchKernelGGL(k_f_k2pre, dim3(g), dim3(FB), 0, s, e0, e1, 0, D, S, cm, (const float*)dinv, e,
+                                  (const float*)r, (const float*)r1, w, (const float*)p0, (const float*)p1, (const float*)q0,
+                                  (const float*)q1, (const double*)partB, g, partC, st);
+        else
+            hipLaunchKernelGGL(k_f_k2pre, dim3(g), dim3(FB), 0, s, D, S, cm, (const float*)dinv, e, (const float*)r,
+                               (const float*)r1, w, (const float*)p0, (const float*)p1, (const float*)q0, (const float*)q1,
+                               (const double*)partB, g, partC, st);
+    }
+};
 
 int learned_fcg(Ctx& c, double shift, double rtol, int maxit, double delta, int* refs, double* relOut) {
     Net& n = *(Net*)c.net;
@@ -523,87 +642,55 @@ int learned_fcg(Ctx& c, double shift, double rtol, int maxit, double delta, int*
     if (!(n.nx == D.nI / 2 && n.ny == D.nJ / 2 && n.nz == D.nK / 2)) net_set_dims(n, D.nI / 2, D.nJ / 2, D.nK / 2, false);
     hipStream_t s = c.stream;
     if (!n.packed) net_pack(n, s);
-    const SymF S = symF(c.Ap32, D);
     const int P = D.P;
-    float* r = n.fw + (size_t)FV_R * P;
-    float* r1 = n.fw + (size_t)FV_R1 * P;
-    float* e = n.fw + (size_t)FV_E * P;
-    float* w = n.fw + (size_t)FV_W * P;
-    float* y = n.fw + (size_t)FV_Y * P;
-    float* z = n.fw + (size_t)FV_Z * P;
-    float* p0 = n.fw + (size_t)FV_P0 * P;
-    float* p1 = n.fw + (size_t)FV_P1 * P;
-    float* q0 = n.fw + (size_t)FV_Q0 * P;
-    float* q1 = n.fw + (size_t)FV_Q1 * P;
-    double* st = c.scal + SC_CG;
-    const double* done = st + F_DONE;
-    double* partA = c.red.part;
-    double* partB = c.red.part + MAX_RED_BLOCKS * RED_STRIDE;
-    double* partC = c.red.part + 2 * MAX_RED_BLOCKS * RED_STRIDE;
     const int g = fgrid(D);
+    auto fv = [&](int slot) { return n.fw + (size_t)slot * P; };
+    const FcgIter I{c, n, symF(c.Ap32, D), colMap(D, g), g,
+                    fv(FV_R), fv(FV_R1), fv(FV_E), fv(FV_W), fv(FV_Y), fv(FV_Z), fv(FV_P0), fv(FV_P1), fv(FV_Q0), fv(FV_Q1),
+                    fv(FV_DI), c.scal + SC_CG, c.red.part, c.red.part + MAX_RED_BLOCKS * RED_STRIDE,
+                    c.red.part + 2 * MAX_RED_BLOCKS * RED_STRIDE, FcgCtl{c.singularP ? (double)D.N : 0.}};
+    double* st = I.st;
+    hipLaunchKernelGGL(k_f_dinv, dim3(blocksFor(P, 256)), dim3(256), 0, s, I.S, I.dinv);
     double rr, bb;
     pcorr_true_resid(c, shift, &rr, &bb);
     double rel = bb > 0. ? rr / bb : 0.;
     int its = 0;
-    FcgCtl lc{delta * delta, maxit, c.singularP ? (double)D.N : 0.};
-    const ColMap cm = colMap(D, g);
     int lastRound = 0;
     while (rel > rtol && its < maxit && *refs < 40) {
         // the inner solve gains `delta`, or only what the last round still needs
         const double need = std::max(delta, 0.5 * rtol / rel);
-        lc.tol2 = need * need;
-        const double scale = 1.0 / rr;
-        hipLaunchKernelGGL(k_f_init, dim3(g), dim3(FB), 0, s, D, (const double*)(c.wp + 6 * P), scale, r, e, partC, st);
-        hipLaunchKernelGGL(k_f_pre0, dim3(g), dim3(FB), 0, s, D, S, (const float*)r, w, (const double*)st);
+        const double tol2 = need * need;
         // a refinement round that needs more than 300 iterations means the cycle does not
         // fit this system: the round ends there and the progress test below decides
-        lc.maxit = std::min(maxit - its, 300);
+        const int roundMax = std::min(maxit - its, 300);
+        const double scale = 1.0 / rr;
+        hipLaunchKernelGGL(k_f_init, dim3(g), dim3(FB), 0, s, D, (const double*)(c.wp + 6 * P), scale, I.r, I.e, I.partC, st,
+                           tol2, roundMax);
+        hipLaunchKernelGGL(k_f_pre0, dim3(g), dim3(FB), 0, s, D, I.S, (const float*)I.dinv, (const float*)I.r, I.w,
+                           (const double*)st);
         // iterations are queued in chunks between convergence checks: the first chunk of a
         // round is the previous round's count, later ones the count the observed rate of
         // decrease still needs (a queued iteration past convergence costs 17 launches
         // that return at once)
         int chunk = lastRound > 0 ? lastRound : 6, it = 0;
         while (true) {
-            for (int k = 0; k < chunk; ++k) {
-                const int nc = (D.nI / 2) * (D.nJ / 2) * (D.nK / 2);
-                hipLaunchKernelGGL(k_f_restrict, dim3((nc + 255) / 256), dim3(256), 0, s, D, (const float*)w, n.vin,
-                                   (const double*)partC, g, st, lc);
-                net_forward(n, s, n.vin, n.vout, done);
-                hipLaunchKernelGGL(k_f_post1, dim3(g), dim3(FB), 0, s, D, S, (const float*)r, (const float*)r1,
-                                   (const float*)n.vout, y, (const double*)st);
-                hipLaunchKernelGGL(k_f_post2, dim3(g), dim3(FB), 0, s, D, S, (const float*)r, (const float*)r1, (const float*)y,
-                                   z, (const float*)q0, (const float*)q1, partA, (const double*)st);
-                hipEvent_t e0, e1;
-                if (c.probe.slot(PROBE_FCG_K1, e0, e1))   // dispatch-bracketing events: the kernel's own time
-                    hipExtLaunchKernelGGL(k_f_k1, dim3(g), dim3(FB), 0, s, e0, e1, 0, D, S, cm, (const float*)z,
-                                          p0, p1, q0, q1, (const double*)partA, g, partB, st, lc);
-                else
-                    hipLaunchKernelGGL(k_f_k1, dim3(g), dim3(FB), 0, s, D, S, cm, z, p0, p1, q0, q1, partA, g, partB, st, lc);
-                if (c.probe.slot(PROBE_FCG_K2PRE, e0, e1))
-                    hipExtLaunchKernelGGL(k_f_k2pre, dim3(g), dim3(FB), 0, s, e0, e1, 0, D, S, e, (const float*)r,
-                                          (const float*)r1, w, (const float*)p0, (const float*)p1, (const float*)q0,
-                                          (const float*)q1, (const double*)partB, g, partC, st);
-                else
-                    hipLaunchKernelGGL(k_f_k2pre, dim3(g), dim3(FB), 0, s, D, S, e, (const float*)r, (const float*)r1, w,
-                                       (const float*)p0, (const float*)p1, (const float*)q0, (const float*)q1,
-                                       (const double*)partB, g, partC, st);
-            }
+            for (int k = 0; k < chunk; ++k) I.launch(s, true);
             HIPCHECK(hipMemcpyAsync(c.hscal + SC_CG, st, 8 * sizeof(double), hipMemcpyDeviceToHost, s));
             HIPCHECK(hipStreamSynchronize(s));
             it = (int)c.hscal[SC_CG + F_IT];
-            if (c.hscal[SC_CG + F_DONE] != 0. || it >= lc.maxit) break;
+            if (c.hscal[SC_CG + F_DONE] != 0. || it >= roundMax) break;
             // F_RR is the residual checked before the last queued iteration (it - 1 done)
             const double rr0 = c.hscal[SC_CG + F_RR], bb0 = c.hscal[SC_CG + F_BB];
             chunk = 4;
             if (it > 1 && rr0 > 0. && bb0 > 0. && rr0 < bb0) {
                 const double rate = std::log(rr0 / bb0) / (it - 1);          // log decrease per iteration
-                const double need = std::log(lc.tol2 * bb0 / rr0) / rate;   // iterations still needed from it - 1
-                chunk = std::max(1, std::min(32, (int)std::ceil(need)));
+                const double still = std::log(tol2 * bb0 / rr0) / rate;    // iterations still needed from it - 1
+                chunk = std::max(1, std::min(32, (int)std::ceil(still)));
             }
         }
         lastRound = it;
         its += it;
-        hipLaunchKernelGGL(k_f_axpy, dim3(g), dim3(FB), 0, s, D, c.xp, (const float*)e, 1.0 / scale);
+        hipLaunchKernelGGL(k_f_axpy, dim3(g), dim3(FB), 0, s, D, c.xp, (const float*)I.e, 1.0 / scale);
         const double prev = rel;
         pcorr_true_resid(c, shift, &rr, &bb);
         rel = rr / bb;
