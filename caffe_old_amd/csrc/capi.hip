@@ -732,6 +732,14 @@ int caffe_kernel_probe_read(caffe_ctx* x, double* avg_ms, long* launches, double
     });
 }
 
+int caffe_ctx_set_fcg_coarse(caffe_ctx* x, int mode) {
+    return guard(x, [&](Ctx& c) {
+        if (mode != 0 && mode != 1) { c.err = "coarse solve: 0 (the network) or 1 (a multigrid V-cycle)"; return CAFFE_ERR_ARG; }
+        c.fcgCoarse = mode;
+        return CAFFE_OK;
+    });
+}
+
 int caffe_sync(caffe_ctx* x) {
     return guard(x, [&](Ctx& c) { HIPCHECK(hipStreamSynchronize(c.stream)); return CAFFE_OK; });
 }

@@ -163,6 +163,7 @@ struct Ctx {
 
     // learned preconditioner (optional)
     void* net = nullptr;
+    int fcgCoarse = 0;        // coarse solve of its two-grid cycle: 0 the network, 1 a multigrid V-cycle
 };
 
 // ---- launchers (fv_kernels.hip)
@@ -208,6 +209,12 @@ void pcorr_make_fp32(Ctx& c);
 // Returns the iteration count, or -1 when the system does not qualify (8 cells or fewer,
 // or an indefinite system)
 int mg_bicgstab(Ctx& c, double shift, double rtol, int maxit, int* refs, double* relOut);
+// The classical coarse solve of the learned two-grid cycle (its ablation): prepare builds the
+// hierarchy's operators for the current p' system (false unless level 1 is the half grid);
+// vcycle runs one V-cycle of levels 1 .. on b1 (level-1 right-hand side, unpadded, i fastest)
+// into z1, returning at once when *done is set
+bool mg_coarse_prepare(Ctx& c);
+void mg_coarse_vcycle(Ctx& c, const float* b1, float* z1, const double* done);
 void add_vec(Ctx& c, double* x, const double* d, int n);
 double time_cg_k1(Ctx& c, bool fp32, int reps);
 

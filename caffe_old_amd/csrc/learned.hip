@@ -96,7 +96,10 @@ __global__ void k_tg_pre(Dims D, SymF S, const float* r, float* w, const double*
 // gathers fine cells 2C-1 .. 2C+2 in each direction; the weight of fine f is 3/4 when
 // f>>1 == C plus 1/4 when its other neighbour is C.  The halo of w is zero, so fine
 // cells just outside the grid (weight 0) are read through the padded layout unmasked.
-__device__ __forceinline__ void restrictAt(const Dims& D, const float* w, float* v, int e) {
+// SC: the fine values are w scaled cell by cell by sc (the classical coarse right-hand side
+// P^T (D w) of the ablation, sc = aP)
+template <bool SC = false>
+__device__ __forceinline__ void restrictAt(const Dims& D, const float* w, float* v, int e, const float* sc = nullptr) {
     const int cx = D.nI >> 1, cy = D.nJ >> 1, cz = D.nK >> 1;
     if (e >= cx * cy * cz) return;
     const int I = e % cx, J = (e / cx) % cy, K = e / (cx * cy);
@@ -125,8 +128,15 @@ __device__ __forceinline__ void restrictAt(const Dims& D, const float* w, float*
         float sy = 0.f;
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            const float* row = w + p0 + c * D.sK + b * D.sJ;
-            sy += wy[b] * (wx[0] * row[0] + wx[1] * row[1] + wx[2] * row[2] + wx[3] * row[3]);
+            const int o = p0 + c * D.sK + b * D.sJ;
+            const float* row = w + o;
+            if (SC) {
+                const float* sr = sc + o;
+                sy += wy[b] * (wx[0] * (row[0] * sr[0]) + wx[1] * (row[1] * sr[1]) + wx[2] * (row[2] * sr[2])
+                               + wx[3] * (row[3] * sr[3]));
+            } else {
+                sy += wy[b] * (wx[0] * row[0] + wx[1] * row[1] + wx[2] * row[2] + wx[3] * row[3]);
+            }
         }
         s += wz[c] * sy;
     }
@@ -360,6 +370,12 @@ __global__ void k_f_restrict(Dims D, const float* w, float* v, const double* par
     }
     if (done) return;
     restrictAt(D, w, v, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// the ablation's classical coarse right-hand side v = P^T (D w) = P^T (r - A z1)
+__global__ void k_f_restrict_dw(Dims D, const float* w, const float* aP, float* v, const double* st) {
+    if (st[F_DONE] != 0.) return;
+    restrictAt<true>(D, w, v, blockIdx.x * blockDim.x + threadIdx.x, aP);
 }
 
 // y = w D^-1 r + P c
@@ -602,6 +618,7 @@ struct FcgIter {
     float *r, *r1, *e, *w, *y, *z, *p0, *p1, *q0, *q1, *dinv;
     double *st, *partA, *partB, *partC;
     FcgCtl lc;
+    bool classical;   // the coarse solve is one multigrid V-cycle (the ablation), not the network
     // probe: record the kernel probe's events around its launches
     void launch(hipStream_t s, bool probe) const {
         const Dims& D = c.D;
@@ -609,7 +626,13 @@ struct FcgIter {
         const double* done = st + F_DONE;
         hipLaunchKernelGGL(k_f_restrict, dim3((nc + 255) / 256), dim3(256), 0, s, D, (const float*)w, n.vin,
                            (const double*)partC, g, st, lc);
-        net_forward(n, s, n.vin, n.vout, done);
+        if (classical) {
+            hipLaunchKernelGGL(k_f_restrict_dw, dim3((nc + 255) / 256), dim3(256), 0, s, D, (const float*)w, S.A, n.vin,
+                               (const double*)st);
+            mg_coarse_vcycle(c, n.vin, n.vout, done);
+        } else {
+            net_forward(n, s, n.vin, n.vout, done);
+        }
         hipLaunchKernelGGL(k_f_post1, dim3(g), dim3(FB), 0, s, D, (const float*)dinv, (const float*)r, (const float*)r1,
                            (const float*)n.vout, y, (const double*)st);
         hipLaunchKernelGGL(k_f_post2, dim3(g), dim3(FB), 0, s, D, S, (const float*)dinv, (const float*)r, (const float*)r1,
@@ -648,7 +671,9 @@ int learned_fcg(Ctx& c, double shift, double rtol, int maxit, double delta, int*
     const FcgIter I{c, n, symF(c.Ap32, D), colMap(D, g), g,
                     fv(FV_R), fv(FV_R1), fv(FV_E), fv(FV_W), fv(FV_Y), fv(FV_Z), fv(FV_P0), fv(FV_P1), fv(FV_Q0), fv(FV_Q1),
                     fv(FV_DI), c.scal + SC_CG, c.red.part, c.red.part + MAX_RED_BLOCKS * RED_STRIDE,
-                    c.red.part + 2 * MAX_RED_BLOCKS * RED_STRIDE, FcgCtl{c.singularP ? (double)D.N : 0.}};
+                    c.red.part + 2 * MAX_RED_BLOCKS * RED_STRIDE, FcgCtl{c.singularP ? (double)D.N : 0.},
+                    c.fcgCoarse == 1};
+    if (I.classical && !mg_coarse_prepare(c)) return -1;
     double* st = I.st;
     hipLaunchKernelGGL(k_f_dinv, dim3(blocksFor(P, 256)), dim3(256), 0, s, I.S, I.dinv);
     double rr, bb;

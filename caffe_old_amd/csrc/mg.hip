@@ -751,23 +751,11 @@ static void mgBuild(Ctx& c) {
     hipLaunchKernelGGL(k_mg_dense, dim3(1), dim3(MBS), 0, c.stream, Lc, Lc.D.N, fixedFace ? 0 : 1, H.cinv);
 }
 
-// the cycle after the level-0 pre-smoothing: z (level 0) holds that smoothing of b
-static void mgCycleRest(Ctx& c, const Sym4<float>& A0, const double* b, float* z, const double* st) {
+// levels 1 .. nl - 1 of the cycle: from level 1's right-hand side L[1].b to its iterate
+// L[1].z (two sweeps from zero, restriction, the coarser levels, prolongation, two sweeps)
+static void mgCoarseLevels(Ctx& c, const double* st) {
     MgHier& H = c.mg;
-    const Dims& D = c.D;
     hipStream_t s = c.stream;
-    const int g0 = kgrid(D);
-    const bool ghosts = A0.ov.rowOf && c.ov.n > 0;
-    const int gg = blocksFor(c.ov.n, MBS);
-    if (ghosts)
-        hipLaunchKernelGGL(k_mg_ghost_pre, dim3(gg), dim3(MBS), 0, s, c.ov.n, (const int*)c.ov.cell, A0, b,
-                           (const float*)H.g1, z, st);
-    if (ghosts)
-        hipLaunchKernelGGL(k_mg_restrict0<true>, dim3(rgrid(H.L[1].D)), dim3(MBS), 0, s, D, (const float*)A0.A,
-                           (const int*)A0.ov.rowOf, b, (const float*)z, H.L[1], st);
-    else
-        hipLaunchKernelGGL(k_mg_restrict0<false>, dim3(rgrid(H.L[1].D)), dim3(MBS), 0, s, D, (const float*)A0.A,
-                           (const int*)nullptr, b, (const float*)z, H.L[1], st);
     const int nl = H.nl;
     // levels lt .. nl - 1 run in one workgroup
     int lt = nl - 1;
@@ -786,6 +774,26 @@ static void mgCycleRest(Ctx& c, const Sym4<float>& A0, const double* b, float* z
         hipLaunchKernelGGL(k_mg_post, dim3(lgrid(L.D)), dim3(MBS), 0, s, L, (const float*)L.z, L.t, st);
         hipLaunchKernelGGL(k_mg_post, dim3(lgrid(L.D)), dim3(MBS), 0, s, L, (const float*)L.t, L.z, st);
     }
+}
+
+// the cycle after the level-0 pre-smoothing: z (level 0) holds that smoothing of b
+static void mgCycleRest(Ctx& c, const Sym4<float>& A0, const double* b, float* z, const double* st) {
+    MgHier& H = c.mg;
+    const Dims& D = c.D;
+    hipStream_t s = c.stream;
+    const int g0 = kgrid(D);
+    const bool ghosts = A0.ov.rowOf && c.ov.n > 0;
+    const int gg = blocksFor(c.ov.n, MBS);
+    if (ghosts)
+        hipLaunchKernelGGL(k_mg_ghost_pre, dim3(gg), dim3(MBS), 0, s, c.ov.n, (const int*)c.ov.cell, A0, b,
+                           (const float*)H.g1, z, st);
+    if (ghosts)
+        hipLaunchKernelGGL(k_mg_restrict0<true>, dim3(rgrid(H.L[1].D)), dim3(MBS), 0, s, D, (const float*)A0.A,
+                           (const int*)A0.ov.rowOf, b, (const float*)z, H.L[1], st);
+    else
+        hipLaunchKernelGGL(k_mg_restrict0<false>, dim3(rgrid(H.L[1].D)), dim3(MBS), 0, s, D, (const float*)A0.A,
+                           (const int*)nullptr, b, (const float*)z, H.L[1], st);
+    mgCoarseLevels(c, st);
     hipLaunchKernelGGL(k_mg_prolong, dim3(lgrid(D)), dim3(MBS), 0, s, D, H.L[1], z, (const int8_t*)c.st, st);
     for (int sweep = 0; sweep < 2; ++sweep) {
         const float* zin = sweep == 0 ? z : H.z0[2];
@@ -926,6 +934,40 @@ static int mgRun(Ctx& c, const double* b, double* e, double rtol, int maxit, int
     }
     *doneOut = (int)c.hscal[SC_BI + B_DONE];
     return it;
+}
+
+// ---- the classical coarse solve of the learned preconditioner's two-grid cycle (its
+// ablation): one V-cycle of this hierarchy below the fine level
+__global__ void k_mg_lvl_in(Dims D, const float* src, float* dst) {   // unpadded -> padded interior
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= D.N) return;
+    int i, j, k;
+    dst[D.cell(e, i, j, k)] = src[e];
+}
+__global__ void k_mg_lvl_out(Dims D, const float* src, float* dst, const double* done) {
+    if (done && *done != 0.) return;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= D.N) return;
+    int i, j, k;
+    dst[e] = src[D.cell(e, i, j, k)];
+}
+
+bool mg_coarse_prepare(Ctx& c) {
+    if (c.indefiniteP || c.hasIB || !mgPlan(c)) return false;
+    const Dims& L1 = c.mg.L[1].D;
+    if (L1.nI * 2 != c.D.nI || L1.nJ * 2 != c.D.nJ || L1.nK * 2 != c.D.nK) return false;
+    mgBuild(c);
+    return true;
+}
+
+void mg_coarse_vcycle(Ctx& c, const float* b1, float* z1, const double* done) {
+    MgHier& H = c.mg;
+    const Dims& D1 = H.L[1].D;
+    // the level kernels read only the done slot B_DONE of their state pointer
+    const double* st = done - B_DONE;
+    hipLaunchKernelGGL(k_mg_lvl_in, dim3(lgrid(D1)), dim3(MBS), 0, c.stream, D1, b1, H.L[1].b);
+    mgCoarseLevels(c, st);
+    hipLaunchKernelGGL(k_mg_lvl_out, dim3(lgrid(D1)), dim3(MBS), 0, c.stream, D1, (const float*)H.L[1].z, z1, done);
 }
 
 int mg_bicgstab(Ctx& c, double shift, double rtol, int maxit, int* refs, double* relOut) {

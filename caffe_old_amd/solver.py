@@ -88,6 +88,11 @@ class Solver:
         self._check(self.lib.caffe_ctx_attach_net(self._h, net._h if net is not None else None), "caffe_ctx_attach_net")
         self._net = net
 
+    def set_fcg_coarse(self, mode: int):
+        """coarse solve of the learned two-grid cycle: 0 the network, 1 a multigrid V-cycle
+        (the ablation)"""
+        self._check(self.lib.caffe_ctx_set_fcg_coarse(self._h, int(mode)), "caffe_ctx_set_fcg_coarse")
+
     def set_solve_opts(self, which: str, mode: int = L.SOLVE_PLAIN_FP64, rtol: float = 1e-8, maxit: int = 50000,
                        inner_rtol: float = 1e-4):
         o = L.SolveOpts(mode, rtol, maxit, inner_rtol)

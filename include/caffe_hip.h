@@ -248,6 +248,12 @@ int caffe_net_eval(caffe_net* net, float* losses, long n);
 /* use the network in CAFFE_SOLVE_LEARNED_FCG solves of ctx (NULL detaches); the network
  * must outlive the attachment */
 int caffe_ctx_attach_net(caffe_ctx* ctx, caffe_net* net);
+/* The coarse solve of the two-grid cycle in CAFFE_SOLVE_LEARNED_FCG solves of ctx: 0 the
+ * attached network (default); 1 one geometric multigrid V-cycle of the CAFFE_SOLVE_MG
+ * hierarchy below the fine level on the classical right-hand side P^T (r - A z1), the same
+ * cycle and FCG otherwise: the ablation that shows what the network gains over a classical
+ * coarse solve.  A network must still be attached (its fine-grid work buffers are used). */
+int caffe_ctx_set_fcg_coarse(caffe_ctx* ctx, int mode);
 
 /* ---------------------------------------------------------------------------------
  * The SparseMatrix / SparseVector compatibility path: for callers that assemble their

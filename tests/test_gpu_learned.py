@@ -89,7 +89,11 @@ def test_learned_solution_matches_plain(system128, trained):
 
 def test_ablation_trained_network_gain(system128, trained):
     """Same flexible CG and cycle: trained network vs the network switched off (two
-    damped-Jacobi sweeps) vs an untrained network; plain FP64 PCG and FP32 IR beside."""
+    damped-Jacobi sweeps) vs an untrained network vs a classical coarse solve (one geometric
+    multigrid V-cycle on the 64^3 coarse grid, on the classical right-hand side
+    P^T (r - A z1)); plain FP64 PCG, red-black SGS PCG, FP32 IR and the multigrid BiCGStab
+    beside.  The classical coarse solve is reported, not asserted against: it shows what the
+    network gains, or loses, against a conventional two-grid."""
     from caffe_old_amd import _lib as L
     from caffe_old_amd.learned import PressureNet
     zero = PressureNet(seed=0)
@@ -101,8 +105,16 @@ def test_ablation_trained_network_gain(system128, trained):
     rows["FCG, network off"] = _solve(system128, L.SOLVE_LEARNED_FCG, zero)[0]
     rows["FCG, untrained network"] = _solve(system128, L.SOLVE_LEARNED_FCG, untrained)[0]
     rows["FCG, trained network"] = _solve(system128, L.SOLVE_LEARNED_FCG, trained)[0]
+    system128.set_fcg_coarse(1)
+    rv, xv = _solve(system128, L.SOLVE_LEARNED_FCG, trained)
+    system128.set_fcg_coarse(0)
+    rows["FCG, V-cycle coarse solve"] = rv
+    rows["SGS PCG (FP64)"] = _solve(system128, L.SOLVE_SGS_FP64)[0]
+    rp, xp = _solve(system128, L.SOLVE_PLAIN_FP64)
+    rows["multigrid BiCGStab"] = _solve(system128, L.SOLVE_MG)[0]
+    assert rv["fell_back"] == 0 and _rel(xv, xp) <= 1e-6
     for k, r in rows.items():
-        print(f"{k:24s} iters {r['iters']:6d} refinements {r['refinements']:3d} relres {r['relres']:.2e} "
+        print(f"{k:26s} iters {r['iters']:6d} refinements {r['refinements']:3d} relres {r['relres']:.2e} "
               f"ms {r['ms']:8.2f} fell_back {r['fell_back']}")
     t = rows["FCG, trained network"]
     assert t["converged"] and t["fell_back"] == 0
