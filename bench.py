@@ -150,9 +150,15 @@ def full_run(n, mode, inner, net, mom_mode):
     g.sync()
     dt = time.perf_counter() - t0
     cont = g.continuityError
+    courant = g.step_report()["courant"]
+    import numpy as np
+    ni, nj, nk = g.shape
+    u = g.get("u").reshape(3, nk + 2, nj + 2, ni + 2)[:, 1:-1, 1:-1, 1:-1]
+    umax = float(np.abs(u).max())   # interior cells; the lid speed is 1
     g.close()
     return {"steps": c.n_solves, "seconds": round(dt, 3), "steps_per_s": round(c.n_solves / dt, 3),
             "p_fell_back_steps": fell, "not_converged_steps": not_conv, "final_continuity": cont,
+            "final_courant": round(courant, 4), "final_max_abs_u_over_lid": round(umax, 4),
             "note": "from rest, including the first solves that allocate work and capture graphs"}
 
 

@@ -798,7 +798,7 @@ Dims fineDims(int nI, int nJ, int nK) {
 }
 
 __global__ void k_collect(Dims D, const double* Ap, const double* bp, const double* xp, const int8_t* st, double shift,
-                          double s, float* A, float* b, float* x) {
+                          double xshift, double s, float* A, float* b, float* x) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= D.P) return;
     for (int m = 0; m < 4; ++m) A[m * D.P + p] = (float)Ap[m * D.P + p];
@@ -806,7 +806,7 @@ __global__ void k_collect(Dims D, const double* Ap, const double* bp, const doub
     const bool in = i >= 1 && i <= D.nI && j >= 1 && j <= D.nJ && k >= 1 && k <= D.nK;
     const bool live = in && st[p] != ST_INACTIVE;
     b[p] = live ? (float)((bp[p] - shift) * s) : 0.f;
-    x[p] = live ? (float)(xp[p] * s) : 0.f;
+    x[p] = live ? (float)((xp[p] - xshift) * s) : 0.f;
 }
 __global__ void k_negate(float* a, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1054,13 +1054,21 @@ int caffe_net_collect(caffe_net* x, caffe_ctx* cx, long* count) {
         const Dims& D = c.D;
         hipStream_t s = c.stream;
         const int g = fgrid(D);
-        double shift = 0.;
+        // A singular (pure-Neumann) system: b is made compatible by removing its mean, and
+        // the mean of x, which A does not see, is removed too.  Near a steady state the
+        // correction is small against its free constant, and x^T A x of the undemeaned FP32
+        // pair is then lost to cancellation.
+        double shift = 0., xshift = 0.;
         if (c.singularP) {
-            hipLaunchKernelGGL(k_sum_live, dim3(g), dim3(FB), 0, s, D, (const double*)c.bp, (const int8_t*)c.st, n.part);
-            hipLaunchKernelGGL(k_final2, dim3(1), dim3(FB), 0, s, n.part, g, n.scal);
-            HIPCHECK(hipMemcpyAsync(n.hscal, n.scal, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-            HIPCHECK(hipStreamSynchronize(s));
-            shift = n.hscal[1] > 0. ? n.hscal[0] / n.hscal[1] : 0.;
+            auto liveMean = [&](const void* v) {
+                hipLaunchKernelGGL(k_sum_live, dim3(g), dim3(FB), 0, s, D, (const double*)v, (const int8_t*)c.st, n.part);
+                hipLaunchKernelGGL(k_final2, dim3(1), dim3(FB), 0, s, n.part, g, n.scal);
+                HIPCHECK(hipMemcpyAsync(n.hscal, n.scal, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+                HIPCHECK(hipStreamSynchronize(s));
+                return n.hscal[1] > 0. ? n.hscal[0] / n.hscal[1] : 0.;
+            };
+            shift = liveMean(c.bp);
+            xshift = liveMean(c.xp);
         }
         hipLaunchKernelGGL(k_sumsq, dim3(g), dim3(FB), 0, s, D, (const double*)c.bp, (const int8_t*)c.st, shift, n.part);
         hipLaunchKernelGGL(k_l_final, dim3(1), dim3(FB), 0, s, n.part, g, n.scal);
@@ -1074,7 +1082,7 @@ int caffe_net_collect(caffe_net* x, caffe_ctx* cx, long* count) {
         HIPCHECK(hipMalloc(&smp.b, (size_t)D.P * sizeof(float)));
         HIPCHECK(hipMalloc(&smp.x, (size_t)D.P * sizeof(float)));
         hipLaunchKernelGGL(k_collect, dim3((D.P + 255) / 256), dim3(256), 0, s, D, (const double*)c.Ap, (const double*)c.bp,
-                           (const double*)c.xp, (const int8_t*)c.st, shift, 1.0 / bn, smp.A, smp.b, smp.x);
+                           (const double*)c.xp, (const int8_t*)c.st, shift, xshift, 1.0 / bn, smp.A, smp.b, smp.x);
         HIPCHECK(hipStreamSynchronize(s));
         tg_ensure_fine(n, D);
         smp.nrm = sampleNorm(n, n.stream, smp);
