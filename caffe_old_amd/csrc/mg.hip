@@ -751,6 +751,110 @@ static void mgBuild(Ctx& c) {
     hipLaunchKernelGGL(k_mg_dense, dim3(1), dim3(MBS), 0, c.stream, Lc, Lc.D.N, fixedFace ? 0 : 1, H.cinv);
 }
 
+// k_mg_tail with the levels' iterates, sweep buffers and right-hand sides in LDS: the same
+// steps in the same order (results bit-identical), but a step between two barriers then
+// waits on LDS instead of on global memory that the previous step wrote, and the tail is
+// about twenty such steps.  The operators and Jacobi weights stay in global memory (read
+// only).  On entry the first level's b is copied in, with every other buffer zeroed (the
+// halos the stencils read are zero, as in global memory); on exit the first level's z is
+// copied out for the prolongation above it.  Used when the levels' 3 P floats fit in
+// MG_TAIL_LDS (padded cells: 16^3 with its coarser levels takes 84 KB).
+constexpr int MG_TAIL_LDS = 24576;   // floats (96 KB)
+// level l of the tail with z, t and b in LDS (levels packed in order, 3 P floats each)
+__device__ __forceinline__ MgLevel ldsLevel(const MgTail& T, float* lds, int l) {
+    MgLevel L = T.L[l];
+    int off = 0;
+    for (int m = 0; m < l; ++m) off += 3 * T.L[m].D.P;
+    L.z = lds + off;
+    L.t = lds + off + L.D.P;
+    L.b = lds + off + 2 * L.D.P;
+    return L;
+}
+__global__ void __launch_bounds__(1024) k_mg_tail_lds(MgTail T, const double* inv, const double* st) {
+    if (mgDone(st)) return;
+    __shared__ float lds[MG_TAIL_LDS];
+    const int tid = threadIdx.x, bd = blockDim.x;
+    int used = 0;
+    for (int l = 0; l < T.n; ++l) used += 3 * T.L[l].D.P;
+    for (int q = tid; q < used; q += bd) lds[q] = 0.f;
+    __syncthreads();
+    {
+        const MgLevel L0 = ldsLevel(T, lds, 0);
+        for (int q = tid; q < L0.D.P; q += bd) L0.b[q] = T.L[0].b[q];
+    }
+    __syncthreads();
+    for (int l = 0; l < T.n - 1; ++l) {
+        const MgLevel L = ldsLevel(T, lds, l);
+        for (int e = tid; e < L.D.N; e += bd) {
+            int i, j, k;
+            const int p = L.D.cell(e, i, j, k);
+            L.z[p] = preSmooth(levOp(L), L.w, [&](int c) { return L.b[c]; }, p);
+        }
+        __syncthreads();
+        const MgLevel C = ldsLevel(T, lds, l + 1);
+        for (int base = 0; base < 8 * C.D.N; base += bd)
+            restrictAgg(L.D, [&](int p) { return (double)levResid(L.A, L.b, L.z, L.D, p); }, C, base + tid);
+        __syncthreads();
+    }
+    {
+        __shared__ double bs[MG_COARSEST];
+        const MgLevel L = ldsLevel(T, lds, T.n - 1);
+        const int n = L.D.N;
+        int i, j, k;
+        if (tid < n) bs[tid] = L.b[L.D.cell(tid, i, j, k)];
+        __syncthreads();
+        if (tid < n) {
+            double acc = 0.;
+            for (int m = 0; m < n; ++m) acc += inv[tid * n + m] * bs[m];
+            L.z[L.D.cell(tid, i, j, k)] = (float)acc;
+        }
+        __syncthreads();
+    }
+    for (int l = T.n - 2; l >= 0; --l) {
+        const MgLevel L = ldsLevel(T, lds, l);
+        const MgLevel C = ldsLevel(T, lds, l + 1);
+        for (int e = tid; e < L.D.N; e += bd) {
+            int i, j, k;
+            const int p = L.D.cell(e, i, j, k);
+            int cx[2], cy[2], cz[2];
+            float wx[2], wy[2], wz[2];
+            lin(i, C.f[0], C.D.nI, cx[0], cx[1], wx[0], wx[1]);
+            lin(j, C.f[1], C.D.nJ, cy[0], cy[1], wy[0], wy[1]);
+            lin(k, C.f[2], C.D.nK, cz[0], cz[1], wz[0], wz[1]);
+            float s = 0.f;
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int d = 0; d < 2; ++d) s += wz[a] * wy[b] * wx[d] * C.z[C.D.pidx(cx[d], cy[b], cz[a])];
+            L.z[p] += s;
+        }
+        __syncthreads();
+        for (int sweep = 0; sweep < 2; ++sweep) {
+            const float* zin = sweep == 0 ? L.z : L.t;
+            float* zout = sweep == 0 ? L.t : L.z;
+            for (int e = tid; e < L.D.N; e += bd) {
+                int i, j, k;
+                const int p = L.D.cell(e, i, j, k);
+                int nb[7];
+                nbrs(p, L.D.sJ, L.D.sK, nb);
+                float x[7];
+#pragma unroll
+                for (int m = 0; m < 7; ++m) x[m] = zin[nb[m]];
+                zout[p] = x[0] + L.w[p] * (L.b[p] - stencil7(L.A, L.D.P, L.D.sJ, L.D.sK, p, x));
+            }
+            __syncthreads();
+        }
+    }
+    const MgLevel L0 = ldsLevel(T, lds, 0);
+    for (int e = tid; e < L0.D.N; e += bd) {
+        int i, j, k;
+        const int p = L0.D.cell(e, i, j, k);
+        T.L[0].z[p] = L0.z[p];
+    }
+}
+
 // levels 1 .. nl - 1 of the cycle: from level 1's right-hand side L[1].b to its iterate
 // L[1].z (two sweeps from zero, restriction, the coarser levels, prolongation, two sweeps)
 static void mgCoarseLevels(Ctx& c, const double* st) {
@@ -766,8 +870,13 @@ static void mgCoarseLevels(Ctx& c, const double* st) {
     }
     MgTail T;
     T.n = nl - lt;
-    for (int l = lt; l < nl; ++l) T.L[l - lt] = H.L[l];
-    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(1024), 0, s, T, (const double*)H.cinv, st);
+    long need = 0;
+    for (int l = lt; l < nl; ++l) {
+        T.L[l - lt] = H.L[l];
+        need += 3L * H.L[l].D.P;
+    }
+    if (need <= MG_TAIL_LDS) hipLaunchKernelGGL(k_mg_tail_lds, dim3(1), dim3(1024), 0, s, T, (const double*)H.cinv, st);
+    else hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(1024), 0, s, T, (const double*)H.cinv, st);
     for (int l = lt - 1; l >= 1; --l) {
         const MgLevel& L = H.L[l];
         hipLaunchKernelGGL(k_mg_prolong, dim3(lgrid(L.D)), dim3(MBS), 0, s, L.D, H.L[l + 1], L.z, (const int8_t*)nullptr, st);

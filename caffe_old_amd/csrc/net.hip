@@ -26,20 +26,13 @@
 #include <cstring>
 
 #include "net.hpp"
+#include "netdev.hpp"
 
 namespace caffe {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ float bf2f(uint32_t h) { return __uint_as_float(h << 16); }
-// round to nearest even (NaN stays NaN)
-__device__ __forceinline__ uint32_t f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (u >> 16) | 0x40u;
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return u >> 16;
-}
 __device__ __forceinline__ uint2 pack4(const float (&v)[4]) {
     uint2 r;
     r.x = f2bf(v[0]) | (f2bf(v[1]) << 16);
@@ -711,29 +704,6 @@ __device__ __forceinline__ int nbr(int i, int j, int k, int t, int nx, int ny, i
     return (cc * ny + bc) * nx + ac;
 }
 
-// The edge layers stage a 16x4x4 output tile with its one-voxel halo in LDS (clamped
-// index and mask: no branch around the loads, which therefore issue together), then one
-// thread computes one output voxel from LDS.
-constexpr int ETX = 16, ETY = 4, ETZ = 4, EHX = ETX + 2, EHY = ETY + 2, EHV = EHX * EHY * (ETZ + 2);
-__device__ __forceinline__ void edgeTile(int nx, int ny, int& ox, int& oy, int& oz) {
-    const int ntx = (nx + ETX - 1) / ETX, nty = (ny + ETY - 1) / ETY;
-    int b = blockIdx.x;
-    ox = (b % ntx) * ETX;
-    b /= ntx;
-    oy = (b % nty) * ETY;
-    oz = (b / nty) * ETZ;
-}
-// halo voxel hv of the tile at (ox,oy,oz): clamped global index and whether it is inside
-__device__ __forceinline__ int haloVox(int hv, int ox, int oy, int oz, int nx, int ny, int nz, bool& in) {
-    const int hx = hv % EHX, hy = (hv / EHX) % EHY, hz = hv / (EHX * EHY);
-    const int gx = ox + hx - 1, gy = oy + hy - 1, gz = oz + hz - 1;
-    in = gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz;
-    return (min(max(gz, 0), nz - 1) * ny + min(max(gy, 0), ny - 1)) * nx + min(max(gx, 0), nx - 1);
-}
-static int edgeBlocks(int nx, int ny, int nz) {
-    return ((nx + ETX - 1) / ETX) * ((ny + ETY - 1) / ETY) * ((nz + ETZ - 1) / ETZ);
-}
-
 // in: 1 -> 16 channels; W [27][16][1]
 __global__ __launch_bounds__(256) void k_in_fwd(const float* x, const float* W, uint16_t* out, int nx, int ny, int nz,
                                                 const double* done) {
@@ -759,32 +729,7 @@ __global__ __launch_bounds__(256) void k_in_fwd(const float* x, const float* W, 
     for (int u = 0; u < NIT; ++u)
         if (tid + u * 256 < EHV) sx[tid + u * 256] = tv[u];
     __syncthreads();
-    const int tx = tid % ETX, ty = (tid / ETX) % ETY, tz = tid / (ETX * ETY);
-    const int gx = ox + tx, gy = oy + ty, gz = oz + tz;
-    const int hb = ((tz + 1) * EHY + (ty + 1)) * EHX + (tx + 1);
-    float acc[16];
-#pragma unroll
-    for (int c = 0; c < 16; ++c) acc[c] = 0.f;
-#pragma unroll 3
-    for (int t = 0; t < 27; ++t) {
-        const float xv = sx[hb + ((t / 9 - 1) * EHY + ((t / 3) % 3 - 1)) * EHX + (t % 3 - 1)];
-#pragma unroll
-        for (int c4 = 0; c4 < 4; ++c4) {
-            const float4 w4 = *(const float4*)&sW[t * 16 + 4 * c4];
-            acc[4 * c4] += w4.x * xv;
-            acc[4 * c4 + 1] += w4.y * xv;
-            acc[4 * c4 + 2] += w4.z * xv;
-            acc[4 * c4 + 3] += w4.w * xv;
-        }
-    }
-    if (gx < nx && gy < ny && gz < nz) {
-        uint32_t w[8];
-#pragma unroll
-        for (int c = 0; c < 8; ++c) w[c] = f2bf(acc[2 * c]) | (f2bf(acc[2 * c + 1]) << 16);
-        uint4* o = (uint4*)(out + ((size_t)(gz * ny + gy) * nx + gx) * 16);
-        o[0] = make_uint4(w[0], w[1], w[2], w[3]);
-        o[1] = make_uint4(w[4], w[5], w[6], w[7]);
-    }
+    inLayerTile(sx, sW, out, ox, oy, oz, nx, ny, nz, tid);
 }
 
 // out: 16 -> 1 channel; W [27][1][16].  The BF16 activations meet the FP32 weights
@@ -1069,15 +1014,20 @@ static void convLaunch(hipStream_t s, const ConvArgs& a) {
     else throw std::string("convolution input channels must be 16, 32 or 64");
 }
 
+// Forward tiles by level size: 16x4x4 on grids of 64 and more, 8x4x4 on 32, 4x4x4 below
+// (more workgroups where the grid is small); the 16-input-channel layers take k_convf16
+// from 32 up.  Measured on the learned solve at 128^3 (coarse grid 64^3): 211 us per FCG
+// iteration with 16x4x4 on 32^3 and 8x4x4 on 16^3, 199 us with this choice, results
+// identical (a tile's outputs do not depend on its shape).
 template <int CIN, int MODE, bool POOLOUT>
 static void convFLaunchC(hipStream_t s, const ConvFArgs& a) {
     if (CIN == 16 && a.nx >= 32) {
         const int nt = ((a.nx + 15) / 16) * ((a.ny + 3) / 4) * ((a.nz + 3) / 4);
         hipLaunchKernelGGL((k_convf16<MODE, POOLOUT>), dim3(nt, a.cout / 16), dim3(NB), 0, s, a);
-    } else if (a.nx >= 32) {
+    } else if (a.nx >= 64) {
         const int nt = ((a.nx + 15) / 16) * ((a.ny + 3) / 4) * ((a.nz + 3) / 4);
         hipLaunchKernelGGL((k_convf<16, 4, 4, CIN, MODE, POOLOUT>), dim3(nt, a.cout / 16), dim3(NB), 0, s, a);
-    } else if (a.nx >= 16) {
+    } else if (a.nx >= 32) {
         const int nt = ((a.nx + 7) / 8) * ((a.ny + 3) / 4) * ((a.nz + 3) / 4);
         hipLaunchKernelGGL((k_convf<8, 4, 4, CIN, MODE, POOLOUT>), dim3(nt, a.cout / 16), dim3(NB), 0, s, a);
     } else {

@@ -15,7 +15,10 @@ energy error of one learned two-grid cycle (Adam, cosine-decayed rate, all on th
 through the library's own forward and backward kernels).  Held-out cavities (seeds from
 10000) measure the loss after training.
 
-    python -m caffe_old_amd.train_pcorr_net --n 128 --runs 16 --steps 12 --train-steps 3000
+    python -m caffe_old_amd.train_pcorr_net --n 128 --runs 80 --steps 25 --pairs 2000 \
+        --train-steps 24000 --select --synthetic 0.75 --lr 5e-4
+
+(the command that made the shipped 128^3 weights)
 """
 from __future__ import annotations
 
