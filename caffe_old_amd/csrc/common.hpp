@@ -210,5 +210,9 @@ __device__ inline int xcdChunk() {
     const int nb = gridDim.x, b = blockIdx.x;
     return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
 }
+// interior cell of a one-thread-per-cell kernel (grid = ceil(N / blockDim)) in the
+// XCD-aware chunk order: an XCD's blocks take one contiguous eighth of the cells, so the
+// +-sJ and +-sK neighbours a cell reads are mostly in that XCD's L2
+__device__ inline long xcdCell() { return (long)xcdChunk() * blockDim.x + threadIdx.x; }
 
 }  // namespace caffe

@@ -102,7 +102,7 @@ __device__ inline void nonOrthFace(const Geo& g, const MomArgs& a, long p, long 
 template <bool U>
 __global__ void k_mom_assemble(Geo g, MomArgs a) {
     const Dims& D = g.D;
-    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long q = xcdCell();
     if (q >= D.N) return;
     int i, j, k;
     const long p = D.cell(q, i, j, k);
@@ -192,7 +192,7 @@ void launch_mom_assemble(Ctx& c, double dt) {
 __global__ void k_mom_copyback(Dims D, const int8_t* st, const double* x, double* u0, double* u1, double* u2,
                                double* h0, double* h1, double* h2, const double* d,
                                const double* g0, const double* g1, const double* g2) {
-    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long q = xcdCell();
     if (q >= D.N) return;
     int i, j, k;
     const long p = D.cell(q, i, j, k);
@@ -241,7 +241,7 @@ struct RcArgs {
 template <bool U>
 __global__ void k_rhie_chow(Geo g, RcArgs a) {
     const Dims& D = g.D;
-    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long q = xcdCell();
     if (q >= D.N) return;
     int ijk[3];
     const long p = D.cell(q, ijk[0], ijk[1], ijk[2]);
@@ -295,7 +295,7 @@ struct PcArgs {
 template <bool U>
 __global__ void k_pc_assemble(Geo g, PcArgs a) {
     const Dims& D = g.D;
-    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long q = xcdCell();
     if (q >= D.N) return;
     int i, j, k;
     const long p = D.cell(q, i, j, k);
@@ -347,7 +347,7 @@ void launch_pc_assemble(Ctx& c) {
 
 // Simple.cpp:277-289
 __global__ void k_copy_noninactive(Dims D, const int8_t* st, const double* src, double* dst) {
-    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long q = xcdCell();
     if (q >= D.N) return;
     int i, j, k;
     const long p = D.cell(q, i, j, k);
@@ -366,7 +366,7 @@ void launch_pc_copyback(Ctx& c) {
 template <bool U>
 __global__ void k_grad_scalar(Geo g, const double* phi, double* o0, double* o1, double* o2) {
     const Dims& D = g.D;
-    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long q = xcdCell();
     if (q >= D.N) return;
     int ijk[3];
     const long p = D.cell(q, ijk[0], ijk[1], ijk[2]);
@@ -400,7 +400,7 @@ struct GuArgs { const double* u[3]; double* o[9]; };
 template <bool U>
 __global__ void k_grad_vector(Geo g, GuArgs a) {
     const Dims& D = g.D;
-    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long q = xcdCell();
     if (q >= D.N) return;
     int ijk[3];
     const long p = D.cell(q, ijk[0], ijk[1], ijk[2]);
@@ -454,7 +454,7 @@ struct CorrArgs {
 template <bool U>
 __global__ void k_correct(Geo g, CorrArgs a) {
     const Dims& D = g.D;
-    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long q = xcdCell();
     if (q >= D.N) return;
     int ijk[3];
     const long p = D.cell(q, ijk[0], ijk[1], ijk[2]);
@@ -700,7 +700,7 @@ struct CpArgs {
 template <bool U>
 __global__ void k_cp_momentum(Geo g, CpArgs a) {
     const Dims& D = g.D;
-    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long q = xcdCell();
     if (q >= D.N) return;
     int i, j, k;
     const long p = D.cell(q, i, j, k);
@@ -771,7 +771,7 @@ __global__ void k_cp_momentum(Geo g, CpArgs a) {
 template <bool U>
 __global__ void k_cp_continuity(Geo g, CpArgs a) {
     const Dims& D = g.D;
-    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long q = xcdCell();
     if (q >= D.N) return;
     int i, j, k;
     const long p = D.cell(q, i, j, k);
@@ -864,7 +864,7 @@ void launch_cp_assemble(Ctx& c, double dt) {
 
 // Coupled.cpp:239-255: u and p from the solution
 __global__ void k_cp_copyback(Dims D, const double* x, double* u0, double* u1, double* u2, double* p) {
-    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long q = xcdCell();
     if (q >= D.N) return;
     int i, j, k;
     const long c = D.cell(q, i, j, k);
@@ -877,7 +877,7 @@ __global__ void k_cp_copyback(Dims D, const double* x, double* u0, double* u1, d
 template <bool U>
 __global__ void k_cp_gradp(Geo g, const double* phi, double* o0, double* o1, double* o2) {
     const Dims& D = g.D;
-    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long q = xcdCell();
     if (q >= D.N) return;
     int ijk[3];
     const long p = D.cell(q, ijk[0], ijk[1], ijk[2]);
@@ -902,7 +902,7 @@ __global__ void k_cp_gradp(Geo g, const double* phi, double* o0, double* o1, dou
 // Coupled.cpp:261-273: h = u + d grad p
 __global__ void k_cp_h(Dims D, const double* u0, const double* u1, const double* u2, const double* d,
                        const double* g0, const double* g1, const double* g2, double* h0, double* h1, double* h2) {
-    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long q = xcdCell();
     if (q >= D.N) return;
     int i, j, k;
     const long p = D.cell(q, i, j, k);
@@ -915,7 +915,7 @@ __global__ void k_cp_h(Dims D, const double* u0, const double* u1, const double*
 template <bool U>
 __global__ void k_cp_rhie_chow(Geo g, RcArgs a) {
     const Dims& D = g.D;
-    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long q = xcdCell();
     if (q >= D.N) return;
     int ijk[3];
     const long p = D.cell(q, ijk[0], ijk[1], ijk[2]);
@@ -995,7 +995,7 @@ struct DiffArgs { const double* phi; double* A; double* b; double dt; int unstea
 template <bool U>
 __global__ void k_diff_assemble(Geo g, DiffArgs a) {
     const Dims& D = g.D;
-    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long q = xcdCell();
     if (q >= D.N) return;
     int i, j, k;
     const long p = D.cell(q, i, j, k);
@@ -1071,7 +1071,7 @@ struct ExtrapArgs { const double* phi; const double* g1[3]; double* o[3]; };
 template <bool U>
 __global__ void k_grad_extrap(Geo g, ExtrapArgs a) {
     const Dims& D = g.D;
-    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long q = xcdCell();
     if (q >= D.N) return;
     int ijk[3];
     const long p = D.cell(q, ijk[0], ijk[1], ijk[2]);
@@ -1153,7 +1153,7 @@ void launch_fill(Ctx& c, double* a, double v, long n) {
 // IbPiso.cpp:74-80: every SOLID cell (INACTIVE, and the IB/GHOST cells picked from
 // them afterwards) starts the time step with u = 0, p = 0
 __global__ void k_zero_inactive(Dims D, const int8_t* st, double* u0, double* u1, double* u2, double* p) {
-    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long q = xcdCell();
     if (q >= D.N) return;
     int i, j, k;
     const long pp = D.cell(q, i, j, k);
