@@ -20,8 +20,8 @@ in the oracle fails a test:
     in y: u_x linear in the cell-centroid y is exact, after one SIMPLE iteration from rest
     with alpha = 1 and as a fixed point with alpha = 0.7.
 
-The mutation record (which planted oracle mistakes each test catches) is kept with the
-oracle's documentation."""
+tests/test_oracle_mutants.py rebuilds the oracle with planted mistakes and checks that
+these pins catch each of them."""
 import numpy as np
 import pytest
 
