@@ -16,7 +16,8 @@ enum {
     SC_BI = 0,     // BiCGStab state, 16 slots per system (up to 3 systems)
     SC_CG = 48,    // CG state
     SC_MISC = 64,  // reductions read back by the host
-    N_SCAL = 128
+    SC_FCGR = 128, // the learned FCG's per-round records (learned.hip)
+    N_SCAL = 192
 };
 enum { FL_MDONE = 0, FL_PDONE = 1, FL_MITER = 2, FL_PITER = 3, N_FLAGS = 8 };
 
@@ -164,6 +165,13 @@ struct Ctx {
     // learned preconditioner (optional)
     void* net = nullptr;
     int fcgCoarse = 0;        // coarse solve of its two-grid cycle: 0 the network, 1 a multigrid V-cycle
+    // the learned FCG's rounds queued without a read-back: count and length of each, in two
+    // slots (PISO alternates two correctors whose systems differ), each with the norm of the
+    // right-hand side it was made on
+    int fcgSolves = 0;
+    int fcgPlanN[2] = {0, 0};
+    int fcgPlan[2][8] = {};
+    double fcgPlanB[2] = {0., 0.};
 };
 
 // ---- launchers (fv_kernels.hip)
@@ -227,11 +235,13 @@ float* falloc(Ctx& c, size_t n);
 
 // FP64 true residual of the p' system (r = b - shift - A x into wp + 6P): ||r||, ||b - shift||
 void pcorr_true_resid(Ctx& c, double shift, double* rr, double* bb);
+// the same residual left on the device: (r,r), (b,b), sum r at c.scal + SC_MISC + 8
+void pcorr_true_resid_dev(Ctx& c, double shift);
 
 // ---- learned preconditioner (learned.hip): flexible CG with the learned two-grid
 // cycle, FP32 iterations inside FP64 refinement; returns the iteration count, or -1
 // when the system does not qualify (ghost rows, dimensions not divisible by 16)
-int learned_fcg(Ctx& c, double shift, double rtol, int maxit, double delta, int* refs, double* relOut);
+int learned_fcg(Ctx& c, double shift, double bNorm, double rtol, int maxit, double delta, int* refs, double* relOut);
 
 #define HIPCHECK(x)                                                        \
     do {                                                                   \

@@ -452,6 +452,22 @@ __global__ void k_finalize(const double* part, int nb, double* out) {
 __global__ void k_add_const(Dims D, const int8_t* st, double* x, double c) {
     FOR_CELLS(D, q, p) { if (st[p] != ST_INACTIVE) x[p] += c; }
 }
+// k_sum with the sum of squares as a third partial
+__global__ void k_sum_sq(Dims D, const int8_t* st, const double* b, double* part) {
+    double acc[3] = {0., 0., 0.};
+    FOR_CELLS(D, q, p) {
+        if (st[p] == ST_INACTIVE) continue;
+        acc[0] += b[p];
+        acc[1] += 1.;
+        acc[2] += b[p] * b[p];
+    }
+    writePartials<KBS, 3>(acc, part);
+}
+// x -= the mean that k_sum + k_finalize<2> left in m (sum, count), on the live cells
+__global__ void k_sub_mean(Dims D, const int8_t* st, double* x, const double* m) {
+    const double mean = m[1] > 0. ? m[0] / m[1] : 0.;
+    FOR_CELLS(D, q, p) { if (st[p] != ST_INACTIVE) x[p] -= mean; }
+}
 __global__ void k_axpy_mixed(Dims D, double* x, const float* e, double s) {
     FOR_CELLS(D, q, p) { x[p] += s * (double)e[p]; }
 }
@@ -481,6 +497,18 @@ void add_vec(Ctx& c, double* x, const double* d, int n) {
 }
 
 // mean over non-inactive cells
+// the mean of v over the live cells and the norm of v minus that mean, in one read-back
+static double meanNorm(Ctx& c, const double* v, double* norm) {
+    const int g = gridFor(c.D);
+    hipLaunchKernelGGL(k_sum_sq, dim3(g), dim3(KBS), 0, c.stream, c.D, c.st, v, c.red.part);
+    hipLaunchKernelGGL(k_finalize<3>, dim3(1), dim3(KBS), 0, c.stream, c.red.part, g, c.scal + SC_MISC + 4);
+    readBack(c, c.scal + SC_MISC + 4, SC_MISC + 4, 3);
+    const double s = c.hscal[SC_MISC + 4], n = c.hscal[SC_MISC + 5], sq = c.hscal[SC_MISC + 6];
+    const double m = n > 0. ? s / n : 0.;
+    *norm = std::sqrt(std::max(sq - n * m * m, 0.));
+    return m;
+}
+
 static double meanOf(Ctx& c, const double* v) {
     const int g = gridFor(c.D);
     hipLaunchKernelGGL(k_sum, dim3(g), dim3(KBS), 0, c.stream, c.D, c.st, v, c.red.part);
@@ -1274,13 +1302,21 @@ void pcorr_true_resid(Ctx& c, double shift, double* rr, double* bb) {
     trueResid<1>(c, symP<double>(c), c.bp, shift, c.xp, c.wp + 6 * c.D.P, rr, bb, c.singularP);
 }
 
+void pcorr_true_resid_dev(Ctx& c, double shift) {
+    const int g = gridFor(c.D);
+    hipLaunchKernelGGL((k_resid<1, Sym4<double>>), dim3(g), dim3(KBS), 0, c.stream, c.D, symP<double>(c), (const double*)c.bp,
+                       shift, (const double*)c.xp, c.wp + 6 * c.D.P, c.red.part);
+    hipLaunchKernelGGL(k_finalize<3>, dim3(1), dim3(KBS), 0, c.stream, c.red.part, g, c.scal + SC_MISC + 8);
+}
+
 int pcorr_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep) {
     HIPCHECK(hipEventRecord(c.ev0, c.stream));
     const Dims& D = c.D;
     const int P = D.P;
     // Neumann null space: the right-hand side loses its (rounding) mean, the solution is
     // returned with zero mean over the cells that carry a row
-    const double shift = c.singularP ? meanOf(c, c.bp) : 0.;
+    double bNorm = 0.;   // |b - shift| where known without another read-back (0: unknown)
+    const double shift = c.singularP ? meanNorm(c, c.bp, &bNorm) : 0.;
     int iters = 0, refs = 0, fell = 0;
     double rel = 1.;
     const bool sym = !c.hasIB && !c.indefiniteP;
@@ -1299,7 +1335,7 @@ int pcorr_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep) {
         if (o.mode == CAFFE_SOLVE_LEARNED_FCG) {
             // learned two-grid cycle inside flexible CG; the inner FP32 solve restarts from
             // the true FP64 residual each time it has gained `inner_rtol` (default 1e-4)
-            const int it = c.net ? learned_fcg(c, shift, o.rtol, o.maxit, o.inner_rtol > 0. ? o.inner_rtol : 1e-4, &refs, &rel) : -1;
+            const int it = c.net ? learned_fcg(c, shift, bNorm, o.rtol, o.maxit, o.inner_rtol > 0. ? o.inner_rtol : 1e-4, &refs, &rel) : -1;
             if (it >= 0) { iters += it; learnedDone = true; }
             else fell = 1;   // no network, or a system it does not apply to: IR below
         }
@@ -1330,9 +1366,11 @@ int pcorr_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep) {
             refs += r2;
         }
     }
-    if (c.singularP) {
-        const double m = meanOf(c, c.xp);
-        hipLaunchKernelGGL(k_add_const, dim3(gridFor(D)), dim3(KBS), 0, c.stream, D, c.st, c.xp, -m);
+    if (c.singularP) {   // the mean stays on the device: no read-back
+        const int g = gridFor(D);
+        hipLaunchKernelGGL(k_sum, dim3(g), dim3(KBS), 0, c.stream, D, c.st, (const double*)c.xp, c.red.part);
+        hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(KBS), 0, c.stream, c.red.part, g, c.scal + SC_MISC + 4);
+        hipLaunchKernelGGL(k_sub_mean, dim3(g), dim3(KBS), 0, c.stream, D, c.st, c.xp, (const double*)(c.scal + SC_MISC + 4));
     }
     HIPCHECK(hipEventRecord(c.ev1, c.stream));
     HIPCHECK(hipEventSynchronize(c.ev1));

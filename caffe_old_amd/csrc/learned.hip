@@ -342,6 +342,69 @@ __global__ void k_f_init(Dims D, const double* r64, double s, float* r, float* e
     }
 }
 
+// Refinement rounds queued without a host read-back.  Each round's record at
+// c.scal + SC_FCGR: the relative residual it started from, whether it was skipped, its
+// scale (1 / |r|) and iteration count, and whether its queued iterations ran out first.
+enum { FR_REL = 0, FR_STOP, FR_SCALE, FR_ITS, FR_CAPPED, FR_STRIDE };
+constexpr int FCG_MAX_PLAN = 8;
+struct RoundCtl {
+    double rtol, delta, nCells;   // nCells > 0: the residual norm drops the mean (pure Neumann)
+    int round, maxit;
+};
+
+// The start of round rc.round from the true residual (r,r), (b,b), sum r that the previous
+// kernels left in fin, with the decisions of the host loop made on the device: the round is
+// skipped when the residual meets rtol, the iteration budget is spent, or the last round
+// gained less than 2x or made no iteration.  Otherwise r = r64 / |r| (live cells), e = 0,
+// partials (r,r), sum r, and the round's loop limits (its tolerance gains `delta`, or only
+// what rtol still needs).  Every block decides from values earlier kernels wrote; block 0
+// writes this round's record and the FCG state, which no block of this kernel reads.
+__global__ void k_f_init_round(Dims D, const double* r64, const double* fin, double* rec, float* r, float* e,
+                               double* part, double* st, RoundCtl rc) {
+    const double r2 = fin[0], b2 = fin[1], rs = fin[2];
+    double itsSoFar = 0., prevRel = 0., prevStop = 0., prevIts = 1.;
+    for (int q = 0; q < rc.round; ++q) itsSoFar += rec[q * FR_STRIDE + FR_ITS];
+    if (rc.round > 0) {
+        const double* pr = rec + (rc.round - 1) * FR_STRIDE;
+        prevRel = pr[FR_REL]; prevStop = pr[FR_STOP]; prevIts = pr[FR_ITS];
+    }
+    const double rr = sqrt(rc.nCells > 0. ? fmax(r2 - rs * rs / rc.nCells, 0.) : r2), bb = sqrt(b2);
+    const double rel = bb > 0. ? rr / bb : 0.;
+    const bool stop = prevStop != 0. || !(rel > rc.rtol) || itsSoFar >= (double)rc.maxit ||
+                      (rc.round > 0 && (!(rel < 0.5 * prevRel) || prevIts == 0.));
+    const double need = fmax(rc.delta, 0.5 * rc.rtol / rel);
+    const double scale = 1. / rr;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        double* my = rec + rc.round * FR_STRIDE;
+        my[FR_REL] = rel; my[FR_STOP] = stop ? 1. : 0.; my[FR_SCALE] = scale; my[FR_ITS] = 0.; my[FR_CAPPED] = 0.;
+        st[F_IT] = 0.; st[F_DONE] = stop ? 1. : 0.; st[F_BB] = -1.; st[F_TOL2] = need * need;
+        st[F_MAXIT] = fmin(300., (double)rc.maxit - itsSoFar);
+    }
+    if (stop) return;
+    double v[2] = {0., 0.};
+    FOR_FINE(D, q, p) {
+        const float rv = (float)(scale * r64[p]);
+        r[p] = rv;
+        e[p] = 0.f;
+        v[0] += (double)rv * (double)rv;
+        v[1] += (double)rv;
+    }
+    writePartials<FB, 2>(v, part);
+}
+
+// x += e / scale at the end of a queued round (nothing for a skipped one); block 0 records
+// the round's iteration count and whether its queued iterations ran out before its test
+__global__ void k_f_axpy_round(Dims D, double* x, const float* e, double* rec, const double* st, int round) {
+    const double* my = rec + round * FR_STRIDE;
+    if (my[FR_STOP] != 0.) return;
+    const double sc = 1. / my[FR_SCALE];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        rec[round * FR_STRIDE + FR_ITS] = st[F_IT];
+        rec[round * FR_STRIDE + FR_CAPPED] = st[F_DONE] == 0. ? 1. : 0.;
+    }
+    FOR_FINE(D, q, p) { x[p] += sc * (double)e[p]; }
+}
+
 // The residual alternates between two buffers by iteration parity (r[F_IT & 1] is the
 // current one), so that k_f_k2pre can write the next residual while its neighbours
 // still read the current one.
@@ -380,37 +443,105 @@ __global__ void k_f_restrict_dw(Dims D, const float* w, const float* aP, float* 
     restrictAt<true>(D, w, v, blockIdx.x * blockDim.x + threadIdx.x, aP);
 }
 
-// y = w D^-1 r + P c
-__global__ void k_f_post1(Dims D, const float* dinv, const float* r0, const float* r1, const float* c, float* y,
-                          const double* st) {
-    const double sDone = st[F_DONE], sIt = st[F_IT];
-    if (sDone != 0.) return;
-    const float* r = ((int)sIt & 1) ? r1 : r0;
-    const int cx = D.nI >> 1, cy = D.nJ >> 1, cz = D.nK >> 1;
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < D.N; q += gridDim.x * blockDim.x) {
-        int i, j, k;
-        const int p = D.cell(q, i, j, k);
-        y[p] = NET_OMEGA * dinv[p] * r[p] + prolongAt(c, cx, cy, cz, i, j, k);
-    }
-}
-
-// z = y + w D^-1 (r - A y); partials (z, q_old), sum z, (z, r)
-__global__ void k_f_post2(Dims D, SymF S, const float* dinv, const float* r0, const float* r1, const float* y, float* z,
-                          const float* q0, const float* q1, double* part, const double* st) {
+// The post-smoothing of the cycle in one pass, without y in memory.  With s = P c and
+// w = D^-1 (r - A z1) from the pre-smoothing (k_f_k2pre / k_f_pre0),
+//   z = y + w D^-1 (r - A y) = z1 + s + w (w_pre - D^-1 A s),   z1 = w D^-1 r,
+// since r - A y = (r - A z1) - A s.  s is formed at the seven stencil points from the
+// coarse correction c (a 1/64-size array that stays in cache): each thread walks its
+// column up k (column tiles as in k_f_k1) keeping, for its own column and the four side
+// columns, the bilinear (x, y) interpolant of c on three coarse planes (k >> 1 - 1 .. + 1);
+// a step to an even k shifts them and loads one coarse plane (a 3x3 patch).  Each s value
+// has prolongAt's products in prolongAt's order, so s is bit-identical to P c.  Cells
+// outside the grid take s = 0 (the halo of y).  Partials (z, q_old), sum z, (z, r).
+__global__ void k_f_post(Dims D, SymF S, ColMap cm, const float* __restrict__ dinv, const float* r0, const float* r1,
+                         const float* __restrict__ wpre, const float* __restrict__ c, float* z, const float* q0,
+                         const float* q1, double* part, const double* st) {
     const double sDone = st[F_DONE], sIt = st[F_IT];
     if (sDone != 0.) return;
     const int it = (int)sIt;
-    const float* qold = (it & 1) ? q0 : q1;
-    const float* r = (it & 1) ? r1 : r0;
+    const float* __restrict__ qold = (it & 1) ? q0 : q1;
+    const float* __restrict__ r = (it & 1) ? r1 : r0;
+    const int P = S.P, sK = S.sK;
+    const float* __restrict__ A = S.A;
+    const int cx = D.nI >> 1, cy = D.nJ >> 1, cz = D.nK >> 1;
+    const float ax = 0.75f, bx = 0.25f;
+    struct Plane { float r, d, w, q, a[7]; };
+    auto loadPlane = [&](int p, Plane& L) {
+        L.r = r[p]; L.d = dinv[p]; L.w = wpre[p]; L.q = qold[p];
+        L.a[0] = A[p]; L.a[1] = A[P + p]; L.a[2] = A[P + p + 1]; L.a[3] = A[2 * P + p];
+        L.a[4] = A[2 * P + p + S.sJ]; L.a[5] = A[3 * P + p]; L.a[6] = A[3 * P + p + sK];
+    };
     double v[3] = {0., 0., 0.};
-    auto yf = [&](int c) { return y[c]; };
-    FOR_FINE(D, q, p) {
-        const float rp = r[p];
-        const float zz = y[p] + NET_OMEGA * dinv[p] * (rp - S.apply(yf, p));
-        z[p] = zz;
-        v[0] += (double)zz * (double)qold[p];
-        v[1] += (double)zz;
-        v[2] += (double)zz * (double)rp;
+    for (int v_ = xcdChunk(), vn = cm.ntx * cm.nty * cm.nseg; v_ < vn; v_ += (int)gridDim.x) {
+        const int t_ = v_ / cm.nseg;
+        const int i = (t_ % cm.ntx) * CTX + (int)(threadIdx.x % CTX), j = (t_ / cm.ntx) * CTY + (int)(threadIdx.x / CTX);
+        int k = (v_ % cm.nseg) * cm.kper;
+        const int ke = min(D.nK, k + cm.kper);
+        if (!(i < D.nI && j < D.nJ) || k >= ke) continue;
+        int p = D.pidx(i, j, k);
+        Plane cur, nxt;
+        loadPlane(p, cur);
+        // the 3x3 coarse patch around (i >> 1, j >> 1), indices clamped as pw clamps them
+        const int X = i >> 1, Y = j >> 1;
+        const int xa = max(X - 1, 0), xb = X, xc = min(X + 1, cx - 1);
+        const int ya = max(Y - 1, 0), yb = Y, yc = min(Y + 1, cy - 1);
+        const bool pi = i & 1, pj = j & 1;
+        const bool inW = i > 0, inE = i + 1 < D.nI, inS = j > 0, inN = j + 1 < D.nJ;
+        // B[col][slot]: bilinear value of columns C, W, E, S, N on coarse planes m-1, m, m+1
+        float B[5][3];
+        struct Row { float l, c, h; };   // x = X - 1, X, X + 1 (clamped), scalars: no array to index
+        auto pick = [](bool b, float t, float f) { return b ? t : f; };
+        auto plane = [&](int m, int slot) {
+            const int mc = min(max(m, 0), cz - 1);
+            const float* c0 = c + (size_t)mc * cy * cx;
+            auto row = [&](int y) { const float* q = c0 + y * cx; return Row{q[xa], q[xb], q[xc]}; };
+            const Row R0 = row(ya), R1 = row(yb), R2 = row(yc);
+            // 1-D pairs (x0, x1) of pw for the column's fine index
+            auto rowC = [&](const Row& R) { return ax * R.c + bx * pick(pi, R.h, R.l); };
+            auto rowW = [&](const Row& R) { return pi ? ax * R.c + bx * R.l : ax * R.l + bx * R.c; };
+            auto rowE = [&](const Row& R) { return pi ? ax * R.h + bx * R.c : ax * R.c + bx * R.h; };
+            // the centre column's second y row: R2 for odd j, else R0
+            const Row& Ry = pj ? R2 : R0;
+            B[0][slot] = ax * rowC(R1) + bx * rowC(Ry);
+            B[1][slot] = ax * rowW(R1) + bx * rowW(Ry);
+            B[2][slot] = ax * rowE(R1) + bx * rowE(Ry);
+            B[3][slot] = pj ? ax * rowC(R1) + bx * rowC(R0) : ax * rowC(R0) + bx * rowC(R1);   // j - 1
+            B[4][slot] = pj ? ax * rowC(R2) + bx * rowC(R1) : ax * rowC(R1) + bx * rowC(R2);   // j + 1
+        };
+        int m = k >> 1;
+        plane(m - 1, 0);
+        plane(m, 1);
+        plane(m + 1, 2);
+        // s of column col at fine plane f, whose coarse pair is (f >> 1, its neighbour):
+        // slots (1, f odd ? 2 : 0) when f >> 1 == m, (2, 1) when f >> 1 == m + 1 (f even)
+        auto sAt = [&](int col, int f) {
+            if ((f >> 1) == m) return (f & 1) ? ax * B[col][1] + bx * B[col][2] : ax * B[col][1] + bx * B[col][0];
+            if ((f >> 1) > m) return ax * B[col][2] + bx * B[col][1];
+            return ax * B[col][0] + bx * B[col][1];   // f = k - 1 odd below an even k
+        };
+        float sd = k > 0 ? sAt(0, k - 1) : 0.f;
+        float sc = sAt(0, k);
+        for (; k < ke; ++k, p += sK) {
+            if (k + 1 < ke) loadPlane(p + sK, nxt);
+            const float su = k + 1 < D.nK ? sAt(0, k + 1) : 0.f;
+            const float sw = inW ? sAt(1, k) : 0.f, se = inE ? sAt(2, k) : 0.f;
+            const float ss = inS ? sAt(3, k) : 0.f, sn = inN ? sAt(4, k) : 0.f;
+            const float As = cur.a[0] * sc + (cur.a[1] * sw + cur.a[2] * se + cur.a[3] * ss + cur.a[4] * sn
+                                              + cur.a[5] * sd + cur.a[6] * su);
+            const float zz = (NET_OMEGA * cur.d * cur.r + sc) + NET_OMEGA * (cur.w - cur.d * As);
+            z[p] = zz;
+            v[0] += (double)zz * (double)cur.q;
+            v[1] += (double)zz;
+            v[2] += (double)zz * (double)cur.r;
+            sd = sc; sc = su;
+            cur = nxt;
+            if (k & 1) {   // the next plane is even: shift the window up one coarse plane
+#pragma unroll
+                for (int col = 0; col < 5; ++col) { B[col][0] = B[col][1]; B[col][1] = B[col][2]; }
+                ++m;
+                if (k + 1 < ke) plane(m + 1, 2);
+            }
+        }
     }
     writePartials<FB, 3>(v, part);
 }
@@ -418,7 +549,7 @@ __global__ void k_f_post2(Dims D, SymF S, const float* dinv, const float* r0, co
 // beta = -(z - zbar, q_old) / (p_old, q_old); p = (z - zbar) + beta p_old formed at the
 // stencil points; q = A p; partials (p,q), sum q.  (p, r) needs no pass of its own:
 // alpha of the previous step makes (p_old, r) = 0, so (p, r) = (z - zbar, r), from the
-// partials of k_f_post2; block 0 records it for k_f_k2pre.
+// partials of k_f_post; block 0 records it for k_f_k2pre.
 __global__ void k_f_k1(Dims D, SymF S, ColMap cm, const float* z, float* p0, float* p1, float* q0, float* q1,
                        const double* partIn, int nb, double* partOut, double* st, FcgCtl lc) {
     // the scalars written by earlier kernels are loaded together, ahead of the partial sums
@@ -617,7 +748,7 @@ struct FcgIter {
     SymF S;
     ColMap cm;
     int g;
-    float *r, *r1, *e, *w, *y, *z, *p0, *p1, *q0, *q1, *dinv;
+    float *r, *r1, *e, *w, *z, *p0, *p1, *q0, *q1, *dinv;
     double *st, *partA, *partB, *partC;
     FcgCtl lc;
     bool classical;   // the coarse solve is one multigrid V-cycle (the ablation), not the network
@@ -638,10 +769,9 @@ struct FcgIter {
         } else {
             net_forward(n, s, n.vin, n.vout, done);
         }
-        hipLaunchKernelGGL(k_f_post1, dim3(g), dim3(FB), 0, s, D, (const float*)dinv, (const float*)r, (const float*)r1,
-                           (const float*)n.vout, y, (const double*)st);
-        hipLaunchKernelGGL(k_f_post2, dim3(g), dim3(FB), 0, s, D, S, (const float*)dinv, (const float*)r, (const float*)r1,
-                           (const float*)y, z, (const float*)q0, (const float*)q1, partA, (const double*)st);
+        hipLaunchKernelGGL(k_f_post, dim3(g), dim3(FB), 0, s, D, S, cm, (const float*)dinv, (const float*)r,
+                           (const float*)r1, (const float*)w, (const float*)n.vout, z, (const float*)q0, (const float*)q1,
+                           partA, (const double*)st);
         hipEvent_t e0, e1;
         if (probe && c.probe.slot(PROBE_FCG_K1, e0, e1))   // dispatch-bracketing events: the kernel's own time
             hipExtLaunchKernelGGL(k_f_k1, dim3(g), dim3(FB), 0, s, e0, e1, 0, D, S, cm, (const float*)z, p0, p1, q0, q1,
@@ -660,7 +790,7 @@ struct FcgIter {
     }
 };
 
-int learned_fcg(Ctx& c, double shift, double rtol, int maxit, double delta, int* refs, double* relOut) {
+int learned_fcg(Ctx& c, double shift, double bNorm, double rtol, int maxit, double delta, int* refs, double* relOut) {
     Net& n = *(Net*)c.net;
     const Dims& D = c.D;
     *refs = 0;
@@ -674,19 +804,83 @@ int learned_fcg(Ctx& c, double shift, double rtol, int maxit, double delta, int*
     const int g = fgrid(D);
     auto fv = [&](int slot) { return n.fw + (size_t)slot * P; };
     const FcgIter I{c, n, symF(c.Ap32, D), colMap(D, g), g,
-                    fv(FV_R), fv(FV_R1), fv(FV_E), fv(FV_W), fv(FV_Y), fv(FV_Z), fv(FV_P0), fv(FV_P1), fv(FV_Q0), fv(FV_Q1),
+                    fv(FV_R), fv(FV_R1), fv(FV_E), fv(FV_W), fv(FV_Z), fv(FV_P0), fv(FV_P1), fv(FV_Q0), fv(FV_Q1),
                     fv(FV_DI), c.scal + SC_CG, c.red.part, c.red.part + MAX_RED_BLOCKS * RED_STRIDE,
                     c.red.part + 2 * MAX_RED_BLOCKS * RED_STRIDE, FcgCtl{c.singularP ? (double)D.N : 0.},
                     c.fcgCoarse == 1};
     if (I.classical && !mg_coarse_prepare(c)) return -1;
     double* st = I.st;
     hipLaunchKernelGGL(k_f_dinv, dim3(blocksFor(P, 256)), dim3(256), 0, s, I.S, I.dinv);
-    double rr, bb;
-    pcorr_true_resid(c, shift, &rr, &bb);
-    double rel = bb > 0. ? rr / bb : 0.;
+    double rr = 0., bb = 0., rel = 0.;
     int its = 0;
     int lastRound = 0;
-    while (rel > rtol && its < maxit && *refs < 40) {
+    bool stalled = false;
+    // this solve's rounds: iterations and whether the queued ones ran out before the test
+    int nRec = 0, recIts[FCG_MAX_PLAN] = {}, recCapped[FCG_MAX_PLAN] = {};
+    auto record = [&](int it_, bool capped) {
+        if (nRec < FCG_MAX_PLAN) { recIts[nRec] = it_; recCapped[nRec] = capped ? 1 : 0; }
+        ++nRec;
+    };
+    // the plan slot: with |b| known, the slot made on the nearest |b| (a slot made on a |b|
+    // more than 3x away is not used; an empty slot then takes this solve's rounds); without
+    // it, the slots alternate
+    int slot = c.fcgSolves & 1;
+    if (bNorm > 0.) {
+        auto dist = [&](int q) { return c.fcgPlanN[q] > 0 && c.fcgPlanB[q] > 0. ? std::fabs(std::log(bNorm / c.fcgPlanB[q])) : 1e300; };
+        const double d0 = dist(0), d1 = dist(1), far = std::log(3.);
+        if (d0 < 1e300 || d1 < 1e300) {
+            slot = d0 <= d1 ? 0 : 1;
+            if (std::min(d0, d1) > far) {   // too far: the other slot's plan is not used
+                if (c.fcgPlanN[1 - slot] == 0) slot = 1 - slot;
+                c.fcgPlanN[slot] = 0;
+            }
+        }
+    }
+    const double nLive = c.singularP ? (double)D.N : 0.;
+    double* rec = c.scal + SC_FCGR;
+    const double* fin = c.scal + SC_MISC + 8;
+    auto relFrom = [&](const double* f) {   // the host formula of pcorr_true_resid
+        rr = std::sqrt(nLive > 0. ? std::max(f[0] - f[2] * f[2] / nLive, 0.) : f[0]);
+        bb = std::sqrt(f[1]);
+        rel = bb > 0. ? rr / bb : 0.;
+    };
+    if (c.fcgPlanN[slot] > 0) {
+        // the rounds of the solve two back, each one iteration longer, queued with their
+        // decisions on the device and one read-back at the end
+        const int nq = c.fcgPlanN[slot];
+        pcorr_true_resid_dev(c, shift);
+        for (int q = 0; q < nq; ++q) {
+            const RoundCtl rc{rtol, delta, nLive, q, maxit};
+            hipLaunchKernelGGL(k_f_init_round, dim3(g), dim3(FB), 0, s, D, (const double*)(c.wp + 6 * P), fin, rec, I.r,
+                               I.e, I.partC, st, rc);
+            hipLaunchKernelGGL(k_f_pre0, dim3(g), dim3(FB), 0, s, D, I.S, (const float*)I.dinv, (const float*)I.r, I.w,
+                               (const double*)st);
+            for (int k = 0; k < c.fcgPlan[slot][q]; ++k) I.launch(s, true);
+            hipLaunchKernelGGL(k_f_axpy_round, dim3(g), dim3(FB), 0, s, D, c.xp, (const float*)I.e, rec, (const double*)st, q);
+            pcorr_true_resid_dev(c, shift);
+        }
+        HIPCHECK(hipMemcpyAsync(c.hscal + SC_MISC + 8, fin, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipMemcpyAsync(c.hscal + SC_FCGR, rec, nq * FR_STRIDE * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        relFrom(c.hscal + SC_MISC + 8);
+        bool anyStop = false;
+        for (int q = 0; q < nq; ++q) {
+            const double* h = c.hscal + SC_FCGR + q * FR_STRIDE;
+            if (h[FR_STOP] != 0.) { anyStop = true; break; }
+            const int it = (int)h[FR_ITS];
+            its += it;
+            ++*refs;
+            lastRound = it;
+            record(it, h[FR_CAPPED] != 0.);
+        }
+        // a round the device skipped with the residual above rtol and iterations left is the
+        // host loop's "no progress" exit
+        stalled = anyStop && rel > rtol && its < maxit;
+    } else {
+        pcorr_true_resid(c, shift, &rr, &bb);
+        rel = bb > 0. ? rr / bb : 0.;
+    }
+    while (!stalled && rel > rtol && its < maxit && *refs < 40) {
         // the inner solve gains `delta`, or only what the last round still needs
         const double need = std::max(delta, 0.5 * rtol / rel);
         const double tol2 = need * need;
@@ -700,7 +894,7 @@ int learned_fcg(Ctx& c, double shift, double rtol, int maxit, double delta, int*
                            (const double*)st);
         // iterations are queued in chunks between convergence checks: the first chunk of a
         // round is the previous round's count, later ones the count the observed rate of
-        // decrease still needs (a queued iteration past convergence costs 17 launches
+        // decrease still needs (a queued iteration past convergence costs 16 launches
         // that return at once)
         int chunk = lastRound > 0 ? lastRound : 6, it = 0;
         while (true) {
@@ -720,6 +914,7 @@ int learned_fcg(Ctx& c, double shift, double rtol, int maxit, double delta, int*
         }
         lastRound = it;
         its += it;
+        record(it, false);
         hipLaunchKernelGGL(k_f_axpy, dim3(g), dim3(FB), 0, s, D, c.xp, (const float*)I.e, 1.0 / scale);
         const double prev = rel;
         pcorr_true_resid(c, shift, &rr, &bb);
@@ -727,6 +922,12 @@ int learned_fcg(Ctx& c, double shift, double rtol, int maxit, double delta, int*
         ++*refs;
         if (!(rel < 0.5 * prev) || it == 0) break;
     }
+    // the plan for the solve two ahead: this solve's rounds if it converged, each one
+    // iteration longer (two if its queued iterations ran out)
+    c.fcgPlanN[slot] = (rel <= rtol && nRec > 0 && nRec <= FCG_MAX_PLAN) ? nRec : 0;
+    c.fcgPlanB[slot] = bNorm;
+    for (int q = 0; q < c.fcgPlanN[slot]; ++q) c.fcgPlan[slot][q] = recIts[q] + (recCapped[q] ? 2 : 1);
+    ++c.fcgSolves;
     *relOut = rel;
     return its;
 }
