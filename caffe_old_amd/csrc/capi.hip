@@ -740,6 +740,14 @@ int caffe_ctx_set_fcg_coarse(caffe_ctx* x, int mode) {
     });
 }
 
+int caffe_ctx_set_fcg_post_sweeps(caffe_ctx* x, int sweeps) {
+    return guard(x, [&](Ctx& c) {
+        if (sweeps < 1 || sweeps > 4) { c.err = "post-smoothing sweeps of the learned cycle: 1 to 4"; return CAFFE_ERR_ARG; }
+        c.fcgPost = sweeps;
+        return CAFFE_OK;
+    });
+}
+
 int caffe_sync(caffe_ctx* x) {
     return guard(x, [&](Ctx& c) { HIPCHECK(hipStreamSynchronize(c.stream)); return CAFFE_OK; });
 }

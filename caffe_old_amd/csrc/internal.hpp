@@ -165,6 +165,7 @@ struct Ctx {
     // learned preconditioner (optional)
     void* net = nullptr;
     int fcgCoarse = 0;        // coarse solve of its two-grid cycle: 0 the network, 1 a multigrid V-cycle
+    int fcgPost = 3;          // damped Jacobi sweeps after its coarse correction (1 to 4)
     // the learned FCG's rounds queued without a read-back: count and length of each, in two
     // slots (PISO alternates two correctors whose systems differ), each with the norm of the
     // right-hand side it was made on

@@ -10,9 +10,13 @@
 //   c  = net(v)                              the multilevel convolution network
 //   y  = z1 + P c                            P = cell-centred trilinear prolongation
 //   z  = y + w D^-1 (r - A y)
-// The network is linear, so z = M r with a fixed linear M.  Training minimises the
-// relative energy error of one cycle applied to a right-hand side b whose converged
-// solution x is known:  L = (x - M b)^T A (x - M b) / x^T A x.
+// The network is linear, so z = M r with a fixed linear M.  Flexible CG applies the cycle
+// with the last line repeated (3 sweeps by default, caffe_ctx_set_fcg_post_sweeps): the
+// high-frequency error the coarse correction leaves is damped there rather than by Krylov
+// iterations (the second PISO corrector's systems: 24 -> 18 iterations at 128^3).
+// Training minimises the relative energy error of one cycle (one post-smoothing sweep)
+// applied to a right-hand side b whose converged solution x is known:
+// L = (x - M b)^T A (x - M b) / x^T A x.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -546,6 +550,56 @@ __global__ void k_f_post(Dims D, SymF S, ColMap cm, const float* __restrict__ di
     writePartials<FB, 3>(v, part);
 }
 
+// One more damped Jacobi sweep after the cycle's post-smoothing: z = y + w D^-1 (r - A y)
+// for y from k_f_post; partials (z, q_old), sum z, (z, r) (they replace k_f_post's).
+// Column tiles and loads one plane ahead as in k_f_k1.
+__global__ void k_f_sweep(Dims D, SymF S, ColMap cm, const float* __restrict__ dinv, const float* r0, const float* r1,
+                          const float* __restrict__ y, float* z, const float* q0, const float* q1, double* part,
+                          const double* st) {
+    const double sDone = st[F_DONE], sIt = st[F_IT];
+    if (sDone != 0.) return;
+    const int it = (int)sIt;
+    const float* __restrict__ qold = (it & 1) ? q0 : q1;
+    const float* __restrict__ r = (it & 1) ? r1 : r0;
+    const int P = S.P, sJ = S.sJ, sK = S.sK;
+    const float* __restrict__ A = S.A;
+    struct Plane { float ys[4], yu, a[7], r, d, q; };
+    auto loadPlane = [&](int p, Plane& L) {
+        const int o[4] = {-1, 1, -sJ, sJ};
+#pragma unroll
+        for (int m = 0; m < 4; ++m) L.ys[m] = y[p + o[m]];
+        L.yu = y[p + sK];
+        L.a[0] = A[p]; L.a[1] = A[P + p]; L.a[2] = A[P + p + 1]; L.a[3] = A[2 * P + p];
+        L.a[4] = A[2 * P + p + sJ]; L.a[5] = A[3 * P + p]; L.a[6] = A[3 * P + p + sK];
+        L.r = r[p]; L.d = dinv[p]; L.q = qold[p];
+    };
+    double v[3] = {0., 0., 0.};
+    for (int v_ = xcdChunk(), vn = cm.ntx * cm.nty * cm.nseg; v_ < vn; v_ += (int)gridDim.x) {
+        const int t_ = v_ / cm.nseg;
+        const int i = (t_ % cm.ntx) * CTX + (int)(threadIdx.x % CTX), j = (t_ / cm.ntx) * CTY + (int)(threadIdx.x / CTX);
+        int k = (v_ % cm.nseg) * cm.kper;
+        const int ke = min(D.nK, k + cm.kper);
+        if (!(i < D.nI && j < D.nJ) || k >= ke) continue;
+        int p = D.pidx(i, j, k);
+        float yd = y[p - sK], yc = y[p];
+        Plane cur, nxt;
+        loadPlane(p, cur);
+        for (; k < ke; ++k, p += sK) {
+            if (k + 1 < ke) loadPlane(p + sK, nxt);
+            const float Ay = cur.a[0] * yc + (cur.a[1] * cur.ys[0] + cur.a[2] * cur.ys[1] + cur.a[3] * cur.ys[2]
+                                              + cur.a[4] * cur.ys[3] + cur.a[5] * yd + cur.a[6] * cur.yu);
+            const float zz = yc + NET_OMEGA * cur.d * (cur.r - Ay);
+            z[p] = zz;
+            v[0] += (double)zz * (double)cur.q;
+            v[1] += (double)zz;
+            v[2] += (double)zz * (double)cur.r;
+            yd = yc; yc = cur.yu;
+            cur = nxt;
+        }
+    }
+    writePartials<FB, 3>(v, part);
+}
+
 // beta = -(z - zbar, q_old) / (p_old, q_old); p = (z - zbar) + beta p_old formed at the
 // stencil points; q = A p; partials (p,q), sum q.  (p, r) needs no pass of its own:
 // alpha of the previous step makes (p_old, r) = 0, so (p, r) = (z - zbar, r), from the
@@ -737,7 +791,8 @@ __global__ void k_f_axpy(Dims D, double* x, const float* e, double s) {
 }
 
 // The launches of one FCG iteration: restriction (with the convergence test), the network,
-// the post-smoothing, the direction and the update with the next pre-smoothing.  The same
+// the post-smoothing (k_f_post, then k_f_sweep for each further sweep), the direction and
+// the update with the next pre-smoothing.  The same
 // launches serve every iteration of every round: the parities and the round's loop limits
 // live in the device state.  (Replaying them from a captured HIP graph measured no faster
 // at 128^3, 3.79 against 3.76 ms per solve: the gaps between the dependent kernels are not
@@ -748,7 +803,7 @@ struct FcgIter {
     SymF S;
     ColMap cm;
     int g;
-    float *r, *r1, *e, *w, *z, *p0, *p1, *q0, *q1, *dinv;
+    float *r, *r1, *e, *w, *z, *p0, *p1, *q0, *q1, *dinv, *y, *h;
     double *st, *partA, *partB, *partC;
     FcgCtl lc;
     bool classical;   // the coarse solve is one multigrid V-cycle (the ablation), not the network
@@ -769,9 +824,20 @@ struct FcgIter {
         } else {
             net_forward(n, s, n.vin, n.vout, done);
         }
+        // the post-smoothing sweeps end in z; the extra ones alternate through y and h
+        const int extra = c.fcgPost - 1;
+        float* bufs[2] = {y, h};
+        float* dst = extra == 0 ? z : bufs[(extra - 1) & 1];
         hipLaunchKernelGGL(k_f_post, dim3(g), dim3(FB), 0, s, D, S, cm, (const float*)dinv, (const float*)r,
-                           (const float*)r1, (const float*)w, (const float*)n.vout, z, (const float*)q0, (const float*)q1,
-                           partA, (const double*)st);
+                           (const float*)r1, (const float*)w, (const float*)n.vout, dst, (const float*)q0,
+                           (const float*)q1, partA, (const double*)st);
+        for (int q = extra - 1; q >= 0; --q) {
+            float* nd = q == 0 ? z : bufs[(q - 1) & 1];
+            hipLaunchKernelGGL(k_f_sweep, dim3(g), dim3(FB), 0, s, D, S, cm, (const float*)dinv, (const float*)r,
+                               (const float*)r1, (const float*)dst, nd, (const float*)q0, (const float*)q1, partA,
+                               (const double*)st);
+            dst = nd;
+        }
         hipEvent_t e0, e1;
         if (probe && c.probe.slot(PROBE_FCG_K1, e0, e1))   // dispatch-bracketing events: the kernel's own time
             hipExtLaunchKernelGGL(k_f_k1, dim3(g), dim3(FB), 0, s, e0, e1, 0, D, S, cm, (const float*)z, p0, p1, q0, q1,
@@ -805,7 +871,7 @@ int learned_fcg(Ctx& c, double shift, double bNorm, double rtol, int maxit, doub
     auto fv = [&](int slot) { return n.fw + (size_t)slot * P; };
     const FcgIter I{c, n, symF(c.Ap32, D), colMap(D, g), g,
                     fv(FV_R), fv(FV_R1), fv(FV_E), fv(FV_W), fv(FV_Z), fv(FV_P0), fv(FV_P1), fv(FV_Q0), fv(FV_Q1),
-                    fv(FV_DI), c.scal + SC_CG, c.red.part, c.red.part + MAX_RED_BLOCKS * RED_STRIDE,
+                    fv(FV_DI), fv(FV_Y), fv(FV_H), c.scal + SC_CG, c.red.part, c.red.part + MAX_RED_BLOCKS * RED_STRIDE,
                     c.red.part + 2 * MAX_RED_BLOCKS * RED_STRIDE, FcgCtl{c.singularP ? (double)D.N : 0.},
                     c.fcgCoarse == 1};
     if (I.classical && !mg_coarse_prepare(c)) return -1;

@@ -93,6 +93,10 @@ class Solver:
         (the ablation)"""
         self._check(self.lib.caffe_ctx_set_fcg_coarse(self._h, int(mode)), "caffe_ctx_set_fcg_coarse")
 
+    def set_fcg_post_sweeps(self, sweeps: int):
+        """damped Jacobi sweeps after the learned cycle's coarse correction (1-4, default 3)"""
+        self._check(self.lib.caffe_ctx_set_fcg_post_sweeps(self._h, int(sweeps)), "caffe_ctx_set_fcg_post_sweeps")
+
     def set_solve_opts(self, which: str, mode: int = L.SOLVE_PLAIN_FP64, rtol: float = 1e-8, maxit: int = 50000,
                        inner_rtol: float = 1e-4):
         o = L.SolveOpts(mode, rtol, maxit, inner_rtol)

@@ -254,6 +254,11 @@ int caffe_ctx_attach_net(caffe_ctx* ctx, caffe_net* net);
  * cycle and FCG otherwise: the ablation that shows what the network gains over a classical
  * coarse solve.  A network must still be attached (its fine-grid work buffers are used). */
 int caffe_ctx_set_fcg_coarse(caffe_ctx* ctx, int mode);
+/* Damped Jacobi sweeps after the coarse correction of the learned cycle in
+ * CAFFE_SOLVE_LEARNED_FCG solves of ctx, 1 to 4 (default 3).  The network is trained with
+ * one; the later ones damp the high-frequency error the coarse correction leaves, which
+ * the second PISO corrector's systems carry most of.  No counterpart in the reference. */
+int caffe_ctx_set_fcg_post_sweeps(caffe_ctx* ctx, int sweeps);
 
 /* ---------------------------------------------------------------------------------
  * The SparseMatrix / SparseVector compatibility path: for callers that assemble their

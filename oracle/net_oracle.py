@@ -261,10 +261,11 @@ def spmv(A, x, n):
     return y
 
 
-def precond_parts(theta, A, r, n):
+def precond_parts(theta, A, r, n, post=1, emulate_bf16=False):
     """The learned two-grid cycle for residual r (padded fine):
          z1 = w D^-1 r;  v = P^T D^-1 (r - A z1);  c = net(v);
-         y = z1 + P c;   z = y + w D^-1 (r - A y).
+         y = z1 + P c;   z = y + w D^-1 (r - A y),
+       the last step `post` times (each sweep from the previous z).
        Returns dict of the intermediates."""
     P = A.size // 4
     aP = A.reshape(4, P)[0]
@@ -272,9 +273,11 @@ def precond_parts(theta, A, r, n):
     z1 = OMEGA * inv * r
     wv = inv * (r - spmv(A, z1, n))
     v = restrict(interior(wv, n))
-    c = forward(theta, v)
+    c = forward(theta, v, emulate_bf16=emulate_bf16)
     y = z1 + pad(prolong(c), n)
-    z = y + OMEGA * inv * (r - spmv(A, y, n))
+    z = y
+    for _ in range(post):
+        z = z + OMEGA * inv * (r - spmv(A, z, n))
     return {"z1": z1, "w": wv, "v": v, "c": c, "y": y, "z": z, "inv": inv}
 
 

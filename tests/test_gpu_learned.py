@@ -77,6 +77,41 @@ def test_training_reduces_loss():
     net.close()
 
 
+@pytest.mark.parametrize("post", [1, 3])
+def test_fcg_first_step_matches_oracle(trained, post):
+    """One flexible-CG step from x = 0 through the FCG kernels (restriction, network,
+    the post-smoothing pass that forms P c at the stencil points, the extra sweeps, the
+    direction and update kernels): x1 = alpha p with p = M b minus its mean and
+    alpha = (p, b) / (p, A p), against the oracle's cycle with the same number of
+    post-smoothing sweeps (network rounded to BF16 where the MFMA path rounds; bar 1e-2
+    relative L2, BF16 operands)."""
+    from caffe_old_amd import _lib as L
+    from oracle import net_oracle as no
+    s = _prepare(32, 10004, 3)
+    try:
+        s.attach_net(trained)
+        s.set_fcg_post_sweeps(post)
+        s.pcorr_assemble()
+        n = s.shape
+        A = s.matrix(1).ravel().astype(np.float32).astype(np.float64)   # the FP32 copy the cycle runs on
+        b = no.interior(s.rhs(1), n)
+        r = s.pcorr_solve(mode=L.SOLVE_LEARNED_FCG, rtol=0.9, maxit=1, inner_rtol=1e-4)
+        print(f"post {post}: first step relres {r['relres']:.3e}")
+        assert r["iters"] == 1 and r["fell_back"] == 0
+        xg = no.interior(s.get("pCorr"), n)
+        rr = no.pad(b - b.mean(), n)
+        z = no.interior(no.precond_parts(trained.get_params().astype(np.float64), A, rr, n, post=post,
+                                         emulate_bf16=True)["z"], n)
+        p = no.pad(z - z.mean(), n)
+        xe = no.interior((p @ rr) / (p @ no.spmv(A, p, n)) * p, n)
+        err = _rel(xg - xg.mean(), xe)
+        print(f"post {post}: x1 rel L2 vs oracle {err:.2e}")
+        assert err <= 1e-2
+    finally:
+        s.attach_net(None)
+        s.close()
+
+
 def test_learned_solution_matches_plain(system128, trained):
     from caffe_old_amd import _lib as L
     rp, xp = _solve(system128, L.SOLVE_PLAIN_FP64)

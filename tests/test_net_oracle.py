@@ -171,6 +171,31 @@ def test_two_grid_cycle_closed_forms():
     assert 0.0 < L0 < 1.0        # damped Jacobi reduces the energy error
 
 
+def test_post_sweeps_converge_to_the_solution():
+    """With the network switched off the cycle with `post` sweeps is damped Jacobi from
+    z1 = w D^-1 b; on a nonsingular system it converges to A^-1 b as post grows (a wrong
+    sign, weight or missing D^-1 in the sweep would not), and each sweep contracts the
+    A-norm error."""
+    n = (16, 16, 16)   # the smallest grid the network's four levels take
+    A = no.random_pcorr_system(n, 12, neumann=False)
+    N = 16 ** 3
+    dense = np.zeros((N, N))
+    for col in range(N):
+        e = np.zeros(N)
+        e[col] = 1.0
+        dense[:, col] = no.interior(no.spmv(A, no.pad(e.reshape(16, 16, 16), n), n), n).ravel()
+    b = _rng(13).standard_normal((16, 16, 16))
+    x = np.linalg.solve(dense, b.ravel())
+    zero = np.zeros(no.param_count())
+    z = no.precond_parts(zero, A, no.pad(b, n), n, post=4000)["z"]
+    assert np.allclose(no.interior(z, n).ravel(), x, rtol=1e-10, atol=1e-12)
+    errs = []
+    for post in (1, 2, 3, 4):
+        e = no.interior(no.precond_parts(zero, A, no.pad(b, n), n, post=post)["z"], n).ravel() - x
+        errs.append(float(e @ dense @ e))
+    assert all(b_ < a_ for a_, b_ in zip(errs, errs[1:])), errs
+
+
 def test_bf16_rounding():
     v = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 1.0 + 2 ** -9, -2.5e-3, 3.0e38])
     r = no.bf16(v)
