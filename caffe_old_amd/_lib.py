@@ -36,6 +36,7 @@ EXPORTS = [
     # learned pressure-correction preconditioner
     "caffe_net_create", "caffe_net_destroy", "caffe_net_last_error", "caffe_net_param_count", "caffe_net_get_params",
     "caffe_net_set_params", "caffe_net_forward", "caffe_net_backward", "caffe_net_precond", "caffe_net_loss_grad",
+    "caffe_net_set_post_sweeps",
     "caffe_net_collect", "caffe_net_clear", "caffe_net_dataset_size", "caffe_net_train", "caffe_net_eval",
     "caffe_ctx_attach_net", "caffe_ctx_set_fcg_coarse", "caffe_ctx_set_fcg_post_sweeps",
     # SparseMatrix / SparseVector compatibility path
@@ -130,6 +131,7 @@ def load(path: str = LIB_PATH):
         "caffe_net_forward": ([vp, C.c_int, C.c_int, C.c_int, fp, fp], C.c_int),
         "caffe_net_backward": ([vp, C.c_int, C.c_int, C.c_int, fp, fp, fp], C.c_int),
         "caffe_net_precond": ([vp, C.c_int, C.c_int, C.c_int, fp, fp, fp], C.c_int),
+        "caffe_net_set_post_sweeps": ([vp, C.c_int], C.c_int),
         "caffe_net_loss_grad": ([vp, C.c_int, C.c_int, C.c_int, fp, fp, fp, dp, fp], C.c_int),
         "caffe_net_collect": ([vp, vp, lp], C.c_int),
         "caffe_net_clear": ([vp], C.c_int),

@@ -80,6 +80,7 @@ static ColMap colMap(const Dims& D, int g) {
 __device__ __forceinline__ bool fdone(const double* done) { return done && *done != 0.; }
 static int fgrid(const Dims& D) { return blocksFor(D.N, FB, FGRID); }
 
+
 // 1-D cell-centred linear prolongation: fine f takes 3/4 of coarse f>>1 and 1/4 of its
 // other neighbour, clamped to [0, nc)
 __device__ __forceinline__ void pw(int f, int nc, int& c0, int& c1) {
@@ -186,6 +187,12 @@ __global__ void k_tg_post2(Dims D, SymF S, const float* r, const float* y, float
     }
 }
 
+// one transposed post-smoothing sweep of the gradient: h = g - w A (D^-1 g)
+__global__ void k_l_sweepT(Dims D, SymF S, const float* g, float* h) {
+    auto gi = [&](int c) { return S.inv(c) * g[c]; };
+    FOR_FINE(D, q, p) { h[p] = g[p] - NET_OMEGA * S.apply(gi, p); }
+}
+
 // ---------------------------------------------------------------- loss and its gradient
 // Ae = A e; partial (e, Ae)
 __global__ void k_l_spmv(Dims D, SymF S, const float* e, float* Ae, double* part) {
@@ -258,6 +265,20 @@ static void restrictLaunch(hipStream_t s, const Dims& D, const float* w, float* 
     hipLaunchKernelGGL(k_tg_restrict, dim3((nc + 255) / 256), dim3(256), 0, s, D, w, v, done);
 }
 
+// the n.post post-smoothing sweeps from y: the last one writes z (or e = xs - z with xs);
+// the earlier ones alternate through the FV_Z and FV_P0 vectors (free outside an FCG solve)
+static void postSweeps(Net& n, hipStream_t s, const Dims& D, const SymF& S, const float* r, const float* y, float* z,
+                       const float* xs) {
+    const int g = fgrid(D);
+    float* t[2] = {n.fw + (size_t)FV_Z * D.P, n.fw + (size_t)FV_P0 * D.P};
+    const float* src = y;
+    for (int q = 0; q < n.post - 1; ++q) {
+        hipLaunchKernelGGL(k_tg_post2, dim3(g), dim3(FB), 0, s, D, S, r, src, t[q & 1], (const float*)nullptr);
+        src = t[q & 1];
+    }
+    hipLaunchKernelGGL(k_tg_post2, dim3(g), dim3(FB), 0, s, D, S, r, src, z, xs);
+}
+
 void tg_apply(Net& n, hipStream_t s, const Dims& D, const float* A, const float* r, float* z, const double* done) {
     float* w = n.fw + (size_t)FV_W * D.P;
     float* y = n.fw + (size_t)FV_Y * D.P;
@@ -267,11 +288,12 @@ void tg_apply(Net& n, hipStream_t s, const Dims& D, const float* A, const float*
     restrictLaunch(s, D, w, n.vin, done);
     net_forward(n, s, n.vin, n.vout, done);
     hipLaunchKernelGGL(k_tg_post1, dim3(g), dim3(FB), 0, s, D, S, r, n.vout, y, done);
-    hipLaunchKernelGGL(k_tg_post2, dim3(g), dim3(FB), 0, s, D, S, r, y, z, (const float*)nullptr);
+    postSweeps(n, s, D, S, r, y, z, nullptr);
 }
 
 // loss of the pair (A, b, x): x^T A x first (device scalar), then the cycle on b, the
-// error e = x - cycle(b), e^T A e; with the gradient, dL/dc = P^T (I - w A D^-1)(-2 A e / x^T A x)
+// error e = x - cycle(b), e^T A e; with the gradient, dL/dc = P^T (I - w A D^-1)^m (-2 A e / x^T A x)
+// for m = n.post post-smoothing sweeps
 static void lossCore(Net& n, hipStream_t s, const Dims& D, const float* A, const float* b, const float* x, bool withGrad,
                      float* lossDev) {
     float* w = n.fw + (size_t)FV_W * D.P;
@@ -288,12 +310,18 @@ static void lossCore(Net& n, hipStream_t s, const Dims& D, const float* A, const
     restrictLaunch(s, D, w, n.vin, nullptr);
     net_forward(n, s, n.vin, n.vout, nullptr);
     hipLaunchKernelGGL(k_tg_post1, dim3(g), dim3(FB), 0, s, D, S, b, n.vout, y, (const double*)nullptr);
-    hipLaunchKernelGGL(k_tg_post2, dim3(g), dim3(FB), 0, s, D, S, b, y, e, x);
+    postSweeps(n, s, D, S, b, y, e, x);
     hipLaunchKernelGGL(k_l_spmv, dim3(g), dim3(FB), 0, s, D, S, e, Ae, n.part);
     hipLaunchKernelGGL(k_l_loss, dim3(1), dim3(FB), 0, s, n.part, g, (const double*)nrm, lossDev);
     if (!withGrad) return;
     hipLaunchKernelGGL(k_l_h, dim3(g), dim3(FB), 0, s, D, S, Ae, (const double*)nrm, h);
-    restrictLaunch(s, D, h, n.gout, nullptr);
+    float* t[2] = {n.fw + (size_t)FV_Q0 * D.P, h};   // the further transposed sweeps: h -> Q0 -> h ...
+    float* hs = h;
+    for (int q = 0; q < n.post - 1; ++q) {
+        hipLaunchKernelGGL(k_l_sweepT, dim3(g), dim3(FB), 0, s, D, S, (const float*)hs, t[q & 1]);
+        hs = t[q & 1];
+    }
+    restrictLaunch(s, D, hs, n.gout, nullptr);
     net_backward(n, s, n.vin, n.gout);
 }
 
@@ -1270,6 +1298,14 @@ int caffe_net_backward(caffe_net* x, int nx, int ny, int nz, const float* in, co
     });
 }
 
+int caffe_net_set_post_sweeps(caffe_net* x, int sweeps) {
+    return nguard(x, [&](Net& n) {
+        if (sweeps < 1 || sweeps > 4) { n.err = "post-smoothing sweeps of the cycle: 1 to 4"; return CAFFE_ERR_ARG; }
+        n.post = sweeps;
+        return CAFFE_OK;
+    });
+}
+
 int caffe_net_precond(caffe_net* x, int nI, int nJ, int nK, const float* A, const float* r, float* z) {
     return nguard(x, [&](Net& n) {
         if (!A || !r || !z) { n.err = "null argument"; return CAFFE_ERR_ARG; }
@@ -1279,7 +1315,7 @@ int caffe_net_precond(caffe_net* x, int nI, int nJ, int nK, const float* A, cons
         Sample smp = uploadSample(n, D, A, r, nullptr);
         tg_ensure_fine(n, D);
         net_set_dims(n, nI / 2, nJ / 2, nK / 2, n.trainBufs);
-        float* zd = n.fw + (size_t)FV_Z * D.P;
+        float* zd = n.fw + (size_t)FV_E * D.P;   // not one of the vectors the cycle works in
         HIPCHECK(hipMemsetAsync(zd, 0, (size_t)D.P * 4, n.stream));
         tg_apply(n, n.stream, D, smp.A, smp.b, zd, nullptr);
         HIPCHECK(hipMemcpyAsync(z, zd, (size_t)D.P * 4, hipMemcpyDeviceToHost, n.stream));

@@ -69,6 +69,7 @@ struct Net {
     float* wpart = nullptr;          // per-chunk weight-gradient partial sums
     long wpartCap = 0;
     int adamT = 0;
+    int post = 1;                    // post-smoothing sweeps of the cycle in training, loss and precond
     bool packed = false;
 
     // activations for the current coarse dimensions

@@ -41,6 +41,7 @@ class PressureNet:
         if rc != L.CAFFE_OK:
             raise L.CaffeError(f"caffe_net_create failed ({rc})")
         self.n_params = int(self.lib.caffe_net_param_count())
+        self.post_sweeps = 1
 
     def _check(self, rc, what):
         if rc != L.CAFFE_OK:
@@ -93,6 +94,11 @@ class PressureNet:
         out = np.empty(self.n_params, dtype=np.float32)
         self._check(self.lib.caffe_net_backward(self._h, I, J, K, _fp(v), _fp(g), _fp(out)), "caffe_net_backward")
         return out
+
+    def set_post_sweeps(self, sweeps: int):
+        """post-smoothing sweeps of the cycle in precond, loss_grad, train and eval (1-4)"""
+        self._check(self.lib.caffe_net_set_post_sweeps(self._h, int(sweeps)), "caffe_net_set_post_sweeps")
+        self.post_sweeps = int(sweeps)
 
     def precond(self, shape, A: np.ndarray, r: np.ndarray) -> np.ndarray:
         nI, nJ, nK = shape

@@ -119,29 +119,35 @@ def generate(net, n, seeds, steps, log, pairs=None, max_courant=MAX_COURANT):
         log(f"  cavity seed {sd}: U {U:.3f} Re {Re:.0f} -> {net.dataset_size} pairs ({time.perf_counter() - t0:.1f} s)")
 
 
-def validation_systems(n, seeds, step, log, max_courant=MAX_COURANT):
-    """Solver contexts holding the first p' system of PISO step `step` of each seeded
-    cavity (seeds apart from the training and held-out ones); a cavity whose flow is no
-    longer bounded by then (Courant number above max_courant) is passed over."""
+def validation_systems(n, seeds, step, log, max_courant=MAX_COURANT, correctors=1):
+    """Solver contexts holding the p' systems of the first `correctors` pressure
+    correctors of PISO step `step` of each seeded cavity (seeds apart from the training and
+    held-out ones), one context per system; a cavity whose flow is no longer bounded by
+    then (Courant number above max_courant) is passed over."""
     out = []
     for sd in seeds:
-        c, U, Re = cavity(n, sd, step + 1)
-        s = make_solver(c)
-        s.set_solve_opts("momentum", rtol=1e-8, maxit=5000)
-        for _ in range(step):
-            piso_step_collect(s, None, collect=False)
-        co = s.step_report()["courant"]
-        if not co <= max_courant:
-            log(f"  validation cavity seed {sd}: passed over (Courant number {co:.3g})")
-            s.close()
-            continue
-        s.save_u0()
-        s.momentum_assemble(s.case.time_step)
-        s.momentum_solve()
-        s.rhie_chow()
-        s.pcorr_assemble()
-        log(f"  validation cavity seed {sd}: U {U:.3f} Re {Re:.0f}, p' system of step {step}")
-        out.append(s)
+        for corr in range(correctors):
+            c, U, Re = cavity(n, sd, step + 1)
+            s = make_solver(c)
+            s.set_solve_opts("momentum", rtol=1e-8, maxit=5000)
+            for _ in range(step):
+                piso_step_collect(s, None, collect=False)
+            co = s.step_report()["courant"]
+            if not co <= max_courant:
+                log(f"  validation cavity seed {sd}: passed over (Courant number {co:.3g})")
+                s.close()
+                break
+            s.save_u0()
+            s.momentum_assemble(s.case.time_step)
+            s.momentum_solve()
+            s.rhie_chow()
+            for _ in range(corr):   # the correctors before this one, as Piso::solve runs them
+                s.pcorr_assemble()
+                s.pcorr_solve(mode=L.SOLVE_IR_FP32, rtol=1e-8, maxit=100000, inner_rtol=1e-2)
+                s.correct()
+            s.pcorr_assemble()
+            log(f"  validation cavity seed {sd}: U {U:.3f} Re {Re:.0f}, p' system of step {step}, corrector {corr + 1}")
+            out.append(s)
     return out
 
 
@@ -178,12 +184,17 @@ def main(argv=None):
                          "flexible-CG iterations on validation cavities (seeds 20000+)")
     ap.add_argument("--val-runs", type=int, default=3)
     ap.add_argument("--val-step", type=int, default=6)
+    ap.add_argument("--val-correctors", type=int, default=1, choices=[1, 2],
+                    help="validation systems from the first corrector only, or from both")
+    ap.add_argument("--post", type=int, default=1,
+                    help="post-smoothing sweeps of the cycle the loss is taken on (the flexible CG applies 3)")
     a = ap.parse_args(argv)
 
     def log(*x):
         print(*x, flush=True)
 
     net = PressureNet(seed=a.seed)
+    net.set_post_sweeps(a.post)
     log(f"generating training pairs: up to {a.runs} cavities x {a.steps} PISO steps x 2 correctors at {a.n}^3"
         + (f", {a.pairs} pairs" if a.pairs else ""))
     generate(net, a.n, range(a.runs), a.steps, log, a.pairs)
@@ -193,7 +204,7 @@ def main(argv=None):
     val, val_curve, best = [], [], None
     if a.select:
         log(f"validation systems: {a.val_runs} cavities (seeds 20000+)")
-        val = validation_systems(a.n, range(20000, 20000 + a.val_runs), a.val_step, log)
+        val = validation_systems(a.n, range(20000, 20000 + a.val_runs), a.val_step, log, correctors=a.val_correctors)
     t0 = time.perf_counter()
     losses = []
     chunk = max(1, a.train_steps // 20)
@@ -235,6 +246,7 @@ def main(argv=None):
     os.makedirs(os.path.dirname(out), exist_ok=True)
     meta = {"grid": a.n, "runs": a.runs, "steps_per_run": a.steps, "pairs": n_pairs,
             "train_steps": a.train_steps, "batch": a.batch, "lr": a.lr, "seed": a.seed, "synthetic": a.synthetic,
+            "post_sweeps": a.post, "val_correctors": a.val_correctors,
             "loss_before": loss0, "loss_train_after": loss_train,
             "loss_heldout_mean": float(ev.mean()), "loss_heldout_max": float(ev.max()),
             "train_seconds": train_s, "selected_step": best[0] if best else None,

@@ -226,6 +226,11 @@ int caffe_net_forward(caffe_net* net, int nx, int ny, int nz, const float* in, f
 int caffe_net_backward(caffe_net* net, int nx, int ny, int nz, const float* in, const float* gout, float* gparams);
 /* z = cycle(r) on the fine grid for the FP32 matrix A */
 int caffe_net_precond(caffe_net* net, int nI, int nJ, int nK, const float* A, const float* r, float* z);
+/* damped Jacobi sweeps after the coarse correction in the cycle of caffe_net_precond,
+ * caffe_net_loss_grad, caffe_net_train and caffe_net_eval: 1 to 4 (default 1).  Training
+ * with the count the flexible CG applies (caffe_ctx_set_fcg_post_sweeps) fits the network
+ * to the cycle it is used in.  No counterpart in the reference. */
+int caffe_net_set_post_sweeps(caffe_net* net, int sweeps);
 /* training loss of one pair: L = e^T A e / x^T A x, e = x - cycle(b); gparams may be NULL */
 int caffe_net_loss_grad(caffe_net* net, int nI, int nJ, int nK, const float* A, const float* b, const float* x,
                         double* loss, float* gparams);

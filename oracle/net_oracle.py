@@ -285,10 +285,11 @@ def apply_precond(theta, A, r, n):
     return precond_parts(theta, A, r, n)["z"]
 
 
-def two_grid_loss(theta, A, b, xs, n, grad=False):
-    """Relative energy-norm error of one learned two-grid cycle applied to b, whose exact
-    solution is xs:  L = e^T A e / xs^T A xs,  e = xs - M b.  With grad, also dL/dtheta."""
-    pp = precond_parts(theta, A, b, n)
+def two_grid_loss(theta, A, b, xs, n, grad=False, post=1):
+    """Relative energy-norm error of one learned two-grid cycle (`post` post-smoothing
+    sweeps) applied to b, whose exact solution is xs:  L = e^T A e / xs^T A xs,
+    e = xs - M b.  With grad, also dL/dtheta."""
+    pp = precond_parts(theta, A, b, n, post=post)
     e = xs - pp["z"]
     Ae = spmv(A, e, n)
     nrm = float(xs @ spmv(A, xs, n))
@@ -296,7 +297,9 @@ def two_grid_loss(theta, A, b, xs, n, grad=False):
     if not grad:
         return L
     g = -2.0 * Ae / nrm                                   # dL/dz
-    h = g - OMEGA * spmv(A, pp["inv"] * g, n)             # dL/dy = (I - w A D^-1) g
+    h = g
+    for _ in range(post):                                 # dL/dy = (I - w A D^-1)^post g
+        h = h - OMEGA * spmv(A, pp["inv"] * h, n)
     gc = restrict(interior(h, n))                         # dL/dc = P^T h
     return L, backward(theta, pp["v"], gc)
 

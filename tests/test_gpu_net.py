@@ -123,19 +123,25 @@ def test_precond_identity_network(net):
     assert _rel(z, ze) <= 5e-3
 
 
-def test_precond_matches_oracle(net):
+@pytest.mark.parametrize("post", [1, 3])
+def test_precond_matches_oracle(net, post):
     n = (32, 32, 32)
     A = _system(n, 25)
     th = _theta(26, 0.5)
     net.set_params(th)
-    r = no.pad(np.random.default_rng(27).standard_normal(n[::-1]), n).astype(np.float32).astype(np.float64)
-    z = net.precond(n, A, r)
-    ze = no.apply_precond(th.astype(np.float64), A, r, n)
-    print(f"cycle rel L2 {_rel(z, ze):.2e}")
+    net.set_post_sweeps(post)
+    try:
+        r = no.pad(np.random.default_rng(27).standard_normal(n[::-1]), n).astype(np.float32).astype(np.float64)
+        z = net.precond(n, A, r)
+    finally:
+        net.set_post_sweeps(1)
+    ze = no.precond_parts(th.astype(np.float64), A, r, n, post=post)["z"]
+    print(f"cycle ({post} post sweeps) rel L2 {_rel(z, ze):.2e}")
     assert _rel(z, ze) <= 3e-2
 
 
-def test_loss_and_gradient_match_oracle(net):
+@pytest.mark.parametrize("post", [1, 3])
+def test_loss_and_gradient_match_oracle(net, post):
     n = (32, 32, 32)
     A = _system(n, 31)
     rng = np.random.default_rng(32)
@@ -143,12 +149,17 @@ def test_loss_and_gradient_match_oracle(net):
     b = no.spmv(A, xs, n).astype(np.float32).astype(np.float64)
     th = _theta(33, 0.5)
     net.set_params(th)
-    L, g = net.loss_grad(n, A, b, xs)
-    Le, ge = no.two_grid_loss(th.astype(np.float64), A, b, xs, n, grad=True)
+    net.set_post_sweeps(post)
+    try:
+        L, g = net.loss_grad(n, A, b, xs)
+        L2 = net.loss_grad(n, A, b, xs, grad=False)
+    finally:
+        net.set_post_sweeps(1)
+    Le, ge = no.two_grid_loss(th.astype(np.float64), A, b, xs, n, grad=True, post=post)
     off = np.cumsum([0] + no.layer_sizes())
     errs = {name: _rel(g[off[i]:off[i + 1]], ge[off[i]:off[i + 1]]) for i, (name, *_r) in enumerate(no.LAYERS)}
     print(f"loss gpu {L:.6f} oracle {Le:.6f}; grad rel L2 per layer", {k: f"{v:.1e}" for k, v in errs.items()})
     assert abs(L - Le) <= 1e-2 * abs(Le)
     assert max(errs.values()) <= 5e-2, errs
     # loss alone (no gradient) is the same number
-    assert abs(net.loss_grad(n, A, b, xs, grad=False) - L) <= 1e-6 * abs(L)
+    assert abs(L2 - L) <= 1e-6 * abs(L)

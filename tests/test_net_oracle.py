@@ -134,14 +134,16 @@ def test_backward_adjoint_identity():
     assert np.isclose(g[:sizes[0]] @ th[:sizes[0]], np.sum(gy * y), rtol=1e-10)
 
 
-def test_two_grid_loss_gradient_finite_differences():
+@pytest.mark.parametrize("post", [1, 3])
+def test_two_grid_loss_gradient_finite_differences(post):
+    """central differences pin dL/dtheta, the transposed post-smoothing sweeps included"""
     n = (16, 16, 16)
     A = no.random_pcorr_system(n, 2)
     r = _rng(8)
     xs = no.pad(r.standard_normal((16, 16, 16)), n)
     b = no.spmv(A, xs, n)
     th = no.init_params(3) * 0.5
-    L, g = no.two_grid_loss(th, A, b, xs, n, grad=True)
+    L, g = no.two_grid_loss(th, A, b, xs, n, grad=True, post=post)
     sizes = np.cumsum([0] + no.layer_sizes())
     for li in range(len(no.LAYERS)):
         idx = r.integers(sizes[li], sizes[li + 1], size=3)
@@ -150,7 +152,7 @@ def test_two_grid_loss_gradient_finite_differences():
             tp, tm = th.copy(), th.copy()
             tp[q] += h
             tm[q] -= h
-            fd = (no.two_grid_loss(tp, A, b, xs, n) - no.two_grid_loss(tm, A, b, xs, n)) / (2 * h)
+            fd = (no.two_grid_loss(tp, A, b, xs, n, post=post) - no.two_grid_loss(tm, A, b, xs, n, post=post)) / (2 * h)
             assert abs(fd - g[q]) <= 1e-6 * max(1.0, abs(g).max()), (no.LAYERS[li][0], q, fd, g[q])
 
 
