@@ -485,6 +485,7 @@ __global__ void k_f_restrict_dw(Dims D, const float* w, const float* aP, float* 
 // a step to an even k shifts them and loads one coarse plane (a 3x3 patch).  Each s value
 // has prolongAt's products in prolongAt's order, so s is bit-identical to P c.  Cells
 // outside the grid take s = 0 (the halo of y).  Partials (z, q_old), sum z, (z, r).
+template <bool PART>   // PART: this pass ends the post-smoothing and writes the partial sums
 __global__ void k_f_post(Dims D, SymF S, ColMap cm, const float* __restrict__ dinv, const float* r0, const float* r1,
                          const float* __restrict__ wpre, const float* __restrict__ c, float* z, const float* q0,
                          const float* q1, double* part, const double* st) {
@@ -499,7 +500,7 @@ __global__ void k_f_post(Dims D, SymF S, ColMap cm, const float* __restrict__ di
     const float ax = 0.75f, bx = 0.25f;
     struct Plane { float r, d, w, q, a[7]; };
     auto loadPlane = [&](int p, Plane& L) {
-        L.r = r[p]; L.d = dinv[p]; L.w = wpre[p]; L.q = qold[p];
+        L.r = r[p]; L.d = dinv[p]; L.w = wpre[p]; L.q = PART ? qold[p] : 0.f;
         L.a[0] = A[p]; L.a[1] = A[P + p]; L.a[2] = A[P + p + 1]; L.a[3] = A[2 * P + p];
         L.a[4] = A[2 * P + p + S.sJ]; L.a[5] = A[3 * P + p]; L.a[6] = A[3 * P + p + sK];
     };
@@ -562,9 +563,11 @@ __global__ void k_f_post(Dims D, SymF S, ColMap cm, const float* __restrict__ di
                                               + cur.a[5] * sd + cur.a[6] * su);
             const float zz = (NET_OMEGA * cur.d * cur.r + sc) + NET_OMEGA * (cur.w - cur.d * As);
             z[p] = zz;
-            v[0] += (double)zz * (double)cur.q;
-            v[1] += (double)zz;
-            v[2] += (double)zz * (double)cur.r;
+            if (PART) {
+                v[0] += (double)zz * (double)cur.q;
+                v[1] += (double)zz;
+                v[2] += (double)zz * (double)cur.r;
+            }
             sd = sc; sc = su;
             cur = nxt;
             if (k & 1) {   // the next plane is even: shift the window up one coarse plane
@@ -575,12 +578,13 @@ __global__ void k_f_post(Dims D, SymF S, ColMap cm, const float* __restrict__ di
             }
         }
     }
-    writePartials<FB, 3>(v, part);
+    if (PART) writePartials<FB, 3>(v, part);
 }
 
 // One more damped Jacobi sweep after the cycle's post-smoothing: z = y + w D^-1 (r - A y)
 // for y from k_f_post; partials (z, q_old), sum z, (z, r) (they replace k_f_post's).
 // Column tiles and loads one plane ahead as in k_f_k1.
+template <bool PART>
 __global__ void k_f_sweep(Dims D, SymF S, ColMap cm, const float* __restrict__ dinv, const float* r0, const float* r1,
                           const float* __restrict__ y, float* z, const float* q0, const float* q1, double* part,
                           const double* st) {
@@ -599,7 +603,7 @@ __global__ void k_f_sweep(Dims D, SymF S, ColMap cm, const float* __restrict__ d
         L.yu = y[p + sK];
         L.a[0] = A[p]; L.a[1] = A[P + p]; L.a[2] = A[P + p + 1]; L.a[3] = A[2 * P + p];
         L.a[4] = A[2 * P + p + sJ]; L.a[5] = A[3 * P + p]; L.a[6] = A[3 * P + p + sK];
-        L.r = r[p]; L.d = dinv[p]; L.q = qold[p];
+        L.r = r[p]; L.d = dinv[p]; L.q = PART ? qold[p] : 0.f;
     };
     double v[3] = {0., 0., 0.};
     for (int v_ = xcdChunk(), vn = cm.ntx * cm.nty * cm.nseg; v_ < vn; v_ += (int)gridDim.x) {
@@ -618,14 +622,16 @@ __global__ void k_f_sweep(Dims D, SymF S, ColMap cm, const float* __restrict__ d
                                               + cur.a[4] * cur.ys[3] + cur.a[5] * yd + cur.a[6] * cur.yu);
             const float zz = yc + NET_OMEGA * cur.d * (cur.r - Ay);
             z[p] = zz;
-            v[0] += (double)zz * (double)cur.q;
-            v[1] += (double)zz;
-            v[2] += (double)zz * (double)cur.r;
+            if (PART) {
+                v[0] += (double)zz * (double)cur.q;
+                v[1] += (double)zz;
+                v[2] += (double)zz * (double)cur.r;
+            }
             yd = yc; yc = cur.yu;
             cur = nxt;
         }
     }
-    writePartials<FB, 3>(v, part);
+    if (PART) writePartials<FB, 3>(v, part);
 }
 
 // beta = -(z - zbar, q_old) / (p_old, q_old); p = (z - zbar) + beta p_old formed at the
@@ -856,12 +862,12 @@ struct FcgIter {
         const int extra = c.fcgPost - 1;
         float* bufs[2] = {y, h};
         float* dst = extra == 0 ? z : bufs[(extra - 1) & 1];
-        hipLaunchKernelGGL(k_f_post, dim3(g), dim3(FB), 0, s, D, S, cm, (const float*)dinv, (const float*)r,
+        hipLaunchKernelGGL(extra == 0 ? k_f_post<true> : k_f_post<false>, dim3(g), dim3(FB), 0, s, D, S, cm, (const float*)dinv, (const float*)r,
                            (const float*)r1, (const float*)w, (const float*)n.vout, dst, (const float*)q0,
                            (const float*)q1, partA, (const double*)st);
         for (int q = extra - 1; q >= 0; --q) {
             float* nd = q == 0 ? z : bufs[(q - 1) & 1];
-            hipLaunchKernelGGL(k_f_sweep, dim3(g), dim3(FB), 0, s, D, S, cm, (const float*)dinv, (const float*)r,
+            hipLaunchKernelGGL(q == 0 ? k_f_sweep<true> : k_f_sweep<false>, dim3(g), dim3(FB), 0, s, D, S, cm, (const float*)dinv, (const float*)r,
                                (const float*)r1, (const float*)dst, nd, (const float*)q0, (const float*)q1, partA,
                                (const double*)st);
             dst = nd;
@@ -1185,12 +1191,13 @@ int caffe_net_create(int device, unsigned long long seed, caffe_net** out) {
             o += L;
             if (l != NET_IN && l != NET_OUT) po += L;
         }
+        n.poff[NET_OUT] = po;   // the forward packing also holds the out layer's BF16 hi/lo columns
         n.nParams = o;
         n.theta = (float*)net_alloc(n, o * 4, n.allocs);
         n.grad = (float*)net_alloc(n, o * 4, n.allocs);
         n.m1 = (float*)net_alloc(n, o * 4, n.allocs);
         n.m2 = (float*)net_alloc(n, o * 4, n.allocs);
-        n.wf = (uint16_t*)net_alloc(n, po * 2, n.allocs);
+        n.wf = (uint16_t*)net_alloc(n, (po + 27 * 256) * 2, n.allocs);
         n.wb = (uint16_t*)net_alloc(n, po * 2, n.allocs);
         n.part = (double*)net_alloc(n, 2 * MAX_RED_BLOCKS * RED_STRIDE * sizeof(double), n.allocs);
         n.scal = (double*)net_alloc(n, 64 * sizeof(double), n.allocs);

@@ -19,7 +19,9 @@
 //   k_conv_wgrad   weight gradient on MFMA: D[co][ci] += G^T X over the voxels of a
 //                  chunk of tiles for all 27 taps, reduced across waves in LDS in a fixed
 //                  order, one partial row per workgroup, then k_reduce_parts.
-//   k_in_*/k_out_* the 1->16 input and 16->1 output layers in FP32 scalar code.
+//   k_inf16/k_outf16  the 1->16 input and 16->1 output layers on MFMA, their FP32 operands
+//                  (input x, weights) split into a BF16 high part and remainder;
+//                  k_out_ginput / k_edge_wgrad their training gradients in FP32 scalar code.
 // Layouts: activations [z][y][x][C] channels last (BF16), gradients FP32 in the same
 // layout, FP32 parameters [27][cout][cin] with tap t = (dz+1)*9 + (dy+1)*3 + (dx+1).
 #include <cmath>
@@ -704,92 +706,185 @@ __device__ __forceinline__ int nbr(int i, int j, int k, int t, int nx, int ny, i
     return (cc * ny + bc) * nx + ac;
 }
 
-// in: 1 -> 16 channels; W [27][16][1]
-__global__ __launch_bounds__(256) void k_in_fwd(const float* x, const float* W, uint16_t* out, int nx, int ny, int nz,
-                                                const double* done) {
+// in: 1 -> 16 channels on MFMA (v_mfma_f32_16x16x32_bf16), tile 16x4x4 as k_outf16.  The
+// reduction index is the tap: lane (n, kq) holds the FP32 input at taps 8kq .. 8kq+7 of
+// voxel n of a 16-voxel x row (taps 27-31 zero) split into a BF16 high part and remainder,
+// x = xh + xl, and the weights W[t][co = n][0] likewise, w = wh + wl; three products
+// xh wh + xh wl + xl wh (each exact in FP32, accumulated in FP32) carry x w to 2^-16
+// relative, well inside the BF16 rounding of the layer's output.  The output tile goes
+// through LDS and leaves in 16-byte stores.
+__global__ __launch_bounds__(NB) void k_inf16(const float* x, const float* W, uint16_t* out, int nx, int ny, int nz,
+                                              const double* done) {
     const double flag = flagOf(done);
-    __shared__ __align__(16) float sW[27 * 16];
-    __shared__ float sx[EHV];
-    int ox, oy, oz;
-    edgeTile(nx, ny, ox, oy, oz);
-    const int tid = threadIdx.x;
-    constexpr int NIT = (EHV + 255) / 256;
+    constexpr int TX = 16, TY = 4, TZ = 4;
+    constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2, HV = HX * HY * HZ;
+    constexpr int TV = TX * TY * TZ, G = TV / 64;
+    __shared__ float sx[HV];
+    __shared__ __align__(16) uint16_t sOut[TV * 16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int n = lane & 15, kq = lane >> 4;
+    const int ntx = (nx + TX - 1) / TX, nty = (ny + TY - 1) / TY;
+    int b = blockIdx.x;
+    const int bx = b % ntx;
+    b /= ntx;
+    const int by = b % nty, bz = b / nty;
+    const int ox = bx * TX, oy = by * TY, oz = bz * TZ;
+    constexpr int NIT = (HV + NB - 1) / NB;
     float tv[NIT];
 #pragma unroll
     for (int u = 0; u < NIT; ++u) {
-        const int hv = tid + u * 256;
-        bool in = false;
-        const int g = haloVox(hv < EHV ? hv : 0, ox, oy, oz, nx, ny, nz, in);
-        const float val = x[g];
-        tv[u] = (in && hv < EHV) ? val : 0.f;
-    }
-    if (flag != 0.) return;
-    for (int i = tid; i < 27 * 16; i += 256) sW[i] = W[i];
-#pragma unroll
-    for (int u = 0; u < NIT; ++u)
-        if (tid + u * 256 < EHV) sx[tid + u * 256] = tv[u];
-    __syncthreads();
-    inLayerTile(sx, sW, out, ox, oy, oz, nx, ny, nz, tid);
-}
-
-// out: 16 -> 1 channel; W [27][1][16].  The BF16 activations meet the FP32 weights
-// through v_dot2_f32_bf16: each weight is split into a BF16 high part and a BF16
-// remainder (w = hi + lo to 2^-17 relative), so a tap is 16 dot2 operations instead
-// of 16 FMAs and 16 BF16 unpacks.
-typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ float dot2(uint32_t a, uint32_t b, float c) {
-    return __builtin_amdgcn_fdot2_f32_bf16(*(const bf16x2*)&a, *(const bf16x2*)&b, c, false);
-}
-__global__ __launch_bounds__(256) void k_out_fwd(const uint16_t* a, const float* W, float* y, int nx, int ny, int nz,
-                                                 const double* done) {
-    const double flag = flagOf(done);
-    __shared__ __align__(16) uint32_t sWh[27 * 8], sWl[27 * 8];   // [t][channel pair]: hi and lo parts
-    __shared__ __align__(16) uint4 sa[EHV * 2];   // 16 BF16 channels per halo voxel
-    int ox, oy, oz;
-    edgeTile(nx, ny, ox, oy, oz);
-    const int tid = threadIdx.x;
-    constexpr int NI = EHV * 2, NIT = (NI + 255) / 256;
-    uint4 tv[NIT];
-#pragma unroll
-    for (int u = 0; u < NIT; ++u) {
-        const int it = tid + u * 256;
-        const int hv = it < NI ? (it >> 1) : 0, h = it & 1;
-        bool in = false;
-        const int g = haloVox(hv, ox, oy, oz, nx, ny, nz, in);
-        const uint4 val = ((const uint4*)(a + (size_t)g * 16))[h];
-        tv[u] = (in && it < NI) ? val : make_uint4(0u, 0u, 0u, 0u);
-    }
-    if (flag != 0.) return;
-    if (tid < 27 * 8) {
-        const float w0 = W[2 * tid], w1 = W[2 * tid + 1];
-        const uint32_t h0 = f2bf(w0), h1 = f2bf(w1);
-        sWh[tid] = h0 | (h1 << 16);
-        sWl[tid] = f2bf(w0 - bf2f(h0)) | (f2bf(w1 - bf2f(h1)) << 16);
-    }
-#pragma unroll
-    for (int u = 0; u < NIT; ++u)
-        if (tid + u * 256 < NI) sa[tid + u * 256] = tv[u];
-    __syncthreads();
-    const int tx = tid % ETX, ty = (tid / ETX) % ETY, tz = tid / (ETX * ETY);
-    const int gx = ox + tx, gy = oy + ty, gz = oz + tz;
-    const int hb = ((tz + 1) * EHY + (ty + 1)) * EHX + (tx + 1);
-    float s[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll 3
-    for (int t = 0; t < 27; ++t) {
-        const int hv = hb + ((t / 9 - 1) * EHY + ((t / 3) % 3 - 1)) * EHX + (t % 3 - 1);
-        const uint4 h0 = sa[2 * hv], h1 = sa[2 * hv + 1];
-        const uint32_t hw[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
-        const uint4 wh0 = *(const uint4*)&sWh[t * 8], wh1 = *(const uint4*)&sWh[t * 8 + 4];
-        const uint4 wl0 = *(const uint4*)&sWl[t * 8], wl1 = *(const uint4*)&sWl[t * 8 + 4];
-        const uint32_t wh[8] = {wh0.x, wh0.y, wh0.z, wh0.w, wh1.x, wh1.y, wh1.z, wh1.w};
-        const uint32_t wl[8] = {wl0.x, wl0.y, wl0.z, wl0.w, wl1.x, wl1.y, wl1.z, wl1.w};
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            s[q & 3] = dot2(hw[q], wh[q], s[q & 3]);
-            s[q & 3] = dot2(hw[q], wl[q], s[q & 3]);
+        const int hv = tid + u * NB;
+        tv[u] = 0.f;
+        if (hv < HV) {
+            const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+            const int gx = ox + hx - 1, gy = oy + hy - 1, gz = oz + hz - 1;
+            if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz) tv[u] = x[(gz * ny + gy) * nx + gx];
         }
     }
-    if (gx < nx && gy < ny && gz < nz) y[(gz * ny + gy) * nx + gx] = (s[0] + s[1]) + (s[2] + s[3]);
+    // weights of taps 8kq .. 8kq+7 for output channel n, split high / remainder
+    bf16x8 wh, wl;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {   // v_cvt_pk_bf16_f32: round to nearest even
+        const int t = 8 * kq + j;
+        const float w = t < 27 ? W[t * 16 + n] : 0.f;
+        wh[j] = (__bf16)w;
+        wl[j] = (__bf16)(w - (float)wh[j]);
+    }
+    if (flag != 0.) return;
+#pragma unroll
+    for (int u = 0; u < NIT; ++u)
+        if (tid + u * NB < HV) sx[tid + u * NB] = tv[u];
+    __syncthreads();
+    int toff[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int t = min(8 * kq + j, 26);   // taps past 26 read any in-bounds row: their weights are zero
+        toff[j] = ((t / 9 - 1) * HY + ((t / 3) % 3 - 1)) * HX + (t % 3 - 1);
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int vi = (wave * G + g) * 16 + n;   // a group is one x row: tx = n
+        const int ty = (vi / TX) % TY, tz = vi / (TX * TY);
+        const int hb = ((tz + 1) * HY + (ty + 1)) * HX + (n + 1);
+        bf16x8 xh, xl;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const float v = sx[hb + toff[j]];
+            xh[j] = (__bf16)v;
+            xl[j] = (__bf16)(v - (float)xh[j]);
+        }
+        f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, wh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, wl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl, wh, acc, 0, 0, 0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int vo = (wave * G + g) * 16 + kq * 4 + q;
+            sOut[vo * 16 + 8 * swz<2>(vo, n >> 3) + (n & 7)] = (uint16_t)f2bf(acc[q]);
+        }
+    }
+    __syncthreads();
+    for (int it = tid; it < TV * 2; it += NB) {
+        const int vi = it >> 1, h = it & 1;
+        const int tx = vi % TX, ty = (vi / TX) % TY, tz = vi / (TX * TY);
+        const int gx = ox + tx, gy = oy + ty, gz = oz + tz;
+        if (gx < nx && gy < ny && gz < nz)
+            *(uint4*)(out + ((size_t)(gz * ny + gy) * nx + gx) * 16 + 8 * h) = *(const uint4*)&sOut[vi * 16 + 8 * swz<2>(vi, h)];
+    }
+}
+
+// out: 16 -> 1 channel on MFMA, in k_convf16's row scheme (tile 16x4x4, 16-voxel x rows,
+// one v_mfma_f32_16x16x16_bf16 per tap, the dx = +-1 operands by DPP row shifts).  The
+// FP32 weights W[t][0][ci] enter as two BF16 output columns, their high part hi = bf16(w)
+// and remainder lo = bf16(w - hi) (w = hi + lo to 2^-17 relative; k_pack_out), so column 0
+// of the product holds sum a hi and column 1 sum a lo, each product exact in FP32 and the
+// sums accumulated in FP32; y = column 0 + column 1.  Columns 2-15 are zero weights.
+__global__ __launch_bounds__(NB) void k_outf16(const uint16_t* in, const uint16_t* wpk, float* y, int nx, int ny, int nz,
+                                               const double* done) {
+    const double flag = flagOf(done);
+    constexpr int TX = 16, TY = 4, TZ = 4, CIN = 16, C8 = 2;
+    constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2, HV = HX * HY * HZ;
+    constexpr int TV = TX * TY * TZ, G = TV / 64;
+    __shared__ __align__(16) uint16_t sIn[HV * CIN];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int n = lane & 15, kq = lane >> 4;
+    const int ntx = (nx + TX - 1) / TX, nty = (ny + TY - 1) / TY;
+    int b = blockIdx.x;
+    const int bx = b % ntx;
+    b /= ntx;
+    const int by = b % nty, bz = b / nty;
+    const int ox = bx * TX, oy = by * TY, oz = bz * TZ;
+
+    constexpr int NI = HV * C8, NIT = (NI + NB - 1) / NB;
+    uint4 vin[NIT];
+#pragma unroll
+    for (int u = 0; u < NIT; ++u) {
+        const int it = tid + u * NB;
+        vin[u] = make_uint4(0u, 0u, 0u, 0u);
+        if (it < NI) {
+            const int hv = it / C8, c = it - hv * C8;
+            const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+            const int gx = ox + hx - 1, gy = oy + hy - 1, gz = oz + hz - 1;
+            if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz)
+                vin[u] = *(const uint4*)(in + ((size_t)(gz * ny + gy) * nx + gx) * CIN + 8 * c);
+        }
+    }
+    if (flag != 0.) return;
+#pragma unroll
+    for (int u = 0; u < NIT; ++u) {
+        const int it = tid + u * NB;
+        if (it < NI) {
+            const int hv = it / C8, c = it - hv * C8;
+            *(uint4*)&sIn[hv * CIN + 8 * swz<C8>(hv, c)] = vin[u];
+        }
+    }
+    // lane (n, kq) holds wpk[t][co = n][ci = 4kq..4kq+3] for all 27 taps
+    const uint16_t* ws = wpk + n * 16 + 4 * kq;
+    uint2 wr[27];
+#pragma unroll
+    for (int t = 0; t < 27; ++t) wr[t] = *(const uint2*)(ws + t * 256);
+    __syncthreads();
+
+    auto frag = [&](int row) { return *(const uint2*)&sIn[row * CIN + 8 * swz<C8>(row, kq >> 1) + 4 * (kq & 1)]; };
+    const int hoff = n < 8 ? -1 : 1;
+    int base[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int vi = (wave * G + g) * 16 + n;
+        const int ty = (vi / TX) % TY, tz = vi / (TX * TY);
+        base[g] = ((tz + 1) * HY + (ty + 1)) * HX + (n + 1);
+    }
+    f32x4 acc[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) acc[g] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int p = 0; p < 9; ++p) {
+        const int dz = p / 3 - 1, dy = p % 3 - 1, t0 = 3 * p;
+        const int poff = (dz * HY + dy) * HX;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int row = base[g] + poff;
+            const uint2 c0 = frag(row);
+            const uint2 h = frag(row + hoff);
+            const bf16x4 am = asbf4(shr1(h.x, c0.x), shr1(h.y, c0.y));
+            const bf16x4 ap = asbf4(shl1(h.x, c0.x), shl1(h.y, c0.y));
+            acc[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am, asbf4(wr[t0].x, wr[t0].y), acc[g], 0, 0, 0);
+            acc[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(asbf4(c0.x, c0.y), asbf4(wr[t0 + 1].x, wr[t0 + 1].y), acc[g], 0, 0, 0);
+            acc[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ap, asbf4(wr[t0 + 2].x, wr[t0 + 2].y), acc[g], 0, 0, 0);
+        }
+    }
+    // lane (0, kq) of group g: y at the row's voxels tx = 4kq .. 4kq+3 = column 0 + column 1
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        float v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = acc[g][q] + __shfl_down(acc[g][q], 1, 64);
+        const int vi = (wave * G + g) * 16;
+        const int ty = (vi / TX) % TY, tz = vi / (TX * TY);
+        const int gx = ox + 4 * kq, gy = oy + ty, gz = oz + tz;
+        if (n == 0 && gx < nx && gy < ny && gz < nz)   // nx is a multiple of 4: the 4 voxels are in
+            *(float4*)(y + ((size_t)(gz * ny + gy) * nx + gx)) = make_float4(v[0], v[1], v[2], v[3]);
+    }
 }
 
 // gradient of the out layer's input: ga[v][ci] += sum_t W[t][0][ci] g[v - off(t)]
@@ -906,6 +1001,17 @@ __global__ void k_pack(const float* W, uint16_t* dst, int cin, int cout, int fli
     dst[e] = (uint16_t)f2bf(v);
 }
 
+// the out layer for k_outf16: dst[t][co][ci] with co 0 = bf16(W[t][0][ci]), co 1 = its BF16
+// remainder, co 2-15 zero
+__global__ void k_pack_out(const float* W, uint16_t* dst) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 27 * 256) return;
+    const int ci = e % 16, co = (e / 16) % 16, t = e / 256;
+    const float w = W[t * 16 + ci];
+    const uint32_t hi = f2bf(w);
+    dst[e] = co == 0 ? (uint16_t)hi : (co == 1 ? (uint16_t)f2bf(w - bf2f(hi)) : (uint16_t)0);
+}
+
 // Gradient clipping: the global norm of the (batch-averaged) gradient is limited to
 // 5x a running average of the previous steps' norms.  A deep linear network trained by
 // Adam otherwise meets rare gradient spikes while the second moments are still small,
@@ -988,6 +1094,7 @@ void net_pack(Net& n, hipStream_t s) {
         hipLaunchKernelGGL(k_pack, dim3((L + 255) / 256), dim3(256), 0, s, n.theta + n.off[l], n.wf + n.poff[l], d.cin, d.cout, 0);
         hipLaunchKernelGGL(k_pack, dim3((L + 255) / 256), dim3(256), 0, s, n.theta + n.off[l], n.wb + n.poff[l], d.cin, d.cout, 1);
     }
+    hipLaunchKernelGGL(k_pack_out, dim3(27), dim3(256), 0, s, n.theta + n.off[NET_OUT], n.wf + n.poff[NET_OUT]);
     n.packed = true;
 }
 
@@ -1045,7 +1152,9 @@ static void convFLaunch(hipStream_t s, const ConvFArgs& a, int cin) {
 
 void net_forward(Net& n, hipStream_t s, const float* vin, float* vout, const double* done) {
     if (!done) done = n.scal + NET_ZERO_SLOT;   // a flag that stays zero
-    hipLaunchKernelGGL(k_in_fwd, dim3(edgeBlocks(n.nx, n.ny, n.nz)), dim3(256), 0, s, vin, n.theta + n.off[0], n.act[0], n.nx, n.ny, n.nz, done);
+    const int nt16 = ((n.nx + 15) / 16) * ((n.ny + 3) / 4) * ((n.nz + 3) / 4);
+    hipLaunchKernelGGL(k_inf16, dim3(nt16), dim3(NB), 0, s, vin, (const float*)(n.theta + n.off[0]), n.act[0], n.nx, n.ny,
+                       n.nz, done);
     for (int l = 1; l < NET_OUT; ++l) {
         const LayerDef& d = kNet[l];
         ConvFArgs a;
@@ -1065,7 +1174,8 @@ void net_forward(Net& n, hipStream_t s, const float* vin, float* vout, const dou
             else convFLaunch<MODE_PLAIN, false>(s, a, d.cin);
         }
     }
-    hipLaunchKernelGGL(k_out_fwd, dim3(edgeBlocks(n.nx, n.ny, n.nz)), dim3(256), 0, s, n.act[NET_OUT - 1], n.theta + n.off[NET_OUT], vout, n.nx, n.ny, n.nz, done);
+    hipLaunchKernelGGL(k_outf16, dim3(nt16), dim3(NB), 0, s, (const uint16_t*)n.act[NET_OUT - 1],
+                       (const uint16_t*)(n.wf + n.poff[NET_OUT]), vout, n.nx, n.ny, n.nz, done);
 }
 
 static void ensurePart(Net& n, long need) {
