@@ -424,9 +424,10 @@ __global__ void k_f_init_round(Dims D, const double* r64, const double* fin, dou
     writePartials<FB, 2>(v, part);
 }
 
-// x += e / scale at the end of a queued round (nothing for a skipped one); block 0 records
-// the round's iteration count and whether its queued iterations ran out before its test
-__global__ void k_f_axpy_round(Dims D, double* x, const float* e, double* rec, const double* st, int round) {
+// x += e / scale at the end of a queued round (nothing for a skipped one), and e = 0, so
+// that the round can go on from its FCG state if its queued iterations ran out; block 0
+// records the round's iteration count and whether they ran out before its test
+__global__ void k_f_axpy_round(Dims D, double* x, float* e, double* rec, const double* st, int round) {
     const double* my = rec + round * FR_STRIDE;
     if (my[FR_STOP] != 0.) return;
     const double sc = 1. / my[FR_SCALE];
@@ -434,7 +435,10 @@ __global__ void k_f_axpy_round(Dims D, double* x, const float* e, double* rec, c
         rec[round * FR_STRIDE + FR_ITS] = st[F_IT];
         rec[round * FR_STRIDE + FR_CAPPED] = st[F_DONE] == 0. ? 1. : 0.;
     }
-    FOR_FINE(D, q, p) { x[p] += sc * (double)e[p]; }
+    FOR_FINE(D, q, p) {
+        x[p] += sc * (double)e[p];
+        e[p] = 0.f;
+    }
 }
 
 // The residual alternates between two buffers by iteration parity (r[F_IT & 1] is the
@@ -945,8 +949,8 @@ int learned_fcg(Ctx& c, double shift, double bNorm, double rtol, int maxit, doub
         rel = bb > 0. ? rr / bb : 0.;
     };
     if (c.fcgPlanN[slot] > 0) {
-        // the rounds of the solve two back, each one iteration longer, queued with their
-        // decisions on the device and one read-back at the end
+        // the rounds of the slot's last solve, queued with their decisions on the device and
+        // one read-back at the end
         const int nq = c.fcgPlanN[slot];
         pcorr_true_resid_dev(c, shift);
         for (int q = 0; q < nq; ++q) {
@@ -956,7 +960,7 @@ int learned_fcg(Ctx& c, double shift, double bNorm, double rtol, int maxit, doub
             hipLaunchKernelGGL(k_f_pre0, dim3(g), dim3(FB), 0, s, D, I.S, (const float*)I.dinv, (const float*)I.r, I.w,
                                (const double*)st);
             for (int k = 0; k < c.fcgPlan[slot][q]; ++k) I.launch(s, true);
-            hipLaunchKernelGGL(k_f_axpy_round, dim3(g), dim3(FB), 0, s, D, c.xp, (const float*)I.e, rec, (const double*)st, q);
+            hipLaunchKernelGGL(k_f_axpy_round, dim3(g), dim3(FB), 0, s, D, c.xp, I.e, rec, (const double*)st, q);
             pcorr_true_resid_dev(c, shift);
         }
         HIPCHECK(hipMemcpyAsync(c.hscal + SC_MISC + 8, fin, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -964,6 +968,7 @@ int learned_fcg(Ctx& c, double shift, double bNorm, double rtol, int maxit, doub
         HIPCHECK(hipStreamSynchronize(s));
         relFrom(c.hscal + SC_MISC + 8);
         bool anyStop = false;
+        int qLast = -1;
         for (int q = 0; q < nq; ++q) {
             const double* h = c.hscal + SC_FCGR + q * FR_STRIDE;
             if (h[FR_STOP] != 0.) { anyStop = true; break; }
@@ -972,10 +977,34 @@ int learned_fcg(Ctx& c, double shift, double bNorm, double rtol, int maxit, doub
             ++*refs;
             lastRound = it;
             record(it, h[FR_CAPPED] != 0.);
+            qLast = q;
         }
         // a round the device skipped with the residual above rtol and iterations left is the
         // host loop's "no progress" exit
         stalled = anyStop && rel > rtol && its < maxit;
+        const double* hl = qLast >= 0 ? c.hscal + SC_FCGR + qLast * FR_STRIDE : nullptr;
+        if (!stalled && rel > rtol && hl && hl[FR_CAPPED] != 0.) {
+            // the last round's queued iterations ran out before its test (this system needs more
+            // than the plan): the round goes on from its FCG state, adding to x what it gains
+            const int it0 = (int)hl[FR_ITS];
+            const double scale = hl[FR_SCALE], prev = hl[FR_REL];
+            int it = it0, chunk = 1;
+            while (true) {
+                for (int k = 0; k < chunk; ++k) I.launch(s, true);
+                HIPCHECK(hipMemcpyAsync(c.hscal + SC_CG, st, 12 * sizeof(double), hipMemcpyDeviceToHost, s));
+                HIPCHECK(hipStreamSynchronize(s));
+                it = (int)c.hscal[SC_CG + F_IT];
+                if (c.hscal[SC_CG + F_DONE] != 0. || it - it0 >= 300) break;
+                chunk = 2;
+            }
+            its += it - it0;
+            lastRound = it;
+            if (nRec - 1 < FCG_MAX_PLAN) { recIts[nRec - 1] = it; recCapped[nRec - 1] = 0; }
+            hipLaunchKernelGGL(k_f_axpy, dim3(g), dim3(FB), 0, s, D, c.xp, (const float*)I.e, 1.0 / scale);
+            pcorr_true_resid(c, shift, &rr, &bb);
+            rel = rr / bb;
+            stalled = !(rel < 0.5 * prev);
+        }
     } else {
         pcorr_true_resid(c, shift, &rr, &bb);
         rel = bb > 0. ? rr / bb : 0.;
@@ -1022,11 +1051,13 @@ int learned_fcg(Ctx& c, double shift, double bNorm, double rtol, int maxit, doub
         ++*refs;
         if (!(rel < 0.5 * prev) || it == 0) break;
     }
-    // the plan for the solve two ahead: this solve's rounds if it converged, each one
-    // iteration longer (two if its queued iterations ran out)
+    // the plan for the next solve of this slot: this solve's rounds if it converged, as long
+    // as they ran (one iteration longer where the queued ones ran out before the round's
+    // test).  A queued iteration past convergence costs ~60 us of launches that return at
+    // once; a round that runs out is continued after the read-back, at one more read-back.
     c.fcgPlanN[slot] = (rel <= rtol && nRec > 0 && nRec <= FCG_MAX_PLAN) ? nRec : 0;
     c.fcgPlanB[slot] = bNorm;
-    for (int q = 0; q < c.fcgPlanN[slot]; ++q) c.fcgPlan[slot][q] = recIts[q] + (recCapped[q] ? 2 : 1);
+    for (int q = 0; q < c.fcgPlanN[slot]; ++q) c.fcgPlan[slot][q] = recIts[q] + (recCapped[q] ? 1 : 0);
     ++c.fcgSolves;
     *relOut = rel;
     return its;
