@@ -592,12 +592,26 @@ static int runBicgstab(Ctx& c, Op A, const double* b, double shift, T* x, T* w, 
         readBack(c, st, SC_BI, B_STRIDE * NV);
         bool all = true;
         it = 0;
+        // the next chunk: the iterations the observed rate of decrease still needs, worst
+        // system, plus one (a system is seen converged by the head of the iteration after
+        // its last); 2 without an estimate.  (Doubling the chunk queued up to 7 iterations
+        // past convergence: their launches return at once but cost the host's launch time.)
+        int need = 0;
         for (int q = 0; q < NV; ++q) {
-            all &= c.hscal[SC_BI + B_STRIDE * q + B_DONE] != 0.;
-            it = std::max(it, (int)c.hscal[SC_BI + B_STRIDE * q + B_IT]);
+            const double* h = c.hscal + SC_BI + B_STRIDE * q;
+            all &= h[B_DONE] != 0.;
+            it = std::max(it, (int)h[B_IT]);
+            if (h[B_DONE] != 0.) continue;
+            const double rr = h[B_RR], bb = h[B_BB], itq = h[B_IT];
+            int est = 2;
+            if (itq > 0. && rr > 0. && bb > 0. && rr < bb) {
+                const double rate = std::log(rr / bb) / itq;
+                est = (int)std::ceil(std::log(lc.tol2 * bb / rr) / rate) + 1;
+            }
+            need = std::max(need, est);
         }
         if (all || it >= maxit) break;
-        if (chunk < 16) chunk *= 2;
+        chunk = std::max(1, std::min(16, need));
     }
     for (int q = 0; q < NV; ++q) doneOut[q] = (int)c.hscal[SC_BI + B_STRIDE * q + B_DONE];
     return it;
@@ -646,7 +660,7 @@ static int momentumIR(Ctx& c, const caffe_solve_opts& o, int* rounds, double* re
         // previous solve needed (4 at first)
         const int slot = std::min(*rounds, 3);
         const int it = runBicgstab<3, float>(c, A32, rhs, 0., dx, w, inner, o.maxit - iters, c.momChunk[slot], done);
-        c.momChunk[slot] = std::max(1, std::min(32, it));
+        c.momChunk[slot] = std::max(1, std::min(32, it + 1));   // + 1: the launch that sees convergence
         iters += it;
         hipLaunchKernelGGL(k_add_f32_3, dim3(g), dim3(KBS), 0, c.stream, D, c.xm, (const float*)dx);
         trueResid<3>(c, A, c.bm, 0., c.xm, rt, rr, bb);
