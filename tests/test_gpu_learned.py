@@ -163,6 +163,24 @@ def test_ablation_trained_network_gain(system128, trained):
     untrained.close()
 
 
+def test_round_plan_runs_short_and_is_continued(system128, trained):
+    """The refinement rounds are queued at the lengths the last solve of their plan slot
+    ran.  Rounds planned on a solve with a looser inner tolerance (1e-3: shorter rounds) run
+    out of queued iterations on the next solve (1e-4); the last one is continued after the
+    read-back, or the rounds after it start from the true residual it left; either way the
+    solve meets rtol and agrees with the plain solve."""
+    from caffe_old_amd import _lib as L
+    rp, xp = _solve(system128, L.SOLVE_PLAIN_FP64)
+    runs = []
+    for inner in (1e-3, 1e-4, 1e-4):
+        r, x = _solve(system128, L.SOLVE_LEARNED_FCG, trained, inner=inner)
+        runs.append((r, x))
+        print(f"inner {inner:g}: {r['iters']} its, {r['refinements']} rounds, relres {r['relres']:.2e}, {r['ms']:.2f} ms")
+    for r, x in runs:
+        assert r["converged"] and r["fell_back"] == 0 and r["relres"] <= RTOL
+        assert _rel(x, xp) <= 1e-6
+
+
 def test_learned_piso_steps_match_plain():
     """Three PISO steps of the 128^3 held-out cavity with learned p' solves reproduce the
     plain-solve fields (every p' solve is judged at 1e-8)."""
