@@ -288,7 +288,8 @@ int momentumSolve(Ctx& c, caffe_solve_report* rep) {
 }
 
 int pcorrSolve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep) {
-    int rc = pcorr_solve(c, o, &c.lastPc);
+    // inside a step (no report asked for) the solve's time is read when the report is
+    int rc = pcorr_solve(c, o, &c.lastPc, rep == nullptr);
     if (rep) *rep = c.lastPc;
     launch_pc_copyback(c);
     double* f[1] = {c.pc};
@@ -378,8 +379,12 @@ int solveStep(Ctx& c, double dt, double* continuity) {
             correctStep(c);
         }
     }
-    if (c.cs.algo != CAFFE_IBPISO) c.courant = courant(c, dt);  // Piso.cpp:26 / Simple.cpp:88
+    // Piso.cpp:26 / Simple.cpp:88: the Courant number's reduction is queued ahead of the
+    // continuity scan, whose read-back brings both values
+    const bool co = c.cs.algo != CAFFE_IBPISO;
+    if (co) courant_queue(c, dt);
     double e = continuity_quirk(c);
+    if (co) c.courant = courant_queued(c);
     if (continuity) *continuity = e;
     return CAFFE_OK;
 }
@@ -426,6 +431,9 @@ int caffe_ctx_create(int device, caffe_ctx** out) {
         HIPCHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
         HIPCHECK(hipEventCreate(&c.ev0));
         HIPCHECK(hipEventCreate(&c.ev1));
+        HIPCHECK(hipEventCreateWithFlags(&c.evR, hipEventDisableTiming));
+        HIPCHECK(hipEventCreate(&c.evP0));
+        HIPCHECK(hipEventCreate(&c.evP1));
         return CAFFE_OK;
     });
     if (rc != CAFFE_OK) {
@@ -446,6 +454,7 @@ int caffe_ctx_destroy(caffe_ctx* x) {
     for (auto e : c.probe.ev) (void)hipEventDestroy(e);
     if (c.ev0) (void)hipEventDestroy(c.ev0);
     if (c.ev1) (void)hipEventDestroy(c.ev1);
+    for (hipEvent_t e : {c.evR, c.evP0, c.evP1}) if (e) (void)hipEventDestroy(e);
     if (c.stream) (void)hipStreamDestroy(c.stream);
     delete x;
     return CAFFE_OK;
@@ -517,7 +526,12 @@ int caffe_step_report(caffe_ctx* x, double dt, double* cref, double* cmax, doubl
     });
 }
 int caffe_last_reports(caffe_ctx* x, caffe_solve_report* m, caffe_solve_report* p) {
-    return guard(x, [&](Ctx& c) { if (m) *m = c.lastMom; if (p) *p = c.lastPc; return CAFFE_OK; });
+    return guard(x, [&](Ctx& c) {
+        pcorr_resolve_ms(c);
+        if (m) *m = c.lastMom;
+        if (p) *p = c.lastPc;
+        return CAFFE_OK;
+    });
 }
 
 int caffe_field_download(caffe_ctx* x, const char* name, double* host, long n) {

@@ -644,7 +644,8 @@ double continuity_quirk(Ctx& c) {
     return c.hscal[SC_MISC];
 }
 
-static double maxReduce(Ctx& c, int mode, double dt) {
+// the max reduction of mode 0 (continuity) or 1 (Courant) into the scalar slot
+static void maxReduceLaunch(Ctx& c, int mode, double dt, int slot) {
     const int nb = blocksFor(c.D.N, BS, 2048);
     double* part = c.red.part;
     if (mode == 0) {
@@ -654,11 +655,21 @@ static double maxReduce(Ctx& c, int mode, double dt) {
         if (c.uniform) hipLaunchKernelGGL((k_maxreduce<true, 1>), dim3(nb), dim3(BS), 0, c.stream, c.geo, c.st, c.mflow, c.u[0], c.u[1], c.u[2], dt, part);
         else hipLaunchKernelGGL((k_maxreduce<false, 1>), dim3(nb), dim3(BS), 0, c.stream, c.geo, c.st, c.mflow, c.u[0], c.u[1], c.u[2], dt, part);
     }
-    hipLaunchKernelGGL(k_max_final, dim3(1), dim3(BS), 0, c.stream, part, nb, c.scal + SC_MISC + 1);
+    hipLaunchKernelGGL(k_max_final, dim3(1), dim3(BS), 0, c.stream, part, nb, c.scal + slot);
+}
+static double maxReduce(Ctx& c, int mode, double dt) {
+    maxReduceLaunch(c, mode, dt, SC_MISC + 1);
     return readScalar(c, SC_MISC + 1);
 }
 double continuity_max(Ctx& c) { return maxReduce(c, 0, 0.); }
 double courant(Ctx& c, double dt) { return maxReduce(c, 1, dt); }
+// a slot no other reduction of the step uses (the continuity scan holds SC_MISC + 0..2)
+constexpr int SC_COURANT = SC_MISC + 16;
+void courant_queue(Ctx& c, double dt) {
+    maxReduceLaunch(c, 1, dt, SC_COURANT);
+    HIPCHECK(hipMemcpyAsync(c.hscal + SC_COURANT, c.scal + SC_COURANT, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+}
+double courant_queued(const Ctx& c) { return c.hscal[SC_COURANT]; }
 
 // ------------------------------------------------------------------ Coupled
 // Coupled::computeMomentumAndPressure (Coupled.cpp:54-280): the four rows of a cell (u, v,

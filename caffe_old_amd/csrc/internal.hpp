@@ -137,6 +137,9 @@ struct Ctx {
     bool ibReady = false;     // ghost rows computed for the current case
     double courant = 0.;      // last Courant number (Simple.cpp:88)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t evR = nullptr;                   // marks a queued read-back into hscal
+    hipEvent_t evP0 = nullptr, evP1 = nullptr;  // the last p' solve, start and end
+    bool pcMsPending = false;                   // lastPc.ms still to be read from evP0/evP1
 
     // live timing of one kernel inside the solves (event pairs around its launches)
     struct Probe {
@@ -194,6 +197,10 @@ void launch_copy_cells(Ctx& c, const double* src, double* dst);   // non-INACTIV
 void launch_grad_extrap(Ctx& c, const double* phi, double* const* g1, double* const* out);
 double diff_resid_inf(Ctx& c, const double* x);
 double courant(Ctx& c, double dt);
+// the Courant number's reduction and its copy to the host mirror, queued; courant_queued
+// reads it once the stream has been synchronised
+void courant_queue(Ctx& c, double dt);
+double courant_queued(const Ctx& c);
 // Coupled (Coupled.cpp:54-338): assembly of the 4-variable system, the field update after its
 // solve (u, p, grad p, h, Rhie-Chow faces), the continuity error
 void launch_cp_assemble(Ctx& c, double dt);
@@ -208,7 +215,10 @@ void compute_metrics(Ctx& c, const double* dNodes);
 
 // ---- Krylov (krylov.hip)
 int momentum_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep);
-int pcorr_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep);
+// lazyMs: the report's ms is left to caffe_last_reports (no wait for the solve's end)
+int pcorr_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep, bool lazyMs = false);
+void pcorr_remove_mean(Ctx& c);
+void pcorr_resolve_ms(Ctx& c);
 // the Coupled system: Jacobi BiCGStab over the joint (u, v, w, p) vector in FP64, to a true
 // relative residual rtol; p is returned with zero mean when no face fixes it
 int coupled_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep);
@@ -242,7 +252,8 @@ void pcorr_true_resid_dev(Ctx& c, double shift);
 // ---- learned preconditioner (learned.hip): flexible CG with the learned two-grid
 // cycle, FP32 iterations inside FP64 refinement; returns the iteration count, or -1
 // when the system does not qualify (ghost rows, dimensions not divisible by 16)
-int learned_fcg(Ctx& c, double shift, double bNorm, double rtol, int maxit, double delta, int* refs, double* relOut);
+int learned_fcg(Ctx& c, double shift, double bNorm, double rtol, int maxit, double delta, int* refs, double* relOut,
+                bool* meanQueued);
 
 #define HIPCHECK(x)                                                        \
     do {                                                                   \

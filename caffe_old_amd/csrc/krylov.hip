@@ -498,11 +498,18 @@ void add_vec(Ctx& c, double* x, const double* d, int n) {
 
 // mean over non-inactive cells
 // the mean of v over the live cells and the norm of v minus that mean, in one read-back
-static double meanNorm(Ctx& c, const double* v, double* norm) {
+// queued in two halves: meanNormQueue launches the reduction and its copy to the host,
+// meanNormWait waits for that copy alone, so work queued in between keeps the GPU busy
+// while the host turns round
+static void meanNormQueue(Ctx& c, const double* v) {
     const int g = gridFor(c.D);
     hipLaunchKernelGGL(k_sum_sq, dim3(g), dim3(KBS), 0, c.stream, c.D, c.st, v, c.red.part);
     hipLaunchKernelGGL(k_finalize<3>, dim3(1), dim3(KBS), 0, c.stream, c.red.part, g, c.scal + SC_MISC + 4);
-    readBack(c, c.scal + SC_MISC + 4, SC_MISC + 4, 3);
+    HIPCHECK(hipMemcpyAsync(c.hscal + SC_MISC + 4, c.scal + SC_MISC + 4, 3 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    HIPCHECK(hipEventRecord(c.evR, c.stream));
+}
+static double meanNormWait(Ctx& c, double* norm) {
+    HIPCHECK(hipEventSynchronize(c.evR));
     const double s = c.hscal[SC_MISC + 4], n = c.hscal[SC_MISC + 5], sq = c.hscal[SC_MISC + 6];
     const double m = n > 0. ? s / n : 0.;
     *norm = std::sqrt(std::max(sq - n * m * m, 0.));
@@ -1323,18 +1330,42 @@ void pcorr_true_resid_dev(Ctx& c, double shift) {
     hipLaunchKernelGGL(k_finalize<3>, dim3(1), dim3(KBS), 0, c.stream, c.red.part, g, c.scal + SC_MISC + 8);
 }
 
-int pcorr_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep) {
-    HIPCHECK(hipEventRecord(c.ev0, c.stream));
+// the solution's mean over the cells that carry a row, removed on the device
+void pcorr_remove_mean(Ctx& c) {
+    const int g = gridFor(c.D);
+    hipLaunchKernelGGL(k_sum, dim3(g), dim3(KBS), 0, c.stream, c.D, c.st, (const double*)c.xp, c.red.part);
+    hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(KBS), 0, c.stream, c.red.part, g, c.scal + SC_MISC + 4);
+    hipLaunchKernelGGL(k_sub_mean, dim3(g), dim3(KBS), 0, c.stream, c.D, c.st, c.xp, (const double*)(c.scal + SC_MISC + 4));
+}
+
+// the time of a solve whose report left it pending
+void pcorr_resolve_ms(Ctx& c) {
+    if (!c.pcMsPending) return;
+    HIPCHECK(hipEventSynchronize(c.evP1));
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, c.evP0, c.evP1));
+    c.lastPc.ms = ms;
+    c.pcMsPending = false;
+}
+
+int pcorr_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep, bool lazyMs) {
+    c.pcMsPending = false;   // a pending time belongs to the report this solve replaces
+    HIPCHECK(hipEventRecord(c.evP0, c.stream));
     const Dims& D = c.D;
     const int P = D.P;
+    const bool fp32 = o.mode != CAFFE_SOLVE_PLAIN_FP64 && o.mode != CAFFE_SOLVE_SGS_FP64;
     // Neumann null space: the right-hand side loses its (rounding) mean, the solution is
-    // returned with zero mean over the cells that carry a row
+    // returned with zero mean over the cells that carry a row.  The mean's read-back is
+    // waited for after the launches that do not need it.
     double bNorm = 0.;   // |b - shift| where known without another read-back (0: unknown)
-    const double shift = c.singularP ? meanNorm(c, c.bp, &bNorm) : 0.;
+    if (c.singularP) meanNormQueue(c, c.bp);
+    hipLaunchKernelGGL(k_zero_d, dim3(blocksFor(P, KBS)), dim3(KBS), 0, c.stream, c.xp, P);
+    if (fp32) pcorr_make_fp32(c);
+    const double shift = c.singularP ? meanNormWait(c, &bNorm) : 0.;
     int iters = 0, refs = 0, fell = 0;
     double rel = 1.;
+    bool meanDone = false;   // the solution's mean already removed (queued by the learned solve)
     const bool sym = !c.hasIB && !c.indefiniteP;
-    hipLaunchKernelGGL(k_zero_d, dim3(blocksFor(P, KBS)), dim3(KBS), 0, c.stream, c.xp, P);
     if (o.mode == CAFFE_SOLVE_PLAIN_FP64) {
         if (sym) iters += cgSolve<double>(c, shift, o.rtol, o.maxit, 0., &refs, &rel);
         else iters += bicgPressure<double>(c, shift, o.rtol, o.maxit, 0., &refs, &rel);
@@ -1344,13 +1375,14 @@ int pcorr_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep) {
         if (sym && !c.hasIB) iters += sgsCgSolve(c, shift, o.rtol, o.maxit, &refs, &rel);
         else { fell = 1; iters += bicgPressure<double>(c, shift, o.rtol, o.maxit, 0., &refs, &rel); }
     } else {
-        pcorr_make_fp32(c);
         bool learnedDone = false;
         if (o.mode == CAFFE_SOLVE_LEARNED_FCG) {
             // learned two-grid cycle inside flexible CG; the inner FP32 solve restarts from
             // the true FP64 residual each time it has gained `inner_rtol` (default 1e-4)
-            const int it = c.net ? learned_fcg(c, shift, bNorm, o.rtol, o.maxit, o.inner_rtol > 0. ? o.inner_rtol : 1e-4, &refs, &rel) : -1;
-            if (it >= 0) { iters += it; learnedDone = true; }
+            bool meanQ = false;
+            const int it = c.net ? learned_fcg(c, shift, bNorm, o.rtol, o.maxit, o.inner_rtol > 0. ? o.inner_rtol : 1e-4, &refs, &rel,
+                                               &meanQ) : -1;
+            if (it >= 0) { iters += it; learnedDone = true; meanDone = meanQ; }
             else fell = 1;   // no network, or a system it does not apply to: IR below
         }
         if (!learnedDone) {
@@ -1373,6 +1405,7 @@ int pcorr_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep) {
         }
         if (!(rel <= o.rtol)) {  // the plain FP64 solve stays as fallback and judge, from x = 0 (also for NaN)
             fell = 1;
+            meanDone = false;
             int r2 = 0;
             hipLaunchKernelGGL(k_zero_d, dim3(blocksFor(P, KBS)), dim3(KBS), 0, c.stream, c.xp, P);
             if (sym) iters += cgSolve<double>(c, shift, o.rtol, o.maxit, 0., &r2, &rel);
@@ -1380,19 +1413,19 @@ int pcorr_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep) {
             refs += r2;
         }
     }
-    if (c.singularP) {   // the mean stays on the device: no read-back
-        const int g = gridFor(D);
-        hipLaunchKernelGGL(k_sum, dim3(g), dim3(KBS), 0, c.stream, D, c.st, (const double*)c.xp, c.red.part);
-        hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(KBS), 0, c.stream, c.red.part, g, c.scal + SC_MISC + 4);
-        hipLaunchKernelGGL(k_sub_mean, dim3(g), dim3(KBS), 0, c.stream, D, c.st, c.xp, (const double*)(c.scal + SC_MISC + 4));
-    }
-    HIPCHECK(hipEventRecord(c.ev1, c.stream));
-    HIPCHECK(hipEventSynchronize(c.ev1));
-    float ms = 0.f;
-    HIPCHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    if (c.singularP && !meanDone) pcorr_remove_mean(c);   // the mean stays on the device: no read-back
+    HIPCHECK(hipEventRecord(c.evP1, c.stream));
     if (rep) {
         rep->iters = iters; rep->refinements = refs; rep->relres = rel; rep->converged = rel <= o.rtol;
-        rep->fell_back = fell; rep->ms = ms;
+        rep->fell_back = fell; rep->ms = 0.;
+    }
+    if (lazyMs && rep == &c.lastPc) {
+        c.pcMsPending = true;
+    } else {
+        HIPCHECK(hipEventSynchronize(c.evP1));
+        float ms = 0.f;
+        HIPCHECK(hipEventElapsedTime(&ms, c.evP0, c.evP1));
+        if (rep) rep->ms = ms;
     }
     return rel <= o.rtol ? CAFFE_OK : CAFFE_ERR_NOT_CONVERGED;
 }

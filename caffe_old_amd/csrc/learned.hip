@@ -894,10 +894,12 @@ struct FcgIter {
     }
 };
 
-int learned_fcg(Ctx& c, double shift, double bNorm, double rtol, int maxit, double delta, int* refs, double* relOut) {
+int learned_fcg(Ctx& c, double shift, double bNorm, double rtol, int maxit, double delta, int* refs, double* relOut,
+                bool* meanQueued) {
     Net& n = *(Net*)c.net;
     const Dims& D = c.D;
     *refs = 0;
+    *meanQueued = false;
     if (!netDimsOk(D) || c.hasIB) return -1;
     HIPCHECK(hipSetDevice(n.device));
     tg_ensure_fine(n, D);
@@ -965,7 +967,15 @@ int learned_fcg(Ctx& c, double shift, double bNorm, double rtol, int maxit, doub
         }
         HIPCHECK(hipMemcpyAsync(c.hscal + SC_MISC + 8, fin, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
         HIPCHECK(hipMemcpyAsync(c.hscal + SC_FCGR, rec, nq * FR_STRIDE * sizeof(double), hipMemcpyDeviceToHost, s));
-        HIPCHECK(hipStreamSynchronize(s));
+        HIPCHECK(hipEventRecord(c.evR, s));
+        // the solution's mean removal goes behind the read-back, so that the GPU runs it while
+        // the host turns round.  It leaves the residual as it is (the residual of the singular
+        // system leaves out its mean); the continuation below changes x and clears the flag.
+        if (c.singularP) {
+            pcorr_remove_mean(c);
+            *meanQueued = true;
+        }
+        HIPCHECK(hipEventSynchronize(c.evR));
         relFrom(c.hscal + SC_MISC + 8);
         bool anyStop = false;
         int qLast = -1;
@@ -986,6 +996,7 @@ int learned_fcg(Ctx& c, double shift, double bNorm, double rtol, int maxit, doub
         if (!stalled && rel > rtol && hl && hl[FR_CAPPED] != 0.) {
             // the last round's queued iterations ran out before its test (this system needs more
             // than the plan): the round goes on from its FCG state, adding to x what it gains
+            *meanQueued = false;
             const int it0 = (int)hl[FR_ITS];
             const double scale = hl[FR_SCALE], prev = hl[FR_REL];
             int it = it0, chunk = 1;

@@ -117,7 +117,8 @@ void tg_ensure_fine(Net& n, const Dims& D);
 void tg_apply(Net& n, hipStream_t s, const Dims& D, const float* A, const float* r, float* z, const double* done);
 // loss of one sample; grad += dL/dtheta when withGrad; the loss lands in *lossDev
 void tg_loss(Net& n, hipStream_t s, const Sample& smp, bool withGrad, float* lossDev);
-int learned_fcg(Ctx& c, double shift, double bNorm, double rtol, int maxit, double delta, int* refs, double* relOut);
+int learned_fcg(Ctx& c, double shift, double bNorm, double rtol, int maxit, double delta, int* refs, double* relOut,
+                bool* meanQueued);
 
 }  // namespace caffe
 
