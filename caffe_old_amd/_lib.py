@@ -28,7 +28,8 @@ EXPORTS = [
     "caffe_ctx_create", "caffe_ctx_destroy", "caffe_last_error", "caffe_case_setup", "caffe_set_solve_opts",
     "caffe_solve", "caffe_save_u0", "caffe_momentum_assemble", "caffe_momentum_solve", "caffe_rhiechow",
     "caffe_pcorr_assemble", "caffe_pcorr_solve", "caffe_correct", "caffe_ib_field", "caffe_ib_coeffs",
-    "caffe_grad_u", "caffe_step_report", "caffe_last_reports", "caffe_field_download", "caffe_field_upload",
+    "caffe_grad_u", "caffe_step_report", "caffe_last_reports", "caffe_last_courant", "caffe_field_download",
+    "caffe_field_upload",
     "caffe_metric_download", "caffe_matrix_download", "caffe_rhs_download", "caffe_ghost_rows", "caffe_time_kernel",
     "caffe_kernel_probe", "caffe_kernel_probe_read",
     "caffe_sync",
@@ -110,6 +111,7 @@ def load(path: str = LIB_PATH):
         "caffe_grad_u": ([vp], C.c_int),
         "caffe_step_report": ([vp, C.c_double, dp, dp, dp], C.c_int),
         "caffe_last_reports": ([vp, C.POINTER(SolveReport), C.POINTER(SolveReport)], C.c_int),
+        "caffe_last_courant": ([vp, dp], C.c_int),
         "caffe_field_download": ([vp, C.c_char_p, dp, C.c_long], C.c_int),
         "caffe_field_upload": ([vp, C.c_char_p, dp, C.c_long], C.c_int),
         "caffe_metric_download": ([vp, C.c_char_p, dp, C.c_long], C.c_int),

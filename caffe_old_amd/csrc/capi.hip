@@ -525,6 +525,13 @@ int caffe_step_report(caffe_ctx* x, double dt, double* cref, double* cmax, doubl
         return CAFFE_OK;
     });
 }
+int caffe_last_courant(caffe_ctx* x, double* co) {
+    return guard(x, [&](Ctx& c) {
+        if (!co) { c.err = "null argument"; return CAFFE_ERR_ARG; }
+        *co = c.courant;
+        return CAFFE_OK;
+    });
+}
 int caffe_last_reports(caffe_ctx* x, caffe_solve_report* m, caffe_solve_report* p) {
     return guard(x, [&](Ctx& c) {
         pcorr_resolve_ms(c);

@@ -106,10 +106,13 @@ class Solver:
     # -------------------------------------------------------------- the reference interface
     def solve(self, dt: float | None = None) -> float:
         """Solver::solve(timeStep): one outer iteration / time step; returns the continuity
-        value of Simple::computeContinuityError (Simple.cpp:358-380)."""
-        e = C.c_double()
+        value of Simple::computeContinuityError (Simple.cpp:358-380) and keeps the Courant
+        number the step computed (Simple.cpp:88, Piso.cpp:26)."""
+        e, co = C.c_double(), C.c_double()
         self._check(self.lib.caffe_solve(self._h, self.case.time_step if dt is None else dt, C.byref(e)), "caffe_solve")
+        self._check(self.lib.caffe_last_courant(self._h, C.byref(co)), "caffe_last_courant")
         self.continuityError = e.value
+        self.courantNumber = co.value
         return e.value
 
     def displayUpdateMessage(self) -> str:

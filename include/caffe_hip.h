@@ -154,6 +154,9 @@ int caffe_ib_coeffs(caffe_ctx* ctx);
 int caffe_grad_u(caffe_ctx* ctx);
 int caffe_step_report(caffe_ctx* ctx, double dt, double* continuity_ref, double* continuity_max, double* courant);
 int caffe_last_reports(caffe_ctx* ctx, caffe_solve_report* momentum, caffe_solve_report* pcorr);
+/* courantNumber_ as the last caffe_solve left it (Simple.cpp:88, Piso.cpp:26; printed by
+ * displayUpdateMessage, Piso.cpp:36); IbPiso::solve does not compute it, so it stays 0 there */
+int caffe_last_courant(caffe_ctx* ctx, double* courant);
 
 /* Field exchange in the padded layout (nI+2)(nJ+2)(nK+2), i fastest, halo = boundary
  * patches; vector fields are component-major.  Names: u u0 h p pCorr d gradP gradPCorr

@@ -220,3 +220,21 @@ def test_ib_channel_small():
     assert np.array_equal(st_g.astype(int), st_o)
     errs = _field_errors(g, o)
     assert max(errs.values()) <= 1e-6, errs
+
+
+def test_step_courant_and_report_time():
+    """solve() keeps the Courant number the step computed (Piso.cpp:26), queued ahead of the
+    continuity scan's read-back: it equals the synchronous step_report() value on the same
+    state.  The p' report read after the step carries the solve's time (read from its events
+    when the report is)."""
+    case = cs.lid_cavity(12, 1e-3, 1.0, "piso", alpha_u=1.0, alpha_p=1.0, time_accurate=True, dt=0.5 / 12,
+                         n_pcorr=2, max_iterations=3)
+    g, o = _pair(case, uniform=True)
+    for _ in range(case.n_solves):
+        g.solve()
+    co = g.courantNumber
+    assert co > 0.
+    assert co == g.step_report()["courant"]
+    m, p = g.reports()
+    assert p["ms"] > 0. and m["ms"] > 0.
+    assert p["converged"] == 1
