@@ -215,4 +215,33 @@ __device__ inline int xcdChunk() {
 // +-sJ and +-sK neighbours a cell reads are mostly in that XCD's L2
 __device__ inline long xcdCell() { return (long)xcdChunk() * blockDim.x + threadIdx.x; }
 
+// Column tiles: a block of CTX x CTY threads takes that many columns of (i, j) through a
+// range of k planes, one thread per column, k ascending, so that the k - 1 and k + 1
+// neighbours a cell reads were, or will next be, loaded by the same thread or block (the
+// plane-chunk order fetches them again from the cache levels below L2, or from HBM on large
+// grids).  Work item v (tile v / nseg, k segment v % nseg) goes to blocks in the
+// XCD-aware order, so an XCD holds whole tiles.
+constexpr int CTX = 64, CTY = 16;
+struct ColMap { int ntx, nty, nseg, kper; };
+inline ColMap colMap(const Dims& D, int g) {
+    ColMap m;
+    m.ntx = (D.nI + CTX - 1) / CTX;
+    m.nty = (D.nJ + CTY - 1) / CTY;
+    const int t = m.ntx * m.nty;
+    m.nseg = g / t > 1 ? g / t : 1;
+    m.kper = (D.nK + m.nseg - 1) / m.nseg;
+    return m;
+}
+// work item v of the map for this thread: the padded index p of its first cell and its k
+// range [k, ke); false when the thread has no cell in the item
+__device__ inline bool colItem(const Dims& D, const ColMap& cm, int v, int& p, int& k, int& ke) {
+    const int t = v / cm.nseg;
+    const int i = (t % cm.ntx) * CTX + (int)(threadIdx.x % CTX), j = (t / cm.ntx) * CTY + (int)(threadIdx.x / CTX);
+    k = (v % cm.nseg) * cm.kper;
+    ke = min(D.nK, k + cm.kper);
+    if (!(i < D.nI && j < D.nJ) || k >= ke) return false;
+    p = D.pidx(i, j, k);
+    return true;
+}
+
 }  // namespace caffe

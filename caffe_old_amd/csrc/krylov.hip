@@ -356,6 +356,112 @@ __global__ void k_bi_b_band(Dims D, BandT<T> A, const T* __restrict__ r, const T
     writePartials<KBS, 6>(acc, partOut);
 }
 
+// KA and KB on the band without ghost rows, on column tiles (ColMap): each thread walks
+// its column up k and keeps the operand it formed at k - 1 and k (p = r + beta (p_old -
+// omega v_old) in KA, s = r - alpha v in KB) and D^-1 there in registers, so a cell loads
+// the operand's arrays at its four side neighbours and above only.  The plane-chunk order
+// of k_bi_a_band re-fetched 1.46x its algorithmic reads from below L2 at 128^3 (k_bi_b_band
+// 1.22x).  The arithmetic of each cell is k_bi_a_band's / k_bi_b_band's.
+static_assert(CTX * CTY == KBS, "one thread per column of a tile");
+template <class T>
+__global__ __launch_bounds__(KBS) void k_bi_a_col(Dims D, BandT<T> A, ColMap cm, const T* __restrict__ r,
+                                                  const T* __restrict__ rh, const T* __restrict__ pold,
+                                                  T* __restrict__ pnew, const T* __restrict__ vold,
+                                                  T* __restrict__ vnew, const T* __restrict__ dinv,
+                                                  const double* partIn, int nb, double* partOut, double* st, LoopCtl lc) {
+    BiState<3> S;
+    S.load(st);
+    if (!S.any()) return;
+    double beta[3], om[3];
+    bool live[3];
+    biaHead<3>(partIn, nb, st, lc, S, beta, om, live);
+    double acc[3] = {0., 0., 0.};
+    const int P = D.P, sJ = D.sJ, sK = D.sK;
+    auto opnd = [&](int s, int q) {
+        const int b = s * P + q;
+        return (double)r[b] + beta[s] * ((double)pold[b] - om[s] * (double)vold[b]);
+    };
+    for (int v = xcdChunk(), vn = cm.ntx * cm.nty * cm.nseg; v < vn; v += (int)gridDim.x) {
+        int p, k, ke;
+        if (!colItem(D, cm, v, p, k, ke)) continue;
+        double pd[3] = {0., 0., 0.}, pc[3] = {0., 0., 0.};   // the operand at k - 1 and k
+#pragma unroll
+        for (int s = 0; s < 3; ++s)
+            if (live[s]) { pd[s] = opnd(s, p - sK); pc[s] = opnd(s, p); }
+        double dd = (double)dinv[p - sK], dc = (double)dinv[p];
+        for (; k < ke; ++k, p += sK) {
+            T a[7];   // converted where used, which keeps the registers of the FP32 values
+#pragma unroll
+            for (int m = 0; m < 7; ++m) a[m] = A.A[m * P + p];
+            const T dE = dinv[p + 1], dW = dinv[p - 1], dN = dinv[p + sJ], dS = dinv[p - sJ], dU = dinv[p + sK];
+#pragma unroll 1
+            for (int s = 0; s < 3; ++s) {   // one component at a time: the three at once need more than 128 registers
+                if (!live[s]) continue;
+                const int b = s * P + p;
+                const double pu = opnd(s, p + sK);
+                const double ph[7] = {pc[s] * dc, opnd(s, p + 1) * (double)dE, opnd(s, p - 1) * (double)dW,
+                                      opnd(s, p + sJ) * (double)dN, opnd(s, p - sJ) * (double)dS, pu * (double)dU, pd[s] * dd};
+                const double vv = (double)a[0] * ph[0] + (double)a[1] * ph[1] + (double)a[2] * ph[2] + (double)a[3] * ph[3]
+                                + (double)a[4] * ph[4] + (double)a[5] * ph[5] + (double)a[6] * ph[6];
+                pnew[b] = (T)pc[s]; vnew[b] = (T)vv;
+                acc[s] += (double)rh[b] * vv;
+                pd[s] = pc[s]; pc[s] = pu;
+            }
+            dd = dc; dc = (double)dU;
+        }
+    }
+    writePartials<KBS, 3>(acc, partOut);
+}
+template <class T>
+__global__ __launch_bounds__(KBS) void k_bi_b_col(Dims D, BandT<T> A, ColMap cm, const T* __restrict__ r,
+                                                  const T* __restrict__ v, T* __restrict__ sv, T* __restrict__ tv,
+                                                  const T* __restrict__ dinv, const double* partIn, int nb, double* partOut,
+                                                  double* st) {
+    BiState<3> S;
+    S.load(st);
+    if (!S.any()) return;
+    double al[3];
+    bool live[3];
+    bibHead<3>(partIn, nb, st, S, al, live);
+    double acc[6] = {0., 0., 0., 0., 0., 0.};
+    const int P = D.P, sJ = D.sJ, sK = D.sK;
+    auto opnd = [&](int s, int q) {
+        const int b = s * P + q;
+        return (double)r[b] - al[s] * (double)v[b];
+    };
+    for (int w = xcdChunk(), wn = cm.ntx * cm.nty * cm.nseg; w < wn; w += (int)gridDim.x) {
+        int p, k, ke;
+        if (!colItem(D, cm, w, p, k, ke)) continue;
+        double sd[3] = {0., 0., 0.}, sc[3] = {0., 0., 0.};
+#pragma unroll
+        for (int s = 0; s < 3; ++s)
+            if (live[s]) { sd[s] = opnd(s, p - sK); sc[s] = opnd(s, p); }
+        double dd = (double)dinv[p - sK], dc = (double)dinv[p];
+        for (; k < ke; ++k, p += sK) {
+            T a[7];   // converted where used, which keeps the registers of the FP32 values
+#pragma unroll
+            for (int m = 0; m < 7; ++m) a[m] = A.A[m * P + p];
+            const T dE = dinv[p + 1], dW = dinv[p - 1], dN = dinv[p + sJ], dS = dinv[p - sJ], dU = dinv[p + sK];
+#pragma unroll 1
+            for (int s = 0; s < 3; ++s) {
+                if (!live[s]) continue;
+                const int b = s * P + p;
+                const double su = opnd(s, p + sK);
+                const double sh[7] = {sc[s] * dc, opnd(s, p + 1) * (double)dE, opnd(s, p - 1) * (double)dW,
+                                      opnd(s, p + sJ) * (double)dN, opnd(s, p - sJ) * (double)dS, su * (double)dU, sd[s] * dd};
+                const double tt = (double)a[0] * sh[0] + (double)a[1] * sh[1] + (double)a[2] * sh[2] + (double)a[3] * sh[3]
+                                + (double)a[4] * sh[4] + (double)a[5] * sh[5] + (double)a[6] * sh[6];
+                sv[b] = (T)sc[s]; tv[b] = (T)tt;
+                acc[2 * s] += tt * sc[s];
+                acc[2 * s + 1] += tt * tt;
+                sd[s] = sc[s]; sc[s] = su;
+            }
+            dd = dc; dc = (double)dU;
+        }
+    }
+    writePartials<KBS, 6>(acc, partOut);
+}
+
 // KC: [omega = (t,s)/(t,t)] x += alpha D^-1 p + omega D^-1 s; r = s - omega t;
 // partials (rh, r), (r, r)
 template <int NV, class T, class Op>
@@ -574,6 +680,7 @@ static int runBicgstab(Ctx& c, Op A, const double* b, double shift, T* x, T* w, 
     double* partA = c.red.part + MAX_RED_BLOCKS * RED_STRIDE;
     double* partB = c.red.part + 2 * MAX_RED_BLOCKS * RED_STRIDE;
     const int g = gridFor(D);
+    const ColMap cm = colMap(D, g);
     LoopCtl lc{rtol * rtol, maxit, 0.};
     hipLaunchKernelGGL((k_bi_init<NV, T, Op>), dim3(g), dim3(KBS), 0, c.stream, D, A, b, shift, r, rh, p0, v0, x, dinv, partC, st);
     T *pc = p0, *pn = p1, *vc = v0, *vn = v1;
@@ -582,9 +689,9 @@ static int runBicgstab(Ctx& c, Op A, const double* b, double shift, T* x, T* w, 
         for (int q = 0; q < chunk; ++q) {
             if constexpr (NV == 3 && std::is_same<Op, BandT<T>>::value) {
                 if (!A.ov.rowOf) {
-                    hipLaunchKernelGGL((k_bi_a_band<T, false>), dim3(g), dim3(KBS), 0, c.stream, D, A, (const T*)r, (const T*)rh,
+                    hipLaunchKernelGGL(k_bi_a_col<T>, dim3(g), dim3(KBS), 0, c.stream, D, A, cm, (const T*)r, (const T*)rh,
                                        (const T*)pc, pn, (const T*)vc, vn, (const T*)dinv, (const double*)partC, g, partA, st, lc);
-                    hipLaunchKernelGGL((k_bi_b_band<T, false>), dim3(g), dim3(KBS), 0, c.stream, D, A, (const T*)r, (const T*)vn, s, t,
+                    hipLaunchKernelGGL(k_bi_b_col<T>, dim3(g), dim3(KBS), 0, c.stream, D, A, cm, (const T*)r, (const T*)vn, s, t,
                                        (const T*)dinv, (const double*)partA, g, partB, st);
                 } else {
                     hipLaunchKernelGGL((k_bi_a_band<T, true>), dim3(g), dim3(KBS), 0, c.stream, D, A, (const T*)r, (const T*)rh,

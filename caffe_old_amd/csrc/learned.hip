@@ -55,22 +55,8 @@ static SymF symF(const float* A, const Dims& D) { return SymF{A, D.P, D.sJ, D.sK
          q < q##e_ && ((p = fineIdx((D), q)), true); q += blockDim.x)
 __device__ inline int fineIdx(const Dims& D, int q) { int i, j, k; return D.cell(q, i, j, k); }
 
-// Column tiles: a block takes (CTX x CTY) columns of (i, j) through a range of k planes,
-// one thread per column, k ascending, so that the k - 1 and k + 1 neighbours a cell reads
-// were, or will next be, loaded by the same block (on large grids the plane-chunk order
-// fetches them from HBM again).  Work item v (tile v / nseg, k segment v % nseg) goes to
-// blocks in the XCD-aware order, so an XCD holds whole tiles.
-constexpr int CTX = 64, CTY = 16;
+// column tiles (ColMap, common.hpp): one thread per column of a tile
 static_assert(CTX * CTY == FB, "one thread per column of a tile");
-struct ColMap { int ntx, nty, nseg, kper; };
-static ColMap colMap(const Dims& D, int g) {
-    ColMap m;
-    m.ntx = (D.nI + CTX - 1) / CTX;
-    m.nty = (D.nJ + CTY - 1) / CTY;
-    m.nseg = std::max(1, g / (m.ntx * m.nty));
-    m.kper = (D.nK + m.nseg - 1) / m.nseg;
-    return m;
-}
 #define FOR_COLS(D, M, p)                                                                           \
     for (int v_ = xcdChunk(), vn_ = (M).ntx * (M).nty * (M).nseg; v_ < vn_; v_ += (int)gridDim.x)   \
         for (int t_ = v_ / (M).nseg, i_ = (t_ % (M).ntx) * CTX + (int)(threadIdx.x % CTX),          \
