@@ -39,7 +39,7 @@ EXPORTS = [
     "caffe_net_set_params", "caffe_net_forward", "caffe_net_backward", "caffe_net_precond", "caffe_net_loss_grad",
     "caffe_net_set_post_sweeps",
     "caffe_net_collect", "caffe_net_clear", "caffe_net_dataset_size", "caffe_net_train", "caffe_net_eval",
-    "caffe_ctx_attach_net", "caffe_ctx_set_fcg_coarse", "caffe_ctx_set_fcg_post_sweeps",
+    "caffe_ctx_attach_net", "caffe_ctx_set_fcg_coarse", "caffe_ctx_set_fcg_post_sweeps", "caffe_ctx_set_fcg_guess",
     # SparseMatrix / SparseVector compatibility path
     "caffe_matrix_create", "caffe_matrix_destroy", "caffe_matrix_last_error", "caffe_matrix_set_rows",
     "caffe_matrix_set_value", "caffe_matrix_zero_entries", "caffe_matrix_assemble", "caffe_matrix_solve",
@@ -143,6 +143,7 @@ def load(path: str = LIB_PATH):
         "caffe_ctx_attach_net": ([vp, vp], C.c_int),
         "caffe_ctx_set_fcg_coarse": ([vp, C.c_int], C.c_int),
         "caffe_ctx_set_fcg_post_sweeps": ([vp, C.c_int], C.c_int),
+        "caffe_ctx_set_fcg_guess": ([vp, C.c_int], C.c_int),
         "caffe_matrix_create": ([C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
                                  C.POINTER(C.c_void_p)], C.c_int),
         "caffe_matrix_destroy": ([vp], C.c_int),

@@ -417,6 +417,7 @@ bool fieldByName(Ctx& c, const std::string& n, FieldRef& f) {
 
 namespace caffe {
 float* falloc(Ctx& c, size_t n) { return talloc<float>(c, n); }
+double* dalloc64(Ctx& c, size_t n) { return talloc<double>(c, n); }
 }  // namespace caffe
 
 extern "C" {
@@ -757,6 +758,15 @@ int caffe_ctx_set_fcg_coarse(caffe_ctx* x, int mode) {
     return guard(x, [&](Ctx& c) {
         if (mode != 0 && mode != 1) { c.err = "coarse solve: 0 (the network) or 1 (a multigrid V-cycle)"; return CAFFE_ERR_ARG; }
         c.fcgCoarse = mode;
+        return CAFFE_OK;
+    });
+}
+
+int caffe_ctx_set_fcg_guess(caffe_ctx* x, int on) {
+    return guard(x, [&](Ctx& c) {
+        if (on != 0 && on != 1) { c.err = "initial guess of the learned solve: 0 (x = 0) or 1"; return CAFFE_ERR_ARG; }
+        c.fcgGuess = on;
+        if (!on) c.fcgPrevGen[0] = c.fcgPrevGen[1] = -1;
         return CAFFE_OK;
     });
 }

@@ -1447,6 +1447,33 @@ void pcorr_true_resid_dev(Ctx& c, double shift) {
     hipLaunchKernelGGL(k_finalize<3>, dim3(1), dim3(KBS), 0, c.stream, c.red.part, g, c.scal + SC_MISC + 8);
 }
 
+// the initial guess x = g xprev: q = A xprev with the partial sums (q, b - shift) and (q, q),
+// then the scaling, g = 0 when q = 0 (inactive cells keep x = 0)
+__global__ void k_guess_dots(Dims D, Sym4<double> A, const double* b, double shift, const double* xp, double* part) {
+    double acc[2] = {0., 0.};
+    FOR_CELLS(D, q, p) {
+        if (A.st && A.st[p] == ST_INACTIVE) continue;
+        auto xf = [&](int c) { return xp[c]; };
+        const double qv = A.apply(xf, p);
+        acc[0] += qv * (b[p] - shift);
+        acc[1] += qv * qv;
+    }
+    writePartials<KBS, 2>(acc, part);
+}
+__global__ void k_guess_scale(Dims D, const int8_t* st, const double* xp, double* x, const double* part, int nb) {
+    double t[2];
+    readPartials<KBS, 2>(part, nb, t);
+    const double g = t[1] > 0. ? t[0] / t[1] : 0.;
+    FOR_CELLS(D, q, p) { x[p] = (st && st[p] == ST_INACTIVE) ? 0. : g * xp[p]; }
+}
+void pcorr_scaled_guess(Ctx& c, const double* xprev, double shift) {
+    const int g = gridFor(c.D);
+    hipLaunchKernelGGL(k_guess_dots, dim3(g), dim3(KBS), 0, c.stream, c.D, symP<double>(c), (const double*)c.bp, shift, xprev,
+                       c.red.part);
+    hipLaunchKernelGGL(k_guess_scale, dim3(g), dim3(KBS), 0, c.stream, c.D, (const int8_t*)c.st, xprev, c.xp,
+                       (const double*)c.red.part, g);
+}
+
 // the solution's mean over the cells that carry a row, removed on the device
 void pcorr_remove_mean(Ctx& c) {
     const int g = gridFor(c.D);

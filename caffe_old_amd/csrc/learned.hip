@@ -936,6 +936,10 @@ int learned_fcg(Ctx& c, double shift, double bNorm, double rtol, int maxit, doub
         bb = std::sqrt(f[1]);
         rel = bb > 0. ? rr / bb : 0.;
     };
+    // the initial guess: the solution this slot's solve reached in the previous time step (or
+    // outer iteration), scaled to the smallest residual (at 128^3 the first corrector's
+    // systems start at a relative residual of 0.1-0.3 and take 11-12 iterations, not 14)
+    if (c.fcgGuess && c.fcgPrev[slot] && c.fcgPrevGen[slot] == c.stepGen - 1) pcorr_scaled_guess(c, c.fcgPrev[slot], shift);
     if (c.fcgPlanN[slot] > 0) {
         // the rounds of the slot's last solve, queued with their decisions on the device and
         // one read-back at the end
@@ -1055,6 +1059,11 @@ int learned_fcg(Ctx& c, double shift, double bNorm, double rtol, int maxit, doub
     c.fcgPlanN[slot] = (rel <= rtol && nRec > 0 && nRec <= FCG_MAX_PLAN) ? nRec : 0;
     c.fcgPlanB[slot] = bNorm;
     for (int q = 0; q < c.fcgPlanN[slot]; ++q) c.fcgPlan[slot][q] = recIts[q] + (recCapped[q] ? 1 : 0);
+    if (c.fcgGuess && rel <= rtol) {   // the next step's guess for this slot
+        if (!c.fcgPrev[slot]) c.fcgPrev[slot] = dalloc64(c, P);
+        HIPCHECK(hipMemcpyAsync(c.fcgPrev[slot], c.xp, (size_t)P * sizeof(double), hipMemcpyDeviceToDevice, s));
+        c.fcgPrevGen[slot] = c.stepGen;
+    }
     ++c.fcgSolves;
     *relOut = rel;
     return its;
