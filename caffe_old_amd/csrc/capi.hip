@@ -766,7 +766,7 @@ int caffe_ctx_set_fcg_guess(caffe_ctx* x, int on) {
     return guard(x, [&](Ctx& c) {
         if (on != 0 && on != 1) { c.err = "initial guess of the learned solve: 0 (x = 0) or 1"; return CAFFE_ERR_ARG; }
         c.fcgGuess = on;
-        if (!on) c.fcgPrevGen[0] = c.fcgPrevGen[1] = -1;
+        if (!on) for (auto& s : c.fcgPrevGen) s[0] = s[1] = -1;
         return CAFFE_OK;
     });
 }

@@ -176,12 +176,13 @@ struct Ctx {
     int fcgPlanN[2] = {0, 0};
     int fcgPlan[2][8] = {};
     double fcgPlanB[2] = {0., 0.};
-    // the initial guess of a learned solve: the solution of the same slot's solve in the
-    // previous time step (or outer iteration), scaled; stepGen counts momentum assemblies
+    // the initial guess of a learned solve: the best combination of the solutions of the
+    // same slot's solves in the two previous time steps (or outer iterations); per slot a
+    // ring of two, stepGen counts momentum assemblies
     int fcgGuess = 1;
     long stepGen = 0;
-    double* fcgPrev[2] = {nullptr, nullptr};
-    long fcgPrevGen[2] = {-1, -1};
+    double* fcgPrev[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    long fcgPrevGen[2][2] = {{-1, -1}, {-1, -1}};
 };
 
 // ---- launchers (fv_kernels.hip)
@@ -224,9 +225,9 @@ int momentum_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep);
 // lazyMs: the report's ms is left to caffe_last_reports (no wait for the solve's end)
 int pcorr_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep, bool lazyMs = false);
 void pcorr_remove_mean(Ctx& c);
-// x = g xprev with g = (q, b - shift) / (q, q), q = A xprev: the multiple of xprev whose
-// residual is smallest in the 2-norm (never above |b - shift|)
-void pcorr_scaled_guess(Ctx& c, const double* xprev, double shift);
+// x = g0 x0 + g1 x1 (x1 = null: g1 = 0), the combination whose residual b - shift - A x is
+// smallest in the 2-norm (never above |b - shift|)
+void pcorr_scaled_guess(Ctx& c, const double* x0, const double* x1, double shift);
 void pcorr_resolve_ms(Ctx& c);
 // the Coupled system: Jacobi BiCGStab over the joint (u, v, w, p) vector in FP64, to a true
 // relative residual rtol; p is returned with zero mean when no face fixes it

@@ -1447,31 +1447,60 @@ void pcorr_true_resid_dev(Ctx& c, double shift) {
     hipLaunchKernelGGL(k_finalize<3>, dim3(1), dim3(KBS), 0, c.stream, c.red.part, g, c.scal + SC_MISC + 8);
 }
 
-// the initial guess x = g xprev: q = A xprev with the partial sums (q, b - shift) and (q, q),
-// then the scaling, g = 0 when q = 0 (inactive cells keep x = 0)
-__global__ void k_guess_dots(Dims D, Sym4<double> A, const double* b, double shift, const double* xp, double* part) {
-    double acc[2] = {0., 0.};
+// The initial guess x = g0 x0 + g1 x1 that minimises |b - shift - A x|: q_i = A x_i with the
+// partial sums of the normal equations (q0,q0) (q0,q1) (q1,q1) (q0,b') (q1,b'), then the
+// 2 x 2 solve and the combination (g1 = 0 without x1, or when q0 and q1 are too close to
+// parallel for the solve; g = 0 when q0 = 0; inactive cells keep x = 0)
+template <bool TWO>
+__global__ void k_guess_dots(Dims D, Sym4<double> A, const double* b, double shift, const double* x0, const double* x1,
+                             double* part) {
+    double acc[5] = {0., 0., 0., 0., 0.};
     FOR_CELLS(D, q, p) {
         if (A.st && A.st[p] == ST_INACTIVE) continue;
-        auto xf = [&](int c) { return xp[c]; };
-        const double qv = A.apply(xf, p);
-        acc[0] += qv * (b[p] - shift);
-        acc[1] += qv * qv;
+        const double bv = b[p] - shift;
+        const double q0 = A.apply([&](int c) { return x0[c]; }, p);
+        acc[0] += q0 * q0;
+        acc[3] += q0 * bv;
+        if (TWO) {
+            const double q1 = A.apply([&](int c) { return x1[c]; }, p);
+            acc[1] += q0 * q1;
+            acc[2] += q1 * q1;
+            acc[4] += q1 * bv;
+        }
     }
-    writePartials<KBS, 2>(acc, part);
+    writePartials<KBS, 5>(acc, part);
 }
-__global__ void k_guess_scale(Dims D, const int8_t* st, const double* xp, double* x, const double* part, int nb) {
-    double t[2];
-    readPartials<KBS, 2>(part, nb, t);
-    const double g = t[1] > 0. ? t[0] / t[1] : 0.;
-    FOR_CELLS(D, q, p) { x[p] = (st && st[p] == ST_INACTIVE) ? 0. : g * xp[p]; }
+template <bool TWO>
+__global__ void k_guess_scale(Dims D, const int8_t* st, const double* x0, const double* x1, double* x, const double* part,
+                              int nb) {
+    double t[5];
+    readPartials<KBS, 5>(part, nb, t);
+    double g0 = t[0] > 0. ? t[3] / t[0] : 0., g1 = 0.;
+    if (TWO) {
+        const double det = t[0] * t[2] - t[1] * t[1];
+        if (det > 1e-10 * t[0] * t[2]) {
+            g0 = (t[3] * t[2] - t[1] * t[4]) / det;
+            g1 = (t[0] * t[4] - t[1] * t[3]) / det;
+        }
+    }
+    FOR_CELLS(D, q, p) {
+        const double v = TWO ? g0 * x0[p] + g1 * x1[p] : g0 * x0[p];
+        x[p] = (st && st[p] == ST_INACTIVE) ? 0. : v;
+    }
 }
-void pcorr_scaled_guess(Ctx& c, const double* xprev, double shift) {
+void pcorr_scaled_guess(Ctx& c, const double* x0, const double* x1, double shift) {
     const int g = gridFor(c.D);
-    hipLaunchKernelGGL(k_guess_dots, dim3(g), dim3(KBS), 0, c.stream, c.D, symP<double>(c), (const double*)c.bp, shift, xprev,
-                       c.red.part);
-    hipLaunchKernelGGL(k_guess_scale, dim3(g), dim3(KBS), 0, c.stream, c.D, (const int8_t*)c.st, xprev, c.xp,
-                       (const double*)c.red.part, g);
+    if (x1) {
+        hipLaunchKernelGGL(k_guess_dots<true>, dim3(g), dim3(KBS), 0, c.stream, c.D, symP<double>(c), (const double*)c.bp, shift,
+                           x0, x1, c.red.part);
+        hipLaunchKernelGGL(k_guess_scale<true>, dim3(g), dim3(KBS), 0, c.stream, c.D, (const int8_t*)c.st, x0, x1, c.xp,
+                           (const double*)c.red.part, g);
+    } else {
+        hipLaunchKernelGGL(k_guess_dots<false>, dim3(g), dim3(KBS), 0, c.stream, c.D, symP<double>(c), (const double*)c.bp,
+                           shift, x0, x0, c.red.part);
+        hipLaunchKernelGGL(k_guess_scale<false>, dim3(g), dim3(KBS), 0, c.stream, c.D, (const int8_t*)c.st, x0, x0, c.xp,
+                           (const double*)c.red.part, g);
+    }
 }
 
 // the solution's mean over the cells that carry a row, removed on the device

@@ -268,9 +268,9 @@ int caffe_ctx_set_fcg_coarse(caffe_ctx* ctx, int mode);
  * the second PISO corrector's systems carry most of.  No counterpart in the reference. */
 int caffe_ctx_set_fcg_post_sweeps(caffe_ctx* ctx, int sweeps);
 /* Initial guess of CAFFE_SOLVE_LEARNED_FCG solves of ctx: 1 (default) starts a solve from
- * the solution its plan slot (the corrector) reached in the previous time step or outer
- * iteration, scaled by g = (q, b)/(q, q), q = A x_prev, so that the starting residual is
- * never above |b|; 0 starts from x = 0 as SparseMatrix::solve does (SparseMatrix.cpp:86-106).
+ * the combination g0 x1 + g1 x2 of the solutions its plan slot (the corrector) reached in
+ * the two previous time steps or outer iterations whose residual |b - A x| is smallest (so
+ * never above |b|); 0 starts from x = 0 as SparseMatrix::solve does (SparseMatrix.cpp:86-106).
  * Either way the solve ends at the same true FP64 relative residual.  No counterpart in the
  * reference. */
 int caffe_ctx_set_fcg_guess(caffe_ctx* ctx, int on);

@@ -214,11 +214,12 @@ def test_learned_piso_steps_match_plain():
 
 
 def test_previous_step_guess():
-    """Learned solves started from the same corrector's solution of the previous step
-    (scaled to the smallest residual) against solves from x = 0, over twelve PISO steps of
-    the 128^3 cavity at Re 1000 from rest: every solve meets rtol, the fields agree to 1e-6,
-    and once the flow has developed (steps 7-12: consecutive corrections are alike) the first
-    corrector's solves take fewer iterations (measured 11-12 against 14)."""
+    """Learned solves started from the best combination of the same corrector's solutions
+    in the two previous steps (smallest residual) against solves from x = 0, over twelve
+    PISO steps of the 128^3 cavity at Re 1000 from rest: every solve meets rtol, the fields
+    agree to 1e-6, and once the flow has developed (steps 7-12: consecutive corrections are
+    alike) the first corrector's solves take fewer iterations (measured 11-13 against 14,
+    10 by step 13)."""
     from caffe_old_amd import _lib as L
     from caffe_old_amd import case as cs
     from caffe_old_amd.learned import PressureNet, trained_weights_path
