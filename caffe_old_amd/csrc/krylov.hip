@@ -1455,14 +1455,28 @@ template <bool TWO>
 __global__ void k_guess_dots(Dims D, Sym4<double> A, const double* b, double shift, const double* x0, const double* x1,
                              double* part) {
     double acc[5] = {0., 0., 0., 0., 0.};
+    const int P = D.P, sJ = D.sJ, sK = D.sK;
     FOR_CELLS(D, q, p) {
         if (A.st && A.st[p] == ST_INACTIVE) continue;
         const double bv = b[p] - shift;
-        const double q0 = A.apply([&](int c) { return x0[c]; }, p);
+        double q0, q1 = 0.;
+        if (A.ov.rowOf) {   // ghost rows (the learned solve takes none): the generic rows
+            q0 = A.apply([&](int c) { return x0[c]; }, p);
+            if (TWO) q1 = A.apply([&](int c) { return x1[c]; }, p);
+        } else {            // the seven coefficients loaded once for both products, in apply's order
+            const double a[7] = {A.A[p], A.A[P + p], A.A[P + p + 1], A.A[2 * P + p], A.A[2 * P + p + sJ], A.A[3 * P + p],
+                                 A.A[3 * P + p + sK]};
+            const int o[7] = {0, -1, 1, -sJ, sJ, -sK, sK};
+            auto prod = [&](const double* x) {
+                return a[0] * x[p + o[0]] + a[1] * x[p + o[1]] + a[2] * x[p + o[2]] + a[3] * x[p + o[3]] + a[4] * x[p + o[4]]
+                     + a[5] * x[p + o[5]] + a[6] * x[p + o[6]];
+            };
+            q0 = prod(x0);
+            if (TWO) q1 = prod(x1);
+        }
         acc[0] += q0 * q0;
         acc[3] += q0 * bv;
         if (TWO) {
-            const double q1 = A.apply([&](int c) { return x1[c]; }, p);
             acc[1] += q0 * q1;
             acc[2] += q1 * q1;
             acc[4] += q1 * bv;
