@@ -237,11 +237,14 @@ def cavity_piso_128(n: int = 128, steps: int = 100) -> Case:
 def channel_ib_sphere(ni: int = 256, nj: int = 64, nk: int = 64, steps: int = 500) -> Case:
     """Box 4x1x1, uniform inlet u = 1 (west), zero-gradient outlet (east, p = 0), no-slip walls,
     sphere diameter 0.2 at (1.0, 0.5, 0.5), Re_D 200 (mu = 1e-3), PISO 2 correctors,
-    dt = 0.5 h / U."""
+    dt = 0.5 h / U, alpha_u = 0.7, alpha_p = 0.3 (recorded choice: the configuration leaves
+    them open; with 1 / 1 IbPiso, the oracle's restatement and this library alike, diverges
+    within about ten steps on the 64x16x16, 128x32x32 and 256x64x64 channels, with 0.7 / 0.3
+    it stays bounded over the 40 steps measured)."""
     nodes = cartesian_nodes(4.0, 1.0, 1.0, ni, nj, nk)
     b = [Boundary("outlet", (0, 0, 0), 0.0), Boundary("inlet", (1.0, 0, 0), 0.0)] + [Boundary("wall") for _ in range(4)]
     h = 4.0 / ni
-    return Case(nodes=nodes, algo="ibpiso", time_accurate=True, rho=1.0, mu=1e-3, alpha_u=1.0, alpha_p=1.0,
+    return Case(nodes=nodes, algo="ibpiso", time_accurate=True, rho=1.0, mu=1e-3, alpha_u=0.7, alpha_p=0.3,
                 n_inner=1, n_pcorr=2, boundaries=b, u_init=(1.0, 0.0, 0.0), p_init=0.0,
                 sphere=(0.1, (1.0, 0.5, 0.5)), time_step=0.5 * h, max_iterations=steps + 1)
 

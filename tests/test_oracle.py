@@ -390,3 +390,19 @@ def test_ib_flags_and_ghost_rows():
         # eight weights are kept (columns of SOLID cells are dropped)
         if len(c) == 9 or (len(c) == 8 and gr in c):
             assert abs(v.sum() - 2.0) < 1e-10
+
+
+def test_ib_channel_relaxation_choice():
+    """The IB channel's recorded relaxation (alpha_u 0.7, alpha_p 0.3; the configuration
+    leaves it open): with 1 / 1 IbPiso (IbPiso.cpp:43-61, Piso's correctors without
+    neighbour correction) diverges on the 64 x 16 x 16 channel within 15 steps, with
+    0.7 / 0.3 the velocity stays bounded."""
+    def max_u(au, ap, steps):
+        c = cs.channel_ib_sphere(64, 16, 16, steps=steps)
+        c.alpha_u, c.alpha_p = au, ap
+        o = po.Oracle(c, rtol=1e-10, n_threads=4, n_blocks=4)
+        for _ in range(steps):
+            o.solve()
+        return float(np.abs(interior(o.get("u"), c.shape)).max())
+    assert max_u(1.0, 1.0, 15) > 100.0
+    assert max_u(0.7, 0.3, 15) < 1.2
