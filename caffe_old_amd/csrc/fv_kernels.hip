@@ -175,7 +175,8 @@ __global__ void k_mom_assemble(Geo g, MomArgs a) {
 }
 
 void launch_mom_assemble(Ctx& c, double dt) {
-    ++c.stepGen;   // a new time step (or outer iteration): the learned solve's guesses age by one
+    ++c.stepGen;   // a new time step (or outer iteration): the p' solves' guesses age by one
+    c.pcSolvesThisStep = 0;
     MomArgs a;
     a.st = c.st; a.mI = c.mF[0]; a.mJ = c.mF[1]; a.mK = c.mF[2];
     for (int q = 0; q < 3; ++q) { a.u[q] = c.u[q]; a.u0[q] = c.u0[q]; a.gp[q] = c.gp[q]; }

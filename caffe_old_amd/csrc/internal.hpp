@@ -133,7 +133,7 @@ struct Ctx {
 
     caffe_solve_opts momOpts{}, pcOpts{};
     caffe_solve_report lastMom{}, lastPc{};
-    int pcSolvesThisStep = 0;
+    int pcSolvesThisStep = 0;  // p' solves since the last momentum assembly (the corrector)
     bool ibReady = false;     // ghost rows computed for the current case
     double courant = 0.;      // last Courant number (Simple.cpp:88)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -176,9 +176,9 @@ struct Ctx {
     int fcgPlanN[2] = {0, 0};
     int fcgPlan[2][8] = {};
     double fcgPlanB[2] = {0., 0.};
-    // the initial guess of a learned solve: the best combination of the solutions of the
-    // same slot's solves in the two previous time steps (or outer iterations); per slot a
-    // ring of two, stepGen counts momentum assemblies
+    // the initial guess of a learned or multigrid p' solve: the best combination of the
+    // solutions of the same corrector's solves in the two previous time steps (or outer
+    // iterations); per corrector a ring of two, stepGen counts momentum assemblies
     int fcgGuess = 1;
     long stepGen = 0;
     double* fcgPrev[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
@@ -228,6 +228,10 @@ void pcorr_remove_mean(Ctx& c);
 // x = g0 x0 + g1 x1 (x1 = null: g1 = 0), the combination whose residual b - shift - A x is
 // smallest in the 2-norm (never above |b - shift|)
 void pcorr_scaled_guess(Ctx& c, const double* x0, const double* x1, double shift);
+// the guess of this p' solve (the solutions of its corrector in the two previous steps), if
+// any; and, after a converged solve, its solution kept for the next steps
+void pcorr_apply_guess(Ctx& c, double shift);
+void pcorr_save_guess(Ctx& c);
 void pcorr_resolve_ms(Ctx& c);
 // the Coupled system: Jacobi BiCGStab over the joint (u, v, w, p) vector in FP64, to a true
 // relative residual rtol; p is returned with zero mean when no face fixes it

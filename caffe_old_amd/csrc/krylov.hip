@@ -1502,6 +1502,32 @@ __global__ void k_guess_scale(Dims D, const int8_t* st, const double* x0, const 
         x[p] = (st && st[p] == ST_INACTIVE) ? 0. : v;
     }
 }
+// The p' solves' initial guesses.  The k-th p' solve after a momentum assembly (the k-th
+// corrector) takes slot min(k, 2) - 1 of a ring per slot holding the solutions of the two
+// previous steps; solves past the algorithm's correctors in a step (a caller solving the same
+// system again) neither take nor leave a guess, so they start from x = 0.
+static int guessSlot(const Ctx& c) {
+    const int nPC = c.cs.algo == CAFFE_SIMPLE ? 1 : std::max(c.cs.nPCorr, 1);
+    const int k = c.pcSolvesThisStep;   // 1 for the first solve after the assembly
+    return c.fcgGuess && k >= 1 && k <= nPC ? std::min(k, 2) - 1 : -1;
+}
+void pcorr_apply_guess(Ctx& c, double shift) {
+    const int slot = guessSlot(c);
+    if (slot < 0) return;
+    const int e1 = (int)((c.stepGen - 1) & 1), e2 = (int)((c.stepGen - 2) & 1);
+    const double* x1 = c.fcgPrevGen[slot][e1] == c.stepGen - 1 ? c.fcgPrev[slot][e1] : nullptr;
+    const double* x2 = c.fcgPrevGen[slot][e2] == c.stepGen - 2 ? c.fcgPrev[slot][e2] : nullptr;
+    if (x1) pcorr_scaled_guess(c, x1, x2, shift);
+}
+void pcorr_save_guess(Ctx& c) {
+    const int slot = guessSlot(c);
+    if (slot < 0) return;
+    const int en = (int)(c.stepGen & 1);
+    if (!c.fcgPrev[slot][en]) c.fcgPrev[slot][en] = dalloc64(c, c.D.P);
+    HIPCHECK(hipMemcpyAsync(c.fcgPrev[slot][en], c.xp, (size_t)c.D.P * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    c.fcgPrevGen[slot][en] = c.stepGen;
+}
+
 void pcorr_scaled_guess(Ctx& c, const double* x0, const double* x1, double shift) {
     const int g = gridFor(c.D);
     if (x1) {
@@ -1537,6 +1563,7 @@ void pcorr_resolve_ms(Ctx& c) {
 
 int pcorr_solve(Ctx& c, const caffe_solve_opts& o, caffe_solve_report* rep, bool lazyMs) {
     c.pcMsPending = false;   // a pending time belongs to the report this solve replaces
+    ++c.pcSolvesThisStep;
     HIPCHECK(hipEventRecord(c.evP0, c.stream));
     const Dims& D = c.D;
     const int P = D.P;

@@ -936,17 +936,12 @@ int learned_fcg(Ctx& c, double shift, double bNorm, double rtol, int maxit, doub
         bb = std::sqrt(f[1]);
         rel = bb > 0. ? rr / bb : 0.;
     };
-    // the initial guess: the combination of the solutions this slot's solves reached in the
-    // two previous time steps (or outer iterations) with the smallest residual.  At 128^3,
-    // once the flow has developed, the first corrector's systems then start at a relative
-    // residual of about 0.03 (0.1 from the last solution alone) and take 10-11 iterations,
-    // not 14.  Ring entry (gen & 1) holds the solution of step gen.
-    if (c.fcgGuess) {
-        const int e1 = (int)((c.stepGen - 1) & 1), e2 = (int)((c.stepGen - 2) & 1);
-        const double* x1 = c.fcgPrevGen[slot][e1] == c.stepGen - 1 ? c.fcgPrev[slot][e1] : nullptr;
-        const double* x2 = c.fcgPrevGen[slot][e2] == c.stepGen - 2 ? c.fcgPrev[slot][e2] : nullptr;
-        if (x1) pcorr_scaled_guess(c, x1, x2, shift);
-    }
+    // the initial guess: the combination of the solutions this corrector's solves reached in
+    // the two previous time steps (or outer iterations) with the smallest residual.  At
+    // 128^3, once the flow has developed, the first corrector's systems then start at a
+    // relative residual of about 0.03 (0.1 from the last solution alone) and take 10-11
+    // iterations, not 14.
+    pcorr_apply_guess(c, shift);
     if (c.fcgPlanN[slot] > 0) {
         // the rounds of the slot's last solve, queued with their decisions on the device and
         // one read-back at the end
@@ -1066,12 +1061,7 @@ int learned_fcg(Ctx& c, double shift, double bNorm, double rtol, int maxit, doub
     c.fcgPlanN[slot] = (rel <= rtol && nRec > 0 && nRec <= FCG_MAX_PLAN) ? nRec : 0;
     c.fcgPlanB[slot] = bNorm;
     for (int q = 0; q < c.fcgPlanN[slot]; ++q) c.fcgPlan[slot][q] = recIts[q] + (recCapped[q] ? 1 : 0);
-    if (c.fcgGuess && rel <= rtol) {   // the next steps' guesses for this slot
-        const int e = (int)(c.stepGen & 1);
-        if (!c.fcgPrev[slot][e]) c.fcgPrev[slot][e] = dalloc64(c, P);
-        HIPCHECK(hipMemcpyAsync(c.fcgPrev[slot][e], c.xp, (size_t)P * sizeof(double), hipMemcpyDeviceToDevice, s));
-        c.fcgPrevGen[slot][e] = c.stepGen;
-    }
+    if (rel <= rtol) pcorr_save_guess(c);   // the next steps' guess for this corrector
     ++c.fcgSolves;
     *relOut = rel;
     return its;

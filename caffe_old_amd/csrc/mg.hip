@@ -1083,11 +1083,13 @@ int mg_bicgstab(Ctx& c, double shift, double rtol, int maxit, int* refs, double*
     *refs = 0;
     if (c.indefiniteP) return -1;   // the coarse Laplacians do not model a reaction term
     if (!mgPlan(c)) return -1;
-    pcorr_make_fp32(c);
-    mgBuild(c);
+    mgBuild(c);   // on the FP32 copy of the matrix pcorr_solve made
     const int P = c.D.P;
     double* e = c.wm + 7 * (size_t)P;
     const double* r64 = c.wp + 6 * (size_t)P;
+    // the initial guess, as in the learned solve: the least-residual combination of the
+    // solutions this corrector reached in the two previous steps
+    pcorr_apply_guess(c, shift);
     double rr, bb;
     pcorr_true_resid(c, shift, &rr, &bb);
     double rel = bb > 0. ? rr / bb : 0.;
@@ -1109,6 +1111,7 @@ int mg_bicgstab(Ctx& c, double shift, double rtol, int maxit, int* refs, double*
         ++*refs;
         if (!(rel < prev)) break;
     }
+    if (rel <= rtol) pcorr_save_guess(c);   // the next steps' guess for this corrector
     *relOut = rel;
     return its;
 }

@@ -98,8 +98,8 @@ class Solver:
         self._check(self.lib.caffe_ctx_set_fcg_post_sweeps(self._h, int(sweeps)), "caffe_ctx_set_fcg_post_sweeps")
 
     def set_fcg_guess(self, on: bool):
-        """learned solves start from the best combination of the same corrector's solutions in
-        the two previous steps (True, default) or from x = 0"""
+        """learned and multigrid p' solves start from the best combination of the same
+        corrector's solutions in the two previous steps (True, default) or from x = 0"""
         self._check(self.lib.caffe_ctx_set_fcg_guess(self._h, 1 if on else 0), "caffe_ctx_set_fcg_guess")
 
     def set_solve_opts(self, which: str, mode: int = L.SOLVE_PLAIN_FP64, rtol: float = 1e-8, maxit: int = 50000,

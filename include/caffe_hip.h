@@ -267,12 +267,13 @@ int caffe_ctx_set_fcg_coarse(caffe_ctx* ctx, int mode);
  * one; the later ones damp the high-frequency error the coarse correction leaves, which
  * the second PISO corrector's systems carry most of.  No counterpart in the reference. */
 int caffe_ctx_set_fcg_post_sweeps(caffe_ctx* ctx, int sweeps);
-/* Initial guess of CAFFE_SOLVE_LEARNED_FCG solves of ctx: 1 (default) starts a solve from
- * the combination g0 x1 + g1 x2 of the solutions its plan slot (the corrector) reached in
- * the two previous time steps or outer iterations whose residual |b - A x| is smallest (so
- * never above |b|); 0 starts from x = 0 as SparseMatrix::solve does (SparseMatrix.cpp:86-106).
- * Either way the solve ends at the same true FP64 relative residual.  No counterpart in the
- * reference. */
+/* Initial guess of the CAFFE_SOLVE_LEARNED_FCG and CAFFE_SOLVE_MG p' solves of ctx: 1
+ * (default) starts the k-th p' solve after a momentum assembly (the k-th corrector) from the
+ * combination g0 x1 + g1 x2 of the solutions the k-th solve reached in the two previous time
+ * steps or outer iterations whose residual |b - A x| is smallest (so never above |b|); solves
+ * past the algorithm's correctors start from x = 0.  0 starts every solve from x = 0 as
+ * SparseMatrix::solve does (SparseMatrix.cpp:86-106).  Either way the solve ends at the same
+ * true FP64 relative residual.  No counterpart in the reference. */
 int caffe_ctx_set_fcg_guess(caffe_ctx* ctx, int on);
 
 /* ---------------------------------------------------------------------------------
