@@ -213,18 +213,20 @@ def test_learned_piso_steps_match_plain():
     net.close()
 
 
-def test_previous_step_guess():
+@pytest.mark.parametrize("mode", ["learned", "mg"])
+def test_previous_step_guess(mode):
     """Learned solves started from the best combination of the same corrector's solutions
     in the two previous steps (smallest residual) against solves from x = 0, over twelve
     PISO steps of the 128^3 cavity at Re 1000 from rest: every solve meets rtol, the fields
     agree to 1e-6, and once the flow has developed (steps 7-12: consecutive corrections are
-    alike) the first corrector's solves take fewer iterations (measured 11-13 against 14,
-    10 by step 13)."""
+    alike) the first corrector's solves take fewer iterations (learned FCG: measured 11-13
+    against 14, 10 by step 13; multigrid BiCGStab: 4 against 5)."""
     from caffe_old_amd import _lib as L
     from caffe_old_amd import case as cs
     from caffe_old_amd.learned import PressureNet, trained_weights_path
     from caffe_old_amd.solver import make_solver
     net = PressureNet(seed=0).load(trained_weights_path(128))
+    md = L.SOLVE_LEARNED_FCG if mode == "learned" else L.SOLVE_MG
     out = {}
     for guess in (False, True):
         c = cs.cavity_piso_128(128, steps=14)
@@ -240,7 +242,7 @@ def test_previous_step_guess():
             s.rhie_chow()
             for corr in range(2):
                 s.pcorr_assemble()
-                r = s.pcorr_solve(mode=L.SOLVE_LEARNED_FCG, rtol=1e-8, maxit=100000, inner_rtol=1e-4)
+                r = s.pcorr_solve(mode=md, rtol=1e-8, maxit=100000, inner_rtol=1e-4)
                 assert r["converged"] and r["fell_back"] == 0 and r["relres"] <= 1e-8, (guess, step, corr, r)
                 its.append(r["iters"])
                 s.correct()
@@ -250,7 +252,7 @@ def test_previous_step_guess():
         s.close()
     a, b = out[True], out[False]
     c1_on, c1_off = a["its"][12::2], b["its"][12::2]   # first correctors of steps 7-12
-    print(f"iterations per solve, guess on {a['its']}, off {b['its']}")
+    print(f"{mode}: iterations per solve, guess on {a['its']}, off {b['its']}")
     assert sum(c1_on) < sum(c1_off)
     for f in ("u", "p"):
         e = _rel(a[f], b[f])
