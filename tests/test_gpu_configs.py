@@ -74,11 +74,15 @@ def test_config1_simple_32():
 
 
 @pytest.mark.timeout(900)
+@pytest.mark.timeout(600)
 def test_config2_piso_128_learned_solves():
+    """Config 2 at full size, 4 steps from rest with the learned p' solves (from step 3 on
+    started from the combination of the two previous steps' solutions) and FP32-round
+    momentum solves, against the tight oracle after every step."""
     from caffe_old_amd import _lib as L
     from caffe_old_amd.learned import PressureNet, trained_weights_path
     from oracle.pyoracle import Oracle
-    case = cs.cavity_piso_128(128, steps=2)
+    case = cs.cavity_piso_128(128, steps=4)
     assert case.shape == (128, 128, 128) and case.mu == 1e-3 and case.time_step == 3.90625e-3
     assert case.n_pcorr == 2 and case.algo == "piso"
     net = PressureNet(seed=0).load(trained_weights_path(128))
