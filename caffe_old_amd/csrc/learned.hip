@@ -489,10 +489,10 @@ __global__ void k_f_post(Dims D, SymF S, ColMap cm, const float* __restrict__ di
     const int cx = D.nI >> 1, cy = D.nJ >> 1, cz = D.nK >> 1;
     const float ax = 0.75f, bx = 0.25f;
     struct Plane { float r, d, w, q, a[7]; };
-    auto loadPlane = [&](int p, Plane& L) {
+    auto loadPlane = [&](int p, Plane& L, bool a5) {   // a5 false: a[5] is the plane below's a[6]
         L.r = r[p]; L.d = dinv[p]; L.w = wpre[p]; L.q = PART ? qold[p] : 0.f;
         L.a[0] = A[p]; L.a[1] = A[P + p]; L.a[2] = A[P + p + 1]; L.a[3] = A[2 * P + p];
-        L.a[4] = A[2 * P + p + S.sJ]; L.a[5] = A[3 * P + p]; L.a[6] = A[3 * P + p + sK];
+        L.a[4] = A[2 * P + p + S.sJ]; if (a5) L.a[5] = A[3 * P + p]; L.a[6] = A[3 * P + p + sK];
     };
     double v[3] = {0., 0., 0.};
     for (int v_ = xcdChunk(), vn = cm.ntx * cm.nty * cm.nseg; v_ < vn; v_ += (int)gridDim.x) {
@@ -503,7 +503,7 @@ __global__ void k_f_post(Dims D, SymF S, ColMap cm, const float* __restrict__ di
         if (!(i < D.nI && j < D.nJ) || k >= ke) continue;
         int p = D.pidx(i, j, k);
         Plane cur, nxt;
-        loadPlane(p, cur);
+        loadPlane(p, cur, true);
         // the 3x3 coarse patch around (i >> 1, j >> 1), indices clamped as pw clamps them
         const int X = i >> 1, Y = j >> 1;
         const int xa = max(X - 1, 0), xb = X, xc = min(X + 1, cx - 1);
@@ -545,7 +545,7 @@ __global__ void k_f_post(Dims D, SymF S, ColMap cm, const float* __restrict__ di
         float sd = k > 0 ? sAt(0, k - 1) : 0.f;
         float sc = sAt(0, k);
         for (; k < ke; ++k, p += sK) {
-            if (k + 1 < ke) loadPlane(p + sK, nxt);
+            if (k + 1 < ke) { loadPlane(p + sK, nxt, false); nxt.a[5] = cur.a[6]; }
             const float su = k + 1 < D.nK ? sAt(0, k + 1) : 0.f;
             const float sw = inW ? sAt(1, k) : 0.f, se = inE ? sAt(2, k) : 0.f;
             const float ss = inS ? sAt(3, k) : 0.f, sn = inN ? sAt(4, k) : 0.f;
@@ -586,13 +586,13 @@ __global__ void k_f_sweep(Dims D, SymF S, ColMap cm, const float* __restrict__ d
     const int P = S.P, sJ = S.sJ, sK = S.sK;
     const float* __restrict__ A = S.A;
     struct Plane { float ys[4], yu, a[7], r, d, q; };
-    auto loadPlane = [&](int p, Plane& L) {
+    auto loadPlane = [&](int p, Plane& L, bool a5) {   // a5 false: a[5] is the plane below's a[6]
         const int o[4] = {-1, 1, -sJ, sJ};
 #pragma unroll
         for (int m = 0; m < 4; ++m) L.ys[m] = y[p + o[m]];
         L.yu = y[p + sK];
         L.a[0] = A[p]; L.a[1] = A[P + p]; L.a[2] = A[P + p + 1]; L.a[3] = A[2 * P + p];
-        L.a[4] = A[2 * P + p + sJ]; L.a[5] = A[3 * P + p]; L.a[6] = A[3 * P + p + sK];
+        L.a[4] = A[2 * P + p + sJ]; if (a5) L.a[5] = A[3 * P + p]; L.a[6] = A[3 * P + p + sK];
         L.r = r[p]; L.d = dinv[p]; L.q = PART ? qold[p] : 0.f;
     };
     double v[3] = {0., 0., 0.};
@@ -605,9 +605,9 @@ __global__ void k_f_sweep(Dims D, SymF S, ColMap cm, const float* __restrict__ d
         int p = D.pidx(i, j, k);
         float yd = y[p - sK], yc = y[p];
         Plane cur, nxt;
-        loadPlane(p, cur);
+        loadPlane(p, cur, true);
         for (; k < ke; ++k, p += sK) {
-            if (k + 1 < ke) loadPlane(p + sK, nxt);
+            if (k + 1 < ke) { loadPlane(p + sK, nxt, false); nxt.a[5] = cur.a[6]; }
             const float Ay = cur.a[0] * yc + (cur.a[1] * cur.ys[0] + cur.a[2] * cur.ys[1] + cur.a[3] * cur.ys[2]
                                               + cur.a[4] * cur.ys[3] + cur.a[5] * yd + cur.a[6] * cur.yu);
             const float zz = yc + NET_OMEGA * cur.d * (cur.r - Ay);
@@ -647,13 +647,13 @@ __global__ void k_f_k1(Dims D, SymF S, ColMap cm, const float* z, float* p0, flo
     const int P = S.P, sJ = S.sJ, sK = S.sK;
     const float* A = S.A;
     struct Plane { float zs[4], ps[4], zu, pu, a[7]; };   // sides W E S N, k + 1 (u), matrix
-    auto loadPlane = [&](int p, Plane& L) {
+    auto loadPlane = [&](int p, Plane& L, bool a5) {   // a5 false: a[5] is the plane below's a[6]
         const int o[4] = {-1, 1, -sJ, sJ};
 #pragma unroll
         for (int m = 0; m < 4; ++m) { L.zs[m] = z[p + o[m]]; L.ps[m] = pold[p + o[m]]; }
         L.zu = z[p + sK]; L.pu = pold[p + sK];
         L.a[0] = A[p]; L.a[1] = A[P + p]; L.a[2] = A[P + p + 1]; L.a[3] = A[2 * P + p];
-        L.a[4] = A[2 * P + p + sJ]; L.a[5] = A[3 * P + p]; L.a[6] = A[3 * P + p + sK];
+        L.a[4] = A[2 * P + p + sJ]; if (a5) L.a[5] = A[3 * P + p]; L.a[6] = A[3 * P + p + sK];
     };
     // work item v: this thread's column (i, j) and its k range [k, ke); false if empty
     auto item = [&](int v_, int& p, int& k, int& ke) {
@@ -672,7 +672,7 @@ __global__ void k_f_k1(Dims D, SymF S, ColMap cm, const float* z, float* p0, flo
     Plane cur, nxt;
     if (live) {
         zd = z[p - sK]; pd = pold[p - sK]; zc = z[p]; pc = pold[p];
-        loadPlane(p, cur);
+        loadPlane(p, cur, true);
     }
     double t[3];
     readPartials<FB, 3>(partIn, nb, t);
@@ -688,12 +688,12 @@ __global__ void k_f_k1(Dims D, SymF S, ColMap cm, const float* z, float* p0, flo
             live = item(v_, p, k, ke);
             if (live) {
                 zd = z[p - sK]; pd = pold[p - sK]; zc = z[p]; pc = pold[p];
-                loadPlane(p, cur);
+                loadPlane(p, cur, true);
             }
         }
         if (!live) continue;
         for (; k < ke; ++k, p += sK) {
-            if (k + 1 < ke) loadPlane(p + sK, nxt);
+            if (k + 1 < ke) { loadPlane(p + sK, nxt, false); nxt.a[5] = cur.a[6]; }
             const float fc = f(zc, pc);
             const float qq = cur.a[0] * fc + (cur.a[1] * f(cur.zs[0], cur.ps[0]) + cur.a[2] * f(cur.zs[1], cur.ps[1])
                                               + cur.a[3] * f(cur.zs[2], cur.ps[2]) + cur.a[4] * f(cur.zs[3], cur.ps[3])
@@ -732,13 +732,13 @@ __global__ void k_f_k2pre(Dims D, SymF S, ColMap cm, const float* __restrict__ d
     const int P = S.P, sJ = S.sJ, sK = S.sK;
     const float* __restrict__ A = S.A;
     struct Plane { float rs[4], qs[4], ds[4], ru, qu, du, a[7], ee, pp; };   // sides W E S N, k + 1 (u)
-    auto loadPlane = [&](int p, Plane& L) {
+    auto loadPlane = [&](int p, Plane& L, bool a5) {   // a5 false: a[5] is the plane below's a[6]
         const int o[4] = {-1, 1, -sJ, sJ};
 #pragma unroll
         for (int m = 0; m < 4; ++m) { L.rs[m] = rc[p + o[m]]; L.qs[m] = qv[p + o[m]]; L.ds[m] = dinv[p + o[m]]; }
         L.ru = rc[p + sK]; L.qu = qv[p + sK]; L.du = dinv[p + sK];
         L.a[0] = A[p]; L.a[1] = A[P + p]; L.a[2] = A[P + p + 1]; L.a[3] = A[2 * P + p];
-        L.a[4] = A[2 * P + p + sJ]; L.a[5] = A[3 * P + p]; L.a[6] = A[3 * P + p + sK];
+        L.a[4] = A[2 * P + p + sJ]; if (a5) L.a[5] = A[3 * P + p]; L.a[6] = A[3 * P + p + sK];
         L.ee = e[p]; L.pp = pv[p];
     };
     auto item = [&](int v_, int& p, int& k, int& ke) {
@@ -759,7 +759,7 @@ __global__ void k_f_k2pre(Dims D, SymF S, ColMap cm, const float* __restrict__ d
     if (live) {
         rd = rc[p - sK]; qd = qv[p - sK]; dd = dinv[p - sK];
         rcc = rc[p]; qcc = qv[p]; dcc = dinv[p];
-        loadPlane(p, cur);
+        loadPlane(p, cur, true);
     }
     double t[2];
     readPartials<FB, 2>(partIn, nb, t);
@@ -775,14 +775,14 @@ __global__ void k_f_k2pre(Dims D, SymF S, ColMap cm, const float* __restrict__ d
             if (live) {
                 rd = rc[p - sK]; qd = qv[p - sK]; dd = dinv[p - sK];
                 rcc = rc[p]; qcc = qv[p]; dcc = dinv[p];
-                loadPlane(p, cur);
+                loadPlane(p, cur, true);
             }
         }
         if (!live) continue;
         float zd = z1(dd, rd, qd);
         float zc = z1(dcc, rcc, qcc);
         for (; k < ke; ++k, p += sK) {
-            if (k + 1 < ke) loadPlane(p + sK, nxt);
+            if (k + 1 < ke) { loadPlane(p + sK, nxt, false); nxt.a[5] = cur.a[6]; }
             const float rp = rnew(rcc, qcc);
             const float zu = z1(cur.du, cur.ru, cur.qu);
             const float Az = cur.a[0] * zc + (cur.a[1] * z1(cur.ds[0], cur.rs[0], cur.qs[0])
