@@ -523,7 +523,7 @@ def main():
         out["config"]["full_run_100_steps"] = full_run(a.n, mode, inner[a.mode], net, mom_mode, a.fcg_post)
     if not a.no_cpu:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(a.n, steps=2, warmup=a.warmup)
+        out["cpu_baseline"] = cpu_baseline(a.n, steps=4, warmup=a.warmup)
         log("field errors against the oracle ...")
         out["field_error_vs_oracle"] = field_errors(a.n, mode, inner[a.mode], net, mom_mode, post=a.fcg_post)
     print(json.dumps(out), flush=True)
