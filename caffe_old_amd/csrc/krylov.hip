@@ -359,9 +359,12 @@ __global__ void k_bi_b_band(Dims D, BandT<T> A, const T* __restrict__ r, const T
 // KA and KB on the band without ghost rows, on column tiles (ColMap): each thread walks
 // its column up k and keeps the operand it formed at k - 1 and k (p = r + beta (p_old -
 // omega v_old) in KA, s = r - alpha v in KB) and D^-1 there in registers, so a cell loads
-// the operand's arrays at its four side neighbours and above only.  The plane-chunk order
-// of k_bi_a_band re-fetched 1.46x its algorithmic reads from below L2 at 128^3 (k_bi_b_band
-// 1.22x).  The arithmetic of each cell is k_bi_a_band's / k_bi_b_band's.
+// the operand's arrays at its four side neighbours and above only (KA: 60 loads per cell
+// for three components instead of 80).  At 128^3 that takes KA from 70-73 to 62-66 us and KB
+// from 52 to 45 us with three live components, although the counted reads from below L2
+// grow (KA 1.46x -> 1.63x its algorithmic reads, KB 1.22x -> 1.44x: the tile-edge
+// neighbours): the kernels are bound by their loads' latency, not by those bytes.  The
+// arithmetic of each cell is k_bi_a_band's / k_bi_b_band's.
 static_assert(CTX * CTY == KBS, "one thread per column of a tile");
 template <class T>
 __global__ __launch_bounds__(KBS) void k_bi_a_col(Dims D, BandT<T> A, ColMap cm, const T* __restrict__ r,
