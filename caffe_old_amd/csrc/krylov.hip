@@ -362,7 +362,7 @@ __global__ void k_bi_b_band(Dims D, BandT<T> A, const T* __restrict__ r, const T
 // the operand's arrays at its four side neighbours and above only (KA: 60 loads per cell
 // for three components instead of 80).  At 128^3 that takes KA from 70-73 to 62-66 us and KB
 // from 52 to 45 us with three live components, although the counted reads from below L2
-// grow (KA 1.43x -> 1.59x its algorithmic reads, KB 1.20x -> 1.41x: the tile-edge
+// grow (KA 1.43x -> 1.59x its algorithmic reads, KB 1.21x -> 1.41x: the tile-edge
 // neighbours): the kernels are bound by their loads' latency, not by those bytes.  The
 // arithmetic of each cell is k_bi_a_band's / k_bi_b_band's.
 static_assert(CTX * CTY == KBS, "one thread per column of a tile");
